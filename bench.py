@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Throughput of the JANC 2-D reacting-Euler step on one MI355X.
+
+One step = the reference's advance_func (simulation/set_simulator.py:444-447):
+CFL dt, three SSP-RK3 stages of WENO5 + global Lax-Friedrichs flux
+divergence with Halley temperature updates, point-implicit 9-species /
+19-reaction H2-air chemistry and the final temperature solve.
+
+Headline (``value``): the FP64 plain step on the 4000 x 400 (1.6M-cell)
+periodic-in-x detonation annulus, inputs resident in HBM, K steps timed
+between two device synchronisations.  Also reported: the mixed-precision and
+FP64 steps on the 8000 x 500 (4M-cell) tube, the per-kernel HIP-event times
+with the roofline of the dominant kernel, and the FP64 CPU oracle timed on a
+200 x 80 crop of the headline workload on this host.
+
+Prints ONE JSON line.  Single GPU only.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = ("cell-updates/s per RK3 step w/ chemistry: mixed vs FP64 at stated err; "
+          "adjoint/fwd")
+HBM_PEAK_GBS = 8000.0
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+
+# Algorithmic HBM words per cell per launch (FP64: 8 B/word, mixed: 4 B/word).
+#   stage 1: read U (12) + T (1), write U1 (12) + aux (2)         = 27
+#   stage 2/3: additionally read U^n (12)                          = 39
+#   chemistry: read U (12) + T (1), write rho*Y (8) + aux (2)      = 23
+WORDS = {"stage": (27 + 39 + 39) / 3.0, "chem": 23.0, "reduce": 13.0, "bc": 0.0}
+
+
+def _run(sim, U, aux, steps, warmup, torch):
+    """Warm up, then time `steps` fused steps; returns (seconds, per-kernel)."""
+    if warmup:
+        sim.advance(U, aux, warmup)
+    torch.cuda.synchronize()
+    sim.timing(U, enable=True)
+    sim.timing(U)                       # drop anything queued before
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, _, dts = sim.advance(U, aux, steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kt = sim.timing(U)
+    sim.timing(U, enable=False)
+    flags = sim.health(U)
+    return t1 - t0, kt, dts, flags
+
+
+def _roofline(kt, cells, word_bytes):
+    out = {}
+    for kind, (ms, n) in kt.items():
+        if n == 0 or WORDS.get(kind, 0.0) == 0.0:
+            continue
+        avg_s = ms / n / 1e3
+        bytes_per_launch = WORDS[kind] * word_bytes * cells
+        ach = bytes_per_launch / avg_s / 1e9
+        out[kind] = {"avg_ms": ms / n, "launches": n, "algorithmic_bytes": bytes_per_launch,
+                     "achieved_GBs": ach, "frac_hbm": ach / HBM_PEAK_GBS}
+    return out
+
+
+def _cpu_baseline(U, aux, geom, theta, steps):
+    """FP64 oracle (numpy, one core) on a 200 x 80 crop of the headline state."""
+    from oracle.step import Oracle
+    from janc_amd.mechanism import DEFAULT_MECH
+    crop = (slice(None), slice(0, 200), slice(0, 80))
+    Uc, ac = U[crop].copy(), aux[crop].copy()
+    o = Oracle(DEFAULT_MECH, geom["boundary_config"], geom["dx"], geom["dy"])
+    t = 0.0
+    o.advance_func(Uc, ac, t, 0.3, theta)          # warm caches
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        Uc, ac, t, _ = o.advance_func(Uc, ac, t, 0.3, theta)
+    el = time.perf_counter() - t0
+    cells = Uc.shape[1] * Uc.shape[2]
+    return {"value": cells * steps / el, "unit": "cell-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": "%d CFL steps of the FP64 numpy oracle on the 200x80 corner crop "
+                      "(seed columns included) of the 4000x400 headline state, same BC kinds; "
+                      "%.1f s" % (steps, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--mixed-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs the single-block step on one GPU; "
+              "--gpus N > 1 is not supported (see DESIGN.md for how it would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import numpy as np
+    import torch
+    from janc_amd import cases
+    from janc_amd.mechanism import Mechanism
+    from janc_amd.solver import Simulator
+
+    mech = Mechanism()
+    # ---- headline: FP64 on the 4000 x 400 annulus ----
+    U0, aux0, geom, theta = cases.rdc_annulus(mech)
+    nx, ny = U0.shape[1], U0.shape[2]
+    sim = Simulator(cases.simulation_config(geom, cfl=0.3, precision="fp64", theta=theta))
+    U = torch.as_tensor(U0).cuda()
+    aux = torch.as_tensor(aux0).cuda()
+    el, kt, dts, flags = _run(sim, U, aux, args.steps, args.warmup, torch)
+    cells = nx * ny
+    value = cells * args.steps / el
+    roof = _roofline(kt, cells, 8)
+    dom = max((k for k in roof), key=lambda k: kt[k][0])
+    r = roof[dom]
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "FP64 plain step, 4000x400 periodic-in-x detonation annulus "
+                               "(ZND-like seed 200 columns 30 atm/3000 K, 0.5% T noise), "
+                               "9sp-19r H2-air, CFL 0.3; bottom slip wall in place of the "
+                               "injection inlet",
+                   "nx": nx, "ny": ny, "cells": cells, "precision": "fp64",
+                   "dt_first": float(dts[0]), "dt_last": float(dts[-1]),
+                   "health_flags": flags},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": r["achieved_GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["frac_hbm"],
+                     "traffic": None},
+        "kernels_fp64": {k: {"ms_total": v[0], "launches": v[1]} for k, v in kt.items()},
+        "roofline_all_fp64": roof,
+    }
+    del U, aux
+    torch.cuda.empty_cache()
+
+    # ---- mixed vs FP64 on the 8000 x 500 tube ----
+    if not args.no_mixed:
+        U3, aux3, g3 = cases.mixed_tube(mech)
+        n3 = U3.shape[1] * U3.shape[2]
+        out = {}
+        for prec, dtype in (("mixed", torch.float32), ("fp64", torch.float64)):
+            s3 = Simulator(cases.simulation_config(g3, cfl=0.3, precision=prec))
+            Ut = torch.as_tensor(U3, dtype=dtype).cuda()
+            at = torch.as_tensor(aux3, dtype=dtype).cuda()
+            e3, k3, d3, f3 = _run(s3, Ut, at, args.mixed_steps, args.warmup, torch)
+            out[prec] = {"value": n3 * args.mixed_steps / e3, "ms_per_step": e3 / args.mixed_steps * 1e3,
+                         "kernels": {k: {"ms_total": v[0], "launches": v[1]} for k, v in k3.items()},
+                         "roofline": _roofline(k3, n3, 4 if prec == "mixed" else 8),
+                         "health_flags": f3}
+            del Ut, at
+            torch.cuda.empty_cache()
+        out["speedup_mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+        out["workload"] = ("8000x500 tube, phi in [0.8,1.2] smooth field, 40-column "
+                           "20 atm/2800 K ignition slab, %d CFL steps" % args.mixed_steps)
+        result["mixed_tube"] = out
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = _cpu_baseline(U0, aux0, geom, theta, args.cpu_steps)
+        result["speedup_vs_cpu"] = value / result["cpu_baseline"]["value"]
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,133 @@
+/*
+ * janc_step.h -- C ABI of the MI355X (gfx950) JANC step library.
+ *
+ * One context = one configured 2-D single-block solver: grid, boundary kinds,
+ * precision form and the mechanism constants.  Every state pointer is a
+ * caller-owned device pointer to a C-order (var, nx, ny) array with y
+ * contiguous -- the reference's U (12, nx, ny) and aux (2, nx, ny) layout,
+ * so torch/JAX contiguous tensors pass through DLPack without a copy.
+ * Element type: double for JH_PREC_FP64, float for JH_PREC_MIXED.
+ *
+ * All calls are asynchronous on the context's HIP stream and return a status
+ * code; jh_last_error() gives the message of the last failure.  A context must
+ * not be used from two threads at once; distinct contexts are independent.
+ * Reference paths below are relative to the luofx23/JANC source tree.
+ */
+#ifndef JANC_STEP_H
+#define JANC_STEP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct jh_ctx jh_ctx;
+
+enum jh_status {
+  JH_OK = 0,
+  JH_ERR_ARG = 1,      /* null pointer, bad size or bad enum */
+  JH_ERR_MECH = 2,     /* mechanism constants do not match the compiled mechanism */
+  JH_ERR_HIP = 3,      /* a HIP runtime call failed */
+};
+
+enum jh_precision { JH_PREC_FP64 = 0, JH_PREC_MIXED = 1 };
+
+/* Boundary kinds, boundary/boundary.py:51-118 and
+ * boundary/boundary_conditions_2D/__init__.py:4-22.  JH_BC_USER leaves the
+ * side's ghost strip to the caller (a Python callable in the reference,
+ * boundary.py:52-57); see jh_ghost_strip(). */
+enum jh_bc_kind {
+  JH_BC_PERIODIC = 0,
+  JH_BC_ZERO_GRADIENT = 1,
+  JH_BC_SLIP_WALL = 2,
+  JH_BC_NO_SLIP_WALL = 3,
+  JH_BC_PRESSURE_OUTLET = 4,
+  JH_BC_USER = 5
+};
+
+/* Sides in the order bc_kind[] / bc_param[] use. */
+enum jh_side { JH_SIDE_LEFT = 0, JH_SIDE_RIGHT = 1, JH_SIDE_BOTTOM = 2, JH_SIDE_TOP = 3 };
+
+/* Number of doubles in the mechanism constant block the library was built
+ * for, and the digest of the compiled reaction structure.  Replaces the
+ * Cantera read of preprocess/load.py:8-135 (the block is produced by
+ * janc_amd/mechanism.py). */
+int jh_mech_const_count(void);
+const char* jh_mech_digest(void);
+int jh_num_species(void);
+
+/* Create a context.  Replaces Simulator.__init__'s set_thermo / set_reaction /
+ * set_flux_solver / set_boundary (simulation/set_simulator.py:387-396).
+ *   bc_kind[4]  : jh_bc_kind per side (left, right, bottom, top)
+ *   bc_param[4] : theta['Pb'] for JH_BC_PRESSURE_OUTLET sides, else ignored
+ *   mech        : host array of jh_mech_const_count() doubles
+ *   stream      : hipStream_t (NULL = default stream) */
+int jh_ctx_create(int nx, int ny, double dx, double dy, const int* bc_kind,
+                  const double* bc_param, int precision, const double* mech,
+                  int n_mech, void* stream, jh_ctx** out);
+int jh_ctx_destroy(jh_ctx* ctx);
+/* Per-call boundary parameter (theta['Pb'] of pressure_outlet.py:24-66 is a
+ * call argument in the reference). */
+int jh_ctx_set_bc_param(jh_ctx* ctx, int side, double value);
+int jh_ctx_set_stream(jh_ctx* ctx, void* stream);
+const char* jh_last_error(const jh_ctx* ctx);
+
+/* Advance nsteps steps.  One step is the reference's advance_func
+ * (simulation/set_simulator.py:444-447): dt = CFL_2D(U, aux) (:438-443) when
+ * dt <= 0, else the fixed dt (:405-408); then advance_one_step (:117-122):
+ * TVD_RK3 (time_step/TVD_RK3.py:5-12) of flux_func (:44-47), point-implicit
+ * chemistry (model/reaction_model.py:183-203) and update_aux.
+ * U_out/aux_out may alias U/aux.  dt_hist (device, nsteps doubles) receives
+ * each step's dt; may be NULL. */
+int jh_advance(jh_ctx* ctx, const void* U, const void* aux, void* U_out, void* aux_out,
+               double dt, double cfl, int nsteps, double* dt_hist);
+
+/* flux_func (set_simulator.py:44-47): rhs = dt * total_flux(BC(U, aux))
+ * on the interior, 12 planes. */
+int jh_rhs(jh_ctx* ctx, const void* U, const void* aux, double dt, void* rhs_out);
+
+/* update_aux (solver_2D/aux_func.py:21-29): Halley solve of T from e with
+ * aux_in's T as the initial guess; writes (gamma, T). aux_out may alias aux_in. */
+int jh_update_aux(jh_ctx* ctx, const void* U, const void* aux_in, void* aux_out);
+
+/* reaction_source_terms (model/reaction_model.py:183-203): S (12 planes;
+ * planes 0-3 are zero). */
+int jh_reaction_source(jh_ctx* ctx, const void* U, const void* aux, double dt, void* S_out);
+
+/* CFL_2D (set_simulator.py:438-443) -> *dt_dev (device double). */
+int jh_cfl_dt(jh_ctx* ctx, const void* U, const void* aux, double cfl, double* dt_dev);
+
+/* The LF maxima of LF.py:10-11 over the ghost-padded field of (U, aux):
+ * alpha_dev[0] = nanmax(|u|+a), alpha_dev[1] = nanmax(|v|+a) (device). */
+int jh_lf_alpha(jh_ctx* ctx, const void* U, const void* aux, double* alpha_dev);
+
+/* Device pointer and element count of one side's ghost strip (14 planes of
+ * 3 x len: vars 0-11 = U, 12-13 = aux; left/right strips are (14, 3, ny) with
+ * layer 0 outermost on the left and innermost on the right; bottom/top are
+ * (14, 3, nx) likewise).  A JH_BC_USER side must be written here before the
+ * stage that reads it (boundary.py:52-57 callables). */
+int jh_ghost_strip(jh_ctx* ctx, int side, void** ptr, int* count);
+
+/* Kernel classes of jh_timing_read(). */
+enum jh_kernel_kind {
+  JH_KERNEL_REDUCE = 0,   /* prologue: CFL and stage-1 LF maxima of a new state */
+  JH_KERNEL_BC = 1,       /* ghost strips + ghost LF maxima, once per RK stage */
+  JH_KERNEL_STAGE = 2,    /* fused WENO5/LF/divergence/RK combine/Halley, per stage */
+  JH_KERNEL_CHEM = 3,     /* point-implicit chemistry + final Halley, per step */
+  JH_KERNEL_KINDS = 4
+};
+
+/* Time every launch of jh_advance with HIP events on the context stream
+ * (no counterpart in the reference, which has no profiling hooks). */
+int jh_timing_enable(jh_ctx* ctx, int on);
+/* Sum of event-measured milliseconds and launch counts per jh_kernel_kind
+ * since the last read (ms[4], counts[4]); synchronises the stream. */
+int jh_timing_read(jh_ctx* ctx, double* ms, int* counts);
+
+/* Health flags accumulated on the device since the last call (JH_FLAG_*
+ * bits: 1 = non-finite value, 2 = rho <= 0); synchronises the stream. */
+int jh_health(jh_ctx* ctx, unsigned int* flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JANC_STEP_H */

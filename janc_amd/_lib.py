@@ -1,0 +1,71 @@
+"""ctypes binding of libjanc_amd.so (declarations: include/janc_step.h).
+
+The library is the only compute path: there is no CPU fallback.  Loading it
+fails loudly when it has not been built (``python -m janc_amd.build``).
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libjanc_amd.so")
+
+JH_OK, JH_ERR_ARG, JH_ERR_MECH, JH_ERR_HIP = 0, 1, 2, 3
+PREC_FP64, PREC_MIXED = 0, 1
+BC_KINDS = {"periodic": 0, "zero_gradient": 1, "slip_wall": 2, "no_slip_wall": 3,
+            "pressure_outlet": 4, "user": 5}
+SIDES = ("left", "right", "bottom", "top")
+KERNEL_KINDS = ("reduce", "bc", "stage", "chem")
+
+_c_int, _c_double, _vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+# name -> (restype, argtypes); every symbol the public header declares
+SIGNATURES = {
+    "jh_mech_const_count": (_c_int, []),
+    "jh_mech_digest": (ctypes.c_char_p, []),
+    "jh_num_species": (_c_int, []),
+    "jh_ctx_create": (_c_int, [_c_int, _c_int, _c_double, _c_double, ctypes.POINTER(_c_int),
+                               ctypes.POINTER(_c_double), _c_int, ctypes.POINTER(_c_double),
+                               _c_int, _vp, ctypes.POINTER(_vp)]),
+    "jh_ctx_destroy": (_c_int, [_vp]),
+    "jh_ctx_set_bc_param": (_c_int, [_vp, _c_int, _c_double]),
+    "jh_ctx_set_stream": (_c_int, [_vp, _vp]),
+    "jh_last_error": (ctypes.c_char_p, [_vp]),
+    "jh_advance": (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_double, _c_double, _c_int, _vp]),
+    "jh_rhs": (_c_int, [_vp, _vp, _vp, _c_double, _vp]),
+    "jh_update_aux": (_c_int, [_vp, _vp, _vp, _vp]),
+    "jh_reaction_source": (_c_int, [_vp, _vp, _vp, _c_double, _vp]),
+    "jh_cfl_dt": (_c_int, [_vp, _vp, _vp, _c_double, _vp]),
+    "jh_lf_alpha": (_c_int, [_vp, _vp, _vp, _vp]),
+    "jh_ghost_strip": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_int)]),
+    "jh_health": (_c_int, [_vp, ctypes.POINTER(ctypes.c_uint)]),
+    "jh_timing_enable": (_c_int, [_vp, _c_int]),
+    "jh_timing_read": (_c_int, [_vp, ctypes.POINTER(_c_double), ctypes.POINTER(_c_int)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the step library once; raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("janc_amd: %s is not built; run `python -m janc_amd.build`"
+                           % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class JancError(RuntimeError):
+    pass
+
+
+def check(rc, ctx=None):
+    if rc != JH_OK:
+        msg = load().jh_last_error(ctx)
+        raise JancError("janc_amd error %d: %s" % (rc, msg.decode() if msg else ""))

@@ -1,0 +1,200 @@
+"""Synthetic initial states for the benchmark and test configurations.
+
+Host-side input preparation (numpy, FP64); the state is built like the
+reference notebooks build theirs (``examples/janc_basic_example2.ipynb``
+``initial_conditions``): T, p, Y -> rho = p/(R T), rho*E = rho*(h - R T) + KE,
+aux = (gamma(T, Y), T).  Species order is the mechanism's phase order
+(H2, O2, OH, O, H, H2O, HO2, H2O2, N2).
+"""
+import numpy as np
+
+from .mechanism import Mechanism
+
+P_ATM = 101325.0
+SLIP = "slip_wall"
+
+
+def premixed_Y(mech, phi):
+    """Mass fractions of H2-air with molar H2:O2:N2 = 2 phi : 1 : 3.76."""
+    phi = np.asarray(phi, dtype=np.float64)
+    X = np.zeros((mech.ns,) + phi.shape)
+    X[mech.species.index("H2")] = 2.0 * phi
+    X[mech.species.index("O2")] = 1.0
+    X[mech.species.index("N2")] = 3.76
+    M = mech.molecular_weight.reshape((-1,) + (1,) * phi.ndim)
+    m = X * M
+    return m / m.sum(axis=0, keepdims=True)
+
+
+def mixture(mech, T, Y9):
+    """(R, cp, h, gamma) of the mixture at T (NASA-7, per unit mass)."""
+    T = np.asarray(T, dtype=np.float64)
+    ext = (1,) * T.ndim
+    R = np.sum((1.0 / mech.species_M).reshape((-1,) + ext) * Y9, axis=0)
+    lo = T < mech.Tcr
+    cp = np.zeros_like(T)
+    h = np.zeros_like(T)
+    for r, sel in ((0, lo), (1, ~lo)):
+        cpk = np.zeros((mech.ns,) + T.shape)
+        hk = np.zeros((mech.ns,) + T.shape)
+        for j in range(5):
+            cpk = cpk * T + mech.cp[r][:, j].reshape((-1,) + ext)
+        for j in range(6):
+            hk = hk * T + mech.h[r][:, j].reshape((-1,) + ext)
+        cp = np.where(sel, np.sum(cpk * Y9, axis=0), cp)
+        h = np.where(sel, np.sum(hk * Y9, axis=0), h)
+    return R, cp, h, cp / (cp - R)
+
+
+def equilibrium_products(mech, phi=1.0):
+    """Ideal complete-combustion products of H2-air (H2O + excess + N2)."""
+    Y = premixed_Y(mech, phi)
+    M = mech.molecular_weight
+    i = {s: mech.species.index(s) for s in mech.species}
+    nH2 = Y[i["H2"]] / M[i["H2"]]
+    nO2 = Y[i["O2"]] / M[i["O2"]]
+    burn = np.minimum(nH2, 2.0 * nO2)
+    out = np.array(Y, copy=True)
+    out[i["H2"]] = (nH2 - burn) * M[i["H2"]]
+    out[i["O2"]] = (nO2 - 0.5 * burn) * M[i["O2"]]
+    out[i["H2O"]] = burn * M[i["H2O"]]
+    return out
+
+
+def build_state(mech, T, p, Y9, u=None, v=None):
+    """(U (12, nx, ny), aux (2, nx, ny)) float64 from T, p, Y9 fields."""
+    R, cp, h, gamma = mixture(mech, T, Y9)
+    rho = p / (R * T)
+    u = np.zeros_like(T) if u is None else u
+    v = np.zeros_like(T) if v is None else v
+    rhoE = rho * h - p + 0.5 * rho * (u * u + v * v)
+    U = np.concatenate([rho[None], (rho * u)[None], (rho * v)[None], rhoE[None],
+                        rho[None] * Y9[:-1]], axis=0)
+    aux = np.stack([gamma, T])
+    return U, aux
+
+
+def _noise(rng, shape, amp):
+    return 1.0 + amp * rng.uniform(-1.0, 1.0, shape)
+
+
+def detonation_tube(mech, nx=400, ny=80, dx=1e-4, seed=1, driver=20,
+                    p_drv=30 * P_ATM, T_drv=3000.0, phi=1.0, rho_noise=0.01):
+    """Config 1 (and the tube of config 3 at other sizes / phi fields).
+
+    Stoichiometric H2-air at 1 atm / 300 K at rest; ``driver`` columns at
+    p_drv / T_drv with the unburned composition; rho *= 1 +- rho_noise with T
+    held (so p = rho R T changes).  BCs: slip left, zero-gradient right, slip
+    bottom and top.
+    """
+    rng = np.random.default_rng(seed)
+    T = np.full((nx, ny), 300.0)
+    p = np.full((nx, ny), P_ATM)
+    T[:driver] = T_drv
+    p[:driver] = p_drv
+    phi = np.broadcast_to(np.asarray(phi, dtype=np.float64), (nx, ny))
+    Y9 = premixed_Y(mech, phi)
+    R, _, _, _ = mixture(mech, T, Y9)
+    rho = p / (R * T) * _noise(rng, (nx, ny), rho_noise)
+    U, aux = build_state(mech, T, rho * R * T, Y9)
+    bc = {"left_boundary": SLIP, "right_boundary": "zero_gradient",
+          "bottom_boundary": SLIP, "top_boundary": SLIP}
+    return U, aux, dict(dx=dx, dy=dx, boundary_config=bc)
+
+
+def rdc_annulus(mech, nx=4000, ny=400, dx=1e-4, seed=2, seed_cols=200,
+                p_seed=30 * P_ATM, T_seed=3000.0, Pb=P_ATM, t_noise=0.005):
+    """Config 2 geometry: a periodic-in-x unrolled annulus of fresh
+    stoichiometric H2-air (1 atm, 300 K) with a ZND-like seed over
+    ``seed_cols`` columns (p_seed / T_seed, complete-combustion products) and
+    T *= 1 +- t_noise.  Top: pressure outlet at Pb.  The bottom injection inlet
+    of the configuration is represented by a slip wall (the inlet model is not
+    built yet).  Returns (U, aux, geometry, theta)."""
+    rng = np.random.default_rng(seed)
+    T = np.full((nx, ny), 300.0)
+    p = np.full((nx, ny), P_ATM)
+    T[:seed_cols] = T_seed
+    p[:seed_cols] = p_seed
+    T = T * _noise(rng, (nx, ny), t_noise)
+    Y9 = premixed_Y(mech, np.ones((nx, ny)))
+    Yb = equilibrium_products(mech, 1.0)
+    Y9[:, :seed_cols] = Yb[:, None, None] if Yb.ndim == 1 else Yb
+    U, aux = build_state(mech, T, p, Y9)
+    bc = {"left_boundary": "periodic", "right_boundary": "periodic",
+          "bottom_boundary": SLIP, "top_boundary": "pressure_outlet"}
+    return U, aux, dict(dx=dx, dy=dx, boundary_config=bc), {"Pb": Pb}
+
+
+def smooth_field(rng, nx, ny, modes=4, kmax=3):
+    """Sum of random-phase Fourier modes (wavenumber <= kmax) scaled to [-1, 1]."""
+    x = np.linspace(0.0, 1.0, nx)[:, None]
+    y = np.linspace(0.0, 1.0, ny)[None, :]
+    s = np.zeros((nx, ny))
+    for _ in range(modes):
+        kx, ky = rng.integers(0, kmax + 1, size=2)
+        ph = rng.uniform(0, 2 * np.pi, size=2)
+        s += np.cos(2 * np.pi * kx * x + ph[0]) * np.cos(2 * np.pi * ky * y + ph[1])
+    s -= s.min()
+    span = s.max()
+    return 2.0 * s / span - 1.0 if span > 0 else s
+
+
+def mixed_tube(mech, nx=8000, ny=500, dx=1e-4, seed=3):
+    """Config 3: phi from a seeded smooth field in [0.8, 1.2]; a 40-column
+    ignition slab at 20 atm / 2800 K; BCs as config 1."""
+    rng = np.random.default_rng(seed)
+    phi = 1.0 + 0.2 * smooth_field(rng, nx, ny)
+    return detonation_tube(mech, nx=nx, ny=ny, dx=dx, seed=seed, driver=40,
+                           p_drv=20 * P_ATM, T_drv=2800.0, phi=phi, rho_noise=0.0)
+
+
+def periodic_box(mech, nx=64, ny=32, dx=1e-4, seed=7, amp=0.05):
+    """Smooth reacting periodic box (all four sides periodic) for parity tests.
+
+    Cells 0 and N-1 of each axis carry equal values, the period-(N-1) field the
+    reference's periodic padding (boundary_padding.py:20-25) assumes.
+    """
+    rng = np.random.default_rng(seed)
+    def per(n):
+        return np.linspace(0.0, 2 * np.pi, n)
+    X, Yg = np.meshgrid(per(nx), per(ny), indexing="ij")
+    T = 1200.0 + 300.0 * np.sin(X + rng.uniform()) * np.cos(Yg)
+    p = P_ATM * (2.0 + amp * np.cos(2 * X) * np.sin(Yg + rng.uniform()))
+    u = 50.0 * np.sin(Yg)
+    v = 30.0 * np.cos(X)
+    Y9 = premixed_Y(mech, 1.0 + 0.1 * np.sin(X) * np.sin(Yg))
+    # a little of each radical so every reaction is active
+    for s, f in (("OH", 1e-3), ("H", 5e-4), ("O", 5e-4), ("H2O", 0.05), ("HO2", 1e-5),
+                 ("H2O2", 1e-6)):
+        Y9[mech.species.index(s)] = f * (1.0 + 0.5 * np.cos(X + Yg))
+    Y9[-1] = 0.0
+    Y9[-1] = 1.0 - Y9[:-1].sum(axis=0)
+    U, aux = build_state(mech, T, p, Y9, u, v)
+    bc = {s + "_boundary": "periodic" for s in ("left", "right", "bottom", "top")}
+    return U, aux, dict(dx=dx, dy=dx, boundary_config=bc)
+
+
+def simulation_config(geom, mech_path=None, cfl=0.3, dt=None, precision="fp64", t_end=1.0,
+                      theta=None):
+    """A reference-style simulation_config dict for janc_amd.Simulator."""
+    from .mechanism import DEFAULT_MECH
+    mech_path = mech_path or DEFAULT_MECH
+    tc = {"temporal_evolution_scheme": "TVD_RK3", "t_end": t_end}
+    if dt is not None:
+        tc["dt"] = dt
+    else:
+        tc["CFL"] = cfl
+    cfg = {
+        "dimension": "2D",
+        "grid_config": {"dx": geom["dx"], "dy": geom["dy"]},
+        "thermo_config": {"thermo_model": "nasa7", "mechanism_directory": mech_path},
+        "reaction_config": {"is_detailed_chemistry": True, "mechanism_directory": mech_path},
+        "flux_config": {"solver_type": "flux_splitting", "interface_reconstruction": "WENO5_JS",
+                        "split_method": "LF", "viscosity": "off"},
+        "boundary_config": geom["boundary_config"],
+        "time_config": tc,
+        "computation_config": {"precision": precision},
+    }
+    if theta is not None:
+        cfg["solver_parameters"] = theta
+    return cfg

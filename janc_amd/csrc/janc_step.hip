@@ -1,0 +1,427 @@
+// C ABI of the gfx950 JANC step library: context management and the launch
+// sequence of one SSP-RK3 + point-implicit-chemistry step.  Declarations and
+// the reference entry points each call replaces are in include/janc_step.h.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/janc_step.h"
+#include "stage.hpp"
+
+using namespace jh;
+
+struct jh_ctx {
+  int nx = 0, ny = 0;
+  double dx = 0, dy = 0;
+  int precision = JH_PREC_FP64;
+  int bc[4] = {0, 0, 0, 0};
+  double bc_p[4] = {0, 0, 0, 0};
+  hipStream_t stream = nullptr;
+  MechConst* mc = nullptr;        // device
+  StepScalars* sc = nullptr;      // device
+  void* ghost[4] = {nullptr, nullptr, nullptr, nullptr};
+  void* tmpU[2] = {nullptr, nullptr};
+  void* tmpA[2] = {nullptr, nullptr};
+  int parity = 0;
+  std::string err;
+  // kernel timing with HIP events on the context stream (jh_timing_*)
+  bool timing = false;
+  std::vector<hipEvent_t> ev_free;
+  struct Mark { int kind; hipEvent_t a, b; };
+  std::vector<Mark> marks;
+};
+
+namespace {
+
+thread_local std::string g_create_err;
+
+int fail(jh_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg; else g_create_err = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, call)                                                     \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail(ctx, JH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+size_t elem_size(const jh_ctx* c) { return c->precision == JH_PREC_FP64 ? 8 : 4; }
+
+int check_launch(jh_ctx* c, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, JH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return JH_OK;
+}
+
+hipEvent_t take_event(jh_ctx* c) {
+  if (!c->ev_free.empty()) {
+    hipEvent_t e = c->ev_free.back();
+    c->ev_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one launch with events when timing is on.
+struct Timed {
+  jh_ctx* c;
+  int kind;
+  hipEvent_t a = nullptr;
+  Timed(jh_ctx* c_, int k) : c(c_), kind(k) {
+    if (c->timing && (a = take_event(c))) (void)hipEventRecord(a, c->stream);
+  }
+  ~Timed() {
+    if (!a) return;
+    hipEvent_t b = take_event(c);
+    if (!b) return;
+    (void)hipEventRecord(b, c->stream);
+    c->marks.push_back({kind, a, b});
+  }
+};
+
+template <typename Real>
+StepArgs<Real> make_args(jh_ctx* c, double dt_fixed, double cfl) {
+  StepArgs<Real> a;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.dx = c->dx;
+  a.dy = c->dy;
+  a.cfl = cfl;
+  a.dt_fixed = dt_fixed;
+  a.parity = c->parity;
+  a.mc = c->mc;
+  a.sc = c->sc;
+  for (int s = 0; s < 4; ++s) {
+    a.g[s] = static_cast<Real*>(c->ghost[s]);
+    a.bc[s] = c->bc[s];
+    a.bc_p[s] = c->bc_p[s];
+  }
+  return a;
+}
+
+int grid_cells(const jh_ctx* c) {
+  const int n = c->nx * c->ny;
+  int b = (n + 255) / 256;
+  return b < 2048 ? b : 2048;
+}
+
+int stage_blocks(const jh_ctx* c) {
+  return ((c->nx + TXR - 1) / TXR) * ((c->ny + TY - 1) / TY);
+}
+
+int bc_blocks(const jh_ctx* c) { return (2 * c->nx + 2 * c->ny + 255) / 256; }
+
+// Prologue for a state handed in by the caller: zero this parity's slots and
+// reduce the interior maxima (stage-1 LF alpha, CFL).
+template <typename Real>
+int prologue(jh_ctx* c, const Real* U, const Real* aux, double dt, double cfl) {
+  Timed tm(c, JH_KERNEL_REDUCE);
+  reset_kernel<<<1, 64, 0, c->stream>>>(c->sc, c->parity);
+  auto a = make_args<Real>(c, dt, cfl);
+  reduce_kernel<Real><<<grid_cells(c), 256, 0, c->stream>>>(a, U, aux, c->parity, 1);
+  return check_launch(c, "reduce_kernel");
+}
+
+template <typename Real>
+int advance_impl(jh_ctx* c, const Real* U, const Real* aux, Real* Uo, Real* auxo, double dt,
+                 double cfl, int nsteps, double* dt_hist) {
+  const size_t n = (size_t)c->nx * c->ny;
+  Real* U1 = static_cast<Real*>(c->tmpU[0]);
+  Real* U2 = static_cast<Real*>(c->tmpU[1]);
+  Real* A1 = static_cast<Real*>(c->tmpA[0]);
+  Real* A2 = static_cast<Real*>(c->tmpA[1]);
+  int rc = prologue<Real>(c, U, aux, dt, cfl);
+  if (rc) return rc;
+  const Real* Un = U;
+  const Real* An = aux;
+  for (int s = 0; s < nsteps; ++s) {
+    auto a = make_args<Real>(c, dt, cfl);
+    const int gb = stage_blocks(c), bb = bc_blocks(c);
+    // stage 1: (Un, An) -> (U1, A1)
+    { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, 256, 0, c->stream>>>(a, Un, An, 0); }
+    { Timed t(c, JH_KERNEL_STAGE);
+      stage_kernel<Real, MODE_RK><<<gb, NT, 0, c->stream>>>(a, 0, Un, An, Un, U1, A1); }
+    // stage 2: (U1, A1) -> (U2, A2)
+    { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, 256, 0, c->stream>>>(a, U1, A1, 1); }
+    { Timed t(c, JH_KERNEL_STAGE);
+      stage_kernel<Real, MODE_RK><<<gb, NT, 0, c->stream>>>(a, 1, U1, A1, Un, U2, A2); }
+    // stage 3: (U2, A2) -> (Uo, auxo); Uo may alias Un (each thread reads
+    // Un at its own cell before writing it)
+    { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, 256, 0, c->stream>>>(a, U2, A2, 2); }
+    { Timed t(c, JH_KERNEL_STAGE);
+      stage_kernel<Real, MODE_RK><<<gb, NT, 0, c->stream>>>(a, 2, U2, A2, Un, Uo, auxo); }
+    // chemistry + final update_aux in place, next step's maxima
+    { Timed t(c, JH_KERNEL_CHEM);
+      chem_kernel<Real, CHEM_STEP><<<grid_cells(c), 256, 0, c->stream>>>(
+          a, Uo, auxo, nullptr, dt_hist ? dt_hist + s : nullptr); }
+    rc = check_launch(c, "step kernels");
+    if (rc) return rc;
+    c->parity ^= 1;
+    Un = Uo;
+    An = auxo;
+  }
+  (void)n;
+  return JH_OK;
+}
+
+template <typename Real>
+int rhs_impl(jh_ctx* c, const Real* U, const Real* aux, double dt, Real* out) {
+  int rc = prologue<Real>(c, U, aux, dt, 0.0);
+  if (rc) return rc;
+  auto a = make_args<Real>(c, dt, 0.0);
+  bc_kernel<Real><<<bc_blocks(c), 256, 0, c->stream>>>(a, U, aux, 0);
+  stage_kernel<Real, MODE_RHS><<<stage_blocks(c), NT, 0, c->stream>>>(a, 0, U, aux, U, out,
+                                                                        nullptr);
+  return check_launch(c, "rhs kernels");
+}
+
+}  // namespace
+
+// Device-side copy of the few scalars that a host-side query needs.
+__global__ void copy_scalars_kernel(const StepScalars* sc, int parity, double cfl, double* dt,
+                                    double* alpha) {
+  if (threadIdx.x != 0) return;
+  if (dt) *dt = cfl * (1.0 / bitsd(sc->cmax[parity]));
+  if (alpha) {
+    alpha[0] = bitsd(sc->amax[parity][0][0]);
+    alpha[1] = bitsd(sc->amax[parity][0][1]);
+  }
+}
+
+extern "C" {
+
+int jh_mech_const_count(void) { return (int)(sizeof(MechConst) / sizeof(double)); }
+const char* jh_mech_digest(void) { return JH_MECH_DIGEST; }
+int jh_num_species(void) { return JH_NS; }
+
+int jh_ctx_create(int nx, int ny, double dx, double dy, const int* bc_kind,
+                  const double* bc_param, int precision, const double* mech, int n_mech,
+                  void* stream, jh_ctx** out) {
+  if (!out) return fail(nullptr, JH_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (nx < 4 || ny < 4) return fail(nullptr, JH_ERR_ARG, "nx and ny must be >= 4");
+  if ((long long)nx * ny * NU >= (1ll << 31))
+    return fail(nullptr, JH_ERR_ARG, "grid too large for 32-bit plane offsets");
+  if (!(dx > 0) || !(dy > 0)) return fail(nullptr, JH_ERR_ARG, "dx, dy must be > 0");
+  if (precision != JH_PREC_FP64 && precision != JH_PREC_MIXED)
+    return fail(nullptr, JH_ERR_ARG, "unknown precision");
+  if (!bc_kind || !mech) return fail(nullptr, JH_ERR_ARG, "bc_kind or mech is NULL");
+  if (n_mech != jh_mech_const_count())
+    return fail(nullptr, JH_ERR_MECH, "mechanism constant count mismatch");
+  for (int s = 0; s < 4; ++s)
+    if (bc_kind[s] < JH_BC_PERIODIC || bc_kind[s] > JH_BC_USER)
+      return fail(nullptr, JH_ERR_ARG, "unknown boundary kind");
+  jh_ctx* c = new jh_ctx();
+  c->nx = nx;
+  c->ny = ny;
+  c->dx = dx;
+  c->dy = dy;
+  c->precision = precision;
+  c->stream = static_cast<hipStream_t>(stream);
+  for (int s = 0; s < 4; ++s) {
+    c->bc[s] = bc_kind[s];
+    c->bc_p[s] = bc_param ? bc_param[s] : 0.0;
+  }
+  const size_t es = elem_size(c), n = (size_t)nx * ny;
+  hipError_t e = hipSuccess;
+  e = e ? e : hipMalloc(&c->mc, sizeof(MechConst));
+  e = e ? e : hipMalloc(&c->sc, sizeof(StepScalars));
+  for (int s = 0; s < 4 && !e; ++s) {
+    const size_t len = (s < 2) ? ny : nx;
+    e = hipMalloc(&c->ghost[s], es * NV * 3 * len);
+  }
+  for (int i = 0; i < 2 && !e; ++i) e = hipMalloc(&c->tmpU[i], es * NU * n);
+  for (int i = 0; i < 2 && !e; ++i) e = hipMalloc(&c->tmpA[i], es * 2 * n);
+  if (!e) e = hipMemcpy(c->mc, mech, sizeof(MechConst), hipMemcpyHostToDevice);
+  if (!e) e = hipMemset(c->sc, 0, sizeof(StepScalars));
+  for (int s = 0; s < 4 && !e; ++s) {
+    const size_t len = (s < 2) ? ny : nx;
+    e = hipMemset(c->ghost[s], 0, es * NV * 3 * len);
+  }
+  if (e) {
+    g_create_err = std::string("allocation: ") + hipGetErrorString(e);
+    jh_ctx_destroy(c);
+    return JH_ERR_HIP;
+  }
+  *out = c;
+  return JH_OK;
+}
+
+int jh_ctx_destroy(jh_ctx* c) {
+  if (!c) return JH_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream); else (void)hipDeviceSynchronize();
+  for (auto& m : c->marks) {
+    c->ev_free.push_back(m.a);
+    c->ev_free.push_back(m.b);
+  }
+  for (auto e : c->ev_free) (void)hipEventDestroy(e);
+  (void)hipFree(c->mc);
+  (void)hipFree(c->sc);
+  for (int s = 0; s < 4; ++s) (void)hipFree(c->ghost[s]);
+  for (int i = 0; i < 2; ++i) {
+    (void)hipFree(c->tmpU[i]);
+    (void)hipFree(c->tmpA[i]);
+  }
+  delete c;
+  return JH_OK;
+}
+
+int jh_ctx_set_bc_param(jh_ctx* c, int side, double value) {
+  if (!c) return JH_ERR_ARG;
+  if (side < 0 || side > 3) return fail(c, JH_ERR_ARG, "side out of range");
+  c->bc_p[side] = value;
+  return JH_OK;
+}
+
+int jh_ctx_set_stream(jh_ctx* c, void* stream) {
+  if (!c) return JH_ERR_ARG;
+  c->stream = static_cast<hipStream_t>(stream);
+  return JH_OK;
+}
+
+const char* jh_last_error(const jh_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+int jh_advance(jh_ctx* c, const void* U, const void* aux, void* U_out, void* aux_out, double dt,
+               double cfl, int nsteps, double* dt_hist) {
+  if (!c) return JH_ERR_ARG;
+  if (!U || !aux || !U_out || !aux_out) return fail(c, JH_ERR_ARG, "NULL state pointer");
+  if (nsteps < 0) return fail(c, JH_ERR_ARG, "nsteps < 0");
+  if (!(dt > 0) && !(cfl > 0)) return fail(c, JH_ERR_ARG, "need dt > 0 or cfl > 0");
+  for (int s = 0; s < 4; ++s)
+    if (c->bc[s] == JH_BC_USER)
+      return fail(c, JH_ERR_ARG, "jh_advance needs built-in boundary kinds on every side");
+  if (nsteps == 0) return JH_OK;
+  if (c->precision == JH_PREC_FP64)
+    return advance_impl<double>(c, (const double*)U, (const double*)aux, (double*)U_out,
+                                (double*)aux_out, dt, cfl, nsteps, dt_hist);
+  return advance_impl<float>(c, (const float*)U, (const float*)aux, (float*)U_out,
+                             (float*)aux_out, dt, cfl, nsteps, dt_hist);
+}
+
+int jh_rhs(jh_ctx* c, const void* U, const void* aux, double dt, void* out) {
+  if (!c) return JH_ERR_ARG;
+  if (!U || !aux || !out) return fail(c, JH_ERR_ARG, "NULL pointer");
+  if (!(dt > 0)) return fail(c, JH_ERR_ARG, "dt must be > 0");
+  for (int s = 0; s < 4; ++s)
+    if (c->bc[s] == JH_BC_USER) return fail(c, JH_ERR_ARG, "user boundary strips not filled");
+  if (c->precision == JH_PREC_FP64)
+    return rhs_impl<double>(c, (const double*)U, (const double*)aux, dt, (double*)out);
+  return rhs_impl<float>(c, (const float*)U, (const float*)aux, dt, (float*)out);
+}
+
+int jh_update_aux(jh_ctx* c, const void* U, const void* aux_in, void* aux_out) {
+  if (!c) return JH_ERR_ARG;
+  if (!U || !aux_in || !aux_out) return fail(c, JH_ERR_ARG, "NULL pointer");
+  if (c->precision == JH_PREC_FP64) {
+    auto a = make_args<double>(c, 1.0, 0.0);
+    update_aux_kernel<double><<<grid_cells(c), 256, 0, c->stream>>>(
+        a, (const double*)U, (const double*)aux_in, (double*)aux_out);
+  } else {
+    auto a = make_args<float>(c, 1.0, 0.0);
+    update_aux_kernel<float><<<grid_cells(c), 256, 0, c->stream>>>(
+        a, (const float*)U, (const float*)aux_in, (float*)aux_out);
+  }
+  return check_launch(c, "update_aux_kernel");
+}
+
+int jh_reaction_source(jh_ctx* c, const void* U, const void* aux, double dt, void* S) {
+  if (!c) return JH_ERR_ARG;
+  if (!U || !aux || !S) return fail(c, JH_ERR_ARG, "NULL pointer");
+  if (!(dt > 0)) return fail(c, JH_ERR_ARG, "dt must be > 0");
+  if (c->precision == JH_PREC_FP64) {
+    auto a = make_args<double>(c, dt, 0.0);
+    chem_kernel<double, CHEM_SOURCE><<<grid_cells(c), 256, 0, c->stream>>>(
+        a, (double*)U, (double*)aux, (double*)S, nullptr);
+  } else {
+    auto a = make_args<float>(c, dt, 0.0);
+    chem_kernel<float, CHEM_SOURCE><<<grid_cells(c), 256, 0, c->stream>>>(
+        a, (float*)U, (float*)aux, (float*)S, nullptr);
+  }
+  return check_launch(c, "chem_kernel");
+}
+
+int jh_cfl_dt(jh_ctx* c, const void* U, const void* aux, double cfl, double* dt_dev) {
+  if (!c) return JH_ERR_ARG;
+  if (!U || !aux || !dt_dev) return fail(c, JH_ERR_ARG, "NULL pointer");
+  int rc = c->precision == JH_PREC_FP64
+               ? prologue<double>(c, (const double*)U, (const double*)aux, 0.0, cfl)
+               : prologue<float>(c, (const float*)U, (const float*)aux, 0.0, cfl);
+  if (rc) return rc;
+  copy_scalars_kernel<<<1, 64, 0, c->stream>>>(c->sc, c->parity, cfl, dt_dev, nullptr);
+  return check_launch(c, "copy_scalars_kernel");
+}
+
+int jh_lf_alpha(jh_ctx* c, const void* U, const void* aux, double* alpha_dev) {
+  if (!c) return JH_ERR_ARG;
+  if (!U || !aux || !alpha_dev) return fail(c, JH_ERR_ARG, "NULL pointer");
+  int rc;
+  if (c->precision == JH_PREC_FP64) {
+    rc = prologue<double>(c, (const double*)U, (const double*)aux, 1.0, 0.0);
+    if (rc) return rc;
+    auto a = make_args<double>(c, 1.0, 0.0);
+    bc_kernel<double><<<bc_blocks(c), 256, 0, c->stream>>>(a, (const double*)U,
+                                                            (const double*)aux, 0);
+  } else {
+    rc = prologue<float>(c, (const float*)U, (const float*)aux, 1.0, 0.0);
+    if (rc) return rc;
+    auto a = make_args<float>(c, 1.0, 0.0);
+    bc_kernel<float><<<bc_blocks(c), 256, 0, c->stream>>>(a, (const float*)U,
+                                                          (const float*)aux, 0);
+  }
+  copy_scalars_kernel<<<1, 64, 0, c->stream>>>(c->sc, c->parity, 0.0, nullptr, alpha_dev);
+  return check_launch(c, "lf_alpha kernels");
+}
+
+int jh_timing_enable(jh_ctx* c, int on) {
+  if (!c) return JH_ERR_ARG;
+  c->timing = on != 0;
+  return JH_OK;
+}
+
+int jh_timing_read(jh_ctx* c, double* ms, int* counts) {
+  if (!c || !ms || !counts) return JH_ERR_ARG;
+  for (int k = 0; k < JH_KERNEL_KINDS; ++k) {
+    ms[k] = 0.0;
+    counts[k] = 0;
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (auto& m : c->marks) {
+    float t = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&t, m.a, m.b));
+    ms[m.kind] += t;
+    counts[m.kind] += 1;
+    c->ev_free.push_back(m.a);
+    c->ev_free.push_back(m.b);
+  }
+  c->marks.clear();
+  return JH_OK;
+}
+
+int jh_ghost_strip(jh_ctx* c, int side, void** ptr, int* count) {
+  if (!c || !ptr || !count || side < 0 || side > 3) return JH_ERR_ARG;
+  *ptr = c->ghost[side];
+  *count = NV * 3 * ((side < 2) ? c->ny : c->nx);
+  return JH_OK;
+}
+
+int jh_health(jh_ctx* c, unsigned int* flags) {
+  if (!c || !flags) return JH_ERR_ARG;
+  StepScalars h;
+  HIP_TRY(c, hipMemcpyAsync(&h, c->sc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *flags = h.flags;
+  const unsigned int zero = 0;
+  HIP_TRY(c, hipMemcpyAsync(&c->sc->flags, &zero, sizeof(zero), hipMemcpyHostToDevice,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return JH_OK;
+}
+
+}  // extern "C"

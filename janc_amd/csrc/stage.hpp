@@ -1,0 +1,631 @@
+// Device kernels of the SSP-RK3 reacting-Euler step (gfx950).
+//
+// Layout in HBM: the state is SoA, C-order (var, x, y) with y contiguous,
+// exactly the reference's (12, nx, ny) / (2, nx, ny) arrays.  Ghost cells are
+// not stored in the state: four ghost strips (left/right: (14, 3, ny),
+// bottom/top: (14, 3, nx); vars 0-11 = U, 12-13 = aux) are filled once per RK
+// stage by bc_kernel and read by the stage kernel's halo loads.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "chem.hpp"
+#include "janc_types.hpp"
+#include "thermo.hpp"
+#include "weno.hpp"
+
+namespace jh {
+
+constexpr int NU = 4 + (JH_NS - 1);   // conserved variables (12)
+constexpr int NV = NU + 2;            // + aux (gamma, T) in ghost strips
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ unsigned long long dbits(double v) {
+  return (unsigned long long)__double_as_longlong(v);
+}
+__device__ __forceinline__ double bitsd(unsigned long long b) {
+  return __longlong_as_double((long long)b);
+}
+
+// Wave-then-block max of non-negative doubles (NaN handling by the caller),
+// then one 64-bit atomicMax per block.  Every thread of the block must call.
+template <int NT>
+__device__ __forceinline__ void block_max_atomic(double v, unsigned long long* dst,
+                                                 double* red_lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red_lds[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red_lds[0];
+#pragma unroll
+    for (int i = 1; i < NT / 64; ++i) m = fmax(m, red_lds[i]);
+    atomicMax(dst, dbits(m));
+  }
+}
+
+// Same reduction that lets a NaN through (CFL_2D uses jnp.min, which
+// propagates NaN).  Values are reduced as bit patterns.
+template <int NT>
+__device__ __forceinline__ void block_max_bits_atomic(unsigned long long v,
+                                                      unsigned long long* dst,
+                                                      unsigned long long* red_lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red_lds[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = red_lds[0];
+#pragma unroll
+    for (int i = 1; i < NT / 64; ++i) m = red_lds[i] > m ? red_lds[i] : m;
+    atomicMax(dst, m);
+  }
+}
+
+__device__ __forceinline__ double nan_to_zero(double v) { return v == v ? v : 0.0; }
+
+// Primitive state of one cell (aux_func.py:39-51), in compute type W.
+template <typename W>
+struct Prim {
+  W rho, u, v, p, a;
+};
+
+template <typename W, bool WITH_A, typename Real>
+__device__ __forceinline__ Prim<W> cell_prim(const MechConst& c, const Real* __restrict__ Up,
+                                             const Real* __restrict__ ap, int stride) {
+  Prim<W> q;
+  q.rho = (W)Up[0];
+  q.u = (W)Up[stride] / q.rho;
+  q.v = (W)Up[2 * stride] / q.rho;
+  W sum = (W)0, R = (W)0;
+  W Y[JH_NS];
+#pragma unroll
+  for (int k = 0; k < JH_NS - 1; ++k) {
+    Y[k] = (W)Up[(4 + k) * stride] / q.rho;
+    sum += Y[k];
+  }
+  Y[JH_NS - 1] = (W)1 - sum;
+#pragma unroll
+  for (int k = 0; k < JH_NS; ++k) R += (W)c.R[k] * Y[k];
+  const W T = (W)ap[stride];
+  q.p = (q.rho * R * T);
+  q.a = (W)0;
+  if (WITH_A) q.a = sqrt((W)ap[0] * q.p / q.rho);
+  return q;
+}
+
+// ------------------------------------------------------------- arguments
+
+template <typename Real>
+struct StepArgs {
+  int nx, ny;
+  double dx, dy;
+  double cfl, dt_fixed;       // dt_fixed > 0 selects a fixed dt
+  int parity;                 // step parity (slot set of StepScalars)
+  const MechConst* __restrict__ mc;
+  StepScalars* sc;
+  Real* g[4];                 // ghost strips L, R, B, T (NV planes each)
+  int bc[4];
+  double bc_p[4];             // pressure_outlet back pressure per side
+};
+
+template <typename Real>
+__device__ __forceinline__ double step_dt(const StepArgs<Real>& a) {
+  return a.dt_fixed > 0.0 ? a.dt_fixed
+                          : a.cfl * (1.0 / bitsd(a.sc->cmax[a.parity]));
+}
+
+// ----------------------------------------------------- reset / reductions
+
+// Zero the reduction slots of one step parity.
+__global__ void reset_kernel(StepScalars* sc, int parity) {
+  const int t = threadIdx.x;
+  if (t < 6) (&sc->amax[parity][0][0])[t] = 0ull;
+  if (t == 6) sc->cmax[parity] = 0ull;
+}
+
+// Interior part of the stage-1 LF maxima and the CFL maximum of a state
+// handed in by the caller (set_simulator.py:438-443, LF.py:10-11).
+template <typename Real>
+__global__ __launch_bounds__(256) void reduce_kernel(StepArgs<Real> a, const Real* __restrict__ U,
+                                                     const Real* __restrict__ aux, int slot_parity,
+                                                     int with_cfl) {
+  __shared__ double red[4];
+  __shared__ unsigned long long redb[4];
+  const MechConst& c = *a.mc;
+  const int n = a.nx * a.ny;
+  double mx = 0.0, my = 0.0;
+  unsigned long long cm = 0ull;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const Prim<double> q = cell_prim<double, true>(c, U + i, aux + i, n);
+    const double sx = fabs(q.u) + q.a, sy = fabs(q.v) + q.a;
+    mx = fmax(mx, nan_to_zero(sx));
+    my = fmax(my, nan_to_zero(sy));
+    const unsigned long long b = dbits(sx / a.dx + sy / a.dy);
+    cm = b > cm ? b : cm;
+  }
+  block_max_atomic<256>(mx, &a.sc->amax[slot_parity][0][0], red);
+  block_max_atomic<256>(my, &a.sc->amax[slot_parity][0][1], red);
+  if (with_cfl) block_max_bits_atomic<256>(cm, &a.sc->cmax[slot_parity], redb);
+}
+
+// ------------------------------------------------------------ ghost cells
+
+// Ghost strips for one RK stage (boundary.py:126-132 with the BC kinds of
+// boundary_conditions_2D/*.py and the periodic pre-pad of
+// boundary_padding.py:20-25), plus the ghost cells' share of the LF maxima.
+// One thread per boundary-normal line: threads [0, ny) left, [ny, 2ny) right,
+// [2ny, 2ny+nx) bottom, [2ny+nx, 2ny+2nx) top.
+template <typename Real>
+__global__ __launch_bounds__(256) void bc_kernel(StepArgs<Real> a, const Real* __restrict__ U,
+                                                 const Real* __restrict__ aux, int stage) {
+  __shared__ double red[4];
+  const MechConst& c = *a.mc;
+  const int nx = a.nx, ny = a.ny, n = nx * ny;
+  if (stage == 0 && blockIdx.x == 0 && threadIdx.x < 7) {
+    // the other parity's slots are written from here on in this step
+    const int o = a.parity ^ 1;
+    if (threadIdx.x < 6) (&a.sc->amax[o][0][0])[threadIdx.x] = 0ull;
+    else a.sc->cmax[o] = 0ull;
+  }
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  double mx = 0.0, my = 0.0;
+  int side = -1, pos = 0;
+  if (tid < ny) { side = JH_LEFT; pos = tid; }
+  else if (tid < 2 * ny) { side = JH_RIGHT; pos = tid - ny; }
+  else if (tid < 2 * ny + nx) { side = JH_BOTTOM; pos = tid - 2 * ny; }
+  else if (tid < 2 * ny + 2 * nx) { side = JH_TOP; pos = tid - 2 * ny - nx; }
+  if (side >= 0) {
+    const bool xside = side <= JH_RIGHT;
+    const int len = xside ? ny : nx;          // strip row length
+    const int N = xside ? nx : ny;            // cells along the normal
+    const int gstride = 3 * len;              // plane stride in the strip
+    Real* gs = a.g[side];
+    const int kind = a.bc[side];
+    // interior cell index (x, y) -> flat
+    auto cell = [&](int along) -> int { return xside ? along * ny + pos : pos * ny + along; };
+    if (kind != JH_BC_USER) {
+      if (kind == JH_BC_PRESSURE_OUTLET) {
+        // pressure_outlet.py:5-66: corrected wall-cell state, 3 copies
+        const int w = (side == JH_LEFT || side == JH_BOTTOM) ? 0 : N - 1;
+        const int ci = cell(w);
+        double Uc[NU];
+#pragma unroll
+        for (int v = 0; v < NU; ++v) Uc[v] = (double)U[v * n + ci];
+        const double g_out = (double)aux[ci], T_out = (double)aux[n + ci];
+        const double rho = Uc[0], u = Uc[1] / rho, vv = Uc[2] / rho;
+        double Y[JH_NS];
+#pragma unroll
+        for (int k = 0; k < JH_NS - 1; ++k) Y[k] = Uc[4 + k] / rho;
+        const double R = fill_Y_get_R(c, Y);
+        const double p_out = rho * (R * T_out);
+        const double a_out = sqrt(g_out * p_out / rho);
+        const double vn = side == JH_LEFT ? -(Uc[1] / Uc[0])
+                        : side == JH_RIGHT ? Uc[1] / Uc[0]
+                        : side == JH_BOTTOM ? -(Uc[2] / Uc[0]) : Uc[2] / Uc[0];
+        const bool mask = (vn / a_out < 1.0);
+        const double Pb = a.bc_p[side];
+        const double rho_c = mask ? Pb / (p_out / rho) : rho;
+        const double p_c = mask ? Pb : p_out;
+        const double T_c = mask ? p_c / (rho_c * R) : T_out;
+        double h;
+        const double gam = mix_gamma(c, Y, R, T_c, &h);
+        double G[NV];
+        G[0] = rho_c;
+        G[1] = rho_c * u;
+        G[2] = rho_c * vv;
+        G[3] = rho_c * h - p_c + 0.5 * rho_c * (u * u + vv * vv);
+#pragma unroll
+        for (int k = 0; k < JH_NS - 1; ++k) G[4 + k] = rho_c * Y[k];
+        G[NU] = gam;
+        G[NU + 1] = T_c;
+        for (int gi = 0; gi < 3; ++gi) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) gs[v * gstride + gi * len + pos] = (Real)G[v];
+        }
+      } else {
+        for (int gi = 0; gi < 3; ++gi) {
+          // source cell along the normal for ghost layer gi (gi = 0 is the
+          // outermost on the low side, the innermost on the high side)
+          const bool low = (side == JH_LEFT || side == JH_BOTTOM);
+          int src;
+          if (kind == JH_BC_PERIODIC) src = low ? N - 4 + gi : 1 + gi;
+          else if (kind == JH_BC_SLIP_WALL) src = low ? 0 : N - 1;
+          else src = low ? 2 - gi : N - 1 - gi;   // zero_gradient, no_slip_wall
+          const int ci = cell(src);
+#pragma unroll
+          for (int v = 0; v < NU; ++v) {
+            Real val = U[v * n + ci];
+            if (kind == JH_BC_SLIP_WALL && v == (xside ? 1 : 2)) val = -val;
+            if (kind == JH_BC_NO_SLIP_WALL && (v == 1 || v == 2)) val = -val;
+            gs[v * gstride + gi * len + pos] = val;
+          }
+          gs[NU * gstride + gi * len + pos] = aux[ci];
+          gs[(NU + 1) * gstride + gi * len + pos] = aux[n + ci];
+        }
+      }
+    }
+    for (int gi = 0; gi < 3; ++gi) {
+      const Real* gp = gs + gi * len + pos;
+      const Prim<double> q = cell_prim<double, true>(c, gp, gp + NU * gstride, gstride);
+      mx = fmax(mx, nan_to_zero(fabs(q.u) + q.a));
+      my = fmax(my, nan_to_zero(fabs(q.v) + q.a));
+    }
+  }
+  block_max_atomic<256>(mx, &a.sc->amax[a.parity][stage][0], red);
+  block_max_atomic<256>(my, &a.sc->amax[a.parity][stage][1], red);
+}
+
+// ------------------------------------------------------------ stage kernel
+
+constexpr int TY = 64;            // tile width along y (one wave)
+constexpr int TXR = 8;            // tile height along x
+constexpr int RX = TXR + 6;       // x-extended region rows
+constexpr int RY = TY + 6;        // y-extended region columns
+constexpr int NT = 256;           // threads per block
+constexpr int NXF = (TXR + 1) * TY;       // x faces per tile (576)
+constexpr int NYF = TXR * (TY + 1);       // y faces per tile (520)
+
+enum StageMode : int { MODE_RK = 0, MODE_RHS = 1 };
+
+// Address of one halo/interior cell: U base, aux base and plane stride.
+template <typename Real>
+struct CellRef {
+  const Real* u;
+  const Real* x;
+  int stride;
+  bool ok;
+};
+
+template <typename Real>
+__device__ __forceinline__ CellRef<Real> locate(const StepArgs<Real>& a, const Real* __restrict__ U,
+                                                const Real* __restrict__ aux, int gx, int gy) {
+  const int nx = a.nx, ny = a.ny;
+  CellRef<Real> r;
+  r.ok = true;
+  const bool inx = gx >= 0 && gx < nx, iny = gy >= 0 && gy < ny;
+  if (inx && iny) {
+    const int o = gx * ny + gy;
+    r.u = U + o;
+    r.x = aux + o;
+    r.stride = nx * ny;
+  } else if (iny && gx < 0 && gx >= -3) {
+    r.stride = 3 * ny;
+    r.u = a.g[JH_LEFT] + (gx + 3) * ny + gy;
+    r.x = r.u + NU * r.stride;
+  } else if (iny && gx >= nx && gx < nx + 3) {
+    r.stride = 3 * ny;
+    r.u = a.g[JH_RIGHT] + (gx - nx) * ny + gy;
+    r.x = r.u + NU * r.stride;
+  } else if (inx && gy < 0 && gy >= -3) {
+    r.stride = 3 * nx;
+    r.u = a.g[JH_BOTTOM] + (gy + 3) * nx + gx;
+    r.x = r.u + NU * r.stride;
+  } else if (inx && gy >= ny && gy < ny + 3) {
+    r.stride = 3 * nx;
+    r.u = a.g[JH_TOP] + (gy - ny) * nx + gx;
+    r.x = r.u + NU * r.stride;
+  } else {
+    r.ok = false;
+    r.u = U;
+    r.x = aux;
+    r.stride = 0;
+  }
+  return r;
+}
+
+// Physical flux of component cv in direction x (dir 0) or y (dir 1) at a
+// cell, LF.py:12 (rho*u, rho*u^2+p, rho*u*v, u*(rhoE+p), rho*u*Y).
+template <typename W>
+__device__ __forceinline__ W phys_flux(int cv, int dir, const Prim<W>& q, W ucv) {
+  const W un = dir == 0 ? q.u : q.v;
+  switch (cv) {
+    case 0: return q.rho * un;
+    case 1: return dir == 0 ? q.rho * (q.u * q.u) + q.p : (q.rho * q.u) * q.v;
+    case 2: return dir == 0 ? (q.rho * q.u) * q.v : q.rho * (q.v * q.v) + q.p;
+    case 3: return un * (ucv + q.p);
+    default: return (q.rho * un) * (ucv / q.rho);
+  }
+}
+
+// One SSP-RK3 stage (TVD_RK3.py:5-12) for a 64 x 8 tile:
+//   rhs = dt * -(dF/dx + dG/dy)   (set_simulator.py:44-47, flux.py:56-68)
+//   stage 0: U1 = U + rhs; stage 1: U2 = 3/4 U + 1/4 (U1 + rhs);
+//   stage 2: U3 = 1/3 U + 2/3 (U2 + rhs)
+// followed by update_aux (Halley) of the new state and the next stage's LF
+// maxima.  MODE_RHS writes rhs only (the reference's flux_func).
+//
+// Work split per conserved variable: phase 1 evaluates the split fluxes
+// H+- = (F +- alpha U)/2 of the tile + halo into LDS, phase 2 reconstructs the
+// 576 x-faces and 520 y-faces (L + R WENO each), phase 3 takes the divergence
+// and the RK combine for the thread's two cells.
+template <typename Real, int MODE>
+__global__ __launch_bounds__(NT) void stage_kernel(StepArgs<Real> a, int stage,
+                                                   const Real* __restrict__ Us,
+                                                   const Real* __restrict__ auxs,
+                                                   const Real* Un, Real* Uo,
+                                                   Real* __restrict__ auxo) {
+  __shared__ Real sHxp[RX][TY], sHxm[RX][TY];
+  __shared__ Real sHyp[TXR][RY], sHym[TXR][RY];
+  __shared__ Real sFx[TXR + 1][TY];
+  __shared__ Real sFy[TXR][TY + 1];
+  __shared__ double red[4];
+
+  const MechConst& c = *a.mc;
+  const int nx = a.nx, ny = a.ny, n = nx * ny;
+  const int t = threadIdx.x, ly = t & 63, w = t >> 6;
+  const int tiles_y = (ny + TY - 1) / TY;
+  const int x0 = (blockIdx.x / tiles_y) * TXR;
+  const int y0 = (blockIdx.x % tiles_y) * TY;
+  const double dt = step_dt(a);
+  const Real ax = (Real)bitsd(a.sc->amax[a.parity][stage][0]);
+  const Real ay = (Real)bitsd(a.sc->amax[a.parity][stage][1]);
+
+  // ---- region cells owned by this thread (see header comment) ----
+  // slots 0,1: interior rows w, w+4; slot 2,3: x-halo rows; slot 4: y-halo
+  constexpr int NS_ = 5;
+  int rrow[NS_], rcol[NS_];       // LDS coordinates: x-region row / y-region col
+  int gxs[NS_], gys[NS_];
+  bool has[NS_];
+  const int hr2[4] = {0, 1, 2, 11};
+  rrow[0] = w + 3;
+  rrow[1] = w + 7;
+  rrow[2] = hr2[w];
+  rrow[3] = (w < 2) ? 12 + w : -1;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    has[s] = rrow[s] >= 0;
+    gxs[s] = x0 - 3 + rrow[s];
+    gys[s] = y0 + ly;
+    rcol[s] = ly + 3;
+  }
+  has[4] = t < 48;
+  {
+    const int xr = t / 6, pos = t % 6;
+    const int col = pos < 3 ? pos : TY + pos;   // 0,1,2 or 67,68,69
+    rrow[4] = xr;                               // y-region row (interior x)
+    rcol[4] = col;
+    gxs[4] = x0 + xr;
+    gys[4] = y0 - 3 + col;
+  }
+  CellRef<Real> ref[NS_];
+  Prim<Real> q[NS_];
+#pragma unroll
+  for (int s = 0; s < NS_; ++s) {
+    ref[s] = locate<Real>(a, Us, auxs, gxs[s], gys[s]);
+    if (!has[s]) ref[s].ok = false;
+    if (ref[s].ok) {
+      q[s] = cell_prim<Real, false>(c, ref[s].u, ref[s].x, ref[s].stride);
+    } else {
+      q[s].rho = (Real)1; q[s].u = (Real)0; q[s].v = (Real)0; q[s].p = (Real)0; q[s].a = (Real)0;
+    }
+  }
+  bool own[2];
+  int oidx[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    own[s] = gxs[s] < nx && gys[s] < ny;
+    oidx[s] = gxs[s] * ny + gys[s];
+  }
+
+  for (int cv = 0; cv < NU; ++cv) {
+    // ---- phase 1: split fluxes into LDS ----
+    Real uc_own[2];
+#pragma unroll
+    for (int s = 0; s < NS_; ++s) {
+      if (!has[s]) continue;
+      const Real ucv = ref[s].ok ? ref[s].u[cv * ref[s].stride] : (Real)0;
+      if (s < 2) uc_own[s] = ucv;
+      const bool xreg = s < 4, yreg = (s < 2) || (s == 4);
+      if (xreg) {
+        Real hp = (Real)0, hm = (Real)0;
+        if (ref[s].ok) {
+          const Real F = phys_flux<Real>(cv, 0, q[s], ucv);
+          hp = (Real)0.5 * (F + ax * ucv);
+          hm = (Real)0.5 * (F - ax * ucv);
+        }
+        sHxp[rrow[s]][ly] = hp;
+        sHxm[rrow[s]][ly] = hm;
+      }
+      if (yreg) {
+        Real hp = (Real)0, hm = (Real)0;
+        if (ref[s].ok) {
+          const Real G = phys_flux<Real>(cv, 1, q[s], ucv);
+          hp = (Real)0.5 * (G + ay * ucv);
+          hm = (Real)0.5 * (G - ay * ucv);
+        }
+        const int yr = s < 2 ? rrow[s] - 3 : rrow[s];
+        sHyp[yr][rcol[s]] = hp;
+        sHym[yr][rcol[s]] = hm;
+      }
+    }
+    __syncthreads();
+    // ---- phase 2: face reconstructions ----
+    for (int it = t; it < NXF + NYF; it += NT) {
+      if (it < NXF) {
+        const int fr = it >> 6, col = it & 63;
+        const Real L = weno5(sHxp[fr][col], sHxp[fr + 1][col], sHxp[fr + 2][col],
+                             sHxp[fr + 3][col], sHxp[fr + 4][col]);
+        const Real R = weno5(sHxm[fr + 5][col], sHxm[fr + 4][col], sHxm[fr + 3][col],
+                             sHxm[fr + 2][col], sHxm[fr + 1][col]);
+        sFx[fr][col] = L + R;
+      } else {
+        const int j = it - NXF;
+        const int xr = j / (TY + 1), fc = j - xr * (TY + 1);
+        const Real L = weno5(sHyp[xr][fc], sHyp[xr][fc + 1], sHyp[xr][fc + 2],
+                             sHyp[xr][fc + 3], sHyp[xr][fc + 4]);
+        const Real R = weno5(sHym[xr][fc + 5], sHym[xr][fc + 4], sHym[xr][fc + 3],
+                             sHym[xr][fc + 2], sHym[xr][fc + 1]);
+        sFy[xr][fc] = L + R;
+      }
+    }
+    __syncthreads();
+    // ---- phase 3: divergence + RK combine ----
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int ir = rrow[s] - 3;
+      const Real dF = sFx[ir + 1][ly] - sFx[ir][ly];
+      const Real dG = sFy[ir][ly + 1] - sFy[ir][ly];
+      const Real rhs = (Real)dt * (-(dF / (Real)a.dx + dG / (Real)a.dy));
+      Real un;
+      if (MODE == MODE_RHS) {
+        un = rhs;
+      } else if (stage == 0) {
+        un = uc_own[s] + rhs;
+      } else {
+        const Real u0 = own[s] ? Un[cv * n + oidx[s]] : (Real)0;
+        if (stage == 1) un = (Real)(3.0 / 4.0) * u0 + (Real)(1.0 / 4.0) * (uc_own[s] + rhs);
+        else un = (Real)(1.0 / 3.0) * u0 + (Real)(2.0 / 3.0) * (uc_own[s] + rhs);
+      }
+      if (own[s]) Uo[cv * n + oidx[s]] = un;
+    }
+  }
+  if (MODE == MODE_RHS) return;
+
+  // ---- update_aux (aux_func.py:21-29) + next stage's LF maxima ----
+  // The new state is read back from this thread's own stores (a runtime
+  // component index into a register array would live in scratch).
+  double mx = 0.0, my = 0.0;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    if (!own[s]) continue;
+    const Real* un = Uo + oidx[s];
+    const double rho = (double)un[0];
+    const double u = (double)un[n] / rho, v = (double)un[2 * n] / rho;
+    const double e = (double)un[3 * n] / rho - 0.5 * (u * u + v * v);
+    double Y[JH_NS];
+#pragma unroll
+    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = (double)un[(4 + k) * n] / rho;
+    const double R = fill_Y_get_R(c, Y);
+    double T = (double)auxs[n + oidx[s]];
+    double gamma;
+    halley_T(c, Y, R, e, T, gamma);
+    auxo[oidx[s]] = (Real)gamma;
+    auxo[n + oidx[s]] = (Real)T;
+    if (stage < 2) {
+      // next stage's alpha from the stored (rounded) values
+      const double gs = (double)(Real)gamma, Ts = (double)(Real)T;
+      const double p = rho * R * Ts;
+      const double snd = sqrt(gs * p / rho);
+      mx = fmax(mx, nan_to_zero(fabs(u) + snd));
+      my = fmax(my, nan_to_zero(fabs(v) + snd));
+    }
+  }
+  if (stage < 2) {
+    block_max_atomic<NT>(mx, &a.sc->amax[a.parity][stage + 1][0], red);
+    block_max_atomic<NT>(my, &a.sc->amax[a.parity][stage + 1][1], red);
+  }
+}
+
+// ---------------------------------------------------------- chemistry
+
+enum ChemMode : int { CHEM_STEP = 0, CHEM_SOURCE = 1 };
+
+// Point-implicit chemistry after RK3 (set_simulator.py:117-122):
+//   U <- U + reaction_source_terms(U, aux, dt); aux <- update_aux(U, aux)
+// plus the next step's CFL and stage-1 LF maxima.  CHEM_SOURCE writes the
+// source S (12 planes, first 4 zero) instead.
+template <typename Real, int MODE>
+__global__ __launch_bounds__(256) void chem_kernel(StepArgs<Real> a, Real* U, Real* aux,
+                                                   Real* S, double* dt_hist) {
+  __shared__ double red[4];
+  __shared__ unsigned long long redb[4];
+  const MechConst& c = *a.mc;
+  const int n = a.nx * a.ny;
+  const double dt = step_dt(a);
+  if (MODE == CHEM_STEP && blockIdx.x == 0 && threadIdx.x == 0) {
+    a.sc->dt_last = dt;
+    if (dt_hist) *dt_hist = dt;
+  }
+  double mx = 0.0, my = 0.0;
+  unsigned long long cm = 0ull;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double Uc[NU];
+#pragma unroll
+    for (int v = 0; v < NU; ++v) Uc[v] = (double)U[v * n + i];
+    double T = (double)aux[n + i];
+    const double rho = Uc[0];
+    double rY[JH_NS - 1];
+#pragma unroll
+    for (int k = 0; k < JH_NS - 1; ++k) rY[k] = Uc[4 + k];
+    double d[JH_NA];
+    chem_increment(c, rho, rY, T, dt, d);
+    if (MODE == CHEM_SOURCE) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) S[v * n + i] = (Real)0;
+#pragma unroll
+      for (int k = 0; k < JH_NA; ++k) S[(4 + k) * n + i] = (Real)d[k];
+      continue;
+    }
+    Real Un[NU];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Un[v] = U[v * n + i];
+#pragma unroll
+    for (int k = 0; k < JH_NA; ++k) {
+      Un[4 + k] = (Real)((double)U[(4 + k) * n + i] + d[k]);
+      U[(4 + k) * n + i] = Un[4 + k];
+    }
+    const double r2 = (double)Un[0];
+    const double u = (double)Un[1] / r2, v = (double)Un[2] / r2;
+    const double e = (double)Un[3] / r2 - 0.5 * (u * u + v * v);
+    double Y[JH_NS];
+#pragma unroll
+    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = (double)Un[4 + k] / r2;
+    const double R = fill_Y_get_R(c, Y);
+    double gamma;
+    halley_T(c, Y, R, e, T, gamma);
+    aux[i] = (Real)gamma;
+    aux[n + i] = (Real)T;
+    const unsigned int f = (r2 > 0.0 ? 0u : (unsigned)JH_FLAG_NEGATIVE_RHO) |
+                           (isfinite(T) && isfinite(gamma) ? 0u : (unsigned)JH_FLAG_NONFINITE);
+    if (f) atomicOr(&a.sc->flags, f);
+    // next step: CFL_2D and stage-1 LF maxima from the stored values
+    const double gs = (double)(Real)gamma, Ts = (double)(Real)T;
+    const double p = r2 * R * Ts;
+    const double snd = sqrt(gs * p / r2);
+    const double sx = fabs(u) + snd, sy = fabs(v) + snd;
+    mx = fmax(mx, nan_to_zero(sx));
+    my = fmax(my, nan_to_zero(sy));
+    const unsigned long long b = dbits(sx / a.dx + sy / a.dy);
+    cm = b > cm ? b : cm;
+  }
+  if (MODE == CHEM_STEP) {
+    const int o = a.parity ^ 1;
+    block_max_atomic<256>(mx, &a.sc->amax[o][0][0], red);
+    block_max_atomic<256>(my, &a.sc->amax[o][0][1], red);
+    block_max_bits_atomic<256>(cm, &a.sc->cmax[o], redb);
+  }
+}
+
+// update_aux alone (aux_func.py:21-29), for the unit entry point.
+template <typename Real>
+__global__ __launch_bounds__(256) void update_aux_kernel(StepArgs<Real> a, const Real* __restrict__ U,
+                                                         const Real* __restrict__ aux_in,
+                                                         Real* __restrict__ aux_out) {
+  const MechConst& c = *a.mc;
+  const int n = a.nx * a.ny;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double rho = (double)U[i];
+    const double u = (double)U[n + i] / rho, v = (double)U[2 * n + i] / rho;
+    const double e = (double)U[3 * n + i] / rho - 0.5 * (u * u + v * v);
+    double Y[JH_NS];
+#pragma unroll
+    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = (double)U[(4 + k) * n + i] / rho;
+    const double R = fill_Y_get_R(c, Y);
+    double T = (double)aux_in[n + i];
+    double gamma;
+    halley_T(c, Y, R, e, T, gamma);
+    aux_out[i] = (Real)gamma;
+    aux_out[n + i] = (Real)T;
+  }
+}
+
+}  // namespace jh

@@ -1,0 +1,112 @@
+// NASA-7 mixture thermo and the Halley temperature solve, one cell at a time.
+// Follows model/thermo_model.py:80-153 of the reference.  All arithmetic is
+// FP64 whatever the storage precision of the state.
+#pragma once
+#include <math.h>
+
+#include "janc_types.hpp"
+
+namespace jh {
+
+constexpr int kHalleyMaxIter = 5;     // thermo_model.py:17
+constexpr double kHalleyTol = 5e-9;   // thermo_model.py:18
+
+// Mixture polynomial of one NASA-7 range: cp and h per mass (highest power
+// first).  Every species shares the switch temperature, so the reference's
+// per-species mask (thermo_model.py:101-104) selects one range per cell and
+// sum_k Y_k P_k(T) can be evaluated as (sum_k Y_k c_k)(T).
+struct MixPoly {
+  double cp[5];
+  double h[6];
+};
+
+__host__ __device__ __forceinline__ void mix_poly(const MechConst& c, int r,
+                                                  const double (&Y)[JH_NS],
+                                                  MixPoly& m) {
+#pragma unroll
+  for (int j = 0; j < 5; ++j) m.cp[j] = 0.0;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) m.h[j] = 0.0;
+#pragma unroll
+  for (int k = 0; k < JH_NS; ++k) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) m.cp[j] += Y[k] * c.cp[r][k][j];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) m.h[j] += Y[k] * c.h[r][k][j];
+  }
+}
+
+// cp, dcp/dT and h of the mixture at T (thermo_model.py:115-126).
+__host__ __device__ __forceinline__ void mix_eval(const MixPoly& m, double T,
+                                                  double& cp, double& dcp,
+                                                  double& h) {
+  cp = m.cp[0];
+#pragma unroll
+  for (int j = 1; j < 5; ++j) cp = cp * T + m.cp[j];
+  dcp = 4.0 * m.cp[0];
+  dcp = dcp * T + 3.0 * m.cp[1];
+  dcp = dcp * T + 2.0 * m.cp[2];
+  dcp = dcp * T + m.cp[3];
+  h = m.h[0];
+#pragma unroll
+  for (int j = 1; j < 6; ++j) h = h * T + m.h[j];
+}
+
+// fill_Y + get_R (thermo_model.py:80-92): Y[NS-1] = 1 - sum(Y[0:NS-1]).
+__host__ __device__ __forceinline__ double fill_Y_get_R(const MechConst& c,
+                                                        double (&Y)[JH_NS]) {
+  double sum = 0.0;
+#pragma unroll
+  for (int k = 0; k < JH_NS - 1; ++k) sum += Y[k];
+  Y[JH_NS - 1] = 1.0 - sum;
+  double R = 0.0;
+#pragma unroll
+  for (int k = 0; k < JH_NS; ++k) R += c.R[k] * Y[k];
+  return R;
+}
+
+// gamma = cp/(cp - R) of the mixture at T (get_thermo_nasa7).
+__host__ __device__ __forceinline__ double mix_gamma(const MechConst& c,
+                                                     const double (&Y)[JH_NS],
+                                                     double R, double T,
+                                                     double* h_out) {
+  MixPoly m;
+  mix_poly(c, T < c.Tcr ? 0 : 1, Y, m);
+  double cp, dcp, h;
+  mix_eval(m, T, cp, dcp, h);
+  if (h_out) *h_out = h;
+  return cp / (cp - R);
+}
+
+// Halley iteration for T from e (get_T_nasa7, thermo_model.py:135-153).
+// The reference stops when the grid-wide max|res| <= tol; here each cell
+// stops on its own residual.  A converged cell's further Halley updates in
+// the reference move T by O(1e-16) relative, so both forms agree to rounding.
+__host__ __device__ __forceinline__ void halley_T(const MechConst& c,
+                                                  const double (&Y)[JH_NS],
+                                                  double R, double e, double& T,
+                                                  double& gamma) {
+  int r = T < c.Tcr ? 0 : 1;
+  MixPoly m;
+  mix_poly(c, r, Y, m);
+  double cp, dcp, h;
+  mix_eval(m, T, cp, dcp, h);
+  double res = (h - R * T) - e;
+  double d1 = cp - R;
+  gamma = cp / (cp - R);
+  for (int i = 0; i < kHalleyMaxIter && fabs(res) > kHalleyTol; ++i) {
+    const double dT = -2.0 * res * d1 / (2.0 * (d1 * d1) - res * dcp);
+    T = T + dT;
+    const int rn = T < c.Tcr ? 0 : 1;
+    if (rn != r) {
+      r = rn;
+      mix_poly(c, r, Y, m);
+    }
+    mix_eval(m, T, cp, dcp, h);
+    res = (h - R * T) - e;
+    d1 = cp - R;
+    gamma = cp / (cp - R);
+  }
+}
+
+}  // namespace jh

@@ -1,0 +1,62 @@
+// WENO5-JS reconstruction of split fluxes at one face.
+// Follows solver_2D/reconstruction/WENO5_JS.py:10-117 of the reference:
+// smoothness IS_k = 1/4(.)^2 + 13/12(.)^2, alpha_k = C_k/(eps+IS_k)^2 with
+// eps = 1e-6, C = (0.1, 0.6, 0.3), candidate stencils (1/3,-7/6,11/6),
+// (-1/6,5/6,1/3), (1/3,5/6,-1/6).  The right-biased form is the same formula
+// with the stencil mirrored, so one routine serves both.
+#pragma once
+#include <math.h>
+
+namespace jh {
+
+// FP64 form.  w_k = alpha_k / sum(alpha) is evaluated as
+// C_k prod_{j!=k} d_j / sum_m C_m prod_{j!=m} d_j with d_k = (eps+IS_k)^2,
+// which is the same ratio with one division instead of six.
+__host__ __device__ __forceinline__ double weno5(double fm2, double fm1, double f0,
+                                                 double fp1, double fp2) {
+  const double a1 = fm2 - 4.0 * fm1 + 3.0 * f0, b1 = fm2 - 2.0 * fm1 + f0;
+  const double a2 = fm1 - fp1, b2 = fm1 - 2.0 * f0 + fp1;
+  const double a3 = 3.0 * f0 - 4.0 * fp1 + fp2, b3 = f0 - 2.0 * fp1 + fp2;
+  const double IS1 = 0.25 * (a1 * a1) + (13.0 / 12.0) * (b1 * b1);
+  const double IS2 = 0.25 * (a2 * a2) + (13.0 / 12.0) * (b2 * b2);
+  const double IS3 = 0.25 * (a3 * a3) + (13.0 / 12.0) * (b3 * b3);
+  double d1 = 1e-6 + IS1, d2 = 1e-6 + IS2, d3 = 1e-6 + IS3;
+  d1 = d1 * d1;
+  d2 = d2 * d2;
+  d3 = d3 * d3;
+  const double p1 = 0.1 * (d2 * d3), p2 = 0.6 * (d1 * d3), p3 = 0.3 * (d1 * d2);
+  const double q1 = (1.0 / 3.0) * fm2 - (7.0 / 6.0) * fm1 + (11.0 / 6.0) * f0;
+  const double q2 = -(1.0 / 6.0) * fm1 + (5.0 / 6.0) * f0 + (1.0 / 3.0) * fp1;
+  const double q3 = (1.0 / 3.0) * f0 + (5.0 / 6.0) * fp1 - (1.0 / 6.0) * fp2;
+  return (p1 * q1 + p2 * q2 + p3 * q3) / (p1 + p2 + p3);
+}
+
+// FP32 form for the mixed-precision step.  In SI units the split energy flux
+// jumps by ~1e11 across a detonation front, so (eps+IS)^2 ~ 1e44 overflows
+// FP32.  The stencil is scaled by s = max|f|: IS' = IS/s^2 and
+// eps' = eps/s^2 give the same weights (the common factor s^4 cancels).
+// eps' is held in [1e-15, 1e15] so every square stays inside FP32's range;
+// the lower clamp acts only where IS/s^2 < 1e-15, i.e. below FP32 rounding of
+// the normalised stencil, and the upper one where the weights are already the
+// linear weights.
+__host__ __device__ __forceinline__ float weno5(float fm2, float fm1, float f0, float fp1,
+                                                float fp2) {
+  float s = fmaxf(fmaxf(fabsf(fm2), fabsf(fm1)), fmaxf(fmaxf(fabsf(f0), fabsf(fp1)), fabsf(fp2)));
+  const float is = 1.0f / fmaxf(s, 1e-20f);
+  const float eps = fminf(fmaxf(1e-6f * is * is, 1e-15f), 1e15f);
+  const float g0 = fm2 * is, g1 = fm1 * is, g2 = f0 * is, g3 = fp1 * is, g4 = fp2 * is;
+  const float a1 = g0 - 4.0f * g1 + 3.0f * g2, b1 = g0 - 2.0f * g1 + g2;
+  const float a2 = g1 - g3, b2 = g1 - 2.0f * g2 + g3;
+  const float a3 = 3.0f * g2 - 4.0f * g3 + g4, b3 = g2 - 2.0f * g3 + g4;
+  const float IS1 = 0.25f * (a1 * a1) + (13.0f / 12.0f) * (b1 * b1);
+  const float IS2 = 0.25f * (a2 * a2) + (13.0f / 12.0f) * (b2 * b2);
+  const float IS3 = 0.25f * (a3 * a3) + (13.0f / 12.0f) * (b3 * b3);
+  float d1 = eps + IS1, d2 = eps + IS2, d3 = eps + IS3;
+  const float al1 = 0.1f / (d1 * d1), al2 = 0.6f / (d2 * d2), al3 = 0.3f / (d3 * d3);
+  const float q1 = (1.0f / 3.0f) * fm2 - (7.0f / 6.0f) * fm1 + (11.0f / 6.0f) * f0;
+  const float q2 = -(1.0f / 6.0f) * fm1 + (5.0f / 6.0f) * f0 + (1.0f / 3.0f) * fp1;
+  const float q3 = (1.0f / 3.0f) * f0 + (5.0f / 6.0f) * fp1 - (1.0f / 6.0f) * fp2;
+  return (al1 * q1 + al2 * q2 + al3 * q3) / (al1 + al2 + al3);
+}
+
+}  // namespace jh

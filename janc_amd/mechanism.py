@@ -1,0 +1,392 @@
+"""Mechanism compiler: Cantera-style YAML -> device constants + HIP source.
+
+Replaces the reference's Cantera-based preprocessing
+(``src/janc_v2/preprocess/load.py:8-135`` and ``preprocess/nondim.py:14-25``)
+for the hot path.  Two products:
+
+* ``Mechanism.const_array()`` - the flat float64 array that fills the device
+  ``MechConst`` struct (layout declared in ``csrc/janc_types.hpp``); it carries
+  every *value* (A, b, Ea/R, third-body efficiencies, NASA-7 polynomials, molar
+  masses, nondimensional scales), so a nondim configuration changes only data.
+* ``Mechanism.generate_source()`` - straight-line HIP/C++ for the reaction
+  rates and the analytic Jacobian.  Only the *structure* (stoichiometry, which
+  reactions are three-body, which exponents are integers) is compiled in, so
+  every zero entry of the 19x8 stoichiometric matrix disappears at compile
+  time.
+
+The YAML is read directly (no Cantera): Cantera's SI conversion of A
+(length^3/quantity -> m^3/kmol per order) and Ea (-> J/kmol), and its element
+weights (H 1.008, O 15.999, N 14.007), are applied here.
+"""
+import hashlib
+import math
+import os
+
+import numpy as np
+import yaml
+
+RU = 8.314463                 # preprocess/load.py:6 (Rg)
+P_ATM = 101325.0              # model/reaction_model.py:113
+MAX_POLY = (5, 6, 6, 5)       # cp, h, h_chem, s coefficient counts
+_ELEMENT_WEIGHT = {"H": 1.008, "O": 15.999, "N": 14.007, "C": 12.011,
+                   "Ar": 39.95, "He": 4.002602}
+_LENGTH_M = {"cm": 0.01, "m": 1.0, "mm": 0.001}
+_QUANTITY_KMOL = {"mol": 0.001, "kmol": 1.0}
+_ENERGY_J_PER_KMOL = {"cal/mol": 4184.0, "kcal/mol": 4.184e6, "J/mol": 1000.0,
+                      "kJ/mol": 1.0e6, "J/kmol": 1.0}
+
+DEFAULT_MECH = os.path.join(os.path.dirname(__file__), "data", "9sp-19r-H2-Air.yaml")
+
+
+class NondimScales:
+    """Reference scales of ``preprocess/nondim.py:3-25`` (all 1 when unset)."""
+
+    def __init__(self, cfg=None):
+        cfg = cfg or {"P0": 1.0, "T0": 1.0, "R0": 1.0, "x0": 1.0}
+        self.P0, self.T0, self.R0, self.x0 = (float(cfg[k]) for k in ("P0", "T0", "R0", "x0"))
+        self.rho0 = self.P0 / (self.R0 * self.T0)
+        self.M0 = RU / self.R0
+        self.e0 = self.P0 / self.rho0
+        self.u0 = math.sqrt(self.P0 / self.rho0)
+        self.t0 = self.x0 / self.u0
+
+
+class Reaction:
+    __slots__ = ("equation", "reactants", "products", "three_body", "A_cgs", "b",
+                 "Ea", "efficiencies")
+
+    def __init__(self, entry, species):
+        self.equation = entry["equation"]
+        sep = "<=>" if "<=>" in self.equation else "="
+        lhs, rhs = self.equation.split(sep)
+        self.reactants, tb_l = _side(lhs, species)
+        self.products, tb_r = _side(rhs, species)
+        self.three_body = entry.get("type") == "three-body"
+        if self.three_body != (tb_l and tb_r):
+            raise ValueError("inconsistent third body in %r" % self.equation)
+        if entry.get("type") not in (None, "three-body", "elementary"):
+            raise NotImplementedError("reaction type %r" % entry.get("type"))
+        rc = entry["rate-constant"]
+        self.A_cgs, self.b, self.Ea = float(rc["A"]), float(rc["b"]), float(rc["Ea"])
+        self.efficiencies = dict(entry.get("efficiencies", {}))
+
+
+def _side(text, species):
+    out, third = {}, False
+    for term in text.split(" + "):
+        term = term.strip()
+        if term == "M":
+            third = True
+            continue
+        bits = term.split()
+        coef, name = (float(bits[0]), bits[1]) if len(bits) == 2 else (1.0, bits[0])
+        k = species.index(name)
+        out[k] = out.get(k, 0.0) + coef
+    return out, third
+
+
+class Mechanism:
+    """Parsed mechanism + nondimensional constants for one solver context."""
+
+    def __init__(self, path=DEFAULT_MECH, nondim_config=None):
+        if os.path.splitext(path)[1].lower() != ".yaml":
+            raise AssertionError("janc only read mech file with .yaml format")
+        if not os.path.isfile(path):
+            raise FileNotFoundError("No mechanism file detected in the specified directory.")
+        self.path = path
+        with open(path) as fh:
+            self.doc = yaml.safe_load(fh)
+        self.nd = NondimScales(nondim_config)
+        units = self.doc.get("units", {})
+        self._conc = _LENGTH_M[units.get("length", "m")] ** 3 / \
+            _QUANTITY_KMOL[units.get("quantity", "kmol")]
+        self._ea = _ENERGY_J_PER_KMOL[units.get("activation-energy", "J/kmol")]
+        self.species = list(self.doc["phases"][0]["species"])
+        self.ns = len(self.species)
+        self.reactions = [Reaction(r, self.species) for r in self.doc["reactions"]]
+        self.nrx = len(self.reactions)
+        active = [any(k in r.reactants or k in r.products for r in self.reactions)
+                  for k in range(self.ns)]
+        self.na = sum(active)
+        if not all(active[:self.na]) or any(active[self.na:]):
+            raise AssertionError("Inert species must be the last elements in species_list")
+        self._thermo()
+        self._kinetics()
+
+    # ---------------- thermo (load.py:91-135) ----------------
+    def _thermo(self):
+        nd = self.nd
+        by_name = {s["name"]: s for s in self.doc["species"]}
+        mw, tcr, lo, hi = [], [], [], []
+        for name in self.species:
+            sp = by_name[name]
+            th = sp["thermo"]
+            if th["model"] != "NASA7":
+                raise NotImplementedError(th["model"])
+            tcr.append(float(th["temperature-ranges"][1]))
+            lo.append([float(v) for v in th["data"][0]])
+            hi.append([float(v) for v in th["data"][1]])
+            mw.append(sum(_ELEMENT_WEIGHT[e] * n for e, n in sp["composition"].items()))
+        self.molecular_weight = np.array(mw)
+        self.species_M = self.molecular_weight / (1000 * nd.M0)
+        self.Tcr_all = np.array(tcr) / nd.T0
+        if not np.all(self.Tcr_all == self.Tcr_all[0]):
+            raise NotImplementedError("species with different NASA-7 switch temperatures")
+        self.Tcr = float(self.Tcr_all[0])
+        sc = np.array([1, nd.T0, nd.T0 ** 2, nd.T0 ** 3, nd.T0 ** 4, 1 / nd.T0, 1])
+        self.nasa = np.stack([np.array(lo) * sc, np.array(hi) * sc])  # (2, ns, 7)
+        a = self.nasa
+        M = self.species_M[None, :, None]
+        # polynomials, highest power first (jnp.polyval order)
+        self.cp = a[:, :, 4::-1] / M                                       # a5..a1
+        hc = np.stack([a[..., 4] / 5, a[..., 3] / 4, a[..., 2] / 3, a[..., 1] / 2,
+                       a[..., 0], a[..., 5]], axis=-1)                     # h/R
+        self.h_chem = hc
+        self.h = hc / M
+        self.s = np.stack([a[..., 4] / 4, a[..., 3] / 3, a[..., 2] / 2, a[..., 1],
+                           a[..., 6]], axis=-1)
+        self.logc = a[..., 0]
+
+    # ---------------- kinetics (load.py:8-88) ----------------
+    def _kinetics(self):
+        nd, ns = self.nd, self.ns
+        A, B, EaR, eff, vsum = [], [], [], [], []
+        for r in self.reactions:
+            order = sum(r.reactants.values()) + (1 if r.three_body else 0)
+            a_si = r.A_cgs * self._conc ** (order - 1)
+            a_mol = a_si * ((1e-3) ** (order - 1))
+            vf_sum = sum(r.reactants.values())
+            vb_sum = sum(r.products.values())
+            A.append((nd.t0 / nd.rho0) * (nd.T0 ** r.b) * ((nd.rho0 / nd.M0) ** vf_sum)
+                     * nd.M0 * a_mol)
+            B.append(r.b)
+            EaR.append((r.Ea * self._ea / 1000) / (nd.e0 * nd.M0))
+            row = np.zeros(ns)
+            if r.three_body:
+                for k, name in enumerate(self.species):
+                    row[k] = float(r.efficiencies.get(name, 1.0))
+            eff.append(row * (nd.rho0 / nd.M0))
+            vsum.append(vb_sum - vf_sum)
+        self.A, self.B, self.EaR = np.array(A), np.array(B), np.array(EaR)
+        self.eff = np.array(eff)
+        self.vsum = np.array(vsum)
+        self.effM = self.eff[:, :self.na] / self.species_M[None, :self.na]
+        for r in self.reactions:
+            for k in set(r.reactants) | set(r.products):
+                d = r.products.get(k, 0.0) - r.reactants.get(k, 0.0)
+                if abs(d) not in (0.0, 1.0, 2.0):
+                    raise NotImplementedError("stoichiometric difference %g" % d)
+            for v in list(r.reactants.values()) + list(r.products.values()):
+                if v not in (1.0, 2.0):
+                    raise NotImplementedError("stoichiometric coefficient %g" % v)
+            if r.three_body is False and r.efficiencies:
+                raise ValueError("efficiencies on a non three-body reaction")
+
+    # ---------------- device constants ----------------
+    def layout(self):
+        """(name, shape) of every MechConst field, in declaration order."""
+        ns, na, nrx = self.ns, self.na, self.nrx
+        return [("R", (ns,)), ("M", (ns,)), ("Tcr", (1,)),
+                ("cp", (2, ns, 5)), ("h", (2, ns, 6)), ("hc", (2, na, 6)),
+                ("s", (2, na, 5)), ("logc", (2, na)),
+                ("A", (nrx,)), ("B", (nrx,)), ("EaR", (nrx,)), ("eff", (nrx, ns)),
+                ("effM", (nrx, na)), ("Pref", (1,)), ("T0", (1,))]
+
+    def const_array(self):
+        vals = {
+            "R": 1.0 / self.species_M, "M": self.species_M, "Tcr": [self.Tcr],
+            "cp": self.cp, "h": self.h, "hc": self.h_chem[:, :self.na],
+            "s": self.s[:, :self.na], "logc": self.logc[:, :self.na],
+            "A": self.A, "B": self.B, "EaR": self.EaR, "eff": self.eff,
+            "effM": self.effM, "Pref": [P_ATM / self.nd.P0], "T0": [self.nd.T0],
+        }
+        parts = []
+        for name, shape in self.layout():
+            arr = np.asarray(vals[name], dtype=np.float64)
+            assert arr.size == int(np.prod(shape)), name
+            parts.append(arr.reshape(-1))
+        return np.ascontiguousarray(np.concatenate(parts))
+
+    def structure_digest(self):
+        """Hash of everything compiled into the generated source."""
+        h = hashlib.sha256()
+        h.update(repr((self.species, self.na)).encode())
+        for r in self.reactions:
+            h.update(repr((sorted(r.reactants.items()), sorted(r.products.items()),
+                           r.three_body, r.b)).encode())
+        return h.hexdigest()[:16]
+
+    # ---------------- code generation ----------------
+    def generate_source(self):
+        return _generate(self)
+
+
+def _fmt(x):
+    return repr(float(x))
+
+
+def _power_expr(k, n, base="Xc"):
+    return " * ".join(["%s[%d]" % (base, k)] * int(n))
+
+
+def _generate(m):
+    na, ns = m.na, m.ns
+    L = []
+    e = L.append
+    e("// Generated by janc_amd/mechanism.py from %s -- do not edit." % os.path.basename(m.path))
+    e("// Structure digest %s: species %s" % (m.structure_digest(), " ".join(m.species)))
+    e("#pragma once")
+    e("")
+    e("// Net production rates w_k and dw_k/d(rho Y_n) for one cell, following")
+    e("// model/reaction_model.py:91-130 (reactionConstant_i summed over i).")
+    e("// X: unclipped rho*Y/M (all species); Xc: clip(X, 1e-50) for active species;")
+    e("// invMX[n] = 1/(M_n*Xc_n); g: Gibbs s/R - h/(RT) of active species.")
+    e("__host__ __device__ __forceinline__ void mech_rates_jacobian(")
+    e("    const MechConst& __restrict__ c, const double T, const double logT,")
+    e("    const double (&X)[JH_NS], const double (&Xc)[JH_NA], const double (&invMX)[JH_NA],")
+    e("    const double (&g)[JH_NA], double (&w)[JH_NA], double (&J)[JH_NA][JH_NA])")
+    e("{")
+    e("  const double Pf = c.Pref / T;   // 101325/P0/T")
+    e("#pragma unroll")
+    e("  for (int k = 0; k < JH_NA; ++k) { w[k] = 0.0;")
+    e("#pragma unroll")
+    e("    for (int n = 0; n < JH_NA; ++n) J[k][n] = 0.0; }")
+    for i, r in enumerate(m.reactions):
+        e("  { // R%d: %s" % (i, r.equation))
+        # kf_i = A * T^B * exp(-Ea/R / T)
+        if r.b == 0.0:
+            tb = None
+        elif r.b == 1.0:
+            tb = "T"
+        elif r.b == 2.0:
+            tb = "(T * T)"
+        elif r.b == -1.0:
+            tb = "(1.0 / T)"
+        elif r.b == -2.0:
+            tb = "(1.0 / (T * T))"
+        else:
+            tb = "exp(c.B[%d] * logT)" % i
+        kfi = "c.A[%d]" % i
+        if tb:
+            kfi = "(%s * %s)" % (kfi, tb)
+        if r.Ea != 0.0:
+            kfi = "%s * exp(-c.EaR[%d] / T)" % (kfi, i)
+        e("    const double kfi = %s;" % kfi)
+        pf = " * ".join(_power_expr(k, n) for k, n in sorted(r.reactants.items()))
+        pb = " * ".join(_power_expr(k, n) for k, n in sorted(r.products.items()))
+        e("    const double kf = kfi * (%s);" % pf)
+        dg = []
+        for k in range(na):
+            d = r.products.get(k, 0.0) - r.reactants.get(k, 0.0)
+            if d == 1.0:
+                dg.append("+ g[%d]" % k)
+            elif d == -1.0:
+                dg.append("- g[%d]" % k)
+            elif d != 0.0:
+                dg.append("%s %s * g[%d]" % ("+" if d > 0 else "-", _fmt(abs(d)), k))
+        sdg = " ".join(dg).lstrip("+ ") if dg else "0.0"
+        if sdg.startswith("- "):
+            sdg = "-" + sdg[2:]
+        vs = sum(r.products.values()) - sum(r.reactants.values())
+        if vs == 0.0:
+            eq = "exp(%s)" % sdg
+        elif vs == 1.0:
+            eq = "(exp(%s) * Pf)" % sdg
+        elif vs == -1.0:
+            eq = "(exp(%s) * (1.0 / Pf))" % sdg
+        else:
+            eq = "(exp(%s) * pow(Pf, %s))" % (sdg, _fmt(vs))
+        e("    const double kb = kfi / %s * (%s);" % (eq, pb))
+        if r.three_body:
+            terms = " + ".join("c.eff[%d][%d] * X[%d]" % (i, k, k) for k in range(ns))
+            e("    const double cM = %s;" % terms)
+        e("    const double q = kf - kb;")
+        cq = "cM * q" if r.three_body else "q"
+        # w_k += dnu_k * cM * q
+        dnu = {}
+        for k in range(na):
+            d = r.products.get(k, 0.0) - r.reactants.get(k, 0.0)
+            if d != 0.0:
+                dnu[k] = d
+        if r.three_body:
+            e("    const double cq = cM * q;")
+            cq = "cq"
+        for k, d in dnu.items():
+            e("    w[%d] %s %s;" % (k, "+=" if d > 0 else "-=",
+                                  cq if abs(d) == 1.0 else "2.0 * " + cq))
+        # d_n = q*effM_n + (invMX_n*cM)*(vf_n kf - vb_n kb)
+        involved = sorted(set(r.reactants) | set(r.products))
+        dn = {}
+        for n in range(na):
+            parts = []
+            vf, vb = r.reactants.get(n, 0.0), r.products.get(n, 0.0)
+            if r.three_body:
+                parts.append("q * c.effM[%d][%d]" % (i, n))
+            if n in involved:
+                if vf and vb:
+                    diff = "(%s - %s)" % (_scaled(vf, "kf"), _scaled(vb, "kb"))
+                elif vf:
+                    diff = _scaled(vf, "kf")
+                else:
+                    diff = "(-%s)" % _scaled(vb, "kb")
+                pre = "(invMX[%d] * cM)" % n if r.three_body else "invMX[%d]" % n
+                parts.append("%s * %s" % (pre, diff))
+            if parts:
+                dn[n] = " + ".join(parts)
+        for n, expr in dn.items():
+            e("    const double d%d = %s;" % (n, expr))
+        for k, d in dnu.items():
+            for n in dn:
+                op = "+=" if d > 0 else "-="
+                val = "d%d" % n if abs(d) == 1.0 else "2.0 * d%d" % n
+                e("    J[%d][%d] %s %s;" % (k, n, op, val))
+        e("  }")
+    e("#pragma unroll")
+    e("  for (int k = 0; k < JH_NA; ++k) { w[k] = c.M[k] * w[k];")
+    e("#pragma unroll")
+    e("    for (int n = 0; n < JH_NA; ++n) J[k][n] = c.M[k] * J[k][n]; }")
+    e("}")
+    return "\n".join(L) + "\n"
+
+
+def generate_dims(m):
+    return "\n".join([
+        "// Generated by janc_amd/mechanism.py from %s -- do not edit." % os.path.basename(m.path),
+        "#pragma once",
+        "#define JH_NS %d" % m.ns,
+        "#define JH_NA %d" % m.na,
+        "#define JH_NRX %d" % m.nrx,
+        '#define JH_MECH_DIGEST "%s"' % m.structure_digest(), ""])
+
+
+def _scaled(v, name):
+    return name if v == 1.0 else "%s * %s" % (_fmt(v), name)
+
+
+def main(argv=None):
+    import argparse
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("yaml", nargs="?", default=DEFAULT_MECH)
+    p.add_argument("-o", "--out", default=os.path.join(os.path.dirname(__file__), "csrc",
+                                                        "mech_gen.inc"))
+    a = p.parse_args(argv)
+    write_generated(Mechanism(a.yaml), os.path.dirname(a.out))
+    print("wrote", a.out)
+
+
+def write_generated(mech, outdir):
+    """Write mech_dims.h and mech_gen.inc; return True if either changed."""
+    changed = False
+    for name, text in (("mech_dims.h", generate_dims(mech)),
+                       ("mech_gen.inc", mech.generate_source())):
+        path = os.path.join(outdir, name)
+        old = open(path).read() if os.path.exists(path) else None
+        if old != text:
+            with open(path, "w") as fh:
+                fh.write(text)
+            changed = True
+    return changed
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,317 @@
+"""Host-side mirror of the reference's 2-D Simulator for the hot path.
+
+``Simulator(simulation_config)`` accepts the reference's configuration dict
+(``src/janc_v2/simulation/set_simulator.py:321-455``) and exposes the same
+callables -- ``advance_func(U, aux, t, theta)``, ``advance_one_step(U, aux,
+dx, dy, dt, theta)``, ``run(U_init, aux_init)`` -- plus the unit pieces
+``flux_func`` (set_simulator.py:44-47), ``update_func`` (aux_func.py:21-29),
+``reaction_source_terms`` (reaction_model.py:202-203) and ``CFL``
+(set_simulator.py:438-443).  Arrays are torch tensors on the GPU laid out like
+the reference's JAX arrays: U (12, nx, ny), aux (2, nx, ny); anything that
+speaks DLPack (a JAX GPU array, a torch tensor) is accepted, numpy arrays are
+copied to the device.  Every computation runs in libjanc_amd.so; there is no
+CPU path.
+
+Configuration errors follow the reference: unsupported keys raise
+``KeyError``/``AssertionError``/``FileNotFoundError`` at construction.
+Paths the library does not implement (Godunov/HLLC, viscosity, 1-D, AMR,
+user callables inside the fused step) raise ``NotImplementedError``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib
+from .mechanism import Mechanism
+
+try:
+    import torch
+except ImportError:  # pragma: no cover - torch is part of the runtime image
+    torch = None
+
+
+def _as_device_tensor(x, dtype, device):
+    if isinstance(x, np.ndarray):
+        return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device=device)
+    if not isinstance(x, torch.Tensor) and hasattr(x, "__dlpack__"):
+        x = torch.from_dlpack(x)
+    if x.device.type != "cuda":
+        x = x.to(device)
+    if x.dtype != dtype:
+        raise TypeError("expected %s state, got %s" % (dtype, x.dtype))
+    return x.contiguous()
+
+
+class _Context:
+    """Owns one jh_ctx (one grid shape, precision and boundary set)."""
+
+    def __init__(self, nx, ny, dx, dy, bc_kinds, bc_params, precision, mech):
+        lib = _lib.load()
+        consts = mech.const_array()
+        if consts.size != lib.jh_mech_const_count():
+            raise _lib.JancError("mechanism constant block has %d values, library expects %d"
+                                 % (consts.size, lib.jh_mech_const_count()))
+        digest = lib.jh_mech_digest().decode()
+        if digest != mech.structure_digest():
+            raise _lib.JancError(
+                "libjanc_amd.so was built for mechanism %s, configuration needs %s; "
+                "rebuild with that mechanism" % (digest, mech.structure_digest()))
+        kinds = (ctypes.c_int * 4)(*bc_kinds)
+        params = (ctypes.c_double * 4)(*bc_params)
+        carr = consts.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        h = ctypes.c_void_p()
+        stream = torch.cuda.current_stream().cuda_stream
+        rc = lib.jh_ctx_create(nx, ny, dx, dy, kinds, params, precision, carr, consts.size,
+                               ctypes.c_void_p(stream), ctypes.byref(h))
+        if rc != _lib.JH_OK:
+            raise _lib.JancError("jh_ctx_create: " + lib.jh_last_error(None).decode())
+        self.h = h
+        self.lib = lib
+        self.nx, self.ny = nx, ny
+
+    def call(self, name, *args):
+        self.lib.jh_ctx_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        rc = getattr(self.lib, name)(self.h, *args)
+        _lib.check(rc, self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.jh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Simulator:
+    """``Simulator`` of simulation/set_simulator.py:320-494, 2-D detailed chemistry."""
+
+    def __init__(self, simulation_config):
+        cfg = simulation_config
+        dim = cfg["dimension"]
+        if dim != "2D":
+            raise NotImplementedError("janc_amd implements the 2-D solver only")
+        grid = cfg["grid_config"]
+        thermo_config = cfg["thermo_config"]
+        reaction_config = cfg["reaction_config"]
+        flux_config = cfg["flux_config"]
+        boundary_config = cfg["boundary_config"]
+        time_control = cfg["time_config"]
+        self.t_end = time_control["t_end"]
+        self.theta = cfg.get("solver_parameters")
+        nondim_config = cfg.get("nondim_config")
+        comp = cfg.get("computation_config", {}) or {}
+        if comp.get("is_parallel", False):
+            raise NotImplementedError("multi-device x-slab decomposition is not built")
+        precision = comp.get("precision", "fp64")
+        if precision not in ("fp64", "mixed"):
+            raise KeyError("precision must be 'fp64' or 'mixed'")
+        self.precision = _lib.PREC_FP64 if precision == "fp64" else _lib.PREC_MIXED
+        self.dtype = torch.float64 if precision == "fp64" else torch.float32
+
+        # thermo_model.py:42-62
+        if thermo_config["thermo_model"] != "nasa7":
+            if thermo_config["thermo_model"] == "constant_gamma":
+                raise NotImplementedError("constant_gamma thermo is not on the built path")
+            raise KeyError("The thermo model you specified is not supported, only 'nasa7' "
+                           "or 'constant_gamma' can be specified.")
+        assert "mechanism_directory" in thermo_config, \
+            "Please specify 'mechanism_directory' in your dict of settings"
+        # reaction_model.py:41-46
+        if not reaction_config["is_detailed_chemistry"]:
+            raise NotImplementedError("only detailed chemistry is on the built path")
+        assert "mechanism_directory" in reaction_config, \
+            "You choosed detailed chemistry without specifying the directory of your " \
+            "mechanism files"
+        self.mech = Mechanism(reaction_config["mechanism_directory"], nondim_config)
+        # flux.py:19-41
+        st = flux_config["solver_type"]
+        if st == "godunov":
+            raise NotImplementedError("the Godunov (HLLC/AUSMPW+) path is not built")
+        if st != "flux_splitting":
+            raise KeyError("JANC only support 'godunov' and 'flux_splitting'")
+        rec = flux_config["interface_reconstruction"]
+        if rec != "WENO5_JS":
+            if rec == "MUSCL":
+                raise NotImplementedError("MUSCL reconstruction is not built")
+            raise KeyError(rec)
+        if flux_config["split_method"] != "LF":
+            if flux_config["split_method"] == "SW":
+                raise NotImplementedError("Steger-Warming splitting is not built")
+            raise KeyError(flux_config["split_method"])
+        if flux_config["viscosity"] == "on":
+            raise NotImplementedError("viscous fluxes are not built")
+        if time_control["temporal_evolution_scheme"] != "TVD_RK3":
+            raise KeyError(time_control["temporal_evolution_scheme"])
+        # boundary.py:51-118
+        self.bc_kinds, self.user_bc = [], {}
+        for side in _lib.SIDES:
+            k = boundary_config[side + "_boundary"]
+            if callable(k):
+                self.user_bc[side] = k
+                self.bc_kinds.append(_lib.BC_KINDS["user"])
+            elif k in _lib.BC_KINDS and k != "user":
+                self.bc_kinds.append(_lib.BC_KINDS[k])
+            else:
+                raise KeyError(k)
+        if self.user_bc:
+            raise NotImplementedError("callable boundary conditions are not on the fused path")
+        self.dx, self.dy = float(grid["dx"]), float(grid["dy"])
+        self.fixed_dt = float(time_control["dt"]) if "dt" in time_control else None
+        self.cfl = float(time_control["CFL"]) if "CFL" in time_control else None
+        if self.fixed_dt is None and self.cfl is None:
+            raise KeyError("time_config needs 'dt' or 'CFL'")
+        self._ctx = {}
+        self.device = torch.device("cuda", torch.cuda.current_device())
+
+    # ------------------------------------------------------------ helpers
+    def _bc_params(self, theta):
+        pb = 0.0
+        if _lib.BC_KINDS["pressure_outlet"] in self.bc_kinds:
+            theta = theta if theta is not None else self.theta
+            pb = float(theta["Pb"])
+        return [pb if k == _lib.BC_KINDS["pressure_outlet"] else 0.0 for k in self.bc_kinds]
+
+    def _context(self, U, theta, dx=None, dy=None):
+        nx, ny = int(U.shape[1]), int(U.shape[2])
+        dx = self.dx if dx is None else float(dx)
+        dy = self.dy if dy is None else float(dy)
+        key = (nx, ny, dx, dy)
+        ctx = self._ctx.get(key)
+        if ctx is None:
+            ctx = _Context(nx, ny, dx, dy, self.bc_kinds, self._bc_params(theta),
+                           self.precision, self.mech)
+            self._ctx[key] = ctx
+        else:
+            for s, v in enumerate(self._bc_params(theta)):
+                _lib.check(ctx.lib.jh_ctx_set_bc_param(ctx.h, s, v), ctx.h)
+        return ctx
+
+    def _state(self, U, aux):
+        U = _as_device_tensor(U, self.dtype, self.device)
+        aux = _as_device_tensor(aux, self.dtype, self.device)
+        nv = 4 + self.mech.ns - 1
+        if U.dim() != 3 or U.shape[0] != nv:
+            raise ValueError("U must be (%d, nx, ny)" % nv)
+        if aux.shape != (2,) + tuple(U.shape[1:]):
+            raise ValueError("aux must be (2, nx, ny)")
+        return U, aux
+
+    # ------------------------------------------------------------ step API
+    def advance_one_step(self, U, aux, dx, dy, dt, theta=None):
+        """set_advance_func.advance_one_step (set_simulator.py:117-122)."""
+        U, aux = self._state(U, aux)
+        ctx = self._context(U, theta, dx, dy)
+        Uo, ao = torch.empty_like(U), torch.empty_like(aux)
+        ctx.call("jh_advance", _ptr(U), _ptr(aux), _ptr(Uo), _ptr(ao), float(dt), 0.0, 1, None)
+        return Uo, ao
+
+    def advance_func(self, U, aux, t, theta=None):
+        """Simulator.advance_func (set_simulator.py:405-408 / :444-447)."""
+        U, aux = self._state(U, aux)
+        ctx = self._context(U, theta)
+        Uo, ao = torch.empty_like(U), torch.empty_like(aux)
+        dt = torch.empty((1,), dtype=torch.float64, device=U.device)
+        if self.fixed_dt is not None:
+            ctx.call("jh_advance", _ptr(U), _ptr(aux), _ptr(Uo), _ptr(ao), self.fixed_dt, 0.0, 1,
+                     _ptr(dt))
+        else:
+            ctx.call("jh_advance", _ptr(U), _ptr(aux), _ptr(Uo), _ptr(ao), 0.0, self.cfl, 1,
+                     _ptr(dt))
+        dt = dt[0]
+        return Uo, ao, t + dt, dt
+
+    def advance(self, U, aux, nsteps, theta=None, dt_hist=None):
+        """nsteps calls of advance_func fused into one library call, in place.
+
+        Returns the per-step dt as a device tensor."""
+        U, aux = self._state(U, aux)
+        ctx = self._context(U, theta)
+        if dt_hist is None:
+            dt_hist = torch.empty((nsteps,), dtype=torch.float64, device=U.device)
+        if self.fixed_dt is not None:
+            ctx.call("jh_advance", _ptr(U), _ptr(aux), _ptr(U), _ptr(aux), self.fixed_dt, 0.0,
+                     int(nsteps), _ptr(dt_hist))
+        else:
+            ctx.call("jh_advance", _ptr(U), _ptr(aux), _ptr(U), _ptr(aux), 0.0, self.cfl,
+                     int(nsteps), _ptr(dt_hist))
+        return U, aux, dt_hist
+
+    def run(self, U_init, aux_init):
+        """Simulator.run (set_simulator.py:457-494) without the HDF5 saver."""
+        U, aux = self._state(U_init, aux_init)
+        U, aux = U.clone(), aux.clone()
+        t = 0.0
+        while t < self.t_end:
+            U, aux, t_dev, dt = self.advance_func(U, aux, t, self.theta)
+            t = float(t_dev)
+        return U, aux, t
+
+    # ------------------------------------------------------------ unit pieces
+    def flux_func(self, U, aux, dx, dy, dt, theta=None):
+        """set_rhs.flux_func (set_simulator.py:44-47): dt * total_flux(BC(U, aux))."""
+        U, aux = self._state(U, aux)
+        ctx = self._context(U, theta, dx, dy)
+        out = torch.empty_like(U)
+        ctx.call("jh_rhs", _ptr(U), _ptr(aux), float(dt), _ptr(out))
+        return out
+
+    def update_func(self, U, aux):
+        """aux_func.update_aux (aux_func.py:21-29)."""
+        U, aux = self._state(U, aux)
+        ctx = self._context(U, None)
+        out = torch.empty_like(aux)
+        ctx.call("jh_update_aux", _ptr(U), _ptr(aux), _ptr(out))
+        return out
+
+    def reaction_source_terms(self, U, aux, dt, theta=None):
+        """reaction_model.reaction_source_terms (reaction_model.py:202-203)."""
+        U, aux = self._state(U, aux)
+        ctx = self._context(U, theta)
+        out = torch.empty_like(U)
+        ctx.call("jh_reaction_source", _ptr(U), _ptr(aux), float(dt), _ptr(out))
+        return out
+
+    def CFL(self, U, aux, cfl=None):
+        """CFL_2D (set_simulator.py:438-443) as a device scalar."""
+        U, aux = self._state(U, aux)
+        ctx = self._context(U, None)
+        out = torch.empty((1,), dtype=torch.float64, device=U.device)
+        ctx.call("jh_cfl_dt", _ptr(U), _ptr(aux), float(cfl if cfl is not None else self.cfl),
+                 _ptr(out))
+        return out[0]
+
+    def lf_alpha(self, U, aux, theta=None):
+        """The two LF maxima of LF.py:10-11 over the ghost-padded field."""
+        U, aux = self._state(U, aux)
+        ctx = self._context(U, theta)
+        out = torch.empty((2,), dtype=torch.float64, device=U.device)
+        ctx.call("jh_lf_alpha", _ptr(U), _ptr(aux), _ptr(out))
+        return out
+
+    def timing(self, U, enable=None):
+        """Enable/disable per-launch HIP-event timing, or read (and reset)
+        {kind: (total_ms, launches)} when ``enable`` is None."""
+        ctx = self._context(U, None)
+        if enable is not None:
+            _lib.check(ctx.lib.jh_timing_enable(ctx.h, int(bool(enable))), ctx.h)
+            return None
+        ms = (ctypes.c_double * 4)()
+        cnt = (ctypes.c_int * 4)()
+        _lib.check(ctx.lib.jh_timing_read(ctx.h, ms, cnt), ctx.h)
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(_lib.KERNEL_KINDS)}
+
+    def health(self, U):
+        """Device health flags since the last query (1 non-finite, 2 rho <= 0)."""
+        ctx = self._context(U, None)
+        f = ctypes.c_uint()
+        _lib.check(ctx.lib.jh_health(ctx.h, ctypes.byref(f)), ctx.h)
+        return int(f.value)

@@ -1,0 +1,158 @@
+"""GPU parity: every C-ABI entry point against the FP64 oracle.
+
+Bar: FP64 form <= 1e-12 relative error per variable (max-norm of the
+difference over max-norm of the oracle's field); mixed form: the stated
+L2/Linf bounds below.  Grid sizes are chosen so the oracle finishes in
+seconds and the tiles are ragged (nx not a multiple of 8, ny not of 64).
+"""
+import numpy as np
+import pytest
+
+from janc_amd import cases
+from janc_amd.mechanism import Mechanism
+from oracle.step import Oracle
+
+torch = pytest.importorskip("torch")
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not torch.cuda.is_available(), reason="needs a GPU")]
+
+FP64_TOL = 1e-12
+
+
+def rel_err(got, ref):
+    got = got.detach().cpu().numpy() if hasattr(got, "detach") else np.asarray(got)
+    ref = np.asarray(ref)
+    d = np.abs(got - ref).reshape(ref.shape[0], -1).max(axis=1)
+    s = np.abs(ref).reshape(ref.shape[0], -1).max(axis=1)
+    return float(np.max(d / np.maximum(s, 1e-300)))
+
+
+def _case(name, mech):
+    if name == "box":
+        U, aux, g = cases.periodic_box(mech, nx=37, ny=70)
+        theta = None
+    elif name == "tube":
+        U, aux, g = cases.detonation_tube(mech, nx=45, ny=22, driver=8)
+        theta = None
+    else:   # walls of every kind + a subsonic pressure outlet
+        U, aux, g = cases.detonation_tube(mech, nx=29, ny=67, driver=6, seed=5)
+        g["boundary_config"] = {"left_boundary": "no_slip_wall",
+                                "right_boundary": "pressure_outlet",
+                                "bottom_boundary": "zero_gradient",
+                                "top_boundary": "pressure_outlet"}
+        theta = {"Pb": 1.5 * 101325.0}
+    return U, aux, g, theta
+
+
+CASES = ["box", "tube", "outlet"]
+
+
+@pytest.fixture(scope="module")
+def mech(mech_path):
+    return Mechanism(mech_path)
+
+
+def _pair(mech_path, mech, name, precision="fp64", dt=None):
+    from janc_amd.solver import Simulator
+    U, aux, g, theta = _case(name, mech)
+    cfg = cases.simulation_config(g, mech_path, cfl=0.3, dt=dt, precision=precision,
+                                  theta=theta)
+    sim = Simulator(cfg)
+    o = Oracle(mech_path, g["boundary_config"], g["dx"], g["dy"])
+    print("case", name, precision, U.shape, flush=True)
+    return sim, o, U, aux, theta
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_flux_func(mech_path, mech, name):
+    sim, o, U, aux, theta = _pair(mech_path, mech, name)
+    dt = 3e-9
+    got = sim.flux_func(U, aux, sim.dx, sim.dy, dt, theta)
+    ref = o.flux_func(U, aux, dt, theta)
+    assert rel_err(got, ref) <= FP64_TOL
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_lf_alpha_and_cfl(mech_path, mech, name):
+    sim, o, U, aux, theta = _pair(mech_path, mech, name)
+    a = sim.lf_alpha(U, aux, theta).cpu().numpy()
+    ax, ay = o.lf_alpha(U, aux, theta)
+    np.testing.assert_allclose(a, [ax, ay], rtol=1e-14)
+    dt = float(sim.CFL(U, aux, 0.3))
+    assert dt == pytest.approx(o.cfl_dt(U, aux, 0.3), rel=1e-14)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_update_aux(mech_path, mech, name):
+    sim, o, U, aux, theta = _pair(mech_path, mech, name)
+    guess = aux.copy()
+    guess[1] *= 1.0 + 0.03 * np.cos(np.arange(guess[1].size)).reshape(guess[1].shape)
+    got = sim.update_func(U, guess)
+    ref = o.update_func(U, guess)
+    assert rel_err(got, ref) <= FP64_TOL
+
+
+@pytest.mark.parametrize("name", ["box", "tube"])
+def test_reaction_source(mech_path, mech, name):
+    sim, o, U, aux, theta = _pair(mech_path, mech, name)
+    for dt in (1e-9, 1e-7):
+        got = sim.reaction_source_terms(U, aux, dt)
+        ref = o.reaction_source_terms(U, aux, dt)
+        assert np.all(got[:4].cpu().numpy() == 0)
+        assert rel_err(got[4:], ref[4:]) <= FP64_TOL
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_advance_func_one_step(mech_path, mech, name):
+    sim, o, U, aux, theta = _pair(mech_path, mech, name)
+    Ug, ag, t, dt = sim.advance_func(U, aux, 0.0, theta)
+    Ur, ar, tr, dtr = o.advance_func(U, aux, 0.0, 0.3, theta)
+    assert float(dt) == pytest.approx(dtr, rel=1e-14)
+    assert rel_err(Ug, Ur) <= FP64_TOL
+    assert rel_err(ag, ar) <= FP64_TOL
+
+
+@pytest.mark.parametrize("name", ["tube", "outlet"])
+def test_fixed_dt_five_steps(mech_path, mech, name):
+    sim, o, U, aux, theta = _pair(mech_path, mech, name, dt=4e-9)
+    Ug, ag = torch.as_tensor(U).cuda(), torch.as_tensor(aux).cuda()
+    Ur, ar = U, aux
+    for _ in range(5):
+        Ug, ag = sim.advance_one_step(Ug, ag, sim.dx, sim.dy, 4e-9, theta)
+        Ur, ar = o.advance_one_step(Ur, ar, 4e-9, theta)
+    assert rel_err(Ug, Ur) <= FP64_TOL
+    assert rel_err(ag, ar) <= FP64_TOL
+
+
+def test_fused_steps_equal_single_steps(mech_path, mech):
+    sim, o, U, aux, theta = _pair(mech_path, mech, "tube")
+    Ua, aa = torch.as_tensor(U).cuda(), torch.as_tensor(aux).cuda()
+    for _ in range(3):
+        Ua, aa, _, _ = sim.advance_func(Ua, aa, 0.0)
+    Ub, ab = torch.as_tensor(U).cuda().clone(), torch.as_tensor(aux).cuda().clone()
+    _, _, dts = sim.advance(Ub, ab, 3)
+    assert torch.equal(Ua, Ub) and torch.equal(aa, ab)
+    assert dts.shape == (3,) and bool(torch.all(dts > 0))
+
+
+def test_mixed_precision_against_fp64_oracle(mech_path, mech):
+    """Mixed form (FP32 state/flux/RK, FP64 chemistry and T solve) after 10
+    CFL steps of the detonation tube: relative L2 error of rho, p, T, Y_k
+    vs the FP64 oracle <= 1e-4 and Linf (relative to the field max) <= 2e-3."""
+    sim, o, U, aux, theta = _pair(mech_path, mech, "tube", precision="mixed")
+    Ug = torch.as_tensor(U, dtype=torch.float32).cuda()
+    ag = torch.as_tensor(aux, dtype=torch.float32).cuda()
+    Ur, ar = U, aux
+    for _ in range(10):
+        Ug, ag, _, _ = sim.advance_func(Ug, ag, 0.0)
+        Ur, ar, _, _ = o.advance_func(Ur, ar, 0.0, 0.3)
+    Ug = Ug.double().cpu().numpy()
+    ag = ag.double().cpu().numpy()
+    fields = {"rho": (Ug[0], Ur[0]), "T": (ag[1], ar[1])}
+    for k in range(8):
+        fields["Y%d" % k] = (Ug[4 + k] / Ug[0], Ur[4 + k] / Ur[0])
+    for name, (g, r) in fields.items():
+        l2 = np.linalg.norm(g - r) / max(np.linalg.norm(r), 1e-300)
+        linf = np.abs(g - r).max() / max(np.abs(r).max(), 1e-300)
+        print(name, "L2", l2, "Linf", linf)
+        assert l2 <= 1e-4 and linf <= 2e-3, name

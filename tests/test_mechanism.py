@@ -1,0 +1,64 @@
+"""The product's mechanism compiler against the oracle's preprocessing, and
+the committed generated sources against a fresh generation."""
+import os
+
+import numpy as np
+import pytest
+
+from janc_amd.mechanism import Mechanism, generate_dims
+from oracle.mech import load_mechanism
+
+CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "janc_amd",
+                    "csrc")
+
+
+@pytest.mark.parametrize("nondim", [None, {"P0": 10 * 101325, "T0": 300, "R0": 369,
+                                           "x0": 0.08}])
+def test_constants_match_oracle(mech_path, nondim):
+    m = Mechanism(mech_path, nondim)
+    rp, th, nd = load_mechanism(mech_path, nondim)
+    np.testing.assert_allclose(m.A, rp["A"], rtol=1e-15)
+    np.testing.assert_array_equal(m.B, rp["B"])
+    np.testing.assert_allclose(m.EaR, rp["Ea/Ru"], rtol=1e-15)
+    np.testing.assert_allclose(m.eff, rp["third_body_coeffs"], rtol=1e-15)
+    np.testing.assert_array_equal(m.vsum, rp["vsum"])
+    np.testing.assert_allclose(m.species_M, th["species_M"], rtol=1e-15)
+    for r, tag in ((0, "low"), (1, "high")):
+        np.testing.assert_allclose(m.cp[r], th["cp_" + tag], rtol=1e-15)
+        np.testing.assert_allclose(m.h[r], th["h_" + tag], rtol=1e-15)
+        np.testing.assert_allclose(m.h_chem[r], th["h_chem_" + tag], rtol=1e-15)
+        np.testing.assert_allclose(m.s[r], th["s_" + tag], rtol=1e-15)
+        np.testing.assert_allclose(m.logc[r], th["logcof_" + tag], rtol=1e-15)
+    assert m.Tcr == th["Tcr"][0]
+    vf = np.zeros((m.nrx, m.na))
+    vb = np.zeros((m.nrx, m.na))
+    for i, r in enumerate(m.reactions):
+        for k, c in r.reactants.items():
+            vf[i, k] = c
+        for k, c in r.products.items():
+            vb[i, k] = c
+    np.testing.assert_array_equal(vf, rp["vf"])
+    np.testing.assert_array_equal(vb, rp["vb"])
+
+
+def test_const_array_layout(mech_path):
+    m = Mechanism(mech_path)
+    a = m.const_array()
+    n = sum(int(np.prod(s)) for _, s in m.layout())
+    assert a.shape == (n,) and a.dtype == np.float64
+    assert a[0] == pytest.approx(1.0 / m.species_M[0])
+
+
+def test_generated_sources_are_current(mech_path):
+    m = Mechanism(mech_path)
+    with open(os.path.join(CSRC, "mech_gen.inc")) as fh:
+        assert fh.read() == m.generate_source()
+    with open(os.path.join(CSRC, "mech_dims.h")) as fh:
+        assert fh.read() == generate_dims(m)
+
+
+def test_config_errors_follow_reference(tmp_path):
+    with pytest.raises(AssertionError):
+        Mechanism(str(tmp_path / "mech.cti"))
+    with pytest.raises(FileNotFoundError):
+        Mechanism(str(tmp_path / "missing.yaml"))
