@@ -64,11 +64,14 @@ def _roofline(kt, cells, word_bytes):
     return out
 
 
+CPU_CROP = (400, 400)
+
+
 def _cpu_baseline(U, aux, geom, theta, steps):
-    """FP64 oracle (numpy, one core) on a 200 x 80 crop of the headline state."""
+    """FP64 oracle (numpy, one core) on a 400 x 400 crop of the headline state."""
     from oracle.step import Oracle
     from janc_amd.mechanism import DEFAULT_MECH
-    crop = (slice(None), slice(0, 200), slice(0, 80))
+    crop = (slice(None), slice(0, CPU_CROP[0]), slice(0, CPU_CROP[1]))
     Uc, ac = U[crop].copy(), aux[crop].copy()
     o = Oracle(DEFAULT_MECH, geom["boundary_config"], geom["dx"], geom["dy"])
     t = 0.0
@@ -80,9 +83,64 @@ def _cpu_baseline(U, aux, geom, theta, steps):
     cells = Uc.shape[1] * Uc.shape[2]
     return {"value": cells * steps / el, "unit": "cell-updates/s", "cores": 1,
             "kind": "port",
-            "sample": "%d CFL steps of the FP64 numpy oracle on the 200x80 corner crop "
+            "sample": "%d CFL steps of the FP64 numpy oracle on the %dx%d corner crop "
                       "(seed columns included) of the 4000x400 headline state, same BC kinds; "
-                      "%.1f s" % (steps, el)}
+                      "%.1f s" % (steps, CPU_CROP[0], CPU_CROP[1], el)}
+
+
+def _fields(U, aux, inv_M):
+    """rho, p, T, Y_k (float64 torch) of a state."""
+    import torch
+    U = U.double()
+    aux = aux.double()
+    rho = U[0]
+    Y = U[4:] / rho
+    Y9 = torch.cat([Y, (1.0 - Y.sum(0))[None]])
+    R = (inv_M[:, None, None] * Y9).sum(0)
+    return {"rho": rho, "p": rho * R * aux[1], "T": aux[1], "Y": Y}
+
+
+def _accuracy(mech, steps, every, torch):
+    """Mixed vs FP64 step on the 8000 x 500 tube at one fixed dt (the initial
+    CFL-0.3 dt): relative L2 and Linf of rho, p, T, Y_k every `every` steps."""
+    from janc_amd import cases
+    from janc_amd.solver import Simulator
+    U3, aux3, g3 = cases.mixed_tube(mech)
+    probe = Simulator(cases.simulation_config(g3, cfl=0.3))
+    dt = float(probe.CFL(U3, aux3, 0.3))
+    del probe
+    inv_M = torch.as_tensor(1.0 / mech.species_M).cuda()
+    sims, states = {}, {}
+    for prec, dtype in (("mixed", torch.float32), ("fp64", torch.float64)):
+        sims[prec] = Simulator(cases.simulation_config(g3, dt=dt, precision=prec))
+        states[prec] = (torch.as_tensor(U3, dtype=dtype).cuda(),
+                        torch.as_tensor(aux3, dtype=dtype).cuda())
+    report = []
+    done = 0
+    while done < steps:
+        n = min(every, steps - done)
+        for prec in sims:
+            sims[prec].advance(*states[prec], n)
+        done += n
+        fm = _fields(*states["mixed"], inv_M)
+        fd = _fields(*states["fp64"], inv_M)
+        row = {"step": done}
+        for k in ("rho", "p", "T"):
+            d = (fm[k] - fd[k])
+            row[k] = {"L2": float(d.norm() / fd[k].norm()),
+                      "Linf": float(d.abs().max() / fd[k].abs().max())}
+        ys = {}
+        for i, name in enumerate(mech.species[:-1]):
+            d = fm["Y"][i] - fd["Y"][i]
+            ys[name] = {"L2": float(d.norm() / max(float(fd["Y"][i].norm()), 1e-300)),
+                        "Linf": float(d.abs().max() / max(float(fd["Y"][i].abs().max()), 1e-300)),
+                        "max": float(fd["Y"][i].abs().max())}
+        row["Y"] = ys
+        report.append(row)
+    flags = {p: sims[p].health(states[p][0]) for p in sims}
+    return {"workload": "8000x500 tube (dx 1.25e-5 m), fixed dt %.4g s, mixed vs FP64 GPU "
+                        "step (the FP64 step is parity-checked against the oracle)" % dt,
+            "dt": dt, "steps": steps, "health_flags": flags, "every": report}
 
 
 def main():
@@ -91,10 +149,15 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--mixed-steps", type=int, default=100)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--accuracy-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the FP64 headline workload (for profiler runs)")
     args = ap.parse_args()
+    if args.headline_only:
+        args.no_mixed = args.no_cpu = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the single-block step on one GPU; "
               "--gpus N > 1 is not supported (see DESIGN.md for how it would shard)",
@@ -167,9 +230,13 @@ def main():
             del Ut, at
             torch.cuda.empty_cache()
         out["speedup_mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
-        out["workload"] = ("8000x500 tube, phi in [0.8,1.2] smooth field, 40-column "
-                           "20 atm/2800 K ignition slab, %d CFL steps" % args.mixed_steps)
+        out["workload"] = ("8000x500 tube (dx 1.25e-5 m), phi in [0.8,1.2] smooth field, "
+                           "40-column 20 atm/2800 K ignition slab, %d CFL steps"
+                           % args.mixed_steps)
         result["mixed_tube"] = out
+
+    if not args.no_mixed and args.accuracy_steps > 0:
+        result["mixed_accuracy"] = _accuracy(mech, args.accuracy_steps, 100, torch)
 
     if not args.no_cpu:
         result["cpu_baseline"] = _cpu_baseline(U0, aux0, geom, theta, args.cpu_steps)

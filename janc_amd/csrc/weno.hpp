@@ -7,6 +7,8 @@
 #pragma once
 #include <math.h>
 
+#include "janc_types.hpp"
+
 namespace jh {
 
 // FP64 form.  w_k = alpha_k / sum(alpha) is evaluated as
@@ -56,7 +58,10 @@ __host__ __device__ __forceinline__ float weno5(float fm2, float fm1, float f0, 
   const float q1 = (1.0f / 3.0f) * fm2 - (7.0f / 6.0f) * fm1 + (11.0f / 6.0f) * f0;
   const float q2 = -(1.0f / 6.0f) * fm1 + (5.0f / 6.0f) * f0 + (1.0f / 3.0f) * fp1;
   const float q3 = (1.0f / 3.0f) * f0 + (5.0f / 6.0f) * fp1 - (1.0f / 6.0f) * fp2;
-  return (al1 * q1 + al2 * q2 + al3 * q3) / (al1 + al2 + al3);
+  // alpha_k reaches 6e29 (eps' = 1e-15): normalise before multiplying by the
+  // unscaled candidates, whose magnitude may be ~1e10
+  const float inv = 1.0f / (al1 + al2 + al3);
+  return (al1 * inv) * q1 + (al2 * inv) * q2 + (al3 * inv) * q3;
 }
 
 }  // namespace jh
