@@ -4,13 +4,14 @@
 #pragma once
 #include <math.h>
 
+#include "fastmath.hpp"
 #include "janc_types.hpp"
 #include "mech_gen.inc"
 
 namespace jh {
 
 // g_k = s_k/R - h_k/(R T) for the active species (get_gibbs, :78-89).
-__host__ __device__ __forceinline__ void gibbs(const MechConst& c, double T,
+__host__ __device__ __forceinline__ void gibbs(const MechConst& c, double T, double invT,
                                                double (&g)[JH_NA]) {
   const int r = T < c.Tcr ? 0 : 1;
   const double lT = log(c.T0 * T);
@@ -23,7 +24,7 @@ __host__ __device__ __forceinline__ void gibbs(const MechConst& c, double T,
 #pragma unroll
     for (int j = 1; j < 5; ++j) s = s * T + c.s[r][k][j];
     s = s + c.logc[r][k] * lT;
-    g[k] = s - h / T;
+    g[k] = s - h * invT;
   }
 }
 
@@ -57,7 +58,7 @@ __host__ __device__ __forceinline__ void lu_solve(double (&A)[N][N], double (&b)
       b[col] = s ? b[r] : t;
       b[r] = s ? t : b[r];
     }
-    const double inv = 1.0 / A[col][col];
+    const double inv = rcp(A[col][col]);
 #pragma unroll
     for (int r = col + 1; r < N; ++r) {
       const double l = A[r][col] * inv;
@@ -71,7 +72,7 @@ __host__ __device__ __forceinline__ void lu_solve(double (&A)[N][N], double (&b)
     double s = b[r];
 #pragma unroll
     for (int j = r + 1; j < N; ++j) s -= A[r][j] * b[j];
-    b[r] = s / A[r][r];
+    b[r] = s * rcp(A[r][r]);
   }
 }
 
@@ -81,27 +82,29 @@ __host__ __device__ __forceinline__ void chem_increment(const MechConst& c, doub
                                                         const double (&rhoY_in)[JH_NS - 1],
                                                         double T, double dt,
                                                         double (&drhoY)[JH_NA]) {
+  const double irho = rcp(rho);
   double Y[JH_NS];
 #pragma unroll
-  for (int k = 0; k < JH_NS - 1; ++k) Y[k] = rhoY_in[k] / rho;
+  for (int k = 0; k < JH_NS - 1; ++k) Y[k] = rhoY_in[k] * irho;
   double sum = 0.0;
 #pragma unroll
   for (int k = 0; k < JH_NS - 1; ++k) sum += Y[k];
   Y[JH_NS - 1] = 1.0 - sum;
   double X[JH_NS];
 #pragma unroll
-  for (int k = 0; k < JH_NS; ++k) X[k] = (rho * Y[k]) / c.M[k];
+  for (int k = 0; k < JH_NS; ++k) X[k] = (rho * Y[k]) * c.R[k];
   double Xc[JH_NA], invMX[JH_NA];
 #pragma unroll
   for (int k = 0; k < JH_NA; ++k) {
     Xc[k] = fmax(X[k], 1e-50);
-    invMX[k] = 1.0 / (c.M[k] * Xc[k]);
+    invMX[k] = rcp(c.M[k] * Xc[k]);
   }
+  const double invT = rcp(T);
   double g[JH_NA];
-  gibbs(c, T, g);
+  gibbs(c, T, invT, g);
   double w[JH_NA], J[JH_NA][JH_NA];
-  mech_rates_jacobian(c, T, log(T), X, Xc, invMX, g, w, J);
-  const double idt = 1.0 / dt;
+  mech_rates_jacobian(c, T, invT, log(T), X, Xc, invMX, g, w, J);
+  const double idt = rcp(dt);
 #pragma unroll
   for (int k = 0; k < JH_NA; ++k) {
 #pragma unroll
@@ -110,7 +113,7 @@ __host__ __device__ __forceinline__ void chem_increment(const MechConst& c, doub
   lu_solve<JH_NA>(J, w);
 #pragma unroll
   for (int k = 0; k < JH_NA; ++k) {
-    double dY = w[k] / rho;
+    double dY = w[k] * irho;
     dY = fmin(fmax(dY, -Y[k]), 1.0 - Y[k]);
     drhoY[k] = rho * dY;
   }

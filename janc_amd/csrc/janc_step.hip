@@ -112,6 +112,9 @@ int grid_cells(const jh_ctx* c) {
   return b < 2048 ? b : 2048;
 }
 
+// one thread per cell (pointwise kernels without a grid-stride loop)
+int grid_all(const jh_ctx* c) { return (c->nx * c->ny + 255) / 256; }
+
 int stage_blocks(const jh_ctx* c) {
   return ((c->nx + TXR - 1) / TXR) * ((c->ny + TY - 1) / TY);
 }
@@ -159,7 +162,7 @@ int advance_impl(jh_ctx* c, const Real* U, const Real* aux, Real* Uo, Real* auxo
       stage_kernel<Real, MODE_RK><<<gb, NT, 0, c->stream>>>(a, 2, U2, A2, Un, Uo, auxo); }
     // chemistry + final update_aux in place, next step's maxima
     { Timed t(c, JH_KERNEL_CHEM);
-      chem_kernel<Real, CHEM_STEP><<<grid_cells(c), 256, 0, c->stream>>>(
+      chem_kernel<Real, CHEM_STEP><<<grid_all(c), 256, 0, c->stream>>>(
           a, Uo, auxo, nullptr, dt_hist ? dt_hist + s : nullptr); }
     rc = check_launch(c, "step kernels");
     if (rc) return rc;
@@ -321,11 +324,11 @@ int jh_update_aux(jh_ctx* c, const void* U, const void* aux_in, void* aux_out) {
   if (!U || !aux_in || !aux_out) return fail(c, JH_ERR_ARG, "NULL pointer");
   if (c->precision == JH_PREC_FP64) {
     auto a = make_args<double>(c, 1.0, 0.0);
-    update_aux_kernel<double><<<grid_cells(c), 256, 0, c->stream>>>(
+    update_aux_kernel<double><<<grid_all(c), 256, 0, c->stream>>>(
         a, (const double*)U, (const double*)aux_in, (double*)aux_out);
   } else {
     auto a = make_args<float>(c, 1.0, 0.0);
-    update_aux_kernel<float><<<grid_cells(c), 256, 0, c->stream>>>(
+    update_aux_kernel<float><<<grid_all(c), 256, 0, c->stream>>>(
         a, (const float*)U, (const float*)aux_in, (float*)aux_out);
   }
   return check_launch(c, "update_aux_kernel");
@@ -337,11 +340,11 @@ int jh_reaction_source(jh_ctx* c, const void* U, const void* aux, double dt, voi
   if (!(dt > 0)) return fail(c, JH_ERR_ARG, "dt must be > 0");
   if (c->precision == JH_PREC_FP64) {
     auto a = make_args<double>(c, dt, 0.0);
-    chem_kernel<double, CHEM_SOURCE><<<grid_cells(c), 256, 0, c->stream>>>(
+    chem_kernel<double, CHEM_SOURCE><<<grid_all(c), 256, 0, c->stream>>>(
         a, (double*)U, (double*)aux, (double*)S, nullptr);
   } else {
     auto a = make_args<float>(c, dt, 0.0);
-    chem_kernel<float, CHEM_SOURCE><<<grid_cells(c), 256, 0, c->stream>>>(
+    chem_kernel<float, CHEM_SOURCE><<<grid_all(c), 256, 0, c->stream>>>(
         a, (float*)U, (float*)aux, (float*)S, nullptr);
   }
   return check_launch(c, "chem_kernel");

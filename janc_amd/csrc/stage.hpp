@@ -83,13 +83,14 @@ __device__ __forceinline__ Prim<W> cell_prim(const MechConst& c, const Real* __r
                                              const Real* __restrict__ ap, int stride) {
   Prim<W> q;
   q.rho = (W)Up[0];
-  q.u = (W)Up[stride] / q.rho;
-  q.v = (W)Up[2 * stride] / q.rho;
+  const W ir = rcp(q.rho);
+  q.u = (W)Up[stride] * ir;
+  q.v = (W)Up[2 * stride] * ir;
   W sum = (W)0, R = (W)0;
   W Y[JH_NS];
 #pragma unroll
   for (int k = 0; k < JH_NS - 1; ++k) {
-    Y[k] = (W)Up[(4 + k) * stride] / q.rho;
+    Y[k] = (W)Up[(4 + k) * stride] * ir;
     sum += Y[k];
   }
   Y[JH_NS - 1] = (W)1 - sum;
@@ -98,7 +99,7 @@ __device__ __forceinline__ Prim<W> cell_prim(const MechConst& c, const Real* __r
   const W T = (W)ap[stride];
   q.p = (q.rho * R * T);
   q.a = (W)0;
-  if (WITH_A) q.a = sqrt((W)ap[0] * q.p / q.rho);
+  if (WITH_A) q.a = sqrt((W)ap[0] * q.p * ir);
   return q;
 }
 
@@ -332,7 +333,7 @@ __device__ __forceinline__ W phys_flux(int cv, int dir, const Prim<W>& q, W ucv)
     case 1: return dir == 0 ? q.rho * (q.u * q.u) + q.p : (q.rho * q.u) * q.v;
     case 2: return dir == 0 ? (q.rho * q.u) * q.v : q.rho * (q.v * q.v) + q.p;
     case 3: return un * (ucv + q.p);
-    default: return (q.rho * un) * (ucv / q.rho);
+    default: return un * ucv;   // rho*u*Y with Y = rhoY/rho
   }
 }
 
@@ -366,6 +367,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(StepArgs<Real> a, int stage,
   const int x0 = (blockIdx.x / tiles_y) * TXR;
   const int y0 = (blockIdx.x % tiles_y) * TY;
   const double dt = step_dt(a);
+  const Real idx = (Real)(1.0 / a.dx), idy = (Real)(1.0 / a.dy);
   const Real ax = (Real)bitsd(a.sc->amax[a.parity][stage][0]);
   const Real ay = (Real)bitsd(a.sc->amax[a.parity][stage][1]);
 
@@ -474,7 +476,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(StepArgs<Real> a, int stage,
       const int ir = rrow[s] - 3;
       const Real dF = sFx[ir + 1][ly] - sFx[ir][ly];
       const Real dG = sFy[ir][ly + 1] - sFy[ir][ly];
-      const Real rhs = (Real)dt * (-(dF / (Real)a.dx + dG / (Real)a.dy));
+      const Real rhs = (Real)dt * (-(dF * idx + dG * idy));
       Real un;
       if (MODE == MODE_RHS) {
         un = rhs;
@@ -498,12 +500,12 @@ __global__ __launch_bounds__(NT) void stage_kernel(StepArgs<Real> a, int stage,
   for (int s = 0; s < 2; ++s) {
     if (!own[s]) continue;
     const Real* un = Uo + oidx[s];
-    const double rho = (double)un[0];
-    const double u = (double)un[n] / rho, v = (double)un[2 * n] / rho;
-    const double e = (double)un[3 * n] / rho - 0.5 * (u * u + v * v);
+    const double rho = (double)un[0], ir = rcp(rho);
+    const double u = (double)un[n] * ir, v = (double)un[2 * n] * ir;
+    const double e = (double)un[3 * n] * ir - 0.5 * (u * u + v * v);
     double Y[JH_NS];
 #pragma unroll
-    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = (double)un[(4 + k) * n] / rho;
+    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = (double)un[(4 + k) * n] * ir;
     const double R = fill_Y_get_R(c, Y);
     double T = (double)auxs[n + oidx[s]];
     double gamma;
@@ -514,7 +516,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(StepArgs<Real> a, int stage,
       // next stage's alpha from the stored (rounded) values
       const double gs = (double)(Real)gamma, Ts = (double)(Real)T;
       const double p = rho * R * Ts;
-      const double snd = sqrt(gs * p / rho);
+      const double snd = sqrt(gs * p * ir);
       mx = fmax(mx, nan_to_zero(fabs(u) + snd));
       my = fmax(my, nan_to_zero(fabs(v) + snd));
     }
@@ -547,7 +549,10 @@ __global__ __launch_bounds__(256) void chem_kernel(StepArgs<Real> a, Real* U, Re
   }
   double mx = 0.0, my = 0.0;
   unsigned long long cm = 0ull;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  // one cell per thread: a grid-stride loop would let the compiler keep every
+  // mechanism constant in registers across iterations
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
     double Uc[NU];
 #pragma unroll
     for (int v = 0; v < NU; ++v) Uc[v] = (double)U[v * n + i];
@@ -563,7 +568,7 @@ __global__ __launch_bounds__(256) void chem_kernel(StepArgs<Real> a, Real* U, Re
       for (int v = 0; v < 4; ++v) S[v * n + i] = (Real)0;
 #pragma unroll
       for (int k = 0; k < JH_NA; ++k) S[(4 + k) * n + i] = (Real)d[k];
-      continue;
+      return;
     }
     Real Un[NU];
 #pragma unroll
@@ -573,12 +578,12 @@ __global__ __launch_bounds__(256) void chem_kernel(StepArgs<Real> a, Real* U, Re
       Un[4 + k] = (Real)((double)U[(4 + k) * n + i] + d[k]);
       U[(4 + k) * n + i] = Un[4 + k];
     }
-    const double r2 = (double)Un[0];
-    const double u = (double)Un[1] / r2, v = (double)Un[2] / r2;
-    const double e = (double)Un[3] / r2 - 0.5 * (u * u + v * v);
+    const double r2 = (double)Un[0], ir = rcp(r2);
+    const double u = (double)Un[1] * ir, v = (double)Un[2] * ir;
+    const double e = (double)Un[3] * ir - 0.5 * (u * u + v * v);
     double Y[JH_NS];
 #pragma unroll
-    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = (double)Un[4 + k] / r2;
+    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = (double)Un[4 + k] * ir;
     const double R = fill_Y_get_R(c, Y);
     double gamma;
     halley_T(c, Y, R, e, T, gamma);
@@ -590,7 +595,7 @@ __global__ __launch_bounds__(256) void chem_kernel(StepArgs<Real> a, Real* U, Re
     // next step: CFL_2D and stage-1 LF maxima from the stored values
     const double gs = (double)(Real)gamma, Ts = (double)(Real)T;
     const double p = r2 * R * Ts;
-    const double snd = sqrt(gs * p / r2);
+    const double snd = sqrt(gs * p * ir);
     const double sx = fabs(u) + snd, sy = fabs(v) + snd;
     mx = fmax(mx, nan_to_zero(sx));
     my = fmax(my, nan_to_zero(sy));
@@ -612,13 +617,14 @@ __global__ __launch_bounds__(256) void update_aux_kernel(StepArgs<Real> a, const
                                                          Real* __restrict__ aux_out) {
   const MechConst& c = *a.mc;
   const int n = a.nx * a.ny;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double rho = (double)U[i];
-    const double u = (double)U[n + i] / rho, v = (double)U[2 * n + i] / rho;
-    const double e = (double)U[3 * n + i] / rho - 0.5 * (u * u + v * v);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double rho = (double)U[i], ir = rcp(rho);
+    const double u = (double)U[n + i] * ir, v = (double)U[2 * n + i] * ir;
+    const double e = (double)U[3 * n + i] * ir - 0.5 * (u * u + v * v);
     double Y[JH_NS];
 #pragma unroll
-    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = (double)U[(4 + k) * n + i] / rho;
+    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = (double)U[(4 + k) * n + i] * ir;
     const double R = fill_Y_get_R(c, Y);
     double T = (double)aux_in[n + i];
     double gamma;

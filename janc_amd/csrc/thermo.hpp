@@ -4,6 +4,7 @@
 #pragma once
 #include <math.h>
 
+#include "fastmath.hpp"
 #include "janc_types.hpp"
 
 namespace jh {
@@ -20,6 +21,9 @@ struct MixPoly {
   double h[6];
 };
 
+// The range index is a runtime (per-lane) value on purpose: with a
+// compile-time index the compiler hoists both ranges' ~200 constants into
+// registers wherever the mixture is evaluated more than once.
 __host__ __device__ __forceinline__ void mix_poly(const MechConst& c, int r,
                                                   const double (&Y)[JH_NS],
                                                   MixPoly& m) {
@@ -75,7 +79,7 @@ __host__ __device__ __forceinline__ double mix_gamma(const MechConst& c,
   double cp, dcp, h;
   mix_eval(m, T, cp, dcp, h);
   if (h_out) *h_out = h;
-  return cp / (cp - R);
+  return cp * rcp(cp - R);
 }
 
 // Halley iteration for T from e (get_T_nasa7, thermo_model.py:135-153).
@@ -93,9 +97,9 @@ __host__ __device__ __forceinline__ void halley_T(const MechConst& c,
   mix_eval(m, T, cp, dcp, h);
   double res = (h - R * T) - e;
   double d1 = cp - R;
-  gamma = cp / (cp - R);
+  gamma = cp * rcp(cp - R);
   for (int i = 0; i < kHalleyMaxIter && fabs(res) > kHalleyTol; ++i) {
-    const double dT = -2.0 * res * d1 / (2.0 * (d1 * d1) - res * dcp);
+    const double dT = -2.0 * res * d1 * rcp(2.0 * (d1 * d1) - res * dcp);
     T = T + dT;
     const int rn = T < c.Tcr ? 0 : 1;
     if (rn != r) {
@@ -105,7 +109,7 @@ __host__ __device__ __forceinline__ void halley_T(const MechConst& c,
     mix_eval(m, T, cp, dcp, h);
     res = (h - R * T) - e;
     d1 = cp - R;
-    gamma = cp / (cp - R);
+    gamma = cp * rcp(cp - R);
   }
 }
 

@@ -7,6 +7,7 @@
 #pragma once
 #include <math.h>
 
+#include "fastmath.hpp"
 #include "janc_types.hpp"
 
 namespace jh {
@@ -30,7 +31,7 @@ __host__ __device__ __forceinline__ double weno5(double fm2, double fm1, double 
   const double q1 = (1.0 / 3.0) * fm2 - (7.0 / 6.0) * fm1 + (11.0 / 6.0) * f0;
   const double q2 = -(1.0 / 6.0) * fm1 + (5.0 / 6.0) * f0 + (1.0 / 3.0) * fp1;
   const double q3 = (1.0 / 3.0) * f0 + (5.0 / 6.0) * fp1 - (1.0 / 6.0) * fp2;
-  return (p1 * q1 + p2 * q2 + p3 * q3) / (p1 + p2 + p3);
+  return (p1 * q1 + p2 * q2 + p3 * q3) * rcp(p1 + p2 + p3);
 }
 
 // FP32 form for the mixed-precision step.  In SI units the split energy flux
@@ -44,7 +45,7 @@ __host__ __device__ __forceinline__ double weno5(double fm2, double fm1, double 
 __host__ __device__ __forceinline__ float weno5(float fm2, float fm1, float f0, float fp1,
                                                 float fp2) {
   float s = fmaxf(fmaxf(fabsf(fm2), fabsf(fm1)), fmaxf(fmaxf(fabsf(f0), fabsf(fp1)), fabsf(fp2)));
-  const float is = 1.0f / fmaxf(s, 1e-20f);
+  const float is = rcp(fmaxf(s, 1e-20f));
   const float eps = fminf(fmaxf(1e-6f * is * is, 1e-15f), 1e15f);
   const float g0 = fm2 * is, g1 = fm1 * is, g2 = f0 * is, g3 = fp1 * is, g4 = fp2 * is;
   const float a1 = g0 - 4.0f * g1 + 3.0f * g2, b1 = g0 - 2.0f * g1 + g2;
@@ -54,13 +55,13 @@ __host__ __device__ __forceinline__ float weno5(float fm2, float fm1, float f0, 
   const float IS2 = 0.25f * (a2 * a2) + (13.0f / 12.0f) * (b2 * b2);
   const float IS3 = 0.25f * (a3 * a3) + (13.0f / 12.0f) * (b3 * b3);
   float d1 = eps + IS1, d2 = eps + IS2, d3 = eps + IS3;
-  const float al1 = 0.1f / (d1 * d1), al2 = 0.6f / (d2 * d2), al3 = 0.3f / (d3 * d3);
+  const float al1 = 0.1f * rcp(d1 * d1), al2 = 0.6f * rcp(d2 * d2), al3 = 0.3f * rcp(d3 * d3);
   const float q1 = (1.0f / 3.0f) * fm2 - (7.0f / 6.0f) * fm1 + (11.0f / 6.0f) * f0;
   const float q2 = -(1.0f / 6.0f) * fm1 + (5.0f / 6.0f) * f0 + (1.0f / 3.0f) * fp1;
   const float q3 = (1.0f / 3.0f) * f0 + (5.0f / 6.0f) * fp1 - (1.0f / 6.0f) * fp2;
   // alpha_k reaches 6e29 (eps' = 1e-15): normalise before multiplying by the
   // unscaled candidates, whose magnitude may be ~1e10
-  const float inv = 1.0f / (al1 + al2 + al3);
+  const float inv = rcp(al1 + al2 + al3);
   return (al1 * inv) * q1 + (al2 * inv) * q2 + (al3 * inv) * q3;
 }
 

@@ -242,11 +242,12 @@ def _generate(m):
     e("// X: unclipped rho*Y/M (all species); Xc: clip(X, 1e-50) for active species;")
     e("// invMX[n] = 1/(M_n*Xc_n); g: Gibbs s/R - h/(RT) of active species.")
     e("__host__ __device__ __forceinline__ void mech_rates_jacobian(")
-    e("    const MechConst& __restrict__ c, const double T, const double logT,")
+    e("    const MechConst& __restrict__ c, const double T, const double invT, const double logT,")
     e("    const double (&X)[JH_NS], const double (&Xc)[JH_NA], const double (&invMX)[JH_NA],")
     e("    const double (&g)[JH_NA], double (&w)[JH_NA], double (&J)[JH_NA][JH_NA])")
     e("{")
-    e("  const double Pf = c.Pref / T;   // 101325/P0/T")
+    e("  const double Pf = c.Pref * invT;   // 101325/P0/T")
+    e("  const double iPf = jh::rcp(Pf);")
     e("#pragma unroll")
     e("  for (int k = 0; k < JH_NA; ++k) { w[k] = 0.0;")
     e("#pragma unroll")
@@ -261,16 +262,16 @@ def _generate(m):
         elif r.b == 2.0:
             tb = "(T * T)"
         elif r.b == -1.0:
-            tb = "(1.0 / T)"
+            tb = "invT"
         elif r.b == -2.0:
-            tb = "(1.0 / (T * T))"
+            tb = "(invT * invT)"
         else:
             tb = "exp(c.B[%d] * logT)" % i
         kfi = "c.A[%d]" % i
         if tb:
             kfi = "(%s * %s)" % (kfi, tb)
         if r.Ea != 0.0:
-            kfi = "%s * exp(-c.EaR[%d] / T)" % (kfi, i)
+            kfi = "%s * exp(-c.EaR[%d] * invT)" % (kfi, i)
         e("    const double kfi = %s;" % kfi)
         pf = " * ".join(_power_expr(k, n) for k, n in sorted(r.reactants.items()))
         pb = " * ".join(_power_expr(k, n) for k, n in sorted(r.products.items()))
@@ -288,15 +289,16 @@ def _generate(m):
         if sdg.startswith("- "):
             sdg = "-" + sdg[2:]
         vs = sum(r.products.values()) - sum(r.reactants.values())
+        # kb = kf_i / (exp(sum dnu g) Pf^vsum) * prod X^nu_b (reaction_model.py:113)
         if vs == 0.0:
-            eq = "exp(%s)" % sdg
+            eq = "exp(-(%s))" % sdg
         elif vs == 1.0:
-            eq = "(exp(%s) * Pf)" % sdg
+            eq = "(exp(-(%s)) * iPf)" % sdg
         elif vs == -1.0:
-            eq = "(exp(%s) * (1.0 / Pf))" % sdg
+            eq = "(exp(-(%s)) * Pf)" % sdg
         else:
-            eq = "(exp(%s) * pow(Pf, %s))" % (sdg, _fmt(vs))
-        e("    const double kb = kfi / %s * (%s);" % (eq, pb))
+            eq = "(exp(-(%s)) * pow(Pf, %s))" % (sdg, _fmt(-vs))
+        e("    const double kb = kfi * %s * (%s);" % (eq, pb))
         if r.three_body:
             terms = " + ".join("c.eff[%d][%d] * X[%d]" % (i, k, k) for k in range(ns))
             e("    const double cM = %s;" % terms)
