@@ -153,6 +153,7 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the FP64 headline workload (for profiler runs)")
     args = ap.parse_args()
@@ -178,6 +179,7 @@ def main():
     sim = Simulator(cases.simulation_config(geom, cfl=0.3, precision="fp64", theta=theta))
     U = torch.as_tensor(U0).cuda()
     aux = torch.as_tensor(aux0).cuda()
+    tuned = sim.autotune(U, aux, theta=theta) if not args.no_tune else None
     el, kt, dts, flags = _run(sim, U, aux, args.steps, args.warmup, torch)
     cells = nx * ny
     value = cells * args.steps / el
@@ -208,6 +210,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["frac_hbm"],
                      "traffic": None},
         "kernels_fp64": {k: {"ms_total": v[0], "launches": v[1]} for k, v in kt.items()},
+        "autotune_fp64": tuned,
         "roofline_all_fp64": roof,
     }
     del U, aux
@@ -222,11 +225,12 @@ def main():
             s3 = Simulator(cases.simulation_config(g3, cfl=0.3, precision=prec))
             Ut = torch.as_tensor(U3, dtype=dtype).cuda()
             at = torch.as_tensor(aux3, dtype=dtype).cuda()
+            t3 = s3.autotune(Ut, at) if not args.no_tune else None
             e3, k3, d3, f3 = _run(s3, Ut, at, args.mixed_steps, args.warmup, torch)
             out[prec] = {"value": n3 * args.mixed_steps / e3, "ms_per_step": e3 / args.mixed_steps * 1e3,
                          "kernels": {k: {"ms_total": v[0], "launches": v[1]} for k, v in k3.items()},
                          "roofline": _roofline(k3, n3, 4 if prec == "mixed" else 8),
-                         "health_flags": f3}
+                         "health_flags": f3, "autotune": t3}
             del Ut, at
             torch.cuda.empty_cache()
         out["speedup_mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]

@@ -69,6 +69,14 @@ int jh_ctx_destroy(jh_ctx* ctx);
  * call argument in the reference). */
 int jh_ctx_set_bc_param(jh_ctx* ctx, int side, double value);
 int jh_ctx_set_stream(jh_ctx* ctx, void* stream);
+/* Stage-kernel tile variant: 0 = 64x8 tile / 256 threads / 3 waves per SIMD,
+ * 1 = 64x8 / 256 / 4 waves per SIMD (<= 128 VGPRs), 2 = 64x16 / 512 / 4 waves
+ * per SIMD.  No counterpart in the reference (XLA picks its own fusion); chosen
+ * by the tile autotuner (janc_amd/autotune.py). */
+int jh_ctx_set_tile(jh_ctx* ctx, int variant);
+int jh_ctx_get_tile(const jh_ctx* ctx);
+int jh_tile_variants(void);
+int jh_tile_info(int variant, int* rows, int* threads, int* waves);
 const char* jh_last_error(const jh_ctx* ctx);
 
 /* Advance nsteps steps.  One step is the reference's advance_func

@@ -28,6 +28,11 @@ SIGNATURES = {
     "jh_ctx_destroy": (_c_int, [_vp]),
     "jh_ctx_set_bc_param": (_c_int, [_vp, _c_int, _c_double]),
     "jh_ctx_set_stream": (_c_int, [_vp, _vp]),
+    "jh_ctx_set_tile": (_c_int, [_vp, _c_int]),
+    "jh_ctx_get_tile": (_c_int, [_vp]),
+    "jh_tile_variants": (_c_int, []),
+    "jh_tile_info": (_c_int, [_c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),
+                              ctypes.POINTER(_c_int)]),
     "jh_last_error": (ctypes.c_char_p, [_vp]),
     "jh_advance": (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_double, _c_double, _c_int, _vp]),
     "jh_rhs": (_c_int, [_vp, _vp, _vp, _c_double, _vp]),

@@ -26,6 +26,7 @@ struct jh_ctx {
   void* tmpU[2] = {nullptr, nullptr};
   void* tmpA[2] = {nullptr, nullptr};
   int parity = 0;
+  int tile = 0;                   // stage-kernel variant (kTileRows / kTileMinBlocks)
   std::string err;
   // kernel timing with HIP events on the context stream (jh_timing_*)
   bool timing = false;
@@ -115,9 +116,35 @@ int grid_cells(const jh_ctx* c) {
 // one thread per cell (pointwise kernels without a grid-stride loop)
 int grid_all(const jh_ctx* c) { return (c->nx * c->ny + 255) / 256; }
 
+int tile_rows(const jh_ctx* c);
+
 int stage_blocks(const jh_ctx* c) {
-  return ((c->nx + TXR - 1) / TXR) * ((c->ny + TY - 1) / TY);
+  const int tx = tile_rows(c);
+  return ((c->nx + tx - 1) / tx) * ((c->ny + TY - 1) / TY);
 }
+
+// Stage-kernel variants (jh_ctx_set_tile): rows along x, and the waves per
+// SIMD the register budget is sized for.
+constexpr int kTileRows[3] = {8, 8, 16};
+constexpr int kTileWaves[3] = {3, 4, 4};
+
+template <typename Real, int MODE>
+void launch_stage(jh_ctx* c, const StepArgs<Real>& a, int stage, const Real* Us, const Real* As,
+                  const Real* Un, Real* Uo, Real* Ao) {
+  const int nb = stage_blocks(c);
+  switch (c->tile) {
+    case 1:
+      stage_kernel<Real, MODE, 8, 4><<<nb, Tile<8>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+      break;
+    case 2:
+      stage_kernel<Real, MODE, 16, 4><<<nb, Tile<16>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+      break;
+    default:
+      stage_kernel<Real, MODE, 8, 3><<<nb, Tile<8>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+  }
+}
+
+int tile_rows(const jh_ctx* c) { return kTileRows[c->tile]; }
 
 int bc_blocks(const jh_ctx* c) { return (2 * c->nx + 2 * c->ny + 255) / 256; }
 
@@ -146,20 +173,20 @@ int advance_impl(jh_ctx* c, const Real* U, const Real* aux, Real* Uo, Real* auxo
   const Real* An = aux;
   for (int s = 0; s < nsteps; ++s) {
     auto a = make_args<Real>(c, dt, cfl);
-    const int gb = stage_blocks(c), bb = bc_blocks(c);
+    const int bb = bc_blocks(c);
     // stage 1: (Un, An) -> (U1, A1)
     { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, 256, 0, c->stream>>>(a, Un, An, 0); }
     { Timed t(c, JH_KERNEL_STAGE);
-      stage_kernel<Real, MODE_RK><<<gb, NT, 0, c->stream>>>(a, 0, Un, An, Un, U1, A1); }
+      launch_stage<Real, MODE_RK>(c, a, 0, Un, An, Un, U1, A1); }
     // stage 2: (U1, A1) -> (U2, A2)
     { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, 256, 0, c->stream>>>(a, U1, A1, 1); }
     { Timed t(c, JH_KERNEL_STAGE);
-      stage_kernel<Real, MODE_RK><<<gb, NT, 0, c->stream>>>(a, 1, U1, A1, Un, U2, A2); }
+      launch_stage<Real, MODE_RK>(c, a, 1, U1, A1, Un, U2, A2); }
     // stage 3: (U2, A2) -> (Uo, auxo); Uo may alias Un (each thread reads
     // Un at its own cell before writing it)
     { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, 256, 0, c->stream>>>(a, U2, A2, 2); }
     { Timed t(c, JH_KERNEL_STAGE);
-      stage_kernel<Real, MODE_RK><<<gb, NT, 0, c->stream>>>(a, 2, U2, A2, Un, Uo, auxo); }
+      launch_stage<Real, MODE_RK>(c, a, 2, U2, A2, Un, Uo, auxo); }
     // chemistry + final update_aux in place, next step's maxima
     { Timed t(c, JH_KERNEL_CHEM);
       chem_kernel<Real, CHEM_STEP><<<grid_all(c), 256, 0, c->stream>>>(
@@ -180,8 +207,7 @@ int rhs_impl(jh_ctx* c, const Real* U, const Real* aux, double dt, Real* out) {
   if (rc) return rc;
   auto a = make_args<Real>(c, dt, 0.0);
   bc_kernel<Real><<<bc_blocks(c), 256, 0, c->stream>>>(a, U, aux, 0);
-  stage_kernel<Real, MODE_RHS><<<stage_blocks(c), NT, 0, c->stream>>>(a, 0, U, aux, U, out,
-                                                                        nullptr);
+  launch_stage<Real, MODE_RHS>(c, a, 0, U, aux, U, out, nullptr);
   return check_launch(c, "rhs kernels");
 }
 
@@ -282,6 +308,25 @@ int jh_ctx_set_bc_param(jh_ctx* c, int side, double value) {
   c->bc_p[side] = value;
   return JH_OK;
 }
+
+int jh_ctx_set_tile(jh_ctx* c, int variant) {
+  if (!c) return JH_ERR_ARG;
+  if (variant < 0 || variant >= jh_tile_variants()) return fail(c, JH_ERR_ARG, "unknown tile variant");
+  c->tile = variant;
+  return JH_OK;
+}
+
+int jh_tile_variants(void) { return 3; }
+
+int jh_tile_info(int variant, int* rows, int* threads, int* waves) {
+  if (variant < 0 || variant >= 3 || !rows || !threads || !waves) return JH_ERR_ARG;
+  *rows = kTileRows[variant];
+  *threads = 32 * kTileRows[variant];
+  *waves = kTileWaves[variant];
+  return JH_OK;
+}
+
+int jh_ctx_get_tile(const jh_ctx* c) { return c ? c->tile : -1; }
 
 int jh_ctx_set_stream(jh_ctx* c, void* stream) {
   if (!c) return JH_ERR_ARG;

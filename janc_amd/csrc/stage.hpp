@@ -268,14 +268,31 @@ __global__ __launch_bounds__(256) void bc_kernel(StepArgs<Real> a, const Real* _
 // ------------------------------------------------------------ stage kernel
 
 constexpr int TY = 64;            // tile width along y (one wave)
-constexpr int TXR = 8;            // tile height along x
-constexpr int RX = TXR + 6;       // x-extended region rows
-constexpr int RY = TY + 6;        // y-extended region columns
-constexpr int NT = 256;           // threads per block
-constexpr int NXF = (TXR + 1) * TY;       // x faces per tile (576)
-constexpr int NYF = TXR * (TY + 1);       // y faces per tile (520)
+
+// Tile geometry for TX rows along x (TX = 8 or 16; 32*TX threads).  The
+// kernel's WPE template argument is the occupancy (waves per SIMD) the
+// register allocation must allow: at most 512/WPE VGPRs per lane.
+template <int TX>
+struct Tile {
+  static constexpr int NT = 32 * TX;             // threads per block
+  static constexpr int NW = NT / 64;             // waves per block
+  static constexpr int RX = TX + 6;              // x-extended region rows
+  static constexpr int RY = TY + 6;              // y-extended region columns
+  static constexpr int NXF = (TX + 1) * TY;      // x faces per tile
+  static constexpr int NYF = TX * (TY + 1);      // y faces per tile
+  static constexpr int NHS = (6 + NW - 1) / NW;  // x-halo rows per wave
+  static constexpr int NSL = 2 + NHS + 1;        // region cells per thread
+};
 
 enum StageMode : int { MODE_RK = 0, MODE_RHS = 1 };
+
+// Blocks go round-robin to the 8 XCDs; give each XCD a contiguous run of
+// tiles (y fastest, then x) so a tile's x-neighbours, whose rows it re-reads
+// as halo, are usually served by the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int xcd = b & 7, k = b >> 3, q = nb >> 3, r = nb & 7;
+  return xcd * q + (xcd < r ? xcd : r) + k;
+}
 
 // Address of one halo/interior cell: U base, aux base and plane stride.
 template <typename Real>
@@ -337,7 +354,7 @@ __device__ __forceinline__ W phys_flux(int cv, int dir, const Prim<W>& q, W ucv)
   }
 }
 
-// One SSP-RK3 stage (TVD_RK3.py:5-12) for a 64 x 8 tile:
+// One SSP-RK3 stage (TVD_RK3.py:5-12) for a 64 x TX tile:
 //   rhs = dt * -(dF/dx + dG/dy)   (set_simulator.py:44-47, flux.py:56-68)
 //   stage 0: U1 = U + rhs; stage 1: U2 = 3/4 U + 1/4 (U1 + rhs);
 //   stage 2: U3 = 1/3 U + 2/3 (U2 + rhs)
@@ -346,68 +363,81 @@ __device__ __forceinline__ W phys_flux(int cv, int dir, const Prim<W>& q, W ucv)
 //
 // Work split per conserved variable: phase 1 evaluates the split fluxes
 // H+- = (F +- alpha U)/2 of the tile + halo into LDS, phase 2 reconstructs the
-// 576 x-faces and 520 y-faces (L + R WENO each), phase 3 takes the divergence
-// and the RK combine for the thread's two cells.
-template <typename Real, int MODE>
-__global__ __launch_bounds__(NT) void stage_kernel(StepArgs<Real> a, int stage,
-                                                   const Real* __restrict__ Us,
-                                                   const Real* __restrict__ auxs,
-                                                   const Real* Un, Real* Uo,
-                                                   Real* __restrict__ auxo) {
+// (TX+1)*64 x-faces and TX*65 y-faces (L + R WENO each), phase 3 takes the
+// divergence and the RK combine for the thread's two cells.  The next
+// variable's loads are issued before phase 2 so their latency is hidden.
+//
+// Region cells of thread t (wave w, lane ly): slots 0,1 = interior rows w and
+// w+NW; slots 2..2+NHS-1 = x-halo rows {0,1,2,TX+3,TX+4,TX+5}[w + h*NW];
+// last slot = y-halo cell t (t < 6*TX).
+template <typename Real, int MODE, int TX, int WPE>
+__global__ __launch_bounds__(Tile<TX>::NT) __attribute__((amdgpu_waves_per_eu(WPE)))
+void stage_kernel(StepArgs<Real> a, int stage,
+                                                             const Real* __restrict__ Us,
+                                                             const Real* __restrict__ auxs,
+                                                             const Real* Un, Real* Uo,
+                                                             Real* __restrict__ auxo) {
+  using G = Tile<TX>;
+  constexpr int NT = G::NT, NW = G::NW, RX = G::RX, RY = G::RY;
+  constexpr int NXF = G::NXF, NYF = G::NYF, NHS = G::NHS, NSL = G::NSL;
   __shared__ Real sHxp[RX][TY], sHxm[RX][TY];
-  __shared__ Real sHyp[TXR][RY], sHym[TXR][RY];
-  __shared__ Real sFx[TXR + 1][TY];
-  __shared__ Real sFy[TXR][TY + 1];
-  __shared__ double red[4];
+  __shared__ Real sHyp[TX][RY], sHym[TX][RY];
+  __shared__ Real sFx[TX + 1][TY];
+  __shared__ Real sFy[TX][TY + 1];
+  __shared__ double red[NW];
 
   const MechConst& c = *a.mc;
   const int nx = a.nx, ny = a.ny, n = nx * ny;
   const int t = threadIdx.x, ly = t & 63, w = t >> 6;
   const int tiles_y = (ny + TY - 1) / TY;
-  const int x0 = (blockIdx.x / tiles_y) * TXR;
-  const int y0 = (blockIdx.x % tiles_y) * TY;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int x0 = (tile / tiles_y) * TX;
+  const int y0 = (tile % tiles_y) * TY;
   const double dt = step_dt(a);
   const Real idx = (Real)(1.0 / a.dx), idy = (Real)(1.0 / a.dy);
   const Real ax = (Real)bitsd(a.sc->amax[a.parity][stage][0]);
   const Real ay = (Real)bitsd(a.sc->amax[a.parity][stage][1]);
 
-  // ---- region cells owned by this thread (see header comment) ----
-  // slots 0,1: interior rows w, w+4; slot 2,3: x-halo rows; slot 4: y-halo
-  constexpr int NS_ = 5;
-  int rrow[NS_], rcol[NS_];       // LDS coordinates: x-region row / y-region col
-  int gxs[NS_], gys[NS_];
-  bool has[NS_];
-  const int hr2[4] = {0, 1, 2, 11};
+  int rrow[NSL], rcol[NSL];       // LDS coordinates: x-region row / y-region col
+  int gxs[NSL], gys[NSL];
+  bool has[NSL];
   rrow[0] = w + 3;
-  rrow[1] = w + 7;
-  rrow[2] = hr2[w];
-  rrow[3] = (w < 2) ? 12 + w : -1;
+  rrow[1] = w + NW + 3;
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
+  for (int h = 0; h < NHS; ++h) {
+    const int hi = w + h * NW;
+    rrow[2 + h] = hi < 6 ? (hi < 3 ? hi : TX + hi) : -1;
+  }
+#pragma unroll
+  for (int s = 0; s < NSL - 1; ++s) {
     has[s] = rrow[s] >= 0;
     gxs[s] = x0 - 3 + rrow[s];
     gys[s] = y0 + ly;
     rcol[s] = ly + 3;
   }
-  has[4] = t < 48;
   {
+    constexpr int s = NSL - 1;
+    has[s] = t < 6 * TX;
     const int xr = t / 6, pos = t % 6;
     const int col = pos < 3 ? pos : TY + pos;   // 0,1,2 or 67,68,69
-    rrow[4] = xr;                               // y-region row (interior x)
-    rcol[4] = col;
-    gxs[4] = x0 + xr;
-    gys[4] = y0 - 3 + col;
+    rrow[s] = xr;                               // y-region row (interior x)
+    rcol[s] = col;
+    gxs[s] = x0 + xr;
+    gys[s] = y0 - 3 + col;
   }
-  CellRef<Real> ref[NS_];
-  Prim<Real> q[NS_];
+  CellRef<Real> ref[NSL];
+  Prim<Real> q[NSL];
+  Real nxt[NSL];
 #pragma unroll
-  for (int s = 0; s < NS_; ++s) {
+  for (int s = 0; s < NSL; ++s) {
     ref[s] = locate<Real>(a, Us, auxs, gxs[s], gys[s]);
     if (!has[s]) ref[s].ok = false;
     if (ref[s].ok) {
       q[s] = cell_prim<Real, false>(c, ref[s].u, ref[s].x, ref[s].stride);
+      nxt[s] = ref[s].u[0];
     } else {
       q[s].rho = (Real)1; q[s].u = (Real)0; q[s].v = (Real)0; q[s].p = (Real)0; q[s].a = (Real)0;
+      nxt[s] = (Real)0;
     }
   }
   bool own[2];
@@ -422,11 +452,11 @@ __global__ __launch_bounds__(NT) void stage_kernel(StepArgs<Real> a, int stage,
     // ---- phase 1: split fluxes into LDS ----
     Real uc_own[2];
 #pragma unroll
-    for (int s = 0; s < NS_; ++s) {
+    for (int s = 0; s < NSL; ++s) {
       if (!has[s]) continue;
-      const Real ucv = ref[s].ok ? ref[s].u[cv * ref[s].stride] : (Real)0;
+      const Real ucv = nxt[s];
       if (s < 2) uc_own[s] = ucv;
-      const bool xreg = s < 4, yreg = (s < 2) || (s == 4);
+      const bool xreg = s < NSL - 1, yreg = (s < 2) || (s == NSL - 1);
       if (xreg) {
         Real hp = (Real)0, hm = (Real)0;
         if (ref[s].ok) {
@@ -440,9 +470,9 @@ __global__ __launch_bounds__(NT) void stage_kernel(StepArgs<Real> a, int stage,
       if (yreg) {
         Real hp = (Real)0, hm = (Real)0;
         if (ref[s].ok) {
-          const Real G = phys_flux<Real>(cv, 1, q[s], ucv);
-          hp = (Real)0.5 * (G + ay * ucv);
-          hm = (Real)0.5 * (G - ay * ucv);
+          const Real Gf = phys_flux<Real>(cv, 1, q[s], ucv);
+          hp = (Real)0.5 * (Gf + ay * ucv);
+          hm = (Real)0.5 * (Gf - ay * ucv);
         }
         const int yr = s < 2 ? rrow[s] - 3 : rrow[s];
         sHyp[yr][rcol[s]] = hp;
@@ -450,6 +480,12 @@ __global__ __launch_bounds__(NT) void stage_kernel(StepArgs<Real> a, int stage,
       }
     }
     __syncthreads();
+    // prefetch the next variable (consumed after two barriers)
+    if (cv + 1 < NU) {
+#pragma unroll
+      for (int s = 0; s < NSL; ++s)
+        if (ref[s].ok) nxt[s] = ref[s].u[(cv + 1) * ref[s].stride];
+    }
     // ---- phase 2: face reconstructions ----
     for (int it = t; it < NXF + NYF; it += NT) {
       if (it < NXF) {

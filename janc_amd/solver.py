@@ -297,6 +297,20 @@ class Simulator:
         ctx.call("jh_lf_alpha", _ptr(U), _ptr(aux), _ptr(out))
         return out
 
+    def set_tile(self, U, variant, theta=None):
+        """Select the stage-kernel variant (jh_ctx_set_tile) for U's grid."""
+        ctx = self._context(U, theta)
+        _lib.check(ctx.lib.jh_ctx_set_tile(ctx.h, int(variant)), ctx.h)
+
+    def get_tile(self, U, theta=None):
+        ctx = self._context(U, theta)
+        return int(ctx.lib.jh_ctx_get_tile(ctx.h))
+
+    def autotune(self, U, aux, steps=4, theta=None, force=False):
+        """Time every stage-kernel variant on a copy of (U, aux); keep the fastest."""
+        from .autotune import tune
+        return tune(self, U, aux, steps=steps, theta=theta, force=force)
+
     def timing(self, U, enable=None):
         """Enable/disable per-launch HIP-event timing, or read (and reset)
         {kind: (total_ms, launches)} when ``enable`` is None."""
