@@ -154,6 +154,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="one 256 MiB device-to-device copy before the timed region "
+                         "(reference dispatch for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the FP64 headline workload (for profiler runs)")
     args = ap.parse_args()
@@ -180,6 +183,12 @@ def main():
     U = torch.as_tensor(U0).cuda()
     aux = torch.as_tensor(aux0).cuda()
     tuned = sim.autotune(U, aux, theta=theta) if not args.no_tune else None
+    if args.calibrate_copy:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     el, kt, dts, flags = _run(sim, U, aux, args.steps, args.warmup, torch)
     cells = nx * ny
     value = cells * args.steps / el

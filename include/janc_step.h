@@ -108,6 +108,18 @@ int jh_cfl_dt(jh_ctx* ctx, const void* U, const void* aux, double cfl, double* d
  * alpha_dev[0] = nanmax(|u|+a), alpha_dev[1] = nanmax(|v|+a) (device). */
 int jh_lf_alpha(jh_ctx* ctx, const void* U, const void* aux, double* alpha_dev);
 
+/* Vector-Jacobian product of one fixed-dt step (the reference's reverse-mode
+ * differentiation of advance_one_step, set_simulator.py:117-122, with
+ * get_T_nasa7's custom VJP, model/thermo_model.py:155-190).  Given the state
+ * (U, aux) the step starts from and cotangents (U_bar, aux_bar) of its
+ * output, writes the cotangents of U and aux (U_in_bar, aux_in_bar); the
+ * step is recomputed, and its output is also written to U_out / aux_out when
+ * they are non-NULL.  FP64 contexts with periodic / zero_gradient /
+ * slip_wall / no_slip_wall boundaries only. */
+int jh_step_vjp(jh_ctx* ctx, const void* U, const void* aux, double dt, const void* U_bar,
+                const void* aux_bar, void* U_in_bar, void* aux_in_bar, void* U_out,
+                void* aux_out);
+
 /* Device pointer and element count of one side's ghost strip (14 planes of
  * 3 x len: vars 0-11 = U, 12-13 = aux; left/right strips are (14, 3, ny) with
  * layer 0 outermost on the left and innermost on the right; bottom/top are

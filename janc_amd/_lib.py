@@ -40,6 +40,7 @@ SIGNATURES = {
     "jh_reaction_source": (_c_int, [_vp, _vp, _vp, _c_double, _vp]),
     "jh_cfl_dt": (_c_int, [_vp, _vp, _vp, _c_double, _vp]),
     "jh_lf_alpha": (_c_int, [_vp, _vp, _vp, _vp]),
+    "jh_step_vjp": (_c_int, [_vp, _vp, _vp, _c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jh_ghost_strip": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_int)]),
     "jh_health": (_c_int, [_vp, ctypes.POINTER(ctypes.c_uint)]),
     "jh_timing_enable": (_c_int, [_vp, _c_int]),

@@ -45,18 +45,26 @@ __host__ __device__ __forceinline__ void lu_solve(double (&A)[N][N], double (&b)
         p = r;
       }
     }
+    // the predicated swap is skipped when no lane of the wave pivots
+#if defined(__HIP_DEVICE_COMPILE__)
+    const bool any_swap = __any(p != col);
+#else
+    const bool any_swap = p != col;
+#endif
+    if (any_swap) {
 #pragma unroll
-    for (int r = col + 1; r < N; ++r) {
-      const bool s = (p == r);
+      for (int r = col + 1; r < N; ++r) {
+        const bool s = (p == r);
 #pragma unroll
-      for (int j = col; j < N; ++j) {
-        const double t = A[col][j];
-        A[col][j] = s ? A[r][j] : t;
-        A[r][j] = s ? t : A[r][j];
+        for (int j = col; j < N; ++j) {
+          const double t = A[col][j];
+          A[col][j] = s ? A[r][j] : t;
+          A[r][j] = s ? t : A[r][j];
+        }
+        const double t = b[col];
+        b[col] = s ? b[r] : t;
+        b[r] = s ? t : b[r];
       }
-      const double t = b[col];
-      b[col] = s ? b[r] : t;
-      b[r] = s ? t : b[r];
     }
     const double inv = rcp(A[col][col]);
 #pragma unroll
@@ -102,8 +110,14 @@ __host__ __device__ __forceinline__ void chem_increment(const MechConst& c, doub
   const double invT = rcp(T);
   double g[JH_NA];
   gibbs(c, T, invT, g);
+  double eg[JH_NA], ieg[JH_NA];
+#pragma unroll
+  for (int k = 0; k < JH_NA; ++k) {
+    eg[k] = exp(g[k]);
+    ieg[k] = rcp(eg[k]);
+  }
   double w[JH_NA], J[JH_NA][JH_NA];
-  mech_rates_jacobian(c, T, invT, log(T), X, Xc, invMX, g, w, J);
+  mech_rates_jacobian(c, T, invT, log(T), X, Xc, invMX, eg, ieg, w, J);
   const double idt = rcp(dt);
 #pragma unroll
   for (int k = 0; k < JH_NA; ++k) {
@@ -116,6 +130,218 @@ __host__ __device__ __forceinline__ void chem_increment(const MechConst& c, doub
     double dY = w[k] * irho;
     dY = fmin(fmax(dY, -Y[k]), 1.0 - Y[k]);
     drhoY[k] = rho * dY;
+  }
+}
+
+}  // namespace jh
+
+namespace jh {
+
+// ---------------------------------------------------------------- adjoint
+// LU with partial pivoting kept for reuse (LAPACK getrf layout: unit-lower
+// multipliers below the diagonal, U on and above it, piv[col] = pivot row).
+template <int N>
+__host__ __device__ __forceinline__ void lu_factor(double (&A)[N][N], int (&piv)[N]) {
+#pragma unroll
+  for (int col = 0; col < N; ++col) {
+    int p = col;
+    double amax = fabs(A[col][col]);
+#pragma unroll
+    for (int r = col + 1; r < N; ++r) {
+      const double v = fabs(A[r][col]);
+      if (v > amax) {
+        amax = v;
+        p = r;
+      }
+    }
+    piv[col] = p;
+#pragma unroll
+    for (int r = col + 1; r < N; ++r) {
+      const bool s = (p == r);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const double t = A[col][j];
+        A[col][j] = s ? A[r][j] : t;
+        A[r][j] = s ? t : A[r][j];
+      }
+    }
+    const double inv = rcp(A[col][col]);
+#pragma unroll
+    for (int r = col + 1; r < N; ++r) {
+      const double l = A[r][col] * inv;
+      A[r][col] = l;
+#pragma unroll
+      for (int j = col + 1; j < N; ++j) A[r][j] -= l * A[col][j];
+    }
+  }
+}
+
+// Swap b[col] <-> b[piv[col]] with compile-time indices.
+template <int N>
+__host__ __device__ __forceinline__ void swap_rows(double (&b)[N], int col, int p) {
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    if (r <= col) continue;
+    const bool s = (p == r);
+    const double t = b[col];
+    b[col] = s ? b[r] : t;
+    b[r] = s ? t : b[r];
+  }
+}
+
+// A x = b with the factors of lu_factor (b <- x).
+template <int N>
+__host__ __device__ __forceinline__ void lu_apply(const double (&A)[N][N], const int (&piv)[N],
+                                                  double (&b)[N]) {
+#pragma unroll
+  for (int col = 0; col < N; ++col) {
+    swap_rows<N>(b, col, piv[col]);
+#pragma unroll
+    for (int r = col + 1; r < N; ++r) b[r] -= A[r][col] * b[col];
+  }
+#pragma unroll
+  for (int r = N - 1; r >= 0; --r) {
+    double s = b[r];
+#pragma unroll
+    for (int j = r + 1; j < N; ++j) s -= A[r][j] * b[j];
+    b[r] = s * rcp(A[r][r]);
+  }
+}
+
+// A^T y = b with the factors of lu_factor (b <- y):  U^T z = b, L^T w = z,
+// y = P^T w (the row swaps undone in reverse order).
+template <int N>
+__host__ __device__ __forceinline__ void lu_apply_T(const double (&A)[N][N], const int (&piv)[N],
+                                                    double (&b)[N]) {
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    double s = b[r];
+#pragma unroll
+    for (int j = 0; j < r; ++j) s -= A[j][r] * b[j];
+    b[r] = s * rcp(A[r][r]);
+  }
+#pragma unroll
+  for (int r = N - 1; r >= 0; --r) {
+    double s = b[r];
+#pragma unroll
+    for (int j = r + 1; j < N; ++j) s -= A[j][r] * b[j];
+    b[r] = s;
+  }
+#pragma unroll
+  for (int col = N - 1; col >= 0; --col) swap_rows<N>(b, col, piv[col]);
+}
+
+// Gradient of one branch of the reference's jnp.clip = min(max(v, lo), hi)
+// with JAX's tie rule (a tie splits the cotangent evenly).
+__host__ __device__ __forceinline__ void clip_vjp(double v, double lo, double hi, double gout,
+                                                  double& gv, double& glo, double& ghi) {
+  const double m = fmax(v, lo);
+  const double gm = m < hi ? gout : (m == hi ? 0.5 * gout : 0.0);
+  ghi = hi < m ? gout : (m == hi ? 0.5 * gout : 0.0);
+  gv = v > lo ? gm : (v == lo ? 0.5 * gm : 0.0);
+  glo = lo > v ? gm : (v == lo ? 0.5 * gm : 0.0);
+}
+
+// Reverse of chem_increment: cotangent Sb of drhoY -> cotangents of rho,
+// rho*Y_k (the state's species densities) and T.  The forward quantities are
+// recomputed; the solve is differentiated as lambda = A^-T x_bar,
+// w_bar = lambda, J_bar = lambda x^T (model/reaction_model.py:147-193 under
+// JAX's reverse mode).
+__host__ __device__ __forceinline__ void chem_increment_vjp(
+    const MechConst& c, double rho, const double (&rhoY_in)[JH_NS - 1], double T, double dt,
+    const double (&Sb)[JH_NA], double& rhob, double (&rYb)[JH_NS - 1], double& Tb) {
+  const double irho = rcp(rho);
+  double Y[JH_NS];
+#pragma unroll
+  for (int k = 0; k < JH_NS - 1; ++k) Y[k] = rhoY_in[k] * irho;
+  double sum = 0.0;
+#pragma unroll
+  for (int k = 0; k < JH_NS - 1; ++k) sum += Y[k];
+  Y[JH_NS - 1] = 1.0 - sum;
+  double X[JH_NS];
+#pragma unroll
+  for (int k = 0; k < JH_NS; ++k) X[k] = (rho * Y[k]) * c.R[k];
+  double Xc[JH_NA], invMX[JH_NA];
+#pragma unroll
+  for (int k = 0; k < JH_NA; ++k) {
+    Xc[k] = fmax(X[k], 1e-50);
+    invMX[k] = rcp(c.M[k] * Xc[k]);
+  }
+  const double invT = rcp(T), logT = log(T);
+  const int rr = T < c.Tcr ? 0 : 1;
+  const double lT = log(c.T0 * T);
+  double g[JH_NA], gp[JH_NA];
+#pragma unroll
+  for (int k = 0; k < JH_NA; ++k) {
+    double h = c.hc[rr][k][0];
+#pragma unroll
+    for (int j = 1; j < 6; ++j) h = h * T + c.hc[rr][k][j];
+    double s = c.s[rr][k][0];
+#pragma unroll
+    for (int j = 1; j < 5; ++j) s = s * T + c.s[rr][k][j];
+    s = s + c.logc[rr][k] * lT;
+    g[k] = s - h * invT;
+    gp[k] = h * invT * invT;
+  }
+  double eg[JH_NA], ieg[JH_NA];
+#pragma unroll
+  for (int k = 0; k < JH_NA; ++k) {
+    eg[k] = exp(g[k]);
+    ieg[k] = rcp(eg[k]);
+  }
+  double w[JH_NA], J[JH_NA][JH_NA];
+  mech_rates_jacobian(c, T, invT, logT, X, Xc, invMX, eg, ieg, w, J);
+  const double idt = rcp(dt);
+#pragma unroll
+  for (int k = 0; k < JH_NA; ++k) {
+#pragma unroll
+    for (int n = 0; n < JH_NA; ++n) J[k][n] = ((k == n) ? idt : 0.0) - J[k][n];
+  }
+  int piv[JH_NA];
+  lu_factor<JH_NA>(J, piv);
+  lu_apply<JH_NA>(J, piv, w);          // w <- x
+  // reverse: S = rho * clip(x/rho, -Y, 1-Y)
+  double Yb[JH_NS], xb[JH_NA];
+#pragma unroll
+  for (int k = 0; k < JH_NS; ++k) Yb[k] = 0.0;
+  rhob = 0.0;
+  Tb = 0.0;
+#pragma unroll
+  for (int k = 0; k < JH_NA; ++k) {
+    const double v = w[k] * irho, lo = -Y[k], hi = 1.0 - Y[k];
+    const double out = fmin(fmax(v, lo), hi);
+    rhob += Sb[k] * out;
+    double gv, glo, ghi;
+    clip_vjp(v, lo, hi, Sb[k] * rho, gv, glo, ghi);
+    Yb[k] -= glo + ghi;
+    xb[k] = gv * irho;
+    rhob -= gv * w[k] * irho * irho;
+  }
+  lu_apply_T<JH_NA>(J, piv, xb);       // xb <- lambda
+  double Xb[JH_NS], Xcb[JH_NA], iMXb[JH_NA];
+#pragma unroll
+  for (int k = 0; k < JH_NS; ++k) Xb[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < JH_NA; ++k) {
+    Xcb[k] = 0.0;
+    iMXb[k] = 0.0;
+  }
+  mech_rates_vjp(c, T, invT, logT, X, Xc, invMX, eg, ieg, gp, xb, w, Xb, Xcb, iMXb, Tb);
+#pragma unroll
+  for (int n = 0; n < JH_NA; ++n) {
+    Xcb[n] -= iMXb[n] * invMX[n] * invMX[n] * c.M[n];
+    Xb[n] += X[n] > 1e-50 ? Xcb[n] : (X[n] == 1e-50 ? 0.5 * Xcb[n] : 0.0);
+  }
+#pragma unroll
+  for (int k = 0; k < JH_NS; ++k) {
+    Yb[k] += Xb[k] * rho * c.R[k];
+    rhob += Xb[k] * Y[k] * c.R[k];
+  }
+#pragma unroll
+  for (int k = 0; k < JH_NS - 1; ++k) {
+    const double yk = Yb[k] - Yb[JH_NS - 1];
+    rYb[k] = yk * irho;
+    rhob -= yk * Y[k] * irho;
   }
 }
 

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/janc_step.h"
+#include "adjoint.hpp"
 #include "stage.hpp"
 
 using namespace jh;
@@ -33,6 +34,9 @@ struct jh_ctx {
   std::vector<hipEvent_t> ev_free;
   struct Mark { int kind; hipEvent_t a, b; };
   std::vector<Mark> marks;
+  // adjoint workspace (allocated on the first jh_step_vjp call)
+  double* adj = nullptr;
+  size_t adj_len = 0;
 };
 
 namespace {
@@ -211,6 +215,156 @@ int rhs_impl(jh_ctx* c, const Real* U, const Real* aux, double dt, Real* out) {
   return check_launch(c, "rhs kernels");
 }
 
+// ------------------------------------------------------------------ adjoint
+
+__global__ void fill_kernel(int m, double val, double* __restrict__ y) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m) y[t] = val;
+}
+
+__global__ void zero_u32_kernel(unsigned int* p, int m) {
+  if ((int)threadIdx.x < m) p[threadIdx.x] = 0u;
+}
+
+int blocks_for(size_t m) { return (int)((m + 255) / 256); }
+
+struct AdjWork {
+  double *U[5], *A[5];          // forward states of the step: 0 = input, 1-3 RK, 4 = output
+  double *Ub, *Uacc, *ab, *rb;  // cotangents
+  double *Up, *ap, *Ubp, *abp;  // padded
+  double *cx, *cy, *part, *scal;
+  unsigned int* cnt;
+  int npad, nblk;
+};
+
+int adj_work(jh_ctx* c, const double* U0, const double* A0, AdjWork& w) {
+  const size_t n = (size_t)c->nx * c->ny;
+  const size_t np = (size_t)(c->nx + 6) * (c->ny + 6);
+  const size_t nfx = (size_t)(c->nx + 1) * c->ny, nfy = (size_t)c->nx * (c->ny + 1);
+  const int nblk = blocks_for(np);
+  const size_t len = 4 * (NU + 2) * n + (3 * NU + 2) * n + 2 * (NU + 2) * np +
+                     (size_t)NU * 10 * (nfx + nfy) + 2 * nblk + 2 + 2;
+  if (c->adj_len < len) {
+    if (c->adj) (void)hipFree(c->adj);
+    c->adj = nullptr;
+    c->adj_len = 0;
+    HIP_TRY(c, hipMalloc(&c->adj, len * sizeof(double)));
+    c->adj_len = len;
+  }
+  double* p = c->adj;
+  w.U[0] = const_cast<double*>(U0);
+  w.A[0] = const_cast<double*>(A0);
+  for (int i = 1; i <= 4; ++i) {
+    w.U[i] = p; p += NU * n;
+    w.A[i] = p; p += 2 * n;
+  }
+  w.Ub = p; p += NU * n;
+  w.Uacc = p; p += NU * n;
+  w.rb = p; p += NU * n;
+  w.ab = p; p += 2 * n;
+  w.Up = p; p += NU * np;
+  w.ap = p; p += 2 * np;
+  w.Ubp = p; p += NU * np;
+  w.abp = p; p += 2 * np;
+  w.cx = p; p += NU * 10 * nfx;
+  w.cy = p; p += NU * 10 * nfy;
+  w.part = p; p += 2 * nblk;
+  w.scal = p; p += 2;
+  w.cnt = reinterpret_cast<unsigned int*>(p);
+  w.npad = (int)np;
+  w.nblk = nblk;
+  return JH_OK;
+}
+
+AdjArgs adj_args(const jh_ctx* c, double dt) {
+  AdjArgs a;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.dx = c->dx;
+  a.dy = c->dy;
+  a.dt = dt;
+  a.mc = c->mc;
+  for (int s = 0; s < 4; ++s) a.bc[s] = c->bc[s];
+  return a;
+}
+
+// Flux VJP of one RK stage with input (Us, As) and rhs cotangent rb; the
+// results are added into (Ut, At).
+void flux_vjp(jh_ctx* c, const AdjArgs& a, AdjWork& w, const double* Us, const double* As,
+              const double* rb, const unsigned long long* alpha, double* Ut, double* At) {
+  const size_t n = (size_t)c->nx * c->ny;
+  const int nfx = (c->nx + 1) * c->ny, nfy = c->nx * (c->ny + 1);
+  hipStream_t st = c->stream;
+  pad_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, Us, As, w.Up, w.ap);
+  face_vjp_kernel<0><<<blocks_for(nfx), 256, 0, st>>>(a, w.Up, w.ap, rb, alpha, w.cx);
+  face_vjp_kernel<1><<<blocks_for(nfy), 256, 0, st>>>(a, w.Up, w.ap, rb, alpha, w.cy);
+  cell_vjp_kernel<<<w.nblk, 256, 0, st>>>(a, w.Up, w.ap, w.cx, w.cy, alpha, w.Ubp, w.abp, w.part);
+  alpha_reduce_kernel<<<1, 256, 0, st>>>(w.part, w.nblk, w.scal);
+  zero_u32_kernel<<<1, 64, 0, st>>>(w.cnt, 2);
+  alpha_count_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, w.Up, w.ap, alpha, w.cnt);
+  alpha_apply_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, w.Up, w.ap, alpha, w.scal, w.cnt,
+                                                         w.Ubp, w.abp);
+  unpad_kernel<<<blocks_for(n), 256, 0, st>>>(a, w.Ubp, w.abp, Ut, At);
+}
+
+int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, const double* Ubar,
+                  const double* Abar, double* Ub0, double* Ab0, double* Uo, double* Ao) {
+  AdjWork w;
+  int rc = adj_work(c, U0, A0, w);
+  if (rc) return rc;
+  const size_t n = (size_t)c->nx * c->ny;
+  const int bn = blocks_for(NU * n), b2 = blocks_for(2 * n), bc1 = blocks_for(n);
+  hipStream_t st = c->stream;
+  // ---- forward recompute, keeping every stage ----
+  rc = prologue<double>(c, U0, A0, dt, 0.0);
+  if (rc) return rc;
+  const int P = c->parity;
+  auto a = make_args<double>(c, dt, 0.0);
+  const int bb = bc_blocks(c);
+  for (int s = 0; s < 3; ++s) {
+    bc_kernel<double><<<bb, 256, 0, st>>>(a, w.U[s], w.A[s], s);
+    launch_stage<double, MODE_RK>(c, a, s, w.U[s], w.A[s], U0, w.U[s + 1], w.A[s + 1]);
+  }
+  HIP_TRY(c, hipMemcpyAsync(w.U[4], w.U[3], NU * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(c, hipMemcpyAsync(w.A[4], w.A[3], 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  chem_kernel<double, CHEM_STEP><<<grid_all(c), 256, 0, st>>>(a, w.U[4], w.A[4], nullptr, nullptr);
+  rc = check_launch(c, "vjp forward");
+  if (rc) return rc;
+  c->parity ^= 1;
+  if (Uo) HIP_TRY(c, hipMemcpyAsync(Uo, w.U[4], NU * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (Ao) HIP_TRY(c, hipMemcpyAsync(Ao, w.A[4], 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  const AdjArgs aa = adj_args(c, dt);
+  const unsigned long long* alpha[3] = {&c->sc->amax[P][0][0], &c->sc->amax[P][1][0],
+                                        &c->sc->amax[P][2][0]};
+  // ---- reverse sweep ----
+  // final update_aux and chemistry: Ub = U4_bar -> U3_bar
+  axpby_kernel<<<bn, 256, 0, st>>>(NU * n, 1.0, Ubar, 0.0, w.Ub);
+  getT_vjp_kernel<<<bc1, 256, 0, st>>>(aa, w.U[4], w.A[4], Abar, w.Ub);
+  chem_vjp_kernel<<<bc1, 256, 0, st>>>(aa, w.U[3], w.A[3], w.Ub, w.Ub, w.ab);
+  getT_vjp_kernel<<<bc1, 256, 0, st>>>(aa, w.U[3], w.A[3], w.ab, w.Ub);
+  // stage 3: U3 = 1/3 U + 2/3 (U2 + r(U2))
+  axpby_kernel<<<bn, 256, 0, st>>>(NU * n, 1.0 / 3.0, w.Ub, 0.0, w.Uacc);
+  axpby_kernel<<<bn, 256, 0, st>>>(NU * n, 2.0 / 3.0, w.Ub, 0.0, w.rb);
+  axpby_kernel<<<bn, 256, 0, st>>>(NU * n, 1.0, w.rb, 0.0, w.Ub);
+  fill_kernel<<<b2, 256, 0, st>>>(2 * n, 0.0, w.ab);
+  flux_vjp(c, aa, w, w.U[2], w.A[2], w.rb, alpha[2], w.Ub, w.ab);
+  getT_vjp_kernel<<<bc1, 256, 0, st>>>(aa, w.U[2], w.A[2], w.ab, w.Ub);
+  // stage 2: U2 = 3/4 U + 1/4 (U1 + r(U1))
+  axpby_kernel<<<bn, 256, 0, st>>>(NU * n, 3.0 / 4.0, w.Ub, 1.0, w.Uacc);
+  axpby_kernel<<<bn, 256, 0, st>>>(NU * n, 1.0 / 4.0, w.Ub, 0.0, w.rb);
+  axpby_kernel<<<bn, 256, 0, st>>>(NU * n, 1.0, w.rb, 0.0, w.Ub);
+  fill_kernel<<<b2, 256, 0, st>>>(2 * n, 0.0, w.ab);
+  flux_vjp(c, aa, w, w.U[1], w.A[1], w.rb, alpha[1], w.Ub, w.ab);
+  getT_vjp_kernel<<<bc1, 256, 0, st>>>(aa, w.U[1], w.A[1], w.ab, w.Ub);
+  // stage 1: U1 = U + r(U)
+  axpby_kernel<<<bn, 256, 0, st>>>(NU * n, 1.0, w.Ub, 1.0, w.Uacc);
+  fill_kernel<<<b2, 256, 0, st>>>(2 * n, 0.0, w.ab);
+  flux_vjp(c, aa, w, w.U[0], w.A[0], w.Ub, alpha[0], w.Uacc, w.ab);
+  HIP_TRY(c, hipMemcpyAsync(Ub0, w.Uacc, NU * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(c, hipMemcpyAsync(Ab0, w.ab, 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  return check_launch(c, "vjp kernels");
+}
+
 }  // namespace
 
 // Device-side copy of the few scalars that a host-side query needs.
@@ -291,6 +445,7 @@ int jh_ctx_destroy(jh_ctx* c) {
     c->ev_free.push_back(m.b);
   }
   for (auto e : c->ev_free) (void)hipEventDestroy(e);
+  if (c->adj) (void)hipFree(c->adj);
   (void)hipFree(c->mc);
   (void)hipFree(c->sc);
   for (int s = 0; s < 4; ++s) (void)hipFree(c->ghost[s]);
@@ -425,6 +580,23 @@ int jh_lf_alpha(jh_ctx* c, const void* U, const void* aux, double* alpha_dev) {
   }
   copy_scalars_kernel<<<1, 64, 0, c->stream>>>(c->sc, c->parity, 0.0, nullptr, alpha_dev);
   return check_launch(c, "lf_alpha kernels");
+}
+
+int jh_step_vjp(jh_ctx* c, const void* U, const void* aux, double dt, const void* U_bar,
+                const void* aux_bar, void* U_in_bar, void* aux_in_bar, void* U_out,
+                void* aux_out) {
+  if (!c) return JH_ERR_ARG;
+  if (!U || !aux || !U_bar || !aux_bar || !U_in_bar || !aux_in_bar)
+    return fail(c, JH_ERR_ARG, "NULL pointer");
+  if (c->precision != JH_PREC_FP64) return fail(c, JH_ERR_ARG, "the adjoint is FP64 only");
+  if (!(dt > 0)) return fail(c, JH_ERR_ARG, "the adjoint needs a fixed dt > 0");
+  for (int s = 0; s < 4; ++s)
+    if (c->bc[s] == JH_BC_PRESSURE_OUTLET || c->bc[s] == JH_BC_USER)
+      return fail(c, JH_ERR_ARG, "the adjoint supports periodic, zero_gradient, slip_wall and "
+                                 "no_slip_wall boundaries");
+  return step_vjp_impl(c, (const double*)U, (const double*)aux, dt, (const double*)U_bar,
+                       (const double*)aux_bar, (double*)U_in_bar, (double*)aux_in_bar,
+                       (double*)U_out, (double*)aux_out);
 }
 
 int jh_timing_enable(jh_ctx* c, int on) {

@@ -114,3 +114,49 @@ __host__ __device__ __forceinline__ void halley_T(const MechConst& c,
 }
 
 }  // namespace jh
+
+namespace jh {
+
+// Reverse of get_T_nasa7 at the converged T (get_T_bwd,
+// thermo_model.py:159-188): (gamma_bar, T_bar) -> (e_bar, Y_bar[0:NS-1]).
+// Y is the filled mass-fraction vector, R its gas constant.
+__host__ __device__ __forceinline__ void getT_vjp(const MechConst& c, const double (&Y)[JH_NS],
+                                                  double R, double T, double gb, double Tbr,
+                                                  double& eb, double (&Yb)[JH_NS - 1]) {
+  const int r = T < c.Tcr ? 0 : 1;
+  double cpk[JH_NS], ek[JH_NS];
+  double cp = 0.0, dcp = 0.0;
+#pragma unroll
+  for (int k = 0; k < JH_NS; ++k) {
+    double a = c.cp[r][k][0];
+#pragma unroll
+    for (int j = 1; j < 5; ++j) a = a * T + c.cp[r][k][j];
+    double d = 4.0 * c.cp[r][k][0];
+    d = d * T + 3.0 * c.cp[r][k][1];
+    d = d * T + 2.0 * c.cp[r][k][2];
+    d = d * T + c.cp[r][k][3];
+    double h = c.h[r][k][0];
+#pragma unroll
+    for (int j = 1; j < 6; ++j) h = h * T + c.h[r][k][j];
+    cpk[k] = a;
+    ek[k] = h - c.R[k] * T;
+    cp += a * Y[k];
+    dcp += d * Y[k];
+  }
+  const double cv = cp - R, icv = rcp(cv);
+  const double dTde = icv;
+  const double dgdT = (dcp * cv - dcp * cp) * icv * icv;
+  eb = gb * dgdT * dTde + Tbr * dTde;
+  constexpr int L = JH_NS - 1;
+#pragma unroll
+  for (int k = 0; k < JH_NS - 1; ++k) {
+    const double dTdY = (-ek[k] + ek[L]) * icv;
+    const double dRdY = c.R[k] - c.R[L];
+    const double dcpdY = cpk[k] - cpk[L];
+    const double dcvdY = dcpdY - dRdY;
+    const double dgdY = dgdT * dTdY + (dcpdY * cv - dcvdY * cp) * icv * icv;
+    Yb[k] = gb * dgdY + Tbr * dTdY;
+  }
+}
+
+}  // namespace jh

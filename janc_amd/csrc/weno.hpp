@@ -66,3 +66,43 @@ __host__ __device__ __forceinline__ float weno5(float fm2, float fm1, float f0, 
 }
 
 }  // namespace jh
+
+namespace jh {
+
+// Value and gradient of the FP64 weno5 with respect to its five arguments
+// (the reverse mode of WENO5_JS.py:17-33 as JAX differentiates it):
+//   dq/dv_j = sum_k w_k dq_k/dv_j + sum_k (q_k - q)(-2 w_k/(eps+IS_k)) dIS_k/dv_j.
+__host__ __device__ __forceinline__ double weno5_grad(double v0, double v1, double v2, double v3,
+                                                      double v4, double (&g)[5]) {
+  const double a1 = v0 - 4.0 * v1 + 3.0 * v2, b1 = v0 - 2.0 * v1 + v2;
+  const double a2 = v1 - v3, b2 = v1 - 2.0 * v2 + v3;
+  const double a3 = 3.0 * v2 - 4.0 * v3 + v4, b3 = v2 - 2.0 * v3 + v4;
+  const double e1 = 1e-6 + (0.25 * (a1 * a1) + (13.0 / 12.0) * (b1 * b1));
+  const double e2 = 1e-6 + (0.25 * (a2 * a2) + (13.0 / 12.0) * (b2 * b2));
+  const double e3 = 1e-6 + (0.25 * (a3 * a3) + (13.0 / 12.0) * (b3 * b3));
+  const double d1 = e1 * e1, d2 = e2 * e2, d3 = e3 * e3;
+  const double p1 = 0.1 * (d2 * d3), p2 = 0.6 * (d1 * d3), p3 = 0.3 * (d1 * d2);
+  const double is = rcp(p1 + p2 + p3);
+  const double w1 = p1 * is, w2 = p2 * is, w3 = p3 * is;
+  const double q1 = (1.0 / 3.0) * v0 - (7.0 / 6.0) * v1 + (11.0 / 6.0) * v2;
+  const double q2 = -(1.0 / 6.0) * v1 + (5.0 / 6.0) * v2 + (1.0 / 3.0) * v3;
+  const double q3 = (1.0 / 3.0) * v2 + (5.0 / 6.0) * v3 - (1.0 / 6.0) * v4;
+  const double q = w1 * q1 + w2 * q2 + w3 * q3;
+  // coefficient of dIS_k/dv in dq/dv
+  const double c1 = (q1 - q) * (-2.0 * w1) * rcp(e1);
+  const double c2 = (q2 - q) * (-2.0 * w2) * rcp(e2);
+  const double c3 = (q3 - q) * (-2.0 * w3) * rcp(e3);
+  // dIS_k/dv = 0.5 a_k da_k/dv + (13/6) b_k db_k/dv
+  const double A1 = 0.5 * a1, B1 = (13.0 / 6.0) * b1;
+  const double A2 = 0.5 * a2, B2 = (13.0 / 6.0) * b2;
+  const double A3 = 0.5 * a3, B3 = (13.0 / 6.0) * b3;
+  g[0] = w1 * (1.0 / 3.0) + c1 * (A1 + B1);
+  g[1] = w1 * (-7.0 / 6.0) + w2 * (-1.0 / 6.0) + c1 * (-4.0 * A1 - 2.0 * B1) + c2 * (A2 + B2);
+  g[2] = w1 * (11.0 / 6.0) + w2 * (5.0 / 6.0) + w3 * (1.0 / 3.0) + c1 * (3.0 * A1 + B1) +
+         c2 * (-2.0 * B2) + c3 * (3.0 * A3 + B3);
+  g[3] = w2 * (1.0 / 3.0) + w3 * (5.0 / 6.0) + c2 * (-A2 + B2) + c3 * (-4.0 * A3 - 2.0 * B3);
+  g[4] = w3 * (-1.0 / 6.0) + c3 * (A3 + B3);
+  return q;
+}
+
+}  // namespace jh

@@ -240,11 +240,13 @@ def _generate(m):
     e("// Net production rates w_k and dw_k/d(rho Y_n) for one cell, following")
     e("// model/reaction_model.py:91-130 (reactionConstant_i summed over i).")
     e("// X: unclipped rho*Y/M (all species); Xc: clip(X, 1e-50) for active species;")
-    e("// invMX[n] = 1/(M_n*Xc_n); g: Gibbs s/R - h/(RT) of active species.")
+    e("// invMX[n] = 1/(M_n*Xc_n); eg/ieg: exp(+-g) of the Gibbs s/R - h/(RT) of the active")
+    e("// species, so exp(-sum_k dnu_k g_k) = prod_k (eg_k or ieg_k)^|dnu_k|.")
     e("__host__ __device__ __forceinline__ void mech_rates_jacobian(")
     e("    const MechConst& __restrict__ c, const double T, const double invT, const double logT,")
     e("    const double (&X)[JH_NS], const double (&Xc)[JH_NA], const double (&invMX)[JH_NA],")
-    e("    const double (&g)[JH_NA], double (&w)[JH_NA], double (&J)[JH_NA][JH_NA])")
+    e("    const double (&eg)[JH_NA], const double (&ieg)[JH_NA], double (&w)[JH_NA],")
+    e("    double (&J)[JH_NA][JH_NA])")
     e("{")
     e("  const double Pf = c.Pref * invT;   // 101325/P0/T")
     e("  const double iPf = jh::rcp(Pf);")
@@ -276,28 +278,23 @@ def _generate(m):
         pf = " * ".join(_power_expr(k, n) for k, n in sorted(r.reactants.items()))
         pb = " * ".join(_power_expr(k, n) for k, n in sorted(r.products.items()))
         e("    const double kf = kfi * (%s);" % pf)
-        dg = []
+        # exp(-sum_k dnu_k g_k) as a product of per-species factors
+        fac = []
         for k in range(na):
             d = r.products.get(k, 0.0) - r.reactants.get(k, 0.0)
-            if d == 1.0:
-                dg.append("+ g[%d]" % k)
-            elif d == -1.0:
-                dg.append("- g[%d]" % k)
-            elif d != 0.0:
-                dg.append("%s %s * g[%d]" % ("+" if d > 0 else "-", _fmt(abs(d)), k))
-        sdg = " ".join(dg).lstrip("+ ") if dg else "0.0"
-        if sdg.startswith("- "):
-            sdg = "-" + sdg[2:]
+            if d != 0.0:
+                fac += [("ieg[%d]" if d > 0 else "eg[%d]") % k] * int(abs(d))
+        eqf = " * ".join(fac) if fac else "1.0"
         vs = sum(r.products.values()) - sum(r.reactants.values())
         # kb = kf_i / (exp(sum dnu g) Pf^vsum) * prod X^nu_b (reaction_model.py:113)
         if vs == 0.0:
-            eq = "exp(-(%s))" % sdg
+            eq = "(%s)" % eqf
         elif vs == 1.0:
-            eq = "(exp(-(%s)) * iPf)" % sdg
+            eq = "(%s * iPf)" % eqf
         elif vs == -1.0:
-            eq = "(exp(-(%s)) * Pf)" % sdg
+            eq = "(%s * Pf)" % eqf
         else:
-            eq = "(exp(-(%s)) * pow(Pf, %s))" % (sdg, _fmt(-vs))
+            eq = "(%s * pow(Pf, %s))" % (eqf, _fmt(-vs))
         e("    const double kb = kfi * %s * (%s);" % (eq, pb))
         if r.three_body:
             terms = " + ".join("c.eff[%d][%d] * X[%d]" % (i, k, k) for k in range(ns))
@@ -347,6 +344,153 @@ def _generate(m):
     e("  for (int k = 0; k < JH_NA; ++k) { w[k] = c.M[k] * w[k];")
     e("#pragma unroll")
     e("    for (int n = 0; n < JH_NA; ++n) J[k][n] = c.M[k] * J[k][n]; }")
+    e("}")
+    return "\n".join(L) + "\n\n" + _generate_vjp(m)
+
+
+def _generate_vjp(m):
+    """Reverse mode of mech_rates_jacobian for the chemistry adjoint.
+
+    The cotangent of the (I/dt - J) x = w solve is w_bar = lambda and
+    J_bar = lambda x^T (rank one), so per reaction both contractions reduce
+    to the scalar s_i = sum_k dnu_ik M_k lambda_k.  Each reaction's forward
+    quantities are recomputed and its reverse applied at once (reactions add
+    independently), accumulating X_bar (all species), Xc_bar, invMX_bar and
+    T_bar.  gp[k] = d g_k / dT = (h_k/R)/T^2.
+    """
+    na, ns = m.na, m.ns
+    L = []
+    e = L.append
+    e("__host__ __device__ __forceinline__ void mech_rates_vjp(")
+    e("    const MechConst& __restrict__ c, const double T, const double invT, const double logT,")
+    e("    const double (&X)[JH_NS], const double (&Xc)[JH_NA], const double (&invMX)[JH_NA],")
+    e("    const double (&eg)[JH_NA], const double (&ieg)[JH_NA], const double (&gp)[JH_NA],")
+    e("    const double (&lam)[JH_NA], const double (&x)[JH_NA],")
+    e("    double (&Xb)[JH_NS], double (&Xcb)[JH_NA], double (&iMXb)[JH_NA], double& Tb)")
+    e("{")
+    e("  const double Pf = c.Pref * invT;")
+    e("  const double iPf = jh::rcp(Pf);")
+    e("  double ml[JH_NA];")
+    e("#pragma unroll")
+    e("  for (int k = 0; k < JH_NA; ++k) ml[k] = c.M[k] * lam[k];")
+    for i, r in enumerate(m.reactions):
+        e("  { // R%d: %s" % (i, r.equation))
+        if r.b == 0.0:
+            tb, dl = None, "0.0"
+        elif r.b == 1.0:
+            tb, dl = "T", "invT"
+        elif r.b == 2.0:
+            tb, dl = "(T * T)", "2.0 * invT"
+        elif r.b == -1.0:
+            tb, dl = "invT", "-invT"
+        elif r.b == -2.0:
+            tb, dl = "(invT * invT)", "-2.0 * invT"
+        else:
+            tb, dl = "exp(c.B[%d] * logT)" % i, "c.B[%d] * invT" % i
+        kfi = "c.A[%d]" % i
+        if tb:
+            kfi = "(%s * %s)" % (kfi, tb)
+        if r.Ea != 0.0:
+            kfi = "%s * exp(-c.EaR[%d] * invT)" % (kfi, i)
+            dl = "(%s + c.EaR[%d] * invT * invT)" % (dl, i)
+        e("    const double kfi = %s;" % kfi)
+        e("    const double dlk = %s;   // d ln kf_i / dT" % dl)
+        pf_terms = []
+        for k, n in sorted(r.reactants.items()):
+            pf_terms += ["Xc[%d]" % k] * int(n)
+        pb_terms = []
+        for k, n in sorted(r.products.items()):
+            pb_terms += ["Xc[%d]" % k] * int(n)
+        e("    const double PF = %s;" % " * ".join(pf_terms))
+        e("    const double PB = %s;" % " * ".join(pb_terms))
+        fac, dgsum = [], []
+        for k in range(na):
+            d = r.products.get(k, 0.0) - r.reactants.get(k, 0.0)
+            if d != 0.0:
+                fac += [("ieg[%d]" if d > 0 else "eg[%d]") % k] * int(abs(d))
+                dgsum.append("%s%s gp[%d]" % ("- " if d > 0 else "+ ", "" if abs(d) == 1 else "2.0 *", k))
+        vs = sum(r.products.values()) - sum(r.reactants.values())
+        eq = " * ".join(fac) if fac else "1.0"
+        if vs == 1.0:
+            eq += " * iPf"
+        elif vs == -1.0:
+            eq += " * Pf"
+        elif vs != 0.0:
+            eq += " * pow(Pf, %s)" % _fmt(-vs)
+        e("    const double EQ = %s;" % eq)
+        e("    const double kf = kfi * PF;")
+        e("    const double kb = kfi * EQ * PB;")
+        dls = " ".join(dgsum)
+        e("    const double dlb = dlk %s + %s * invT;   // d ln kb / dT at fixed X" %
+          (dls, _fmt(vs)))
+        e("    const double q = kf - kb;")
+        dnu = {}
+        for k in range(na):
+            d = r.products.get(k, 0.0) - r.reactants.get(k, 0.0)
+            if d != 0.0:
+                dnu[k] = d
+        terms = []
+        for k, d in dnu.items():
+            terms.append(("+ " if d > 0 else "- ") + ("" if abs(d) == 1 else "2.0 * ") + "ml[%d]" % k)
+        sexpr = " ".join(terms).lstrip("+ ")
+        if sexpr.startswith("- "):
+            sexpr = "-" + sexpr[2:]
+        e("    const double s = %s;" % sexpr)
+        involved = sorted(set(r.reactants) | set(r.products))
+        if r.three_body:
+            e("    const double cM = %s;" % " + ".join("c.eff[%d][%d] * X[%d]" % (i, k, k)
+                                                     for k in range(ns)))
+            e("    double sx = 0.0;   // sum_n x_n effM_in")
+            for n in range(na):
+                e("    sx += x[%d] * c.effM[%d][%d];" % (n, i, n))
+            e("    double qb = s * cM + s * sx;")
+            e("    double cMb = s * q;")
+            cexpr = "cM"
+        else:
+            e("    double qb = s;")
+            cexpr = None
+        e("    double kfb = qb, kbb = -qb;")
+        for n in involved:
+            vf, vb = r.reactants.get(n, 0.0), r.products.get(n, 0.0)
+            diff_terms = []
+            if vf:
+                diff_terms.append(_scaled(vf, "kf"))
+            if vb:
+                diff_terms.append("-" + _scaled(vb, "kb"))
+            diff = " ".join(diff_terms) if len(diff_terms) == 1 else \
+                "%s - %s" % (diff_terms[0], diff_terms[1][1:])
+            e("    { const double Db = s * x[%d];" % n)
+            if r.three_body:
+                e("      const double cf = invMX[%d] * cM;" % n)
+                e("      cMb += Db * invMX[%d] * (%s);" % (n, diff))
+                e("      iMXb[%d] += Db * cM * (%s);" % (n, diff))
+            else:
+                e("      const double cf = invMX[%d];" % n)
+                e("      iMXb[%d] += Db * (%s);" % (n, diff))
+            if vf:
+                e("      kfb += Db * cf * %s;" % _fmt(vf))
+            if vb:
+                e("      kbb -= Db * cf * %s;" % _fmt(vb))
+            e("    }")
+        if r.three_body:
+            for k in range(ns):
+                e("    Xb[%d] += cMb * c.eff[%d][%d];" % (k, i, k))
+        # kf = kfi * PF, kb = kfi * EQ * PB
+        e("    Tb += kfb * kf * dlk + kbb * kb * dlb;")
+        def prod_grad(terms, coef):
+            # d(prod)/d Xc[k] for each distinct k, times coef
+            from collections import Counter
+            cnt = Counter(terms)
+            for t, mult in cnt.items():
+                k = int(t[3:-1])
+                others = list(terms)
+                others.remove(t)
+                rest = " * ".join(others) if others else "1.0"
+                fac2 = "" if mult == 1 else "%s * " % _fmt(mult)
+                e("    Xcb[%d] += %s%s * %s;" % (k, fac2, coef, rest))
+        prod_grad(pf_terms, "kfb * kfi")
+        prod_grad(pb_terms, "kbb * kfi * EQ")
+        e("  }")
     e("}")
     return "\n".join(L) + "\n"
 

@@ -297,6 +297,25 @@ class Simulator:
         ctx.call("jh_lf_alpha", _ptr(U), _ptr(aux), _ptr(out))
         return out
 
+    def step_vjp(self, U, aux, dt, U_bar, aux_bar, with_output=False):
+        """Reverse-mode derivative of advance_one_step at fixed dt (FP64).
+
+        Returns (U_in_bar, aux_in_bar), plus the step's output (U, aux) when
+        ``with_output``: the VJP of set_simulator.py:117-122 under JAX's
+        rules, get_T_nasa7's custom VJP (thermo_model.py:155-190) included."""
+        if self.precision != _lib.PREC_FP64:
+            raise NotImplementedError("the adjoint is built for the FP64 form")
+        U, aux = self._state(U, aux)
+        Ub, ab = self._state(U_bar, aux_bar)
+        ctx = self._context(U, None)
+        gU, ga = torch.empty_like(U), torch.empty_like(aux)
+        Uo = torch.empty_like(U) if with_output else None
+        ao = torch.empty_like(aux) if with_output else None
+        ctx.call("jh_step_vjp", _ptr(U), _ptr(aux), float(dt), _ptr(Ub), _ptr(ab), _ptr(gU),
+                 _ptr(ga), _ptr(Uo) if with_output else None,
+                 _ptr(ao) if with_output else None)
+        return (gU, ga, Uo, ao) if with_output else (gU, ga)
+
     def set_tile(self, U, variant, theta=None):
         """Select the stage-kernel variant (jh_ctx_set_tile) for U's grid."""
         ctx = self._context(U, theta)

@@ -1,0 +1,150 @@
+"""GPU parity of the hand-written adjoint (jh_step_vjp, janc_amd.adjoint).
+
+Oracle: reverse mode of the differentiable torch twin of the oracle step
+(oracle/torch_step.py, itself pinned against finite differences of the numpy
+oracle).  Bars: per-variable max-norm relative error <= 1e-9 against the
+twin; directional derivative within 1e-6 of a central finite difference of
+the GPU step (step 1e-7 relative per component); VJP linear in the cotangent
+to 1e-12; checkpointed rollout identical to the chained single-step VJPs.
+"""
+import numpy as np
+import pytest
+
+from janc_amd import cases
+from janc_amd.mechanism import Mechanism
+
+torch = pytest.importorskip("torch")
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not torch.cuda.is_available(), reason="needs a GPU")]
+
+from oracle.torch_step import TorchOracle  # noqa: E402
+from test_oracle_adjoint import tie_free_state  # noqa: E402
+
+TWIN_TOL = 1e-9
+DT = 2e-9
+
+
+@pytest.fixture(scope="module")
+def mech(mech_path):
+    return Mechanism(mech_path)
+
+
+def rel_err(got, ref):
+    got = got.detach().cpu().numpy() if hasattr(got, "detach") else np.asarray(got)
+    ref = ref.detach().cpu().numpy() if hasattr(ref, "detach") else np.asarray(ref)
+    d = np.abs(got - ref).reshape(ref.shape[0], -1).max(axis=1)
+    s = np.abs(ref).reshape(ref.shape[0], -1).max(axis=1)
+    return d / np.maximum(s, 1e-300)
+
+
+def _setup(mech_path, mech, name):
+    from janc_amd.solver import Simulator
+    if name == "box_ties":      # cells 0 and N-1 equal: exact LF-maximum ties
+        U, aux, g = cases.periodic_box(mech, nx=23, ny=19)
+    elif name == "tube":
+        U, aux, g = tie_free_state(mech_path, mech, "tube", 26, 13)
+    elif name == "walls":
+        U, aux, g = tie_free_state(mech_path, mech, "tube", 21, 17)
+        g["boundary_config"] = {"left_boundary": "no_slip_wall",
+                                "right_boundary": "slip_wall",
+                                "bottom_boundary": "zero_gradient",
+                                "top_boundary": "slip_wall"}
+    else:
+        U, aux, g = tie_free_state(mech_path, mech, "box", 22, 15)
+    cfg = cases.simulation_config(g, mech_path, cfl=0.3, dt=DT)
+    sim = Simulator(cfg)
+    print("case", name, U.shape, flush=True)
+    return sim, g, U, aux
+
+
+def _cotangents(U, aux, seed=0):
+    rng = np.random.default_rng(seed)
+    sU = np.abs(U).reshape(U.shape[0], -1).max(1)[:, None, None]
+    sa = np.abs(aux).reshape(2, -1).max(1)[:, None, None]
+    return rng.standard_normal(U.shape) / sU, rng.standard_normal(aux.shape) / sa
+
+
+@pytest.mark.parametrize("name", ["box", "box_ties", "tube", "walls"])
+def test_step_vjp_matches_twin(mech_path, mech, name):
+    sim, g, U, aux = _setup(mech_path, mech, name)
+    W, Wa = _cotangents(U, aux)
+    to = TorchOracle(mech_path, g["boundary_config"], g["dx"], g["dy"])
+    Uv = torch.tensor(U, requires_grad=True)
+    av = torch.tensor(aux, requires_grad=True)
+    U4, a4 = to.advance_one_step(Uv, av, DT)
+    ((U4 * torch.tensor(W)).sum() + (a4 * torch.tensor(Wa)).sum()).backward()
+    gU, ga, Uo, ao = sim.step_vjp(U, aux, DT, W, Wa, with_output=True)
+    eU, ea = rel_err(gU, Uv.grad), rel_err(ga, av.grad)
+    print("U_bar rel err", eU.max(), "aux_bar rel err", ea.max(), flush=True)
+    assert eU.max() <= TWIN_TOL and ea.max() <= TWIN_TOL
+    assert rel_err(Uo, U4).max() <= 1e-12 and rel_err(ao, a4).max() <= 1e-12
+
+
+@pytest.mark.parametrize("name,nx,ny", [("tube", 26, 13), ("box", 256, 160)])
+def test_step_vjp_directional_fd(mech_path, mech, name, nx, ny):
+    from janc_amd.solver import Simulator
+    U, aux, g = tie_free_state(mech_path, mech, name, nx, ny)
+    sim = Simulator(cases.simulation_config(g, mech_path, cfl=0.3, dt=DT))
+    print("case", name, U.shape, flush=True)
+    W, Wa = _cotangents(U, aux, 1)
+    Wt, Wat = torch.tensor(W, device="cuda"), torch.tensor(Wa, device="cuda")
+    gU, ga = sim.step_vjp(U, aux, DT, Wt, Wat)
+    rng = np.random.default_rng(2)
+    Ut, at = torch.tensor(U, device="cuda"), torch.tensor(aux, device="cuda")
+    dU = torch.tensor(rng.standard_normal(U.shape) * np.abs(U) * 1e-7, device="cuda")
+
+    def L(Ux):
+        A, B = sim.advance_one_step(Ux, at, sim.dx, sim.dy, DT)
+        return float((A * Wt).sum() + (B * Wat).sum())
+
+    fd = (L(Ut + dU) - L(Ut - dU)) / 2
+    ad = float((gU * dU).sum())
+    print("fd", fd, "ad", ad, flush=True)
+    assert abs(fd - ad) <= 1e-6 * abs(fd)
+
+
+def test_step_vjp_linear_in_cotangent(mech_path, mech):
+    sim, g, U, aux = _setup(mech_path, mech, "walls")
+    W1, Wa1 = _cotangents(U, aux, 3)
+    W2, Wa2 = _cotangents(U, aux, 4)
+    g1, a1 = sim.step_vjp(U, aux, DT, W1, Wa1)
+    g2, a2 = sim.step_vjp(U, aux, DT, W2, Wa2)
+    g3, a3 = sim.step_vjp(U, aux, DT, 2.0 * W1 - W2, 2.0 * Wa1 - Wa2)
+    assert rel_err(g3, 2.0 * g1 - g2).max() <= 1e-12
+    assert rel_err(a3, 2.0 * a1 - a2).max() <= 1e-12
+
+
+def test_rollout_checkpointing_matches_chained_steps(mech_path, mech):
+    from janc_amd import adjoint
+    sim, g, U, aux = _setup(mech_path, mech, "tube")
+    W, Wa = _cotangents(U, aux, 5)
+    Wt, Wat = torch.tensor(W, device="cuda"), torch.tensor(Wa, device="cuda")
+    n = 5
+    U0 = torch.tensor(U, device="cuda", requires_grad=True)
+    a0 = torch.tensor(aux, device="cuda", requires_grad=True)
+    Ux, ax = U0, a0
+    for _ in range(n):
+        Ux, ax = adjoint.step(sim, Ux, ax, DT)
+    ((Ux * Wt).sum() + (ax * Wat).sum()).backward()
+    U1 = torch.tensor(U, device="cuda", requires_grad=True)
+    a1 = torch.tensor(aux, device="cuda", requires_grad=True)
+    Uy, ay = adjoint.rollout(sim, U1, a1, DT, n, checkpoint_every=2)
+    ((Uy * Wt).sum() + (ay * Wat).sum()).backward()
+    assert torch.equal(Ux.detach(), Uy.detach())
+    assert torch.equal(U0.grad, U1.grad) and torch.equal(a0.grad, a1.grad)
+
+
+def test_step_vjp_rejects_unsupported(mech_path, mech):
+    from janc_amd import _lib
+    from janc_amd.solver import Simulator
+    U, aux, g = cases.detonation_tube(mech, nx=20, ny=12, driver=5)
+    W, Wa = _cotangents(U, aux)
+    g["boundary_config"]["top_boundary"] = "pressure_outlet"
+    sim = Simulator(cases.simulation_config(g, mech_path, cfl=0.3, dt=DT,
+                                            theta={"Pb": 101325.0}))
+    with pytest.raises(_lib.JancError):
+        sim.step_vjp(U, aux, DT, W, Wa)
+    g["boundary_config"]["top_boundary"] = "slip_wall"
+    sim = Simulator(cases.simulation_config(g, mech_path, cfl=0.3, dt=DT, precision="mixed"))
+    with pytest.raises(NotImplementedError):
+        sim.step_vjp(U.astype(np.float32), aux.astype(np.float32), DT, W, Wa)
