@@ -143,6 +143,125 @@ def _accuracy(mech, steps, every, torch):
             "dt": dt, "steps": steps, "health_flags": flags, "every": report}
 
 
+def _batch_leg(mech, steps, tune, torch):
+    """64 uneven mixed-precision tubes advanced together (batched launches
+    captured in one HIP graph over `steps` steps) vs the same cases advanced
+    one after another, each in one fused library call."""
+    from janc_amd import cases
+    from janc_amd.batch import BatchSimulator
+    from janc_amd.solver import Simulator
+    cs = cases.batch_cases(mech)
+    cfgs = [cases.simulation_config(g, cfl=0.3, precision="mixed") for _, _, g in cs]
+    f32 = torch.float32
+    U0 = [torch.as_tensor(U, dtype=f32).cuda() for U, _, _ in cs]
+    A0 = [torch.as_tensor(a, dtype=f32).cuda() for _, a, _ in cs]
+    cells = sum(U.shape[1] * U.shape[2] for U in U0)
+    Us = [U.clone() for U in U0]
+    As = [a.clone() for a in A0]
+    b = BatchSimulator(cfgs, Us, As)
+    tuned = b.autotune(steps=steps) if tune else None
+    for U, a, u0, a0 in zip(Us, As, U0, A0):
+        U.copy_(u0)
+        a.copy_(a0)
+    b.advance(steps, graph=True)            # capture + first replay
+    for U, a, u0, a0 in zip(Us, As, U0, A0):
+        U.copy_(u0)
+        a.copy_(a0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.advance(steps, graph=True)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    flags = b.health()
+    # sequential: one Simulator per case, same tile variant, warm contexts
+    sims = [Simulator(c) for c in cfgs]
+    Uq = [U.clone() for U in U0]
+    Aq = [a.clone() for a in A0]
+    for sim, U, a in zip(sims, Uq, Aq):
+        sim.set_tile(U, b.get_tile())
+        Uw, aw = U.clone(), a.clone()
+        sim.advance(Uw, aw, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sim, U, a in zip(sims, Uq, Aq):
+        sim.advance(U, a, steps)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    same = all(torch.equal(x, y) for x, y in zip(Us, Uq))
+    return {"workload": "64 tubes, grids drawn from {128x32, 256x64, 512x128, 1024x256}, "
+                        "phi U[0.6,1.4], p0 U[0.5,3] atm, driver T U[2000,3200] K, mixed "
+                        "precision, CFL 0.3, %d steps in one HIP graph" % steps,
+            "cases": len(cs), "cells": cells,
+            "value": cells * steps / t_batch, "unit": "cell-updates/s",
+            "ms_per_step": t_batch / steps * 1e3,
+            "sequential_value": cells * steps / t_seq,
+            "speedup_vs_sequential": t_seq / t_batch,
+            "bit_identical_to_sequential": bool(same),
+            "tile_variant": b.get_tile(), "autotune": tuned, "health_flags": flags}
+
+
+def _adjoint_leg(mech, steps, every, torch):
+    """Forward vs forward+adjoint of a 100-step FP64 rollout on a 1024 x 256
+    tube (checkpoints every 10 steps, RK stages recomputed), plus one step's
+    VJP against one forward step."""
+    from janc_amd import adjoint, cases
+    from janc_amd.solver import Simulator
+    U, aux, g = cases.detonation_tube(mech, nx=1024, ny=256, driver=50, seed=4)
+    probe = Simulator(cases.simulation_config(g, cfl=0.3, precision="fp64"))
+    Ut = torch.as_tensor(U).cuda()
+    at = torch.as_tensor(aux).cuda()
+    dt = 0.8 * float(probe.CFL(Ut, at))
+    sim = Simulator(cases.simulation_config(g, dt=dt, precision="fp64"))
+    cells = U.shape[1] * U.shape[2]
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    W = torch.randn(Ut.shape, dtype=torch.float64, device="cuda", generator=gen)
+    Wa = torch.zeros_like(at)
+    # one step: forward vs VJP (which recomputes the step's stages)
+    Uw, aw = Ut.clone(), at.clone()
+    sim.advance(Uw, aw, 2)
+    sim.step_vjp(Ut, at, dt, W, Wa)
+    torch.cuda.synchronize()
+    reps = 10
+    t0 = time.perf_counter()
+    sim.advance(Uw, aw, reps)
+    torch.cuda.synchronize()
+    t_fwd1 = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.step_vjp(Ut, at, dt, W, Wa)
+    torch.cuda.synchronize()
+    t_vjp1 = (time.perf_counter() - t0) / reps
+    # rollout
+    Uf, af = Ut.clone(), at.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.advance(Uf, af, steps)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    U0 = Ut.clone().requires_grad_(True)
+    a0 = at.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Uy, ay = adjoint.rollout(sim, U0, a0, dt, steps, checkpoint_every=every)
+    torch.cuda.synchronize()
+    t_roll_fwd = time.perf_counter() - t0
+    loss = (Uy * W).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t0
+    finite = bool(torch.isfinite(U0.grad).all() and torch.isfinite(a0.grad).all())
+    return {"workload": "1024x256 tube (dx 1.25e-5 m, 50-column 30 atm/3000 K driver), FP64, "
+                        "fixed dt %.3e s, %d steps, checkpoints every %d" % (dt, steps, every),
+            "cells": cells,
+            "step_fwd_ms": t_fwd1 * 1e3, "step_vjp_ms": t_vjp1 * 1e3,
+            "adjoint_over_fwd_step": t_vjp1 / t_fwd1,
+            "rollout_fwd_ms": t_fwd * 1e3,
+            "rollout_checkpointed_fwd_ms": t_roll_fwd * 1e3,
+            "rollout_fwd_plus_adjoint_ms": t_total * 1e3,
+            "adjoint_over_fwd_rollout": (t_total - t_roll_fwd) / t_fwd,
+            "gradients_finite": finite}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +273,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
+    ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=100)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="one 256 MiB device-to-device copy before the timed region "
                          "(reference dispatch for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -161,7 +284,7 @@ def main():
                     help="only the FP64 headline workload (for profiler runs)")
     args = ap.parse_args()
     if args.headline_only:
-        args.no_mixed = args.no_cpu = True
+        args.no_mixed = args.no_cpu = args.no_batch = args.no_adjoint = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the single-block step on one GPU; "
               "--gpus N > 1 is not supported (see DESIGN.md for how it would shard)",
@@ -250,6 +373,13 @@ def main():
 
     if not args.no_mixed and args.accuracy_steps > 0:
         result["mixed_accuracy"] = _accuracy(mech, args.accuracy_steps, 100, torch)
+
+    if not args.no_batch:
+        result["batch"] = _batch_leg(mech, args.batch_steps, not args.no_tune, torch)
+        torch.cuda.empty_cache()
+    if not args.no_adjoint:
+        result["adjoint"] = _adjoint_leg(mech, args.adjoint_steps, 10, torch)
+        torch.cuda.empty_cache()
 
     if not args.no_cpu:
         result["cpu_baseline"] = _cpu_baseline(U0, aux0, geom, theta, args.cpu_steps)

@@ -147,6 +147,44 @@ int jh_timing_read(jh_ctx* ctx, double* ms, int* counts);
  * bits: 1 = non-finite value, 2 = rho <= 0); synchronises the stream. */
 int jh_health(jh_ctx* ctx, unsigned int* flags);
 
+/* ---------------------------------------------------------------- batches
+ * Many independent cases of uneven grid sizes advanced together: one launch
+ * per kernel kind covers every case (the stage kernel is persistent over the
+ * tile list of all cases), and jh_batch_advance can capture its launch
+ * sequence in a HIP graph and replay it.  Each case evolves exactly as
+ * jh_advance would advance it alone (bit-identical results).  The reference
+ * batches same-size blocks with jax.vmap over advance_one_step
+ * (simulation/set_simulator.py:37-42, 66-70) and has no uneven-size batch;
+ * against it this replaces a Python loop of advance_func calls
+ * (set_simulator.py:444-447), one per case. */
+typedef struct jh_batch jh_batch;
+
+/* ncase cases; nx, ny, dx, dy have ncase entries; bc_kind / bc_param have
+ * 4 per case (JH_SIDE order, no JH_BC_USER; bc_param may be NULL).  All cases
+ * share the precision form and the mechanism.  Launches and graph replays
+ * run on an internal stream ordered after / before `stream` with events. */
+int jh_batch_create(int ncase, const int* nx, const int* ny, const double* dx, const double* dy,
+                    const int* bc_kind, const double* bc_param, int precision,
+                    const double* mech_consts, int n_consts, void* stream, jh_batch** out);
+int jh_batch_destroy(jh_batch* batch);
+/* Device state pointers of every case (U[k]: (12, nx, ny), aux[k]:
+ * (2, nx, ny)); advanced in place.  Invalidates a captured graph. */
+int jh_batch_bind(jh_batch* batch, void* const* U, void* const* aux);
+int jh_batch_set_stream(jh_batch* batch, void* stream);
+/* Stage-kernel variant for every case (as jh_ctx_set_tile). */
+int jh_batch_set_tile(jh_batch* batch, int variant);
+int jh_batch_get_tile(const jh_batch* batch);
+/* nsteps steps of every case: fixed dt if dt > 0, else each case's own CFL
+ * dt (set_simulator.py:438-447).  dt_hist (device, optional) receives
+ * (ncase, nsteps) per-case dt.  use_graph != 0 captures the sequence in a
+ * HIP graph on first use and replays it while (nsteps, dt, cfl, dt_hist,
+ * tile variant, bound pointers) stay the same. */
+int jh_batch_advance(jh_batch* batch, double dt, double cfl, int nsteps, double* dt_hist,
+                     int use_graph);
+/* OR of every case's health flags since the last call; synchronises. */
+int jh_batch_health(jh_batch* batch, unsigned int* flags);
+const char* jh_batch_last_error(const jh_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

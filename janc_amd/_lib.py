@@ -45,6 +45,18 @@ SIGNATURES = {
     "jh_health": (_c_int, [_vp, ctypes.POINTER(ctypes.c_uint)]),
     "jh_timing_enable": (_c_int, [_vp, _c_int]),
     "jh_timing_read": (_c_int, [_vp, ctypes.POINTER(_c_double), ctypes.POINTER(_c_int)]),
+    "jh_batch_create": (_c_int, [_c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),
+                                 ctypes.POINTER(_c_double), ctypes.POINTER(_c_double),
+                                 ctypes.POINTER(_c_int), ctypes.POINTER(_c_double), _c_int,
+                                 ctypes.POINTER(_c_double), _c_int, _vp, ctypes.POINTER(_vp)]),
+    "jh_batch_destroy": (_c_int, [_vp]),
+    "jh_batch_bind": (_c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "jh_batch_set_stream": (_c_int, [_vp, _vp]),
+    "jh_batch_set_tile": (_c_int, [_vp, _c_int]),
+    "jh_batch_get_tile": (_c_int, [_vp]),
+    "jh_batch_advance": (_c_int, [_vp, _c_double, _c_double, _c_int, _vp, _c_int]),
+    "jh_batch_health": (_c_int, [_vp, ctypes.POINTER(ctypes.c_uint)]),
+    "jh_batch_last_error": (ctypes.c_char_p, [_vp]),
 }
 
 _lib = None

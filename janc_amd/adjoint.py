@@ -78,3 +78,76 @@ def rollout(sim, U, aux, dt, nsteps, checkpoint_every=10):
     ``checkpoint_every`` steps (memory: one state per checkpoint plus one
     segment during the backward sweep)."""
     return _Rollout.apply(U, aux, sim, float(dt), int(nsteps), int(checkpoint_every))
+
+
+# ---------------------------------------------------------------- objectives
+# Differentiable (torch) state construction and losses for gradients with
+# respect to initial temperature and composition fields.
+
+def _poly(coef, T):
+    """Horner evaluation, coef (ns, m) highest power first, T (nx, ny)."""
+    acc = torch.zeros((coef.shape[0],) + tuple(T.shape), dtype=T.dtype, device=T.device)
+    for j in range(coef.shape[1]):
+        acc = acc * T + coef[:, j].reshape(-1, 1, 1)
+    return acc
+
+
+def _consts(mech, ref):
+    import numpy as np
+    f = lambda a: torch.as_tensor(np.asarray(a), dtype=ref.dtype, device=ref.device)  # noqa
+    return f(1.0 / mech.species_M), f(mech.cp), f(mech.h)
+
+
+def mixture(mech, T, Y9):
+    """(R, cp, h, gamma) of the mixture (NASA-7, per unit mass), as in the
+    reference's get_thermo_nasa7 (model/thermo_model.py:94-132)."""
+    invM, cp_c, h_c = _consts(mech, T)
+    R = (invM.reshape(-1, 1, 1) * Y9).sum(0)
+    lo = T < mech.Tcr
+    cp = torch.where(lo, (_poly(cp_c[0], T) * Y9).sum(0), (_poly(cp_c[1], T) * Y9).sum(0))
+    h = torch.where(lo, (_poly(h_c[0], T) * Y9).sum(0), (_poly(h_c[1], T) * Y9).sum(0))
+    return R, cp, h, cp / (cp - R)
+
+
+def initial_state(mech, T, p, Y, u=None, v=None):
+    """(U, aux) from T, p (nx, ny) and the first ns-1 mass fractions Y
+    (ns-1, nx, ny), differentiably: rho = p/(R T), rho*E = rho (h - R T) +
+    kinetic energy, aux = (gamma, T)."""
+    Y9 = torch.cat([Y, 1.0 - Y.sum(0, keepdim=True)], 0)
+    R, cp, h, gamma = mixture(mech, T, Y9)
+    rho = p / (R * T)
+    u = torch.zeros_like(T) if u is None else u
+    v = torch.zeros_like(T) if v is None else v
+    rhoE = rho * h - p + 0.5 * rho * (u * u + v * v)
+    U = torch.cat([rho[None], (rho * u)[None], (rho * v)[None], rhoE[None], rho[None] * Y], 0)
+    return U, torch.stack([gamma, T])
+
+
+def formation_enthalpy(mech, ref, T_ref=298.15):
+    """h_k(T_ref) per unit mass of every species (NASA-7)."""
+    _, _, h_c = _consts(mech, ref)
+    r = 0 if T_ref < mech.Tcr else 1
+    T = torch.full((1, 1), T_ref, dtype=ref.dtype, device=ref.device)
+    return _poly(h_c[r], T).reshape(-1)
+
+
+def heat_release(mech, U0, UN, dx, dy):
+    """Chemical heat released between states U0 and UN: the decrease of
+    sum_k h_k(298.15 K) rho Y_k integrated over the domain (N2 by closure)."""
+    hf = formation_enthalpy(mech, U0)
+
+    def chem(U):
+        rY = U[4:]
+        rN2 = U[0] - rY.sum(0)
+        return (hf[:-1].reshape(-1, 1, 1) * rY).sum(0) + hf[-1] * rN2
+
+    return (chem(U0) - chem(UN)).sum() * dx * dy
+
+
+def pressure(mech, U, aux):
+    """p = rho R T (aux_func.py:39-51)."""
+    invM, _, _ = _consts(mech, U)
+    Y = U[4:] / U[0]
+    Y9 = torch.cat([Y, 1.0 - Y.sum(0, keepdim=True)], 0)
+    R = (invM.reshape(-1, 1, 1) * Y9).sum(0)
+    return U[0] * R * aux[1]

@@ -86,17 +86,17 @@ TUBE_DX = 1.25e-5
 
 
 def detonation_tube(mech, nx=400, ny=80, dx=TUBE_DX, seed=1, driver=20,
-                    p_drv=30 * P_ATM, T_drv=3000.0, phi=1.0, rho_noise=0.01):
+                    p_drv=30 * P_ATM, T_drv=3000.0, phi=1.0, rho_noise=0.01, p0=P_ATM):
     """Config 1 (and the tube of config 3 at other sizes / phi fields).
 
-    Stoichiometric H2-air at 1 atm / 300 K at rest; ``driver`` columns at
+    Stoichiometric H2-air at p0 (1 atm) / 300 K at rest; ``driver`` columns at
     p_drv / T_drv with the unburned composition; rho *= 1 +- rho_noise with T
     held (so p = rho R T changes).  BCs: slip left, zero-gradient right, slip
     bottom and top.
     """
     rng = np.random.default_rng(seed)
     T = np.full((nx, ny), 300.0)
-    p = np.full((nx, ny), P_ATM)
+    p = np.full((nx, ny), float(p0))
     T[:driver] = T_drv
     p[:driver] = p_drv
     phi = np.broadcast_to(np.asarray(phi, dtype=np.float64), (nx, ny))
@@ -107,6 +107,26 @@ def detonation_tube(mech, nx=400, ny=80, dx=TUBE_DX, seed=1, driver=20,
     bc = {"left_boundary": SLIP, "right_boundary": "zero_gradient",
           "bottom_boundary": SLIP, "top_boundary": SLIP}
     return U, aux, dict(dx=dx, dy=dx, boundary_config=bc)
+
+
+BATCH_SIZES = ((128, 32), (256, 64), (512, 128), (1024, 256))
+
+
+def batch_cases(mech, ncase=64, seed=5, sizes=BATCH_SIZES):
+    """Config 5: ``ncase`` independent detonation tubes whose grid is drawn
+    from ``sizes``, equivalence ratio from U[0.6, 1.4], initial pressure from
+    U[0.5, 3] atm and driver temperature from U[2000, 3200] K (driver at
+    30 atm over nx/20 columns).  Returns a list of (U, aux, geometry)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(ncase):
+        nx, ny = sizes[int(rng.integers(0, len(sizes)))]
+        phi = float(rng.uniform(0.6, 1.4))
+        p0 = float(rng.uniform(0.5, 3.0)) * P_ATM
+        T_drv = float(rng.uniform(2000.0, 3200.0))
+        out.append(detonation_tube(mech, nx=nx, ny=ny, seed=seed * 1000 + k,
+                                   driver=max(4, nx // 20), T_drv=T_drv, phi=phi, p0=p0))
+    return out
 
 
 def rdc_annulus(mech, nx=4000, ny=400, dx=1e-4, seed=2, seed_cols=200,

@@ -3,6 +3,7 @@
 // the reference entry points each call replaces are in include/janc_step.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -10,6 +11,7 @@
 
 #include "../../include/janc_step.h"
 #include "adjoint.hpp"
+#include "batch.hpp"
 #include "stage.hpp"
 
 using namespace jh;
@@ -378,6 +380,193 @@ __global__ void copy_scalars_kernel(const StepScalars* sc, int parity, double cf
   }
 }
 
+
+// ------------------------------------------------------------------ batch
+
+struct jh_batch {
+  int ncase = 0;
+  int precision = JH_PREC_FP64;
+  std::vector<int> nx, ny, bc;        // bc: 4 per case
+  std::vector<double> dx, dy, bcp;    // bcp: 4 per case
+  hipStream_t user = nullptr;         // the caller's stream
+  hipStream_t own = nullptr;          // launches and graph capture
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  MechConst* mc = nullptr;
+  StepScalars* sc = nullptr;          // one per case
+  void* bufs = nullptr;               // stage buffers and ghost strips of all cases
+  std::vector<size_t> buf_off;        // per case: byte offset of its block in bufs
+  void* desc = nullptr;               // device CaseDesc[ncase]
+  int* off = nullptr;                 // device block tables
+  std::vector<int> h_off;
+  int t_stage[3] = {0, 0, 0}, t_bc = 0, t_red = 0, t_chem = 0;   // table starts in off
+  int persist[3] = {0, 0, 0};         // persistent grid of each stage variant
+  int tile = 0;
+  std::vector<void*> U, aux;
+  bool bound = false;
+  unsigned gen = 0;                   // bumped by every bind
+  hipGraphExec_t gexec = nullptr;
+  struct Key {
+    int nsteps = -1;
+    double dt = 0, cfl = 0;
+    double* hist = nullptr;
+    int tile = -1;
+    unsigned gen = 0;
+  } key;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_batch_err;
+
+int bfail(jh_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg; else g_batch_err = msg;
+  return code;
+}
+
+#define BHIP_TRY(b, call)                                                      \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return bfail(b, JH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Per-case buffer block: U1, U2 (NU planes), A1, A2 (2 planes), 4 strips.
+size_t case_bytes(size_t es, int nx, int ny) {
+  const size_t n = (size_t)nx * ny;
+  size_t t = 2 * align256(es * NU * n) + 2 * align256(es * 2 * n);
+  for (int s = 0; s < 4; ++s) t += align256(es * NV * 3 * (s < 2 ? ny : nx));
+  return t;
+}
+
+template <typename Real>
+void fill_desc(const jh_batch* b, int k, CaseDesc<Real>& d) {
+  std::memset(&d, 0, sizeof(d));
+  char* base = static_cast<char*>(b->bufs) + b->buf_off[k];
+  const size_t es = sizeof(Real), n = (size_t)b->nx[k] * b->ny[k];
+  StepArgs<Real>& a = d.a;
+  a.nx = b->nx[k];
+  a.ny = b->ny[k];
+  a.dx = b->dx[k];
+  a.dy = b->dy[k];
+  a.mc = b->mc;
+  a.sc = b->sc + k;
+  d.U1 = reinterpret_cast<Real*>(base);
+  base += align256(es * NU * n);
+  d.U2 = reinterpret_cast<Real*>(base);
+  base += align256(es * NU * n);
+  d.A1 = reinterpret_cast<Real*>(base);
+  base += align256(es * 2 * n);
+  d.A2 = reinterpret_cast<Real*>(base);
+  base += align256(es * 2 * n);
+  for (int s = 0; s < 4; ++s) {
+    a.g[s] = reinterpret_cast<Real*>(base);
+    base += align256(es * NV * 3 * (s < 2 ? b->ny[k] : b->nx[k]));
+    a.bc[s] = b->bc[4 * k + s];
+    a.bc_p[s] = b->bcp[4 * k + s];
+  }
+  d.U = b->bound ? static_cast<Real*>(b->U[k]) : nullptr;
+  d.aux = b->bound ? static_cast<Real*>(b->aux[k]) : nullptr;
+}
+
+template <typename Real>
+int upload_desc(jh_batch* b) {
+  std::vector<CaseDesc<Real>> h(b->ncase);
+  for (int k = 0; k < b->ncase; ++k) fill_desc<Real>(b, k, h[k]);
+  BHIP_TRY(b, hipStreamSynchronize(b->own));
+  BHIP_TRY(b, hipMemcpy(b->desc, h.data(), sizeof(CaseDesc<Real>) * b->ncase,
+                        hipMemcpyHostToDevice));
+  return JH_OK;
+}
+
+template <typename Real, int TX, int WPE>
+int persistent_grid(jh_batch* b, int v, int cus) {
+  int per_cu = 0;
+  BHIP_TRY(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                  &per_cu, stage_batch_kernel<Real, TX, WPE>, Tile<TX>::NT, 0));
+  if (per_cu < 1) per_cu = 1;
+  int g = (per_cu * cus) & ~7;
+  if (g < 8) g = 8;
+  b->persist[v] = g;
+  return JH_OK;
+}
+
+template <typename Real>
+int setup_grids(jh_batch* b) {
+  int dev = 0, cus = 0;
+  BHIP_TRY(b, hipGetDevice(&dev));
+  BHIP_TRY(b, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  int rc = persistent_grid<Real, 8, 3>(b, 0, cus);
+  if (!rc) rc = persistent_grid<Real, 8, 4>(b, 1, cus);
+  if (!rc) rc = persistent_grid<Real, 16, 4>(b, 2, cus);
+  return rc;
+}
+
+template <typename Real>
+void launch_stage_batch(jh_batch* b, hipStream_t st, int stage, int parity, double dt,
+                        double cfl) {
+  const auto* d = static_cast<const CaseDesc<Real>*>(b->desc);
+  const int v = b->tile;
+  const int total = b->h_off[b->t_stage[v] + b->ncase];
+  const int g = std::min(b->persist[v], total);
+  const int* off = b->off + b->t_stage[v];
+  switch (v) {
+    case 1:
+      stage_batch_kernel<Real, 8, 4><<<g, Tile<8>::NT, 0, st>>>(d, off, b->ncase, stage, parity,
+                                                                dt, cfl);
+      break;
+    case 2:
+      stage_batch_kernel<Real, 16, 4><<<g, Tile<16>::NT, 0, st>>>(d, off, b->ncase, stage,
+                                                                  parity, dt, cfl);
+      break;
+    default:
+      stage_batch_kernel<Real, 8, 3><<<g, Tile<8>::NT, 0, st>>>(d, off, b->ncase, stage, parity,
+                                                                dt, cfl);
+  }
+}
+
+// The launch sequence of nsteps batched steps (also what a graph captures).
+template <typename Real>
+int batch_enqueue(jh_batch* b, hipStream_t st, double dt, double cfl, int nsteps,
+                  double* hist) {
+  const auto* d = static_cast<const CaseDesc<Real>*>(b->desc);
+  const int nc = b->ncase;
+  const int n_red = b->h_off[b->t_red + nc], n_bc = b->h_off[b->t_bc + nc];
+  const int n_chem = b->h_off[b->t_chem + nc];
+  reset_batch_kernel<Real><<<(nc + 63) / 64, 64, 0, st>>>(d, nc, 0);
+  reduce_batch_kernel<Real><<<n_red, 256, 0, st>>>(d, b->off + b->t_red, nc, 0, dt, cfl);
+  for (int s = 0; s < nsteps; ++s) {
+    const int parity = s & 1;
+    for (int stage = 0; stage < 3; ++stage) {
+      bc_batch_kernel<Real><<<n_bc, 256, 0, st>>>(d, b->off + b->t_bc, nc, stage, parity, dt,
+                                                  cfl);
+      launch_stage_batch<Real>(b, st, stage, parity, dt, cfl);
+    }
+    chem_batch_kernel<Real><<<n_chem, 256, 0, st>>>(d, b->off + b->t_chem, nc, parity, dt, cfl,
+                                                     hist, nsteps, s);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return bfail(b, JH_ERR_HIP, std::string("batch kernels: ") +
+                                                       hipGetErrorString(e));
+  return JH_OK;
+}
+
+int batch_enqueue_any(jh_batch* b, hipStream_t st, double dt, double cfl, int nsteps,
+                      double* hist) {
+  return b->precision == JH_PREC_FP64 ? batch_enqueue<double>(b, st, dt, cfl, nsteps, hist)
+                                      : batch_enqueue<float>(b, st, dt, cfl, nsteps, hist);
+}
+
+void drop_graph(jh_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  b->key = jh_batch::Key();
+}
+
+}  // namespace
+
 extern "C" {
 
 int jh_mech_const_count(void) { return (int)(sizeof(MechConst) / sizeof(double)); }
@@ -642,6 +831,212 @@ int jh_health(jh_ctx* c, unsigned int* flags) {
                             c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return JH_OK;
+}
+
+// ------------------------------------------------------------------ batch API
+
+int jh_batch_create(int ncase, const int* nx, const int* ny, const double* dx, const double* dy,
+                    const int* bc_kind, const double* bc_param, int precision,
+                    const double* mech, int n_mech, void* stream, jh_batch** out) {
+  if (!out) return bfail(nullptr, JH_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (ncase < 1) return bfail(nullptr, JH_ERR_ARG, "ncase must be >= 1");
+  if (!nx || !ny || !dx || !dy || !bc_kind || !mech)
+    return bfail(nullptr, JH_ERR_ARG, "NULL argument");
+  if (precision != JH_PREC_FP64 && precision != JH_PREC_MIXED)
+    return bfail(nullptr, JH_ERR_ARG, "unknown precision");
+  if (n_mech != jh_mech_const_count())
+    return bfail(nullptr, JH_ERR_MECH, "mechanism constant count mismatch");
+  for (int k = 0; k < ncase; ++k) {
+    if (nx[k] < 4 || ny[k] < 4) return bfail(nullptr, JH_ERR_ARG, "nx and ny must be >= 4");
+    if ((long long)nx[k] * ny[k] * NU >= (1ll << 31))
+      return bfail(nullptr, JH_ERR_ARG, "grid too large for 32-bit plane offsets");
+    if (!(dx[k] > 0) || !(dy[k] > 0)) return bfail(nullptr, JH_ERR_ARG, "dx, dy must be > 0");
+    for (int s = 0; s < 4; ++s) {
+      const int kind = bc_kind[4 * k + s];
+      if (kind < JH_BC_PERIODIC || kind >= JH_BC_USER)
+        return bfail(nullptr, JH_ERR_ARG, "batched cases take built-in boundary kinds only");
+    }
+  }
+  jh_batch* b = new jh_batch();
+  b->ncase = ncase;
+  b->precision = precision;
+  b->user = static_cast<hipStream_t>(stream);
+  b->nx.assign(nx, nx + ncase);
+  b->ny.assign(ny, ny + ncase);
+  b->dx.assign(dx, dx + ncase);
+  b->dy.assign(dy, dy + ncase);
+  b->bc.assign(bc_kind, bc_kind + 4 * ncase);
+  b->bcp.assign(4 * ncase, 0.0);
+  if (bc_param) b->bcp.assign(bc_param, bc_param + 4 * ncase);
+  b->U.assign(ncase, nullptr);
+  b->aux.assign(ncase, nullptr);
+  // block tables: stage tiles per variant, bc lines, reduce, chemistry cells
+  const int stride = ncase + 1;
+  b->h_off.assign(6 * stride, 0);
+  for (int v = 0; v < 3; ++v) b->t_stage[v] = v * stride;
+  b->t_bc = 3 * stride;
+  b->t_red = 4 * stride;
+  b->t_chem = 5 * stride;
+  for (int k = 0; k < ncase; ++k) {
+    const long long n = (long long)nx[k] * ny[k];
+    for (int v = 0; v < 3; ++v) {
+      const int tx = kTileRows[v];
+      b->h_off[b->t_stage[v] + k + 1] =
+          b->h_off[b->t_stage[v] + k] + ((nx[k] + tx - 1) / tx) * ((ny[k] + TY - 1) / TY);
+    }
+    b->h_off[b->t_bc + k + 1] = b->h_off[b->t_bc + k] + (2 * nx[k] + 2 * ny[k] + 255) / 256;
+    const int cb = (int)((n + 255) / 256);
+    b->h_off[b->t_red + k + 1] = b->h_off[b->t_red + k] + std::min(cb, 64);
+    b->h_off[b->t_chem + k + 1] = b->h_off[b->t_chem + k] + cb;
+  }
+  const size_t es = precision == JH_PREC_FP64 ? 8 : 4;
+  size_t total = 0;
+  b->buf_off.resize(ncase);
+  for (int k = 0; k < ncase; ++k) {
+    b->buf_off[k] = total;
+    total += case_bytes(es, nx[k], ny[k]);
+  }
+  const size_t desc_bytes = precision == JH_PREC_FP64 ? sizeof(CaseDesc<double>) * ncase
+                                                      : sizeof(CaseDesc<float>) * ncase;
+  hipError_t e = hipStreamCreateWithFlags(&b->own, hipStreamNonBlocking);
+  if (!e) e = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming);
+  if (!e) e = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming);
+  if (!e) e = hipMalloc(&b->mc, sizeof(MechConst));
+  if (!e) e = hipMalloc(&b->sc, sizeof(StepScalars) * ncase);
+  if (!e) e = hipMalloc(&b->bufs, total);
+  if (!e) e = hipMalloc(&b->desc, desc_bytes);
+  if (!e) e = hipMalloc(&b->off, sizeof(int) * b->h_off.size());
+  if (!e) e = hipMemcpy(b->mc, mech, sizeof(MechConst), hipMemcpyHostToDevice);
+  if (!e) e = hipMemset(b->sc, 0, sizeof(StepScalars) * ncase);
+  if (!e) e = hipMemset(b->bufs, 0, total);
+  if (!e)
+    e = hipMemcpy(b->off, b->h_off.data(), sizeof(int) * b->h_off.size(), hipMemcpyHostToDevice);
+  if (e) {
+    g_batch_err = std::string("allocation: ") + hipGetErrorString(e);
+    jh_batch_destroy(b);
+    return JH_ERR_HIP;
+  }
+  int rc = precision == JH_PREC_FP64 ? setup_grids<double>(b) : setup_grids<float>(b);
+  if (!rc) rc = precision == JH_PREC_FP64 ? upload_desc<double>(b) : upload_desc<float>(b);
+  if (rc) {
+    g_batch_err = b->err;
+    jh_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return JH_OK;
+}
+
+int jh_batch_destroy(jh_batch* b) {
+  if (!b) return JH_OK;
+  if (b->own) (void)hipStreamSynchronize(b->own);
+  drop_graph(b);
+  (void)hipFree(b->mc);
+  (void)hipFree(b->sc);
+  (void)hipFree(b->bufs);
+  (void)hipFree(b->desc);
+  (void)hipFree(b->off);
+  if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+  if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+  if (b->own) (void)hipStreamDestroy(b->own);
+  delete b;
+  return JH_OK;
+}
+
+int jh_batch_bind(jh_batch* b, void* const* U, void* const* aux) {
+  if (!b) return JH_ERR_ARG;
+  if (!U || !aux) return bfail(b, JH_ERR_ARG, "NULL pointer array");
+  for (int k = 0; k < b->ncase; ++k)
+    if (!U[k] || !aux[k]) return bfail(b, JH_ERR_ARG, "NULL state pointer");
+  b->U.assign(U, U + b->ncase);
+  b->aux.assign(aux, aux + b->ncase);
+  b->bound = true;
+  ++b->gen;
+  return b->precision == JH_PREC_FP64 ? upload_desc<double>(b) : upload_desc<float>(b);
+}
+
+int jh_batch_set_stream(jh_batch* b, void* stream) {
+  if (!b) return JH_ERR_ARG;
+  b->user = static_cast<hipStream_t>(stream);
+  return JH_OK;
+}
+
+int jh_batch_set_tile(jh_batch* b, int variant) {
+  if (!b) return JH_ERR_ARG;
+  if (variant < 0 || variant >= jh_tile_variants())
+    return bfail(b, JH_ERR_ARG, "unknown tile variant");
+  b->tile = variant;
+  return JH_OK;
+}
+
+int jh_batch_get_tile(const jh_batch* b) { return b ? b->tile : -1; }
+
+int jh_batch_advance(jh_batch* b, double dt, double cfl, int nsteps, double* dt_hist,
+                     int use_graph) {
+  if (!b) return JH_ERR_ARG;
+  if (!b->bound) return bfail(b, JH_ERR_ARG, "no state bound (jh_batch_bind)");
+  if (nsteps < 1) return bfail(b, JH_ERR_ARG, "nsteps must be >= 1");
+  if (!(dt > 0) && !(cfl > 0)) return bfail(b, JH_ERR_ARG, "need dt > 0 or cfl > 0");
+  if (dt > 0) cfl = 0.0;
+  else dt = 0.0;
+  BHIP_TRY(b, hipEventRecord(b->ev_in, b->user));
+  BHIP_TRY(b, hipStreamWaitEvent(b->own, b->ev_in, 0));
+  if (use_graph) {
+    const jh_batch::Key& k = b->key;
+    if (!(b->gexec && k.nsteps == nsteps && k.dt == dt && k.cfl == cfl && k.hist == dt_hist &&
+          k.tile == b->tile && k.gen == b->gen)) {
+      drop_graph(b);
+      hipGraph_t g = nullptr;
+      BHIP_TRY(b, hipStreamBeginCapture(b->own, hipStreamCaptureModeThreadLocal));
+      const int rc = batch_enqueue_any(b, b->own, dt, cfl, nsteps, dt_hist);
+      const hipError_t e = hipStreamEndCapture(b->own, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess)
+        return bfail(b, JH_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      const hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) {
+        b->gexec = nullptr;
+        return bfail(b, JH_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+      }
+      b->key.nsteps = nsteps;
+      b->key.dt = dt;
+      b->key.cfl = cfl;
+      b->key.hist = dt_hist;
+      b->key.tile = b->tile;
+      b->key.gen = b->gen;
+    }
+    BHIP_TRY(b, hipGraphLaunch(b->gexec, b->own));
+  } else {
+    const int rc = batch_enqueue_any(b, b->own, dt, cfl, nsteps, dt_hist);
+    if (rc) return rc;
+  }
+  BHIP_TRY(b, hipEventRecord(b->ev_out, b->own));
+  BHIP_TRY(b, hipStreamWaitEvent(b->user, b->ev_out, 0));
+  return JH_OK;
+}
+
+int jh_batch_health(jh_batch* b, unsigned int* flags) {
+  if (!b || !flags) return JH_ERR_ARG;
+  std::vector<StepScalars> h(b->ncase);
+  BHIP_TRY(b, hipStreamSynchronize(b->own));
+  BHIP_TRY(b, hipMemcpy(h.data(), b->sc, sizeof(StepScalars) * b->ncase, hipMemcpyDeviceToHost));
+  unsigned int f = 0;
+  for (int k = 0; k < b->ncase; ++k) {
+    f |= h[k].flags;
+    h[k].flags = 0;
+  }
+  BHIP_TRY(b, hipMemcpy(b->sc, h.data(), sizeof(StepScalars) * b->ncase, hipMemcpyHostToDevice));
+  *flags = f;
+  return JH_OK;
+}
+
+const char* jh_batch_last_error(const jh_batch* b) {
+  return b ? b->err.c_str() : g_batch_err.c_str();
 }
 
 }  // extern "C"

@@ -135,17 +135,17 @@ __global__ void reset_kernel(StepScalars* sc, int parity) {
 
 // Interior part of the stage-1 LF maxima and the CFL maximum of a state
 // handed in by the caller (set_simulator.py:438-443, LF.py:10-11).
+// Block blk of nblk (grid-stride over the cells).
 template <typename Real>
-__global__ __launch_bounds__(256) void reduce_kernel(StepArgs<Real> a, const Real* __restrict__ U,
-                                                     const Real* __restrict__ aux, int slot_parity,
-                                                     int with_cfl) {
-  __shared__ double red[4];
-  __shared__ unsigned long long redb[4];
+__device__ __forceinline__ void reduce_body(const StepArgs<Real>& a, const Real* __restrict__ U,
+                                            const Real* __restrict__ aux, int slot_parity,
+                                            int with_cfl, int blk, int nblk, double* red,
+                                            unsigned long long* redb) {
   const MechConst& c = *a.mc;
   const int n = a.nx * a.ny;
   double mx = 0.0, my = 0.0;
   unsigned long long cm = 0ull;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (int i = blk * blockDim.x + threadIdx.x; i < n; i += nblk * blockDim.x) {
     const Prim<double> q = cell_prim<double, true>(c, U + i, aux + i, n);
     const double sx = fabs(q.u) + q.a, sy = fabs(q.v) + q.a;
     mx = fmax(mx, nan_to_zero(sx));
@@ -158,26 +158,35 @@ __global__ __launch_bounds__(256) void reduce_kernel(StepArgs<Real> a, const Rea
   if (with_cfl) block_max_bits_atomic<256>(cm, &a.sc->cmax[slot_parity], redb);
 }
 
+template <typename Real>
+__global__ __launch_bounds__(256) void reduce_kernel(StepArgs<Real> a, const Real* __restrict__ U,
+                                                     const Real* __restrict__ aux, int slot_parity,
+                                                     int with_cfl) {
+  __shared__ double red[4];
+  __shared__ unsigned long long redb[4];
+  reduce_body<Real>(a, U, aux, slot_parity, with_cfl, blockIdx.x, gridDim.x, red, redb);
+}
+
 // ------------------------------------------------------------ ghost cells
 
 // Ghost strips for one RK stage (boundary.py:126-132 with the BC kinds of
 // boundary_conditions_2D/*.py and the periodic pre-pad of
 // boundary_padding.py:20-25), plus the ghost cells' share of the LF maxima.
 // One thread per boundary-normal line: threads [0, ny) left, [ny, 2ny) right,
-// [2ny, 2ny+nx) bottom, [2ny+nx, 2ny+2nx) top.
+// [2ny, 2ny+nx) bottom, [2ny+nx, 2ny+2nx) top.  blk = block index.
 template <typename Real>
-__global__ __launch_bounds__(256) void bc_kernel(StepArgs<Real> a, const Real* __restrict__ U,
-                                                 const Real* __restrict__ aux, int stage) {
-  __shared__ double red[4];
+__device__ __forceinline__ void bc_body(const StepArgs<Real>& a, const Real* __restrict__ U,
+                                        const Real* __restrict__ aux, int stage, int blk,
+                                        double* red) {
   const MechConst& c = *a.mc;
   const int nx = a.nx, ny = a.ny, n = nx * ny;
-  if (stage == 0 && blockIdx.x == 0 && threadIdx.x < 7) {
+  if (stage == 0 && blk == 0 && threadIdx.x < 7) {
     // the other parity's slots are written from here on in this step
     const int o = a.parity ^ 1;
     if (threadIdx.x < 6) (&a.sc->amax[o][0][0])[threadIdx.x] = 0ull;
     else a.sc->cmax[o] = 0ull;
   }
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = blk * blockDim.x + threadIdx.x;
   double mx = 0.0, my = 0.0;
   int side = -1, pos = 0;
   if (tid < ny) { side = JH_LEFT; pos = tid; }
@@ -263,6 +272,13 @@ __global__ __launch_bounds__(256) void bc_kernel(StepArgs<Real> a, const Real* _
   }
   block_max_atomic<256>(mx, &a.sc->amax[a.parity][stage][0], red);
   block_max_atomic<256>(my, &a.sc->amax[a.parity][stage][1], red);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void bc_kernel(StepArgs<Real> a, const Real* __restrict__ U,
+                                                 const Real* __restrict__ aux, int stage) {
+  __shared__ double red[4];
+  bc_body<Real>(a, U, aux, stage, blockIdx.x, red);
 }
 
 // ------------------------------------------------------------ stage kernel
@@ -370,27 +386,39 @@ __device__ __forceinline__ W phys_flux(int cv, int dir, const Prim<W>& q, W ucv)
 // Region cells of thread t (wave w, lane ly): slots 0,1 = interior rows w and
 // w+NW; slots 2..2+NHS-1 = x-halo rows {0,1,2,TX+3,TX+4,TX+5}[w + h*NW];
 // last slot = y-halo cell t (t < 6*TX).
-template <typename Real, int MODE, int TX, int WPE>
-__global__ __launch_bounds__(Tile<TX>::NT) __attribute__((amdgpu_waves_per_eu(WPE)))
-void stage_kernel(StepArgs<Real> a, int stage,
-                                                             const Real* __restrict__ Us,
-                                                             const Real* __restrict__ auxs,
-                                                             const Real* Un, Real* Uo,
-                                                             Real* __restrict__ auxo) {
+// LDS of one stage-kernel block.
+template <typename Real, int TX>
+struct StageLds {
+  Real Hxp[TX + 6][TY], Hxm[TX + 6][TY];
+  Real Hyp[TX][TY + 6], Hym[TX][TY + 6];
+  Real Fx[TX + 1][TY];
+  Real Fy[TX][TY + 1];
+  double red[Tile<TX>::NW];
+};
+
+// One tile (index `tile` in y-fastest order) of one RK stage.
+template <typename Real, int MODE, int TX>
+__device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
+                                           const Real* __restrict__ Us,
+                                           const Real* __restrict__ auxs, const Real* Un,
+                                           Real* Uo, Real* __restrict__ auxo, int tile,
+                                           StageLds<Real, TX>& L) {
   using G = Tile<TX>;
-  constexpr int NT = G::NT, NW = G::NW, RX = G::RX, RY = G::RY;
+  constexpr int NT = G::NT, NW = G::NW;
   constexpr int NXF = G::NXF, NYF = G::NYF, NHS = G::NHS, NSL = G::NSL;
-  __shared__ Real sHxp[RX][TY], sHxm[RX][TY];
-  __shared__ Real sHyp[TX][RY], sHym[TX][RY];
-  __shared__ Real sFx[TX + 1][TY];
-  __shared__ Real sFy[TX][TY + 1];
-  __shared__ double red[NW];
+  auto& sHxp = L.Hxp;
+  auto& sHxm = L.Hxm;
+  auto& sHyp = L.Hyp;
+  auto& sHym = L.Hym;
+  auto& sFx = L.Fx;
+  auto& sFy = L.Fy;
+  double* red = L.red;
+  (void)NW;
 
   const MechConst& c = *a.mc;
   const int nx = a.nx, ny = a.ny, n = nx * ny;
   const int t = threadIdx.x, ly = t & 63, w = t >> 6;
   const int tiles_y = (ny + TY - 1) / TY;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int x0 = (tile / tiles_y) * TX;
   const int y0 = (tile % tiles_y) * TY;
   const double dt = step_dt(a);
@@ -563,6 +591,15 @@ void stage_kernel(StepArgs<Real> a, int stage,
   }
 }
 
+template <typename Real, int MODE, int TX, int WPE>
+__global__ __launch_bounds__(Tile<TX>::NT) __attribute__((amdgpu_waves_per_eu(WPE)))
+void stage_kernel(StepArgs<Real> a, int stage, const Real* __restrict__ Us,
+                  const Real* __restrict__ auxs, const Real* Un, Real* Uo,
+                  Real* __restrict__ auxo) {
+  __shared__ StageLds<Real, TX> L;
+  stage_body<Real, MODE, TX>(a, stage, Us, auxs, Un, Uo, auxo, xcd_tile(blockIdx.x, gridDim.x), L);
+}
+
 // ---------------------------------------------------------- chemistry
 
 enum ChemMode : int { CHEM_STEP = 0, CHEM_SOURCE = 1 };
@@ -572,14 +609,13 @@ enum ChemMode : int { CHEM_STEP = 0, CHEM_SOURCE = 1 };
 // plus the next step's CFL and stage-1 LF maxima.  CHEM_SOURCE writes the
 // source S (12 planes, first 4 zero) instead.
 template <typename Real, int MODE>
-__global__ __launch_bounds__(256) void chem_kernel(StepArgs<Real> a, Real* U, Real* aux,
-                                                   Real* S, double* dt_hist) {
-  __shared__ double red[4];
-  __shared__ unsigned long long redb[4];
+__device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real* aux, Real* S,
+                                          double* dt_hist, int blk, double* red,
+                                          unsigned long long* redb) {
   const MechConst& c = *a.mc;
   const int n = a.nx * a.ny;
   const double dt = step_dt(a);
-  if (MODE == CHEM_STEP && blockIdx.x == 0 && threadIdx.x == 0) {
+  if (MODE == CHEM_STEP && blk == 0 && threadIdx.x == 0) {
     a.sc->dt_last = dt;
     if (dt_hist) *dt_hist = dt;
   }
@@ -587,7 +623,7 @@ __global__ __launch_bounds__(256) void chem_kernel(StepArgs<Real> a, Real* U, Re
   unsigned long long cm = 0ull;
   // one cell per thread: a grid-stride loop would let the compiler keep every
   // mechanism constant in registers across iterations
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blk * blockDim.x + threadIdx.x;
   if (i < n) {
     double Uc[NU];
 #pragma unroll
@@ -644,6 +680,14 @@ __global__ __launch_bounds__(256) void chem_kernel(StepArgs<Real> a, Real* U, Re
     block_max_atomic<256>(my, &a.sc->amax[o][0][1], red);
     block_max_bits_atomic<256>(cm, &a.sc->cmax[o], redb);
   }
+}
+
+template <typename Real, int MODE>
+__global__ __launch_bounds__(256) void chem_kernel(StepArgs<Real> a, Real* U, Real* aux,
+                                                   Real* S, double* dt_hist) {
+  __shared__ double red[4];
+  __shared__ unsigned long long redb[4];
+  chem_body<Real, MODE>(a, U, aux, S, dt_hist, blockIdx.x, red, redb);
 }
 
 // update_aux alone (aux_func.py:21-29), for the unit entry point.

@@ -148,3 +148,97 @@ def test_step_vjp_rejects_unsupported(mech_path, mech):
     sim = Simulator(cases.simulation_config(g, mech_path, cfl=0.3, dt=DT, precision="mixed"))
     with pytest.raises(NotImplementedError):
         sim.step_vjp(U.astype(np.float32), aux.astype(np.float32), DT, W, Wa)
+
+
+# ------------------------------------------------- gradients w.r.t. T and Y
+def _tY_case(mech, nx, ny, seed=9):
+    """Tube with 0.5 % temperature noise (so the LF maxima have single
+    maximisers) as (T, p, Y) fields."""
+    U, aux, g = cases.detonation_tube(mech, nx=nx, ny=ny, driver=max(4, nx // 8), seed=seed)
+    rng = np.random.default_rng(seed)
+    T = aux[1] * (1.0 + 0.005 * rng.uniform(-1, 1, aux[1].shape))
+    rho = U[0]
+    Y = U[4:] / rho
+    Y9 = np.concatenate([Y, (1.0 - Y.sum(0))[None]])
+    R, _, _, _ = cases.mixture(mech, aux[1], Y9)
+    p = rho * R * aux[1]
+    return T, p, Y, g
+
+
+def _loss(mech, U0, UN, aN, dx, dy, p_target, q_scale, p_ref):
+    from janc_amd import adjoint
+    q = adjoint.heat_release(mech, U0, UN, dx, dy) / q_scale
+    pN = adjoint.pressure(mech, UN, aN)
+    return q + (((pN - p_target) / p_ref) ** 2).mean()
+
+
+def test_initial_field_gradient_matches_twin(mech_path, mech):
+    """d loss / d (T, Y) through the GPU rollout vs autograd of the torch twin."""
+    from janc_amd import adjoint
+    from janc_amd.solver import Simulator
+    T, p, Y, g = _tY_case(mech, 40, 16)
+    steps, dt = 3, 2e-9
+    sim = Simulator(cases.simulation_config(g, mech_path, dt=dt))
+    to = TorchOracle(mech_path, g["boundary_config"], g["dx"], g["dy"])
+    rng = np.random.default_rng(1)
+    p_target = p * (1.0 + 0.01 * rng.standard_normal(p.shape))
+    grads = []
+    for dev in ("cuda", "cpu"):
+        Tt = torch.tensor(T, device=dev, requires_grad=True)
+        Yt = torch.tensor(Y, device=dev, requires_grad=True)
+        U0, a0 = adjoint.initial_state(mech, Tt, torch.tensor(p, device=dev), Yt)
+        if dev == "cuda":
+            UN, aN = adjoint.rollout(sim, U0, a0, dt, steps, checkpoint_every=2)
+        else:
+            UN, aN = U0, a0
+            for _ in range(steps):
+                UN, aN = to.advance_one_step(UN, aN, dt)
+        L = _loss(mech, U0, UN, aN, g["dx"], g["dy"], torch.tensor(p_target, device=dev),
+                  1.0, 101325.0)
+        L.backward()
+        grads.append((Tt.grad.cpu(), Yt.grad.cpu()))
+    eT = rel_err(grads[0][0][None], grads[1][0][None]).max()
+    eY = rel_err(grads[0][1], grads[1][1]).max()
+    print("dL/dT rel err", eT, "dL/dY rel err", eY, flush=True)
+    assert eT <= TWIN_TOL and eY <= TWIN_TOL
+
+
+def test_initial_field_gradient_matches_fd(mech_path, mech):
+    """Central differences (eps 1e-6) on random directions in (T, Y) space
+    through a checkpointed multi-step rollout."""
+    from janc_amd import adjoint
+    from janc_amd.solver import Simulator
+    T, p, Y, g = _tY_case(mech, 96, 24)
+    probe = Simulator(cases.simulation_config(g, mech_path, cfl=0.3))
+    steps = 12
+    Tt = torch.tensor(T, device="cuda")
+    Yt = torch.tensor(Y, device="cuda")
+    pt = torch.tensor(p, device="cuda")
+    U0, a0 = adjoint.initial_state(mech, Tt, pt, Yt)
+    dt = 0.8 * float(probe.CFL(U0, a0))
+    sim = Simulator(cases.simulation_config(g, mech_path, dt=dt))
+    rng = np.random.default_rng(2)
+    p_target = pt * (1.0 + 0.01 * torch.tensor(rng.standard_normal(p.shape), device="cuda"))
+    UN, aN = sim.advance(U0.clone(), a0.clone(), steps)[:2]
+    q_scale = float(adjoint.heat_release(mech, U0, UN, g["dx"], g["dy"]).abs()) + 1e-30
+
+    def L(Tx, Yx, grad=False):
+        U0, a0 = adjoint.initial_state(mech, Tx, pt, Yx)
+        if grad:
+            UN, aN = adjoint.rollout(sim, U0, a0, dt, steps, checkpoint_every=5)
+        else:
+            UN, aN = sim.advance(U0.clone(), a0.clone(), steps)[:2]
+        return _loss(mech, U0, UN, aN, g["dx"], g["dy"], p_target, q_scale, 101325.0)
+
+    Tg = Tt.clone().requires_grad_(True)
+    Yg = Yt.clone().requires_grad_(True)
+    L(Tg, Yg, grad=True).backward()
+    eps = 1e-6
+    for d in range(4):
+        dT = Tt * torch.tensor(rng.standard_normal(T.shape), device="cuda")
+        dY = Yt * torch.tensor(rng.standard_normal(Y.shape), device="cuda")
+        fd = (float(L(Tt + eps * dT, Yt + eps * dY)) -
+              float(L(Tt - eps * dT, Yt - eps * dY))) / (2 * eps)
+        ad = float((Tg.grad * dT).sum() + (Yg.grad * dY).sum())
+        print("direction", d, "fd", fd, "ad", ad, flush=True)
+        assert abs(fd - ad) <= 1e-6 * max(abs(fd), abs(ad))
