@@ -73,7 +73,16 @@ def _cpu_baseline(U, aux, geom, theta, steps):
     from janc_amd.mechanism import DEFAULT_MECH
     crop = (slice(None), slice(0, CPU_CROP[0]), slice(0, CPU_CROP[1]))
     Uc, ac = U[crop].copy(), aux[crop].copy()
-    o = Oracle(DEFAULT_MECH, geom["boundary_config"], geom["dx"], geom["dy"])
+    bcfg = dict(geom["boundary_config"])
+    inlets = [k for k, v in bcfg.items() if v == "injection_inlet"]
+    for k in inlets:
+        bcfg[k] = "slip_wall"
+    o = Oracle(DEFAULT_MECH, bcfg, geom["dx"], geom["dy"])
+    if inlets:
+        from oracle.inlet import attach_inlet
+        for k in inlets:
+            attach_inlet(o, k.replace("_boundary", ""), theta["P0_inj"], theta["T0_inj"],
+                         theta["Yinj"])
     t = 0.0
     o.advance_func(Uc, ac, t, 0.3, theta)          # warm caches
     t0 = time.perf_counter()
@@ -333,8 +342,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "FP64 plain step, 4000x400 periodic-in-x detonation annulus "
                                "(ZND-like seed 200 columns 30 atm/3000 K, 0.5% T noise), "
-                               "9sp-19r H2-air, CFL 0.3; bottom slip wall in place of the "
-                               "injection inlet",
+                               "9sp-19r H2-air, CFL 0.3; bottom injection inlet (0.2 MPa / "
+                               "300 K manifold), top pressure outlet 1 atm",
                    "nx": nx, "ny": ny, "cells": cells, "precision": "fp64",
                    "dt_first": float(dts[0]), "dt_last": float(dts[-1]),
                    "health_flags": flags},

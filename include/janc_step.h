@@ -41,7 +41,13 @@ enum jh_bc_kind {
   JH_BC_SLIP_WALL = 2,
   JH_BC_NO_SLIP_WALL = 3,
   JH_BC_PRESSURE_OUTLET = 4,
-  JH_BC_USER = 5
+  JH_BC_USER = 5,
+  /* Injection inlet: the pressure-feedback injector boundary of the
+   * reference's rotating-detonation example (examples/
+   * janc_basic_example2.ipynb, bottom_boundary + inj_model, a user callable
+   * there), with the manifold total pressure P0 as the side's bc_param and
+   * T0 / injected composition from jh_ctx_set_inlet(). */
+  JH_BC_INJECTION_INLET = 6
 };
 
 /* Sides in the order bc_kind[] / bc_param[] use. */
@@ -58,7 +64,9 @@ int jh_num_species(void);
 /* Create a context.  Replaces Simulator.__init__'s set_thermo / set_reaction /
  * set_flux_solver / set_boundary (simulation/set_simulator.py:387-396).
  *   bc_kind[4]  : jh_bc_kind per side (left, right, bottom, top)
- *   bc_param[4] : theta['Pb'] for JH_BC_PRESSURE_OUTLET sides, else ignored
+ *   bc_param[4] : theta['Pb'] for JH_BC_PRESSURE_OUTLET sides, the manifold
+ *                 total pressure P0 for JH_BC_INJECTION_INLET sides, else
+ *                 ignored
  *   mech        : host array of jh_mech_const_count() doubles
  *   stream      : hipStream_t (NULL = default stream) */
 int jh_ctx_create(int nx, int ny, double dx, double dy, const int* bc_kind,
@@ -68,6 +76,11 @@ int jh_ctx_destroy(jh_ctx* ctx);
 /* Per-call boundary parameter (theta['Pb'] of pressure_outlet.py:24-66 is a
  * call argument in the reference). */
 int jh_ctx_set_bc_param(jh_ctx* ctx, int side, double value);
+
+/* Manifold total temperature T0 and injected mass fractions Y_inj (the first
+ * n = jh_num_species()-1 species) of JH_BC_INJECTION_INLET sides (theta['Yinj']
+ * of janc_basic_example2.ipynb; the example's nondimensional T0 = 1). */
+int jh_ctx_set_inlet(jh_ctx* ctx, double T0, const double* Y_inj, int n);
 int jh_ctx_set_stream(jh_ctx* ctx, void* stream);
 /* Stage-kernel tile variant: 0 = 64x8 tile / 256 threads / 3 waves per SIMD,
  * 1 = 64x8 / 256 / 4 waves per SIMD (<= 128 VGPRs), 2 = 64x16 / 512 / 4 waves
@@ -126,6 +139,30 @@ int jh_step_vjp(jh_ctx* ctx, const void* U, const void* aux, double dt, const vo
  * (14, 3, nx) likewise).  A JH_BC_USER side must be written here before the
  * stage that reads it (boundary.py:52-57 callables). */
 int jh_ghost_strip(jh_ctx* ctx, int side, void** ptr, int* count);
+/* Copy a caller-computed strip (device, same layout and element type) into a
+ * JH_BC_USER side's ghost strip on the context stream.  Each call that reads
+ * ghost cells (jh_rhs, jh_lf_alpha, jh_step_stage) needs every JH_BC_USER
+ * side written since the previous such call (by this call, or through the
+ * pointer from jh_ghost_strip), else it fails with JH_ERR_ARG. */
+int jh_set_ghost_strip(jh_ctx* ctx, int side, const void* strip);
+
+/* One step in pieces, for boundaries computed by the caller between the RK
+ * stages (the reference calls its boundary callables inside every stage,
+ * boundary.py:127-131 from flux_func, set_simulator.py:44-47):
+ *   jh_step_begin  : the step's CFL and stage-1 LF maxima of (U, aux)
+ *                    (set_simulator.py:438-443);
+ *   jh_step_stage  : ghost fill of the built-in sides + one SSP-RK3 stage
+ *                    (TVD_RK3.py:5-12) with update_aux: stage 0 (U,aux)->(U1,A1),
+ *                    1 (U1,A1)->(U2,A2), 2 (U2,A2)->(Uo,Ao); Un = the step's
+ *                    input U; Uo may alias Un;
+ *   jh_step_finish : chemistry + update_aux in place (set_simulator.py:
+ *                    117-122), dt of the step to *dt_dev if non-NULL.
+ * dt > 0 selects a fixed dt, else the CFL dt with cfl; pass the same
+ * (dt, cfl) to all three. */
+int jh_step_begin(jh_ctx* ctx, const void* U, const void* aux, double dt, double cfl);
+int jh_step_stage(jh_ctx* ctx, int stage, const void* Us, const void* As, const void* Un,
+                  void* Uo, void* Ao, double dt, double cfl);
+int jh_step_finish(jh_ctx* ctx, void* U, void* aux, double dt, double cfl, double* dt_dev);
 
 /* Kernel classes of jh_timing_read(). */
 enum jh_kernel_kind {
@@ -173,6 +210,8 @@ int jh_batch_bind(jh_batch* batch, void* const* U, void* const* aux);
 int jh_batch_set_stream(jh_batch* batch, void* stream);
 /* Stage-kernel variant for every case (as jh_ctx_set_tile). */
 int jh_batch_set_tile(jh_batch* batch, int variant);
+/* jh_ctx_set_inlet for case k of the batch. */
+int jh_batch_set_inlet(jh_batch* batch, int k, double T0, const double* Y_inj, int n);
 int jh_batch_get_tile(const jh_batch* batch);
 /* nsteps steps of every case: fixed dt if dt > 0, else each case's own CFL
  * dt (set_simulator.py:438-447).  dt_hist (device, optional) receives

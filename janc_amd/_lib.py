@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libjanc_amd.so")
 JH_OK, JH_ERR_ARG, JH_ERR_MECH, JH_ERR_HIP = 0, 1, 2, 3
 PREC_FP64, PREC_MIXED = 0, 1
 BC_KINDS = {"periodic": 0, "zero_gradient": 1, "slip_wall": 2, "no_slip_wall": 3,
-            "pressure_outlet": 4, "user": 5}
+            "pressure_outlet": 4, "user": 5, "injection_inlet": 6}
 SIDES = ("left", "right", "bottom", "top")
 KERNEL_KINDS = ("reduce", "bc", "stage", "chem")
 
@@ -28,6 +28,7 @@ SIGNATURES = {
     "jh_ctx_destroy": (_c_int, [_vp]),
     "jh_ctx_set_bc_param": (_c_int, [_vp, _c_int, _c_double]),
     "jh_ctx_set_stream": (_c_int, [_vp, _vp]),
+    "jh_ctx_set_inlet": (_c_int, [_vp, _c_double, ctypes.POINTER(_c_double), _c_int]),
     "jh_ctx_set_tile": (_c_int, [_vp, _c_int]),
     "jh_ctx_get_tile": (_c_int, [_vp]),
     "jh_tile_variants": (_c_int, []),
@@ -42,6 +43,10 @@ SIGNATURES = {
     "jh_lf_alpha": (_c_int, [_vp, _vp, _vp, _vp]),
     "jh_step_vjp": (_c_int, [_vp, _vp, _vp, _c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jh_ghost_strip": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_int)]),
+    "jh_set_ghost_strip": (_c_int, [_vp, _c_int, _vp]),
+    "jh_step_begin": (_c_int, [_vp, _vp, _vp, _c_double, _c_double]),
+    "jh_step_stage": (_c_int, [_vp, _c_int, _vp, _vp, _vp, _vp, _vp, _c_double, _c_double]),
+    "jh_step_finish": (_c_int, [_vp, _vp, _vp, _c_double, _c_double, _vp]),
     "jh_health": (_c_int, [_vp, ctypes.POINTER(ctypes.c_uint)]),
     "jh_timing_enable": (_c_int, [_vp, _c_int]),
     "jh_timing_read": (_c_int, [_vp, ctypes.POINTER(_c_double), ctypes.POINTER(_c_int)]),
@@ -53,6 +58,7 @@ SIGNATURES = {
     "jh_batch_bind": (_c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "jh_batch_set_stream": (_c_int, [_vp, _vp]),
     "jh_batch_set_tile": (_c_int, [_vp, _c_int]),
+    "jh_batch_set_inlet": (_c_int, [_vp, _c_int, _c_double, ctypes.POINTER(_c_double), _c_int]),
     "jh_batch_get_tile": (_c_int, [_vp]),
     "jh_batch_advance": (_c_int, [_vp, _c_double, _c_double, _c_int, _vp, _c_int]),
     "jh_batch_health": (_c_int, [_vp, ctypes.POINTER(ctypes.c_uint)]),

@@ -77,6 +77,11 @@ class BatchSimulator:
         if rc != _lib.JH_OK:
             raise _lib.JancError("jh_batch_create: " + lib.jh_batch_last_error(None).decode())
         self.h = h
+        for k, s in enumerate(sims):
+            if _lib.BC_KINDS["injection_inlet"] in s.bc_kinds:
+                T0, Y = s._inlet(None)
+                arr = (ctypes.c_double * len(Y))(*Y)
+                self._check(lib.jh_batch_set_inlet(h, k, T0, arr, len(Y)))
         self.bind(Us, auxs)
 
     # ------------------------------------------------------------ plumbing

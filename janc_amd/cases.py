@@ -130,13 +130,15 @@ def batch_cases(mech, ncase=64, seed=5, sizes=BATCH_SIZES):
 
 
 def rdc_annulus(mech, nx=4000, ny=400, dx=1e-4, seed=2, seed_cols=200,
-                p_seed=30 * P_ATM, T_seed=3000.0, Pb=P_ATM, t_noise=0.005):
+                p_seed=30 * P_ATM, T_seed=3000.0, Pb=P_ATM, t_noise=0.005,
+                P0_inj=2.0e5, T0_inj=300.0):
     """Config 2 geometry: a periodic-in-x unrolled annulus of fresh
     stoichiometric H2-air (1 atm, 300 K) with a ZND-like seed over
     ``seed_cols`` columns (p_seed / T_seed, complete-combustion products) and
-    T *= 1 +- t_noise.  Top: pressure outlet at Pb.  The bottom injection inlet
-    of the configuration is represented by a slip wall (the inlet model is not
-    built yet).  Returns (U, aux, geometry, theta)."""
+    T *= 1 +- t_noise.  Bottom: injection inlet (the rotating-detonation
+    example's injector model) fed from a manifold at P0_inj / T0_inj with the
+    example's stoichiometric H2-air; top: pressure outlet at Pb.
+    Returns (U, aux, geometry, theta)."""
     rng = np.random.default_rng(seed)
     T = np.full((nx, ny), 300.0)
     p = np.full((nx, ny), P_ATM)
@@ -148,8 +150,9 @@ def rdc_annulus(mech, nx=4000, ny=400, dx=1e-4, seed=2, seed_cols=200,
     Y9[:, :seed_cols] = Yb[:, None, None] if Yb.ndim == 1 else Yb
     U, aux = build_state(mech, T, p, Y9)
     bc = {"left_boundary": "periodic", "right_boundary": "periodic",
-          "bottom_boundary": SLIP, "top_boundary": "pressure_outlet"}
-    return U, aux, dict(dx=dx, dy=dx, boundary_config=bc), {"Pb": Pb}
+          "bottom_boundary": "injection_inlet", "top_boundary": "pressure_outlet"}
+    theta = {"Pb": Pb, "P0_inj": P0_inj, "T0_inj": T0_inj, "Yinj": mech.injection_Y()}
+    return U, aux, dict(dx=dx, dy=dx, boundary_config=bc), theta
 
 
 def smooth_field(rng, nx, ny, modes=4, kmax=3):

@@ -340,10 +340,30 @@ __global__ __launch_bounds__(256) void cell_vjp_kernel(AdjArgs a, const double* 
 // alpha_bar = sum of the partials (one block, fixed order), and the number
 // of padded cells attaining each LF maximum.  scal[0..1] = alpha_bar,
 // cnt[0..1] = tie counts.
-__device__ __forceinline__ bool is_max(double val, double alpha) {
-  // the maxima were reduced from the same arithmetic; allow 4 ulp for
-  // contraction differences between the producing kernels
-  return val == val && val >= alpha - 4.0 * 2.220446049250313e-16 * alpha;
+// Exact tie test against the maxima alpha_max_kernel reduced from the same
+// arithmetic: JAX's max VJP splits the cotangent among the elements equal to
+// the maximum.
+__device__ __forceinline__ bool is_max(double val, double alpha) { return val == alpha; }
+
+// LF maxima of the padded field recomputed with the arithmetic of
+// alpha_count_kernel / alpha_apply_kernel (NaN ignored, nanmax of LF.py:10-11).
+__global__ __launch_bounds__(256) void alpha_max_kernel(AdjArgs a, const double* __restrict__ Up,
+                                                        const double* __restrict__ ap,
+                                                        unsigned long long* amx) {
+  __shared__ double red[4];
+  const MechConst& c = *a.mc;
+  const int PY = a.ny + 6, np = (a.nx + 6) * PY;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double mx = 0.0, my = 0.0;
+  int sx, sy, flip;
+  if (t < np && pad_src(a, t / PY, t % PY, sx, sy, flip)) {
+    const PrimD q = prim_at(c, Up, ap, np, t);
+    const double snd = sqrt(q.gamma * q.p * q.ir);
+    mx = nan_to_zero(fabs(q.u) + snd);
+    my = nan_to_zero(fabs(q.v) + snd);
+  }
+  block_max_atomic<256>(mx, &amx[0], red);
+  block_max_atomic<256>(my, &amx[1], red);
 }
 
 __global__ __launch_bounds__(256) void alpha_reduce_kernel(const double* __restrict__ part,

@@ -189,13 +189,16 @@ __host__ __device__ __forceinline__ void swap_rows(double (&b)[N], int col, int 
   }
 }
 
-// A x = b with the factors of lu_factor (b <- x).
+// A x = b with the factors of lu_factor (b <- x): P b (every row swap, in
+// order, before the substitutions: the stored multipliers are in the final
+// row order), L y = P b, U x = y.
 template <int N>
 __host__ __device__ __forceinline__ void lu_apply(const double (&A)[N][N], const int (&piv)[N],
                                                   double (&b)[N]) {
 #pragma unroll
+  for (int col = 0; col < N; ++col) swap_rows<N>(b, col, piv[col]);
+#pragma unroll
   for (int col = 0; col < N; ++col) {
-    swap_rows<N>(b, col, piv[col]);
 #pragma unroll
     for (int r = col + 1; r < N; ++r) b[r] -= A[r][col] * b[col];
   }

@@ -22,6 +22,8 @@ struct jh_ctx {
   int precision = JH_PREC_FP64;
   int bc[4] = {0, 0, 0, 0};
   double bc_p[4] = {0, 0, 0, 0};
+  double inj_T0 = 300.0;
+  double inj_Y[JH_NS - 1] = {};
   hipStream_t stream = nullptr;
   MechConst* mc = nullptr;        // device
   StepScalars* sc = nullptr;      // device
@@ -29,6 +31,7 @@ struct jh_ctx {
   void* tmpU[2] = {nullptr, nullptr};
   void* tmpA[2] = {nullptr, nullptr};
   int parity = 0;
+  bool user_filled[4] = {false, false, false, false};   // JH_BC_USER strips written
   int tile = 0;                   // stage-kernel variant (kTileRows / kTileMinBlocks)
   std::string err;
   // kernel timing with HIP events on the context stream (jh_timing_*)
@@ -110,6 +113,8 @@ StepArgs<Real> make_args(jh_ctx* c, double dt_fixed, double cfl) {
     a.bc[s] = c->bc[s];
     a.bc_p[s] = c->bc_p[s];
   }
+  a.inj_T0 = c->inj_T0;
+  for (int k = 0; k < JH_NS - 1; ++k) a.inj_Y[k] = c->inj_Y[k];
   return a;
 }
 
@@ -230,12 +235,23 @@ __global__ void zero_u32_kernel(unsigned int* p, int m) {
 
 int blocks_for(size_t m) { return (int)((m + 255) / 256); }
 
+// JH_BC_USER sides must have been written (jh_set_ghost_strip or through
+// jh_ghost_strip's pointer) since their last use; consumes the marks.
+int take_user_strips(jh_ctx* c) {
+  for (int s = 0; s < 4; ++s)
+    if (c->bc[s] == JH_BC_USER && !c->user_filled[s])
+      return fail(c, JH_ERR_ARG, "user boundary strip not filled for this stage");
+  for (int s = 0; s < 4; ++s) c->user_filled[s] = false;
+  return JH_OK;
+}
+
 struct AdjWork {
   double *U[5], *A[5];          // forward states of the step: 0 = input, 1-3 RK, 4 = output
   double *Ub, *Uacc, *ab, *rb;  // cotangents
   double *Up, *ap, *Ubp, *abp;  // padded
   double *cx, *cy, *part, *scal;
   unsigned int* cnt;
+  unsigned long long* amx;      // recomputed LF maxima (bit patterns)
   int npad, nblk;
 };
 
@@ -245,7 +261,7 @@ int adj_work(jh_ctx* c, const double* U0, const double* A0, AdjWork& w) {
   const size_t nfx = (size_t)(c->nx + 1) * c->ny, nfy = (size_t)c->nx * (c->ny + 1);
   const int nblk = blocks_for(np);
   const size_t len = 4 * (NU + 2) * n + (3 * NU + 2) * n + 2 * (NU + 2) * np +
-                     (size_t)NU * 10 * (nfx + nfy) + 2 * nblk + 2 + 2;
+                     (size_t)NU * 10 * (nfx + nfy) + 2 * nblk + 2 + 2 + 2;
   if (c->adj_len < len) {
     if (c->adj) (void)hipFree(c->adj);
     c->adj = nullptr;
@@ -273,6 +289,8 @@ int adj_work(jh_ctx* c, const double* U0, const double* A0, AdjWork& w) {
   w.part = p; p += 2 * nblk;
   w.scal = p; p += 2;
   w.cnt = reinterpret_cast<unsigned int*>(p);
+  p += 2;
+  w.amx = reinterpret_cast<unsigned long long*>(p);
   w.npad = (int)np;
   w.nblk = nblk;
   return JH_OK;
@@ -302,9 +320,10 @@ void flux_vjp(jh_ctx* c, const AdjArgs& a, AdjWork& w, const double* Us, const d
   face_vjp_kernel<1><<<blocks_for(nfy), 256, 0, st>>>(a, w.Up, w.ap, rb, alpha, w.cy);
   cell_vjp_kernel<<<w.nblk, 256, 0, st>>>(a, w.Up, w.ap, w.cx, w.cy, alpha, w.Ubp, w.abp, w.part);
   alpha_reduce_kernel<<<1, 256, 0, st>>>(w.part, w.nblk, w.scal);
-  zero_u32_kernel<<<1, 64, 0, st>>>(w.cnt, 2);
-  alpha_count_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, w.Up, w.ap, alpha, w.cnt);
-  alpha_apply_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, w.Up, w.ap, alpha, w.scal, w.cnt,
+  zero_u32_kernel<<<1, 64, 0, st>>>(w.cnt, 8);   // counts (+ pad) and the two maxima
+  alpha_max_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, w.Up, w.ap, w.amx);
+  alpha_count_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, w.Up, w.ap, w.amx, w.cnt);
+  alpha_apply_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, w.Up, w.ap, w.amx, w.scal, w.cnt,
                                                          w.Ubp, w.abp);
   unpad_kernel<<<blocks_for(n), 256, 0, st>>>(a, w.Ubp, w.abp, Ut, At);
 }
@@ -388,6 +407,7 @@ struct jh_batch {
   int precision = JH_PREC_FP64;
   std::vector<int> nx, ny, bc;        // bc: 4 per case
   std::vector<double> dx, dy, bcp;    // bcp: 4 per case
+  std::vector<double> inj;            // per case: T0, Y_inj[NS-1]
   hipStream_t user = nullptr;         // the caller's stream
   hipStream_t own = nullptr;          // launches and graph capture
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -467,6 +487,8 @@ void fill_desc(const jh_batch* b, int k, CaseDesc<Real>& d) {
     a.bc[s] = b->bc[4 * k + s];
     a.bc_p[s] = b->bcp[4 * k + s];
   }
+  a.inj_T0 = b->inj[(size_t)k * JH_NS];
+  for (int j = 0; j < JH_NS - 1; ++j) a.inj_Y[j] = b->inj[(size_t)k * JH_NS + 1 + j];
   d.U = b->bound ? static_cast<Real*>(b->U[k]) : nullptr;
   d.aux = b->bound ? static_cast<Real*>(b->aux[k]) : nullptr;
 }
@@ -588,7 +610,7 @@ int jh_ctx_create(int nx, int ny, double dx, double dy, const int* bc_kind,
   if (n_mech != jh_mech_const_count())
     return fail(nullptr, JH_ERR_MECH, "mechanism constant count mismatch");
   for (int s = 0; s < 4; ++s)
-    if (bc_kind[s] < JH_BC_PERIODIC || bc_kind[s] > JH_BC_USER)
+    if (bc_kind[s] < JH_BC_PERIODIC || bc_kind[s] > JH_BC_INJECTION_INLET)
       return fail(nullptr, JH_ERR_ARG, "unknown boundary kind");
   jh_ctx* c = new jh_ctx();
   c->nx = nx;
@@ -653,6 +675,15 @@ int jh_ctx_set_bc_param(jh_ctx* c, int side, double value) {
   return JH_OK;
 }
 
+int jh_ctx_set_inlet(jh_ctx* c, double T0, const double* Y, int n) {
+  if (!c) return JH_ERR_ARG;
+  if (!Y || n != JH_NS - 1) return fail(c, JH_ERR_ARG, "Y_inj needs num_species-1 values");
+  if (!(T0 > 0)) return fail(c, JH_ERR_ARG, "T0 must be > 0");
+  c->inj_T0 = T0;
+  for (int k = 0; k < n; ++k) c->inj_Y[k] = Y[k];
+  return JH_OK;
+}
+
 int jh_ctx_set_tile(jh_ctx* c, int variant) {
   if (!c) return JH_ERR_ARG;
   if (variant < 0 || variant >= jh_tile_variants()) return fail(c, JH_ERR_ARG, "unknown tile variant");
@@ -701,8 +732,8 @@ int jh_rhs(jh_ctx* c, const void* U, const void* aux, double dt, void* out) {
   if (!c) return JH_ERR_ARG;
   if (!U || !aux || !out) return fail(c, JH_ERR_ARG, "NULL pointer");
   if (!(dt > 0)) return fail(c, JH_ERR_ARG, "dt must be > 0");
-  for (int s = 0; s < 4; ++s)
-    if (c->bc[s] == JH_BC_USER) return fail(c, JH_ERR_ARG, "user boundary strips not filled");
+  int rc = take_user_strips(c);
+  if (rc) return rc;
   if (c->precision == JH_PREC_FP64)
     return rhs_impl<double>(c, (const double*)U, (const double*)aux, dt, (double*)out);
   return rhs_impl<float>(c, (const float*)U, (const float*)aux, dt, (float*)out);
@@ -753,7 +784,8 @@ int jh_cfl_dt(jh_ctx* c, const void* U, const void* aux, double cfl, double* dt_
 int jh_lf_alpha(jh_ctx* c, const void* U, const void* aux, double* alpha_dev) {
   if (!c) return JH_ERR_ARG;
   if (!U || !aux || !alpha_dev) return fail(c, JH_ERR_ARG, "NULL pointer");
-  int rc;
+  int rc = take_user_strips(c);
+  if (rc) return rc;
   if (c->precision == JH_PREC_FP64) {
     rc = prologue<double>(c, (const double*)U, (const double*)aux, 1.0, 0.0);
     if (rc) return rc;
@@ -780,7 +812,8 @@ int jh_step_vjp(jh_ctx* c, const void* U, const void* aux, double dt, const void
   if (c->precision != JH_PREC_FP64) return fail(c, JH_ERR_ARG, "the adjoint is FP64 only");
   if (!(dt > 0)) return fail(c, JH_ERR_ARG, "the adjoint needs a fixed dt > 0");
   for (int s = 0; s < 4; ++s)
-    if (c->bc[s] == JH_BC_PRESSURE_OUTLET || c->bc[s] == JH_BC_USER)
+    if (c->bc[s] == JH_BC_PRESSURE_OUTLET || c->bc[s] == JH_BC_USER ||
+        c->bc[s] == JH_BC_INJECTION_INLET)
       return fail(c, JH_ERR_ARG, "the adjoint supports periodic, zero_gradient, slip_wall and "
                                  "no_slip_wall boundaries");
   return step_vjp_impl(c, (const double*)U, (const double*)aux, dt, (const double*)U_bar,
@@ -817,7 +850,73 @@ int jh_ghost_strip(jh_ctx* c, int side, void** ptr, int* count) {
   if (!c || !ptr || !count || side < 0 || side > 3) return JH_ERR_ARG;
   *ptr = c->ghost[side];
   *count = NV * 3 * ((side < 2) ? c->ny : c->nx);
+  if (c->bc[side] == JH_BC_USER) c->user_filled[side] = true;   // the caller writes it
   return JH_OK;
+}
+
+int jh_set_ghost_strip(jh_ctx* c, int side, const void* src) {
+  if (!c) return JH_ERR_ARG;
+  if (side < 0 || side > 3 || !src) return fail(c, JH_ERR_ARG, "bad side or NULL strip");
+  if (c->bc[side] != JH_BC_USER) return fail(c, JH_ERR_ARG, "side is not JH_BC_USER");
+  const size_t bytes = elem_size(c) * NV * 3 * ((side < 2) ? c->ny : c->nx);
+  HIP_TRY(c, hipMemcpyAsync(c->ghost[side], src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  c->user_filled[side] = true;
+  return JH_OK;
+}
+
+int jh_step_begin(jh_ctx* c, const void* U, const void* aux, double dt, double cfl) {
+  if (!c) return JH_ERR_ARG;
+  if (!U || !aux) return fail(c, JH_ERR_ARG, "NULL state pointer");
+  if (!(dt > 0) && !(cfl > 0)) return fail(c, JH_ERR_ARG, "need dt > 0 or cfl > 0");
+  return c->precision == JH_PREC_FP64
+             ? prologue<double>(c, (const double*)U, (const double*)aux, dt, cfl)
+             : prologue<float>(c, (const float*)U, (const float*)aux, dt, cfl);
+}
+
+int jh_step_stage(jh_ctx* c, int stage, const void* Us, const void* As, const void* Un,
+                  void* Uo, void* Ao, double dt, double cfl) {
+  if (!c) return JH_ERR_ARG;
+  if (stage < 0 || stage > 2) return fail(c, JH_ERR_ARG, "stage must be 0, 1 or 2");
+  if (!Us || !As || !Un || !Uo || !Ao) return fail(c, JH_ERR_ARG, "NULL state pointer");
+  if (!(dt > 0) && !(cfl > 0)) return fail(c, JH_ERR_ARG, "need dt > 0 or cfl > 0");
+  if (dt > 0) cfl = 0.0;
+  else dt = 0.0;
+  int rc = take_user_strips(c);
+  if (rc) return rc;
+  if (c->precision == JH_PREC_FP64) {
+    auto a = make_args<double>(c, dt, cfl);
+    bc_kernel<double><<<bc_blocks(c), 256, 0, c->stream>>>(a, (const double*)Us,
+                                                            (const double*)As, stage);
+    launch_stage<double, MODE_RK>(c, a, stage, (const double*)Us, (const double*)As,
+                                  (const double*)Un, (double*)Uo, (double*)Ao);
+  } else {
+    auto a = make_args<float>(c, dt, cfl);
+    bc_kernel<float><<<bc_blocks(c), 256, 0, c->stream>>>(a, (const float*)Us,
+                                                          (const float*)As, stage);
+    launch_stage<float, MODE_RK>(c, a, stage, (const float*)Us, (const float*)As,
+                                 (const float*)Un, (float*)Uo, (float*)Ao);
+  }
+  return check_launch(c, "step stage kernels");
+}
+
+int jh_step_finish(jh_ctx* c, void* U, void* aux, double dt, double cfl, double* dt_dev) {
+  if (!c) return JH_ERR_ARG;
+  if (!U || !aux) return fail(c, JH_ERR_ARG, "NULL state pointer");
+  if (!(dt > 0) && !(cfl > 0)) return fail(c, JH_ERR_ARG, "need dt > 0 or cfl > 0");
+  if (dt > 0) cfl = 0.0;
+  else dt = 0.0;
+  if (c->precision == JH_PREC_FP64) {
+    auto a = make_args<double>(c, dt, cfl);
+    chem_kernel<double, CHEM_STEP><<<grid_all(c), 256, 0, c->stream>>>(
+        a, (double*)U, (double*)aux, nullptr, dt_dev);
+  } else {
+    auto a = make_args<float>(c, dt, cfl);
+    chem_kernel<float, CHEM_STEP><<<grid_all(c), 256, 0, c->stream>>>(
+        a, (float*)U, (float*)aux, nullptr, dt_dev);
+  }
+  const int rc = check_launch(c, "chem_kernel");
+  c->parity ^= 1;
+  return rc;
 }
 
 int jh_health(jh_ctx* c, unsigned int* flags) {
@@ -854,7 +953,7 @@ int jh_batch_create(int ncase, const int* nx, const int* ny, const double* dx, c
     if (!(dx[k] > 0) || !(dy[k] > 0)) return bfail(nullptr, JH_ERR_ARG, "dx, dy must be > 0");
     for (int s = 0; s < 4; ++s) {
       const int kind = bc_kind[4 * k + s];
-      if (kind < JH_BC_PERIODIC || kind >= JH_BC_USER)
+      if (kind < JH_BC_PERIODIC || kind > JH_BC_INJECTION_INLET || kind == JH_BC_USER)
         return bfail(nullptr, JH_ERR_ARG, "batched cases take built-in boundary kinds only");
     }
   }
@@ -871,6 +970,8 @@ int jh_batch_create(int ncase, const int* nx, const int* ny, const double* dx, c
   if (bc_param) b->bcp.assign(bc_param, bc_param + 4 * ncase);
   b->U.assign(ncase, nullptr);
   b->aux.assign(ncase, nullptr);
+  b->inj.assign((size_t)ncase * JH_NS, 0.0);
+  for (int k = 0; k < ncase; ++k) b->inj[(size_t)k * JH_NS] = 300.0;
   // block tables: stage tiles per variant, bc lines, reduce, chemistry cells
   const int stride = ncase + 1;
   b->h_off.assign(6 * stride, 0);
@@ -971,6 +1072,16 @@ int jh_batch_set_tile(jh_batch* b, int variant) {
 }
 
 int jh_batch_get_tile(const jh_batch* b) { return b ? b->tile : -1; }
+
+int jh_batch_set_inlet(jh_batch* b, int k, double T0, const double* Y, int n) {
+  if (!b) return JH_ERR_ARG;
+  if (k < 0 || k >= b->ncase) return bfail(b, JH_ERR_ARG, "case index out of range");
+  if (!Y || n != JH_NS - 1) return bfail(b, JH_ERR_ARG, "Y_inj needs num_species-1 values");
+  if (!(T0 > 0)) return bfail(b, JH_ERR_ARG, "T0 must be > 0");
+  b->inj[(size_t)k * JH_NS] = T0;
+  for (int j = 0; j < n; ++j) b->inj[(size_t)k * JH_NS + 1 + j] = Y[j];
+  return b->precision == JH_PREC_FP64 ? upload_desc<double>(b) : upload_desc<float>(b);
+}
 
 int jh_batch_advance(jh_batch* b, double dt, double cfl, int nsteps, double* dt_hist,
                      int use_graph) {

@@ -115,7 +115,9 @@ struct StepArgs {
   StepScalars* sc;
   Real* g[4];                 // ghost strips L, R, B, T (NV planes each)
   int bc[4];
-  double bc_p[4];             // pressure_outlet back pressure per side
+  double bc_p[4];             // pressure_outlet back pressure / inlet P0 per side
+  double inj_T0;              // injection inlet: manifold total temperature
+  double inj_Y[JH_NS - 1];    // injection inlet: injected mass fractions
 };
 
 template <typename Real>
@@ -168,6 +170,87 @@ __global__ __launch_bounds__(256) void reduce_kernel(StepArgs<Real> a, const Rea
 }
 
 // ------------------------------------------------------------ ghost cells
+
+// Pressure-feedback injector model (examples/janc_basic_example2.ipynb,
+// inj_model; Fievisohn et al. 2017) with the manifold total state (P0, T0)
+// explicit instead of the example's nondimensional 1.0: 20 bisection steps
+// on the throat Mach number, then the chamber-side velocity V3 and density.
+__device__ __forceinline__ void inj_model(double p, double R, double P0, double T0, double& V3,
+                                          double& rho) {
+  constexpr double g = 1.29, A1 = 1.0, A3 = 5.0, A2 = A3 - A1;
+  const double C0 = sqrt(g * R * T0);
+  const double P3 = p;
+  double M1 = 0.0, M2 = 1.0, MFC = 0.0, V = 0.0;
+  for (int i = 0; i < 20; ++i) {
+    const double M = 0.5 * (M1 + M2);
+    const double f = 1 + (g - 1) / 2 * (M * M);
+    const double P1 = P0 * pow(f, -g / (g - 1));
+    const double V1 = M * pow(f, -0.5) * C0;
+    MFC = A1 * P0 / sqrt(T0) * sqrt(g / R) * M * pow(f, -(g + 1) / 2 / (g - 1));
+    const double A = 0.5;
+    const double B = g / (g - 1) * P3 * A3 / MFC;
+    const double C = -g / (g - 1) * R * T0;
+    V = (-B + sqrt(B * B - 4 * A * C)) / (2 * A);
+    const double P2 = (MFC * (V - V1) - P1 * A1 + P3 * A3) / A2;
+    if (P2 >= P1) M2 = M;
+    else M1 = M;
+  }
+  V3 = V;
+  rho = MFC / V / A3;
+}
+
+// Ghost state of an injection-inlet side from its wall cell ci
+// (bottom_boundary of the same example): injected stream where the wall
+// cell's pressure is below P0, else the wall cell with its normal velocity
+// negated; the pressure is the wall cell's in both cases.
+template <typename Real>
+__device__ __forceinline__ void inlet_ghost(const MechConst& c, const StepArgs<Real>& a,
+                                            const Real* __restrict__ U,
+                                            const Real* __restrict__ aux, int n, int ci,
+                                            bool xside, bool low, double P0, double (&G)[NV]) {
+  double Uc[NU];
+#pragma unroll
+  for (int v = 0; v < NU; ++v) Uc[v] = (double)U[v * n + ci];
+  const double g_in = (double)aux[ci], T_in = (double)aux[n + ci];
+  const double rho = Uc[0], u = Uc[1] / rho, vv = Uc[2] / rho;
+  double Y[JH_NS];
+#pragma unroll
+  for (int k = 0; k < JH_NS - 1; ++k) Y[k] = Uc[4 + k] / rho;
+  const double R = fill_Y_get_R(c, Y);
+  const double p = rho * R * T_in;
+  double Yi[JH_NS];
+#pragma unroll
+  for (int k = 0; k < JH_NS - 1; ++k) Yi[k] = a.inj_Y[k];
+  const double Ri = fill_Y_get_R(c, Yi);
+  const bool blocked = p >= P0;
+  double rho_c, un_c, ut_c, T_c, gam, h;
+  const double un = xside ? u : vv, ut = xside ? vv : u;   // normal, tangential
+  if (blocked) {
+    rho_c = rho;
+    un_c = -un;
+    ut_c = ut;
+    T_c = T_in;
+    gam = g_in;
+    (void)mix_gamma(c, Y, R, T_in, &h);
+  } else {
+    double V3, rho_m;
+    inj_model(p, Ri, P0, a.inj_T0, V3, rho_m);
+    T_c = p / (Ri * rho_m);
+    gam = mix_gamma(c, Yi, Ri, T_c, &h);
+    rho_c = p / (Ri * T_c);
+    un_c = low ? V3 : -V3;
+    ut_c = 0.0;
+  }
+  const double uc = xside ? un_c : ut_c, vc = xside ? ut_c : un_c;
+  G[0] = rho_c;
+  G[1] = rho_c * uc;
+  G[2] = rho_c * vc;
+  G[3] = rho_c * h - p + 0.5 * rho_c * (uc * uc + vc * vc);
+#pragma unroll
+  for (int k = 0; k < JH_NS - 1; ++k) G[4 + k] = rho_c * (blocked ? Y[k] : Yi[k]);
+  G[NU] = gam;
+  G[NU + 1] = T_c;
+}
 
 // Ghost strips for one RK stage (boundary.py:126-132 with the BC kinds of
 // boundary_conditions_2D/*.py and the periodic pre-pad of
@@ -237,6 +320,14 @@ __device__ __forceinline__ void bc_body(const StepArgs<Real>& a, const Real* __r
         for (int k = 0; k < JH_NS - 1; ++k) G[4 + k] = rho_c * Y[k];
         G[NU] = gam;
         G[NU + 1] = T_c;
+        for (int gi = 0; gi < 3; ++gi) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) gs[v * gstride + gi * len + pos] = (Real)G[v];
+        }
+      } else if (kind == JH_BC_INJECTION_INLET) {
+        const bool low = (side == JH_LEFT || side == JH_BOTTOM);
+        double G[NV];
+        inlet_ghost(c, a, U, aux, n, cell(low ? 0 : N - 1), xside, low, a.bc_p[side], G);
         for (int gi = 0; gi < 3; ++gi) {
 #pragma unroll
           for (int v = 0; v < NV; ++v) gs[v * gstride + gi * len + pos] = (Real)G[v];

@@ -147,6 +147,18 @@ class Mechanism:
                            a[..., 6]], axis=-1)
         self.logc = a[..., 0]
 
+    def injection_Y(self, ratio=1.0):
+        """Injected mass fractions of the rotating-detonation example
+        (janc_basic_example2.ipynb): H2 and O2 of H2-air at equivalence
+        ``ratio`` (air 0.232 O2 by mass), 1e-20 for the other reacting
+        species; the last species (N2) by closure."""
+        yH2 = (ratio / 8 * 0.232) / (1 + ratio / 8 * 0.232)
+        yO2 = 0.232 / (1 + ratio / 8 * 0.232)
+        Y = np.full(self.ns - 1, 1e-20)
+        Y[self.species.index("H2")] = yH2
+        Y[self.species.index("O2")] = yO2
+        return Y
+
     # ---------------- kinetics (load.py:8-88) ----------------
     def _kinetics(self):
         nd, ns = self.nd, self.ns

@@ -162,8 +162,6 @@ class Simulator:
                 self.bc_kinds.append(_lib.BC_KINDS[k])
             else:
                 raise KeyError(k)
-        if self.user_bc:
-            raise NotImplementedError("callable boundary conditions are not on the fused path")
         self.dx, self.dy = float(grid["dx"]), float(grid["dy"])
         self.fixed_dt = float(time_control["dt"]) if "dt" in time_control else None
         self.cfl = float(time_control["CFL"]) if "CFL" in time_control else None
@@ -174,11 +172,36 @@ class Simulator:
 
     # ------------------------------------------------------------ helpers
     def _bc_params(self, theta):
-        pb = 0.0
-        if _lib.BC_KINDS["pressure_outlet"] in self.bc_kinds:
-            theta = theta if theta is not None else self.theta
-            pb = float(theta["Pb"])
-        return [pb if k == _lib.BC_KINDS["pressure_outlet"] else 0.0 for k in self.bc_kinds]
+        theta = theta if theta is not None else self.theta
+        out = []
+        for k in self.bc_kinds:
+            if k == _lib.BC_KINDS["pressure_outlet"]:
+                out.append(float(theta["Pb"]))
+            elif k == _lib.BC_KINDS["injection_inlet"]:
+                out.append(float(theta["P0_inj"]))
+            else:
+                out.append(0.0)
+        return out
+
+    def _inlet(self, theta):
+        """(T0, Y_inj) of injection-inlet sides: theta['T0_inj'] (default
+        300 K) and theta['Yinj'] (default: stoichiometric H2-air as the
+        rotating-detonation example, janc_basic_example2.ipynb)."""
+        theta = theta if theta is not None else self.theta
+        theta = theta or {}
+        T0 = float(theta.get("T0_inj", 300.0))
+        Y = theta.get("Yinj")
+        if Y is None:
+            Y = self.mech.injection_Y()
+        Y = np.asarray(Y, dtype=np.float64)
+        if Y.ndim > 1:
+            flat = Y.reshape(Y.shape[0], -1)
+            if not np.all(flat == flat[:, :1]):
+                raise NotImplementedError("injected composition must be uniform along the inlet")
+            Y = flat[:, 0]
+        if Y.shape != (self.mech.ns - 1,):
+            raise ValueError("Yinj needs %d mass fractions" % (self.mech.ns - 1))
+        return T0, Y
 
     def _context(self, U, theta, dx=None, dy=None):
         nx, ny = int(U.shape[1]), int(U.shape[2])
@@ -193,6 +216,10 @@ class Simulator:
         else:
             for s, v in enumerate(self._bc_params(theta)):
                 _lib.check(ctx.lib.jh_ctx_set_bc_param(ctx.h, s, v), ctx.h)
+        if _lib.BC_KINDS["injection_inlet"] in self.bc_kinds:
+            T0, Y = self._inlet(theta)
+            arr = (ctypes.c_double * len(Y))(*Y)
+            _lib.check(ctx.lib.jh_ctx_set_inlet(ctx.h, T0, arr, len(Y)), ctx.h)
         return ctx
 
     def _state(self, U, aux):
@@ -205,12 +232,55 @@ class Simulator:
             raise ValueError("aux must be (2, nx, ny)")
         return U, aux
 
+    # ------------------------------------------------------------ callables
+    def _fill_user(self, ctx, U, aux, theta):
+        """Call each side's boundary function on its three boundary-adjacent
+        layers (boundary.py:52-57, 127-131: U_bd (12, 3, ny) for left/right,
+        (12, nx, 3) for bottom/top) and copy the ghost layers it returns into
+        the side's strip."""
+        th = theta if theta is not None else self.theta
+        for j, side in enumerate(_lib.SIDES):
+            fn = self.user_bc.get(side)
+            if fn is None:
+                continue
+            sl = {"left": (slice(None), slice(0, 3)), "right": (slice(None), slice(-3, None)),
+                  "bottom": (slice(None), slice(None), slice(0, 3)),
+                  "top": (slice(None), slice(None), slice(-3, None))}[side]
+            Ug, ag = fn(U[sl], aux[sl], th)
+            strip = torch.cat([torch.as_tensor(Ug), torch.as_tensor(ag)], 0).to(
+                device=U.device, dtype=self.dtype)
+            if side in ("bottom", "top"):
+                strip = strip.permute(0, 2, 1)
+            n = U.shape[2] if side in ("left", "right") else U.shape[1]
+            if tuple(strip.shape) != (U.shape[0] + 2, 3, n):
+                raise ValueError("boundary function for %s returned shapes %s, %s"
+                                 % (side, tuple(Ug.shape), tuple(ag.shape)))
+            strip = strip.contiguous()
+            ctx.call("jh_set_ghost_strip", j, _ptr(strip))
+
+    def _step_pieces(self, ctx, U, aux, Uo, ao, dt, cfl, dt_out, theta):
+        """One step through jh_step_begin / _stage / _finish, calling the
+        boundary functions before every RK stage."""
+        U1, A1 = torch.empty_like(U), torch.empty_like(aux)
+        U2, A2 = torch.empty_like(U), torch.empty_like(aux)
+        ctx.call("jh_step_begin", _ptr(U), _ptr(aux), dt, cfl)
+        for stage, (Us, As, Ud, Ad) in enumerate(((U, aux, U1, A1), (U1, A1, U2, A2),
+                                                  (U2, A2, Uo, ao))):
+            self._fill_user(ctx, Us, As, theta)
+            ctx.call("jh_step_stage", stage, _ptr(Us), _ptr(As), _ptr(U), _ptr(Ud), _ptr(Ad),
+                     dt, cfl)
+        ctx.call("jh_step_finish", _ptr(Uo), _ptr(ao), dt, cfl,
+                 _ptr(dt_out) if dt_out is not None else None)
+
     # ------------------------------------------------------------ step API
     def advance_one_step(self, U, aux, dx, dy, dt, theta=None):
         """set_advance_func.advance_one_step (set_simulator.py:117-122)."""
         U, aux = self._state(U, aux)
         ctx = self._context(U, theta, dx, dy)
         Uo, ao = torch.empty_like(U), torch.empty_like(aux)
+        if self.user_bc:
+            self._step_pieces(ctx, U, aux, Uo, ao, float(dt), 0.0, None, theta)
+            return Uo, ao
         ctx.call("jh_advance", _ptr(U), _ptr(aux), _ptr(Uo), _ptr(ao), float(dt), 0.0, 1, None)
         return Uo, ao
 
@@ -220,7 +290,11 @@ class Simulator:
         ctx = self._context(U, theta)
         Uo, ao = torch.empty_like(U), torch.empty_like(aux)
         dt = torch.empty((1,), dtype=torch.float64, device=U.device)
-        if self.fixed_dt is not None:
+        if self.user_bc:
+            fixed = self.fixed_dt is not None
+            self._step_pieces(ctx, U, aux, Uo, ao, self.fixed_dt if fixed else 0.0,
+                              0.0 if fixed else self.cfl, dt, theta)
+        elif self.fixed_dt is not None:
             ctx.call("jh_advance", _ptr(U), _ptr(aux), _ptr(Uo), _ptr(ao), self.fixed_dt, 0.0, 1,
                      _ptr(dt))
         else:
@@ -237,6 +311,12 @@ class Simulator:
         ctx = self._context(U, theta)
         if dt_hist is None:
             dt_hist = torch.empty((nsteps,), dtype=torch.float64, device=U.device)
+        if self.user_bc:
+            fixed = self.fixed_dt is not None
+            for s in range(nsteps):
+                self._step_pieces(ctx, U, aux, U, aux, self.fixed_dt if fixed else 0.0,
+                                  0.0 if fixed else self.cfl, dt_hist[s:s + 1], theta)
+            return U, aux, dt_hist
         if self.fixed_dt is not None:
             ctx.call("jh_advance", _ptr(U), _ptr(aux), _ptr(U), _ptr(aux), self.fixed_dt, 0.0,
                      int(nsteps), _ptr(dt_hist))
@@ -261,6 +341,7 @@ class Simulator:
         U, aux = self._state(U, aux)
         ctx = self._context(U, theta, dx, dy)
         out = torch.empty_like(U)
+        self._fill_user(ctx, U, aux, theta)
         ctx.call("jh_rhs", _ptr(U), _ptr(aux), float(dt), _ptr(out))
         return out
 
@@ -294,6 +375,7 @@ class Simulator:
         U, aux = self._state(U, aux)
         ctx = self._context(U, theta)
         out = torch.empty((2,), dtype=torch.float64, device=U.device)
+        self._fill_user(ctx, U, aux, theta)
         ctx.call("jh_lf_alpha", _ptr(U), _ptr(aux), _ptr(out))
         return out
 

@@ -1,0 +1,64 @@
+// Host build of the per-cell device functions (they are __host__ __device__)
+// for CPU tests against the oracle: compiled with g++ by tests/test_host_kernels.py.
+#include <cmath>
+#include <algorithm>
+using std::fmax;
+using std::fmin;
+#include "../../janc_amd/csrc/chem.hpp"
+#include "../../janc_amd/csrc/thermo.hpp"
+#include "../../janc_amd/csrc/weno.hpp"
+
+extern "C" {
+
+// chem_increment per cell: U (12, n), T (n) -> drhoY (8, n)
+void host_chem(const double* mc, int n, const double* U, const double* T, double dt,
+               double* d_out) {
+  const MechConst& c = *reinterpret_cast<const MechConst*>(mc);
+  for (int i = 0; i < n; ++i) {
+    double rY[JH_NS - 1];
+    for (int k = 0; k < JH_NS - 1; ++k) rY[k] = U[(4 + k) * n + i];
+    double d[JH_NA];
+    jh::chem_increment(c, U[i], rY, T[i], dt, d);
+    for (int k = 0; k < JH_NA; ++k) d_out[k * n + i] = d[k];
+  }
+}
+
+// chem_increment_vjp per cell: cotangent Sb (8, n) -> rho_bar (n), rhoY_bar (8, n), T_bar (n)
+void host_chem_vjp(const double* mc, int n, const double* U, const double* T, double dt,
+                   const double* Sb, double* rhob, double* rYb, double* Tb) {
+  const MechConst& c = *reinterpret_cast<const MechConst*>(mc);
+  for (int i = 0; i < n; ++i) {
+    double rY[JH_NS - 1], sb[JH_NA], yb[JH_NS - 1], rb, tb;
+    for (int k = 0; k < JH_NS - 1; ++k) rY[k] = U[(4 + k) * n + i];
+    for (int k = 0; k < JH_NA; ++k) sb[k] = Sb[k * n + i];
+    jh::chem_increment_vjp(c, U[i], rY, T[i], dt, sb, rb, yb, tb);
+    rhob[i] = rb;
+    for (int k = 0; k < JH_NS - 1; ++k) rYb[k * n + i] = yb[k];
+    Tb[i] = tb;
+  }
+}
+
+// getT_vjp per cell at the converged T: (gamma_bar, T_bar) -> (e_bar, Y_bar (8))
+void host_getT_vjp(const double* mc, int n, const double* Y8, const double* T, const double* gb,
+                   const double* Tbar, double* eb, double* Yb) {
+  const MechConst& c = *reinterpret_cast<const MechConst*>(mc);
+  for (int i = 0; i < n; ++i) {
+    double Y[JH_NS];
+    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = Y8[k * n + i];
+    const double R = jh::fill_Y_get_R(c, Y);
+    double e, yb[JH_NS - 1];
+    jh::getT_vjp(c, Y, R, T[i], gb[i], Tbar[i], e, yb);
+    eb[i] = e;
+    for (int k = 0; k < JH_NS - 1; ++k) Yb[k * n + i] = yb[k];
+  }
+}
+
+double host_weno(const double* v) { return jh::weno5(v[0], v[1], v[2], v[3], v[4]); }
+float host_weno_f32(const float* v) { return jh::weno5(v[0], v[1], v[2], v[3], v[4]); }
+double host_weno_grad(const double* v, double* g) {
+  double gg[5];
+  const double q = jh::weno5_grad(v[0], v[1], v[2], v[3], v[4], gg);
+  for (int k = 0; k < 5; ++k) g[k] = gg[k];
+  return q;
+}
+}

@@ -34,17 +34,37 @@ def _case(name, mech):
     elif name == "tube":
         U, aux, g = cases.detonation_tube(mech, nx=45, ny=22, driver=8)
         theta = None
-    else:   # walls of every kind + a subsonic pressure outlet
+    elif name == "outlet":   # walls of every kind + a subsonic pressure outlet
         U, aux, g = cases.detonation_tube(mech, nx=29, ny=67, driver=6, seed=5)
         g["boundary_config"] = {"left_boundary": "no_slip_wall",
                                 "right_boundary": "pressure_outlet",
                                 "bottom_boundary": "zero_gradient",
                                 "top_boundary": "pressure_outlet"}
         theta = {"Pb": 1.5 * 101325.0}
+    else:   # annulus crop: periodic x, injection inlet below (open and blocked
+        # cells), pressure outlet above
+        U, aux, g, theta = cases.rdc_annulus(mech, nx=61, ny=37, seed_cols=12)
     return U, aux, g, theta
 
 
-CASES = ["box", "tube", "outlet"]
+CASES = ["box", "tube", "outlet", "inlet"]
+
+
+def make_oracle(mech_path, g, theta):
+    """Oracle for a geometry; injection-inlet sides get the restated example
+    boundary function (a user callable in the reference)."""
+    bcfg = dict(g["boundary_config"])
+    inlets = [s for s in ("left", "right", "bottom", "top")
+              if bcfg[s + "_boundary"] == "injection_inlet"]
+    for s in inlets:
+        bcfg[s + "_boundary"] = "slip_wall"
+    o = Oracle(mech_path, bcfg, g["dx"], g["dy"])
+    if inlets:
+        from oracle.inlet import attach_inlet, stoichiometric_injection_Y
+        Y = theta.get("Yinj", stoichiometric_injection_Y(len(o.thermo.Mex)))
+        for s in inlets:
+            attach_inlet(o, s, theta["P0_inj"], theta.get("T0_inj", 300.0), Y)
+    return o
 
 
 @pytest.fixture(scope="module")
@@ -58,7 +78,7 @@ def _pair(mech_path, mech, name, precision="fp64", dt=None):
     cfg = cases.simulation_config(g, mech_path, cfl=0.3, dt=dt, precision=precision,
                                   theta=theta)
     sim = Simulator(cfg)
-    o = Oracle(mech_path, g["boundary_config"], g["dx"], g["dy"])
+    o = make_oracle(mech_path, g, theta)
     print("case", name, precision, U.shape, flush=True)
     return sim, o, U, aux, theta
 
@@ -112,7 +132,7 @@ def test_advance_func_one_step(mech_path, mech, name):
     assert rel_err(ag, ar) <= FP64_TOL
 
 
-@pytest.mark.parametrize("name", ["tube", "outlet"])
+@pytest.mark.parametrize("name", ["tube", "outlet", "inlet"])
 def test_fixed_dt_five_steps(mech_path, mech, name):
     sim, o, U, aux, theta = _pair(mech_path, mech, name, dt=4e-9)
     Ug, ag = torch.as_tensor(U).cuda(), torch.as_tensor(aux).cuda()
@@ -207,3 +227,69 @@ def test_mixed_precision_against_fp64_oracle(mech_path, mech):
         print(name, "max", np.abs(r).max(), "L2", l2, "Linf", linf)
         if np.abs(r).max() > 1e-4:       # rho, T and the major species
             assert l2 <= 1e-4 and linf <= 2e-3, name
+
+
+# ---------------------------------------------------------- callable sides
+def _blowing(xp_copy):
+    """Bottom-side boundary function: mirrored layers with the wall-normal
+    momentum reflected about a blowing velocity theta['vb']."""
+    def fn(U_bd, aux_bd, theta):
+        Ug = xp_copy(U_bd[:, :, [2, 1, 0]])
+        ag = xp_copy(aux_bd[:, :, [2, 1, 0]])
+        Ug[2] = -Ug[2] + 2.0 * Ug[0] * theta["vb"]
+        return Ug, ag
+    return fn
+
+
+def _mirror_x(xp_copy):
+    """Left-side boundary function: zero-gradient mirror written by hand."""
+    def fn(U_bd, aux_bd, theta):
+        return xp_copy(U_bd[:, [2, 1, 0], :]), xp_copy(aux_bd[:, [2, 1, 0], :])
+    return fn
+
+
+def _user_pair(mech_path, mech, precision="fp64", dt=None):
+    from janc_amd.solver import Simulator
+    U, aux, g = cases.detonation_tube(mech, nx=33, ny=26, driver=6, seed=8)
+    theta = {"vb": 15.0}
+    bc_np = dict(g["boundary_config"], bottom_boundary=_blowing(np.copy),
+                 left_boundary=_mirror_x(np.copy))
+    bc_t = dict(g["boundary_config"], bottom_boundary=_blowing(torch.clone),
+                left_boundary=_mirror_x(torch.clone))
+    cfg = cases.simulation_config(dict(g, boundary_config=bc_t), mech_path, cfl=0.3, dt=dt,
+                                  precision=precision, theta=theta)
+    sim = Simulator(cfg)
+    o = Oracle(mech_path, bc_np, g["dx"], g["dy"])
+    print("case user", precision, U.shape, flush=True)
+    return sim, o, U, aux, theta
+
+
+def test_callable_boundaries(mech_path, mech):
+    sim, o, U, aux, theta = _user_pair(mech_path, mech)
+    got = sim.flux_func(U, aux, sim.dx, sim.dy, 3e-9)
+    assert rel_err(got, o.flux_func(U, aux, 3e-9, theta)) <= FP64_TOL
+    a = sim.lf_alpha(U, aux).cpu().numpy()
+    np.testing.assert_allclose(a, o.lf_alpha(U, aux, theta), rtol=1e-14)
+    Ug, ag, _, dt = sim.advance_func(U, aux, 0.0)
+    Ur, ar, _, dtr = o.advance_func(U, aux, 0.0, 0.3, theta)
+    assert float(dt) == pytest.approx(dtr, rel=1e-14)
+    assert rel_err(Ug, Ur) <= FP64_TOL and rel_err(ag, ar) <= FP64_TOL
+    Ug, ag = torch.as_tensor(U).cuda(), torch.as_tensor(aux).cuda()
+    _, _, dts = sim.advance(Ug, ag, 3)
+    Ur, ar, t = U, aux, 0.0
+    for k in range(3):
+        Ur, ar, t, dtr = o.advance_func(Ur, ar, t, 0.3, theta)
+        assert float(dts[k]) == pytest.approx(dtr, rel=1e-13)
+    assert rel_err(Ug, Ur) <= FP64_TOL and rel_err(ag, ar) <= FP64_TOL
+
+
+def test_callable_boundaries_need_fresh_strips(mech_path, mech):
+    """A stage whose user strip was not written since the last use fails."""
+    from janc_amd import _lib
+    from janc_amd.solver import _ptr
+    sim, o, U, aux, theta = _user_pair(mech_path, mech, dt=2e-9)
+    Ut, at = torch.as_tensor(U).cuda(), torch.as_tensor(aux).cuda()
+    ctx = sim._context(Ut, None)
+    out = torch.empty_like(Ut)
+    with pytest.raises(_lib.JancError):
+        ctx.call("jh_rhs", _ptr(Ut), _ptr(at), 2e-9, _ptr(out))

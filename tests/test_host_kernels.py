@@ -1,0 +1,144 @@
+"""The per-cell device functions (``__host__ __device__`` in janc_amd/csrc)
+compiled for the host with g++ and checked on the CPU: chemistry increment
+against the oracle, its VJP against autograd of the torch twin, the
+temperature-solve VJP against the oracle's get_T_bwd, WENO5 (FP64, FP32)
+against the oracle and the WENO5 gradient against autograd."""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from janc_amd import cases
+from janc_amd.mechanism import Mechanism
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+
+
+@pytest.fixture(scope="module")
+def shim(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("host") / "shim.so")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__",
+                    "-I/opt/rocm/include", os.path.join(HERE, "host", "shim.cpp"), "-o", out],
+                   check=True)
+    lib = ctypes.CDLL(out)
+    lib.host_weno.restype = ctypes.c_double
+    lib.host_weno_f32.restype = ctypes.c_float
+    lib.host_weno_grad.restype = ctypes.c_double
+    return lib
+
+
+P = ctypes.POINTER(ctypes.c_double)
+
+
+def _p(a):
+    return np.ascontiguousarray(a).ctypes.data_as(P)
+
+
+@pytest.fixture(scope="module")
+def mech(mech_path):
+    return Mechanism(mech_path)
+
+
+def _cells(mech):
+    """Unburned / burned / partly reacted cells over 1-30 atm and 900-3000 K
+    (with and without radicals, so pivoting and the clip both occur)."""
+    rng = np.random.default_rng(3)
+    n = 64
+    T = rng.uniform(900.0, 3000.0, (n, 1))
+    p = rng.uniform(1.0, 30.0, (n, 1)) * 101325.0
+    Y9 = cases.premixed_Y(mech, rng.uniform(0.6, 1.4, (n, 1)))
+    rad = np.where(rng.uniform(size=(n, 1)) < 0.3, 0.0, 10 ** rng.uniform(-8, -2, (n, 1)))
+    Y9[2:8] = rad * rng.uniform(0.1, 1.0, (6, n, 1))
+    Y9 /= Y9.sum(0, keepdims=True)
+    return cases.build_state(mech, T, p, Y9)
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("dt", [2e-9, 1e-7])
+def test_chemistry_increment_and_vjp(shim, mech, mech_path, dt):
+    from oracle.step import Oracle
+    from oracle.torch_step import TorchOracle
+    U, aux = _cells(mech)
+    n = U.shape[1] * U.shape[2]
+    mc = mech.const_array()
+    d = np.zeros((8, n))
+    shim.host_chem(_p(mc), n, _p(U), _p(aux[1]), ctypes.c_double(dt), _p(d))
+    bc = {s + "_boundary": "periodic" for s in ("left", "right", "bottom", "top")}
+    S = Oracle(mech_path, bc, 1e-4, 1e-4).reaction_source_terms(U, aux, dt)
+    assert _rel(d, S[4:].reshape(8, -1)) < 1e-12
+    to = TorchOracle(mech_path, bc, 1e-4, 1e-4)
+    Sb = np.random.default_rng(1).standard_normal((8,) + U.shape[1:])
+    Ut = torch.tensor(U, requires_grad=True)
+    at = torch.tensor(aux, requires_grad=True)
+    (to.reaction(Ut, at, dt)[4:] * torch.tensor(Sb)).sum().backward()
+    rb, yb, tb = np.zeros(n), np.zeros((8, n)), np.zeros(n)
+    shim.host_chem_vjp(_p(mc), n, _p(U), _p(aux[1]), ctypes.c_double(dt), _p(Sb), _p(rb),
+                       _p(yb), _p(tb))
+    g = Ut.grad.numpy().reshape(12, -1)
+    assert _rel(rb, g[0]) < 1e-11
+    assert _rel(tb, at.grad.numpy()[1].ravel()) < 1e-11
+    # species with a non-negligible share: the gradient with respect to a
+    # ~1e-9 radical cancels terms of ~1/X in both forms and is not compared
+    for k in range(8):
+        big = (U[4 + k].ravel() / U[0].ravel()) > 1e-6
+        assert _rel(yb[k][big], g[4 + k][big]) < 1e-9, k
+
+
+def test_temperature_vjp(shim, mech, mech_path):
+    from oracle.mech import load_mechanism
+    from oracle.thermo import Thermo
+    U, aux = _cells(mech)
+    n = U.shape[1] * U.shape[2]
+    Y = (U[4:] / U[0]).reshape(8, -1)
+    rng = np.random.default_rng(2)
+    gb, Tbar = rng.standard_normal(n), rng.standard_normal(n)
+    eb, Yb = np.zeros(n), np.zeros((8, n))
+    shim.host_getT_vjp(_p(mech.const_array()), n, _p(Y), _p(aux[1].ravel()), _p(gb), _p(Tbar),
+                       _p(eb), _p(Yb))
+    _, th, _ = load_mechanism(mech_path)
+    de, dY, _ = Thermo(th).get_T_bwd(None, Y, aux.reshape(2, -1), np.stack([gb, Tbar]))
+    assert _rel(eb, de.ravel()) < 1e-12
+    assert _rel(Yb, dY) < 1e-12
+
+
+def test_weno(shim):
+    from oracle.flux import _weno_core
+    rng = np.random.default_rng(0)
+    for _ in range(500):
+        v = rng.standard_normal(5) * 10 ** rng.uniform(-3, 10)
+        ref = _weno_core(*v)
+        assert abs(shim.host_weno(_p(v)) - ref) <= 1e-13 * max(np.abs(v).max(), 1e-300)
+        v32 = v.astype(np.float32)
+        got32 = shim.host_weno_f32(np.ascontiguousarray(v32).ctypes.data_as(
+            ctypes.POINTER(ctypes.c_float)))
+        assert abs(got32 - _weno_core(*v32.astype(np.float64))) <= 2e-6 * np.abs(v32).max()
+
+
+def test_weno_gradient(shim):
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        v = rng.standard_normal(5) * 10 ** rng.uniform(-4, 4)
+        t = torch.tensor(v, requires_grad=True)
+        f = t
+        IS1 = 0.25 * (f[0] - 4 * f[1] + 3 * f[2]) ** 2 + 13 / 12 * (f[0] - 2 * f[1] + f[2]) ** 2
+        IS2 = 0.25 * (f[1] - f[3]) ** 2 + 13 / 12 * (f[1] - 2 * f[2] + f[3]) ** 2
+        IS3 = 0.25 * (3 * f[2] - 4 * f[3] + f[4]) ** 2 + 13 / 12 * (f[2] - 2 * f[3] + f[4]) ** 2
+        a = torch.stack([0.1 / (1e-6 + IS1) ** 2, 0.6 / (1e-6 + IS2) ** 2,
+                         0.3 / (1e-6 + IS3) ** 2])
+        w = a / a.sum()
+        q = torch.stack([f[0] / 3 - 7 * f[1] / 6 + 11 * f[2] / 6,
+                         -f[1] / 6 + 5 * f[2] / 6 + f[3] / 3,
+                         f[2] / 3 + 5 * f[3] / 6 - f[4] / 6])
+        (w * q).sum().backward()
+        g = np.zeros(5)
+        shim.host_weno_grad(_p(v), _p(g))
+        np.testing.assert_allclose(g, t.grad.numpy(), rtol=1e-9, atol=1e-12)
