@@ -209,26 +209,68 @@ def _batch_leg(mech, steps, tune, torch):
             "tile_variant": b.get_tile(), "autotune": tuned, "health_flags": flags}
 
 
-def _adjoint_leg(mech, steps, every, torch):
-    """Forward vs forward+adjoint of a 100-step FP64 rollout on a 1024 x 256
-    tube (checkpoints every 10 steps, RK stages recomputed), plus one step's
-    VJP against one forward step."""
+def _adjoint_leg(mech, steps, every, ndir, torch):
+    """Config 4: J = heat released + 1/2 |p_final/p_ref - p_target/p_ref|^2 /
+    cells after `steps` FP64 steps of a 1024 x 256 tube, gradients with
+    respect to the initial T and Y_k fields through the checkpointed adjoint
+    rollout; timed against the forward rollout, and checked against central
+    differences (eps 1e-6) on `ndir` seeded random directions."""
+    import numpy as np
     from janc_amd import adjoint, cases
     from janc_amd.solver import Simulator
-    U, aux, g = cases.detonation_tube(mech, nx=1024, ny=256, driver=50, seed=4)
+    T, p, Y, u, v, g = cases.adjoint_case(mech)
+    dev = "cuda"
+    Tt, pt, Yt = (torch.tensor(x, device=dev) for x in (T, p, Y))
+    ut, vt = torch.tensor(u, device=dev), torch.tensor(v, device=dev)
+    U0, a0 = adjoint.initial_state(mech, Tt, pt, Yt, ut, vt)
     probe = Simulator(cases.simulation_config(g, cfl=0.3, precision="fp64"))
-    Ut = torch.as_tensor(U).cuda()
-    at = torch.as_tensor(aux).cuda()
-    dt = 0.8 * float(probe.CFL(Ut, at))
+    dt = 0.8 * float(probe.CFL(U0, a0))
     sim = Simulator(cases.simulation_config(g, dt=dt, precision="fp64"))
-    cells = U.shape[1] * U.shape[2]
-    gen = torch.Generator(device="cuda").manual_seed(3)
-    W = torch.randn(Ut.shape, dtype=torch.float64, device="cuda", generator=gen)
-    Wa = torch.zeros_like(at)
-    # one step: forward vs VJP (which recomputes the step's stages)
-    Uw, aw = Ut.clone(), at.clone()
-    sim.advance(Uw, aw, 2)
-    sim.step_vjp(Ut, at, dt, W, Wa)
+    cells = T.size
+    dx, dy = g["dx"], g["dy"]
+    UN, aN = sim.advance(U0.clone(), a0.clone(), steps)[:2]
+    rng = np.random.default_rng(4)
+    s_field = torch.tensor(cases.smooth_field(rng, *T.shape), device=dev)
+    p_ref = 101325.0
+    p_target = adjoint.pressure(mech, UN, aN) * (1.0 + 0.01 * s_field)
+    q_scale = float(adjoint.heat_release(mech, U0, UN, dx, dy).abs()) + 1e-300
+
+    def J(U0, UN, aN):
+        q = adjoint.heat_release(mech, U0, UN, dx, dy) / q_scale
+        r = (adjoint.pressure(mech, UN, aN) - p_target) / p_ref
+        return q + 0.5 * (r * r).sum() / cells
+
+    def J_fwd(Tx, Yx):
+        U0, a0 = adjoint.initial_state(mech, Tx, pt, Yx, ut, vt)
+        UN, aN = sim.advance(U0.clone(), a0.clone(), steps)[:2]
+        return float(J(U0, UN, aN))
+
+    # forward rollout alone (fused library call)
+    Uf, af = U0.clone(), a0.clone()
+    sim.advance(Uf, af, 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.advance(Uf, af, steps)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    # forward + adjoint
+    Tg = Tt.clone().requires_grad_(True)
+    Yg = Yt.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    U0g, a0g = adjoint.initial_state(mech, Tg, pt, Yg, ut, vt)
+    UNg, aNg = adjoint.rollout(sim, U0g, a0g, dt, steps, checkpoint_every=every)
+    torch.cuda.synchronize()
+    t_roll = time.perf_counter() - t0
+    J(U0g, UNg, aNg).backward()
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t0
+    # one step: forward vs VJP
+    W = torch.randn(U0.shape, dtype=torch.float64, device=dev,
+                    generator=torch.Generator(device=dev).manual_seed(3))
+    Wa = torch.zeros_like(a0)
+    Uw, aw = U0.clone(), a0.clone()
+    sim.step_vjp(U0, a0, dt, W, Wa)
     torch.cuda.synchronize()
     reps = 10
     t0 = time.perf_counter()
@@ -237,38 +279,30 @@ def _adjoint_leg(mech, steps, every, torch):
     t_fwd1 = (time.perf_counter() - t0) / reps
     t0 = time.perf_counter()
     for _ in range(reps):
-        sim.step_vjp(Ut, at, dt, W, Wa)
+        sim.step_vjp(U0, a0, dt, W, Wa)
     torch.cuda.synchronize()
     t_vjp1 = (time.perf_counter() - t0) / reps
-    # rollout
-    Uf, af = Ut.clone(), at.clone()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sim.advance(Uf, af, steps)
-    torch.cuda.synchronize()
-    t_fwd = time.perf_counter() - t0
-    U0 = Ut.clone().requires_grad_(True)
-    a0 = at.clone().requires_grad_(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    Uy, ay = adjoint.rollout(sim, U0, a0, dt, steps, checkpoint_every=every)
-    torch.cuda.synchronize()
-    t_roll_fwd = time.perf_counter() - t0
-    loss = (Uy * W).sum()
-    loss.backward()
-    torch.cuda.synchronize()
-    t_total = time.perf_counter() - t0
-    finite = bool(torch.isfinite(U0.grad).all() and torch.isfinite(a0.grad).all())
-    return {"workload": "1024x256 tube (dx 1.25e-5 m, 50-column 30 atm/3000 K driver), FP64, "
-                        "fixed dt %.3e s, %d steps, checkpoints every %d" % (dt, steps, every),
+    # central differences on random directions
+    eps, errs = 1e-6, []
+    for d in range(ndir):
+        dT = Tt * torch.tensor(rng.standard_normal(T.shape), device=dev)
+        dY = Yt * torch.tensor(rng.standard_normal(Y.shape), device=dev)
+        fd = (J_fwd(Tt + eps * dT, Yt + eps * dY) - J_fwd(Tt - eps * dT, Yt - eps * dY)) / (2 * eps)
+        ad = float((Tg.grad * dT).sum() + (Yg.grad * dY).sum())
+        errs.append(abs(fd - ad) / max(abs(fd), abs(ad), 1e-300))
+    return {"workload": "1024x256 tube (dx 1.25e-5 m, 51-column 30 atm/3000 K driver, 0.5%% T "
+                        "and 3 m/s velocity noise), FP64, fixed dt %.3e s, %d steps, checkpoints "
+                        "every %d; J = heat release + pressure mismatch; grad wrt initial T, Y_k"
+                        % (dt, steps, every),
             "cells": cells,
             "step_fwd_ms": t_fwd1 * 1e3, "step_vjp_ms": t_vjp1 * 1e3,
             "adjoint_over_fwd_step": t_vjp1 / t_fwd1,
             "rollout_fwd_ms": t_fwd * 1e3,
-            "rollout_checkpointed_fwd_ms": t_roll_fwd * 1e3,
+            "rollout_checkpointed_fwd_ms": t_roll * 1e3,
             "rollout_fwd_plus_adjoint_ms": t_total * 1e3,
-            "adjoint_over_fwd_rollout": (t_total - t_roll_fwd) / t_fwd,
-            "gradients_finite": finite}
+            "adjoint_over_fwd_rollout": (t_total - t_roll) / t_fwd,
+            "fd_directions": ndir, "fd_eps": eps, "fd_max_rel_err": max(errs),
+            "fd_median_rel_err": float(np.median(errs))}
 
 
 def main():
@@ -282,10 +316,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
+    ap.add_argument("--tile", type=int, default=None,
+                    help="force the headline stage-kernel variant (skips its autotuning)")
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--fd-directions", type=int, default=32)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="one 256 MiB device-to-device copy before the timed region "
                          "(reference dispatch for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -314,7 +351,11 @@ def main():
     sim = Simulator(cases.simulation_config(geom, cfl=0.3, precision="fp64", theta=theta))
     U = torch.as_tensor(U0).cuda()
     aux = torch.as_tensor(aux0).cuda()
-    tuned = sim.autotune(U, aux, theta=theta) if not args.no_tune else None
+    if args.tile is not None:
+        sim.set_tile(U, args.tile, theta)
+        tuned = {"variant": args.tile, "forced": True}
+    else:
+        tuned = sim.autotune(U, aux, theta=theta) if not args.no_tune else None
     if args.calibrate_copy:
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -387,7 +428,7 @@ def main():
         result["batch"] = _batch_leg(mech, args.batch_steps, not args.no_tune, torch)
         torch.cuda.empty_cache()
     if not args.no_adjoint:
-        result["adjoint"] = _adjoint_leg(mech, args.adjoint_steps, 10, torch)
+        result["adjoint"] = _adjoint_leg(mech, args.adjoint_steps, 10, args.fd_directions, torch)
         torch.cuda.empty_cache()
 
     if not args.no_cpu:

@@ -109,6 +109,25 @@ def detonation_tube(mech, nx=400, ny=80, dx=TUBE_DX, seed=1, driver=20,
     return U, aux, dict(dx=dx, dy=dx, boundary_config=bc)
 
 
+def adjoint_case(mech, nx=1024, ny=256, seed=4, t_noise=0.005, u_noise=3.0):
+    """Config 4 initial fields (T, p, Y (ns-1), u, v) and geometry: the config-1
+    tube (driver over nx/20 columns) with T *= 1 +- t_noise and Gaussian
+    velocity noise of u_noise m/s.  The noise keeps every LF maximum a single
+    cell and every |u|, |v| away from its kink at 0, where reverse mode and a
+    finite difference legitimately disagree."""
+    U, aux, g = detonation_tube(mech, nx=nx, ny=ny, driver=max(4, nx // 20), seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    rho = U[0]
+    Y = U[4:] / rho
+    Y9 = np.concatenate([Y, (1.0 - Y.sum(0))[None]])
+    R, _, _, _ = mixture(mech, aux[1], Y9)
+    p = rho * R * aux[1]
+    T = aux[1] * _noise(rng, aux[1].shape, t_noise)
+    u = u_noise * rng.standard_normal(T.shape)
+    v = u_noise * rng.standard_normal(T.shape)
+    return T, p, Y, u, v, g
+
+
 BATCH_SIZES = ((128, 32), (256, 64), (512, 128), (1024, 256))
 
 

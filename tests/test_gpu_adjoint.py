@@ -168,7 +168,8 @@ def test_step_vjp_rejects_unsupported(mech_path, mech):
 # ------------------------------------------------- gradients w.r.t. T and Y
 def _tY_case(mech, nx, ny, seed=9):
     """Tube with 0.5 % temperature noise (so the LF maxima have single
-    maximisers) as (T, p, Y) fields."""
+    maximisers) and a few m/s of velocity noise (so no |u| or |v| sits at its
+    kink at 0) as (T, p, Y, u, v) fields."""
     U, aux, g = cases.detonation_tube(mech, nx=nx, ny=ny, driver=max(4, nx // 8), seed=seed)
     rng = np.random.default_rng(seed)
     T = aux[1] * (1.0 + 0.005 * rng.uniform(-1, 1, aux[1].shape))
@@ -177,7 +178,9 @@ def _tY_case(mech, nx, ny, seed=9):
     Y9 = np.concatenate([Y, (1.0 - Y.sum(0))[None]])
     R, _, _, _ = cases.mixture(mech, aux[1], Y9)
     p = rho * R * aux[1]
-    return T, p, Y, g
+    u = 3.0 * rng.standard_normal(T.shape)
+    v = 3.0 * rng.standard_normal(T.shape)
+    return T, p, Y, u, v, g
 
 
 def _loss(mech, U0, UN, aN, dx, dy, p_target, q_scale, p_ref):
@@ -191,7 +194,7 @@ def test_initial_field_gradient_matches_twin(mech_path, mech):
     """d loss / d (T, Y) through the GPU rollout vs autograd of the torch twin."""
     from janc_amd import adjoint
     from janc_amd.solver import Simulator
-    T, p, Y, g = _tY_case(mech, 40, 16)
+    T, p, Y, u, v, g = _tY_case(mech, 40, 16)
     steps, dt = 3, 2e-9
     sim = Simulator(cases.simulation_config(g, mech_path, dt=dt))
     to = TorchOracle(mech_path, g["boundary_config"], g["dx"], g["dy"])
@@ -201,7 +204,8 @@ def test_initial_field_gradient_matches_twin(mech_path, mech):
     for dev in ("cuda", "cpu"):
         Tt = torch.tensor(T, device=dev, requires_grad=True)
         Yt = torch.tensor(Y, device=dev, requires_grad=True)
-        U0, a0 = adjoint.initial_state(mech, Tt, torch.tensor(p, device=dev), Yt)
+        U0, a0 = adjoint.initial_state(mech, Tt, torch.tensor(p, device=dev), Yt,
+                                       torch.tensor(u, device=dev), torch.tensor(v, device=dev))
         if dev == "cuda":
             UN, aN = adjoint.rollout(sim, U0, a0, dt, steps, checkpoint_every=2)
         else:
@@ -223,13 +227,14 @@ def test_initial_field_gradient_matches_fd(mech_path, mech):
     through a checkpointed multi-step rollout."""
     from janc_amd import adjoint
     from janc_amd.solver import Simulator
-    T, p, Y, g = _tY_case(mech, 96, 24)
+    T, p, Y, u, v, g = _tY_case(mech, 96, 24)
     probe = Simulator(cases.simulation_config(g, mech_path, cfl=0.3))
     steps = 12
     Tt = torch.tensor(T, device="cuda")
     Yt = torch.tensor(Y, device="cuda")
     pt = torch.tensor(p, device="cuda")
-    U0, a0 = adjoint.initial_state(mech, Tt, pt, Yt)
+    ut, vt = torch.tensor(u, device="cuda"), torch.tensor(v, device="cuda")
+    U0, a0 = adjoint.initial_state(mech, Tt, pt, Yt, ut, vt)
     dt = 0.8 * float(probe.CFL(U0, a0))
     sim = Simulator(cases.simulation_config(g, mech_path, dt=dt))
     rng = np.random.default_rng(2)
@@ -238,7 +243,7 @@ def test_initial_field_gradient_matches_fd(mech_path, mech):
     q_scale = float(adjoint.heat_release(mech, U0, UN, g["dx"], g["dy"]).abs()) + 1e-30
 
     def L(Tx, Yx, grad=False):
-        U0, a0 = adjoint.initial_state(mech, Tx, pt, Yx)
+        U0, a0 = adjoint.initial_state(mech, Tx, pt, Yx, ut, vt)
         if grad:
             UN, aN = adjoint.rollout(sim, U0, a0, dt, steps, checkpoint_every=5)
         else:
