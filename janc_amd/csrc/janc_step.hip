@@ -32,7 +32,7 @@ struct jh_ctx {
   void* tmpA[2] = {nullptr, nullptr};
   int parity = 0;
   bool user_filled[4] = {false, false, false, false};   // JH_BC_USER strips written
-  int tile = 0;                   // stage-kernel variant (kTileRows / kTileMinBlocks)
+  int tile = 0;                   // stage-kernel variant (kTileRows / kTileWaves)
   std::string err;
   // kernel timing with HIP events on the context stream (jh_timing_*)
   bool timing = false;
@@ -134,24 +134,27 @@ int stage_blocks(const jh_ctx* c) {
   return ((c->nx + tx - 1) / tx) * ((c->ny + TY - 1) / TY);
 }
 
-// Stage-kernel variants (jh_ctx_set_tile): rows along x, and the waves per
-// SIMD the register budget is sized for.
-constexpr int kTileRows[3] = {8, 8, 16};
-constexpr int kTileWaves[3] = {3, 4, 4};
+// Stage-kernel variants (jh_ctx_set_tile): X(index, rows along x, waves per
+// SIMD the register budget is sized for).
+#define JH_TILE_VARIANTS(X) X(0, 8, 3) X(1, 8, 4) X(2, 16, 4) X(3, 16, 5) X(4, 32, 4)
+constexpr int kTileVariants = 5;
+constexpr int kTileRows[kTileVariants] = {8, 8, 16, 16, 32};
+constexpr int kTileWaves[kTileVariants] = {3, 4, 4, 5, 4};
 
 template <typename Real, int MODE>
 void launch_stage(jh_ctx* c, const StepArgs<Real>& a, int stage, const Real* Us, const Real* As,
                   const Real* Un, Real* Uo, Real* Ao) {
   const int nb = stage_blocks(c);
   switch (c->tile) {
-    case 1:
-      stage_kernel<Real, MODE, 8, 4><<<nb, Tile<8>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+#define JH_LAUNCH(V, TX, WPE)                                                          \
+    case V:                                                                            \
+      stage_kernel<Real, MODE, TX, WPE><<<nb, Tile<TX>::NT, 0, c->stream>>>(a, stage, Us, As, \
+                                                                           Un, Uo, Ao); \
       break;
-    case 2:
-      stage_kernel<Real, MODE, 16, 4><<<nb, Tile<16>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
-      break;
+    JH_TILE_VARIANTS(JH_LAUNCH)
+#undef JH_LAUNCH
     default:
-      stage_kernel<Real, MODE, 8, 3><<<nb, Tile<8>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+      break;
   }
 }
 
@@ -418,8 +421,8 @@ struct jh_batch {
   void* desc = nullptr;               // device CaseDesc[ncase]
   int* off = nullptr;                 // device block tables
   std::vector<int> h_off;
-  int t_stage[3] = {0, 0, 0}, t_bc = 0, t_red = 0, t_chem = 0;   // table starts in off
-  int persist[3] = {0, 0, 0};         // persistent grid of each stage variant
+  int t_stage[kTileVariants] = {}, t_bc = 0, t_red = 0, t_chem = 0;   // table starts in off
+  int persist[kTileVariants] = {};    // persistent grid of each stage variant
   int tile = 0;
   std::vector<void*> U, aux;
   bool bound = false;
@@ -520,9 +523,11 @@ int setup_grids(jh_batch* b) {
   int dev = 0, cus = 0;
   BHIP_TRY(b, hipGetDevice(&dev));
   BHIP_TRY(b, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  int rc = persistent_grid<Real, 8, 3>(b, 0, cus);
-  if (!rc) rc = persistent_grid<Real, 8, 4>(b, 1, cus);
-  if (!rc) rc = persistent_grid<Real, 16, 4>(b, 2, cus);
+  int rc = JH_OK;
+#define JH_GRID(V, TX, WPE) \
+  if (!rc) rc = persistent_grid<Real, TX, WPE>(b, V, cus);
+  JH_TILE_VARIANTS(JH_GRID)
+#undef JH_GRID
   return rc;
 }
 
@@ -535,17 +540,15 @@ void launch_stage_batch(jh_batch* b, hipStream_t st, int stage, int parity, doub
   const int g = std::min(b->persist[v], total);
   const int* off = b->off + b->t_stage[v];
   switch (v) {
-    case 1:
-      stage_batch_kernel<Real, 8, 4><<<g, Tile<8>::NT, 0, st>>>(d, off, b->ncase, stage, parity,
-                                                                dt, cfl);
+#define JH_LAUNCH(V, TX, WPE)                                                           \
+    case V:                                                                             \
+      stage_batch_kernel<Real, TX, WPE><<<g, Tile<TX>::NT, 0, st>>>(d, off, b->ncase, stage, \
+                                                                   parity, dt, cfl);     \
       break;
-    case 2:
-      stage_batch_kernel<Real, 16, 4><<<g, Tile<16>::NT, 0, st>>>(d, off, b->ncase, stage,
-                                                                  parity, dt, cfl);
-      break;
+    JH_TILE_VARIANTS(JH_LAUNCH)
+#undef JH_LAUNCH
     default:
-      stage_batch_kernel<Real, 8, 3><<<g, Tile<8>::NT, 0, st>>>(d, off, b->ncase, stage, parity,
-                                                                dt, cfl);
+      break;
   }
 }
 
@@ -691,7 +694,7 @@ int jh_ctx_set_tile(jh_ctx* c, int variant) {
   return JH_OK;
 }
 
-int jh_tile_variants(void) { return 3; }
+int jh_tile_variants(void) { return kTileVariants; }
 
 int jh_tile_info(int variant, int* rows, int* threads, int* waves) {
   if (variant < 0 || variant >= 3 || !rows || !threads || !waves) return JH_ERR_ARG;
@@ -974,14 +977,14 @@ int jh_batch_create(int ncase, const int* nx, const int* ny, const double* dx, c
   for (int k = 0; k < ncase; ++k) b->inj[(size_t)k * JH_NS] = 300.0;
   // block tables: stage tiles per variant, bc lines, reduce, chemistry cells
   const int stride = ncase + 1;
-  b->h_off.assign(6 * stride, 0);
-  for (int v = 0; v < 3; ++v) b->t_stage[v] = v * stride;
-  b->t_bc = 3 * stride;
-  b->t_red = 4 * stride;
-  b->t_chem = 5 * stride;
+  b->h_off.assign((kTileVariants + 3) * stride, 0);
+  for (int v = 0; v < kTileVariants; ++v) b->t_stage[v] = v * stride;
+  b->t_bc = kTileVariants * stride;
+  b->t_red = (kTileVariants + 1) * stride;
+  b->t_chem = (kTileVariants + 2) * stride;
   for (int k = 0; k < ncase; ++k) {
     const long long n = (long long)nx[k] * ny[k];
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < kTileVariants; ++v) {
       const int tx = kTileRows[v];
       b->h_off[b->t_stage[v] + k + 1] =
           b->h_off[b->t_stage[v] + k] + ((nx[k] + tx - 1) / tx) * ((ny[k] + TY - 1) / TY);
