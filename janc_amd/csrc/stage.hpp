@@ -599,11 +599,18 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
       }
     }
     __syncthreads();
-    // prefetch the next variable (consumed after two barriers)
+    // prefetch the next variable (consumed after two barriers) and this
+    // variable's U^n for the RK combine (consumed in phase 3)
     if (cv + 1 < NU) {
 #pragma unroll
       for (int s = 0; s < NSL; ++s)
         if (ref[s].ok) nxt[s] = ref[s].u[(cv + 1) * ref[s].stride];
+    }
+    Real u0n[2] = {(Real)0, (Real)0};
+    if (MODE != MODE_RHS && stage != 0) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+        if (own[s]) u0n[s] = Un[cv * n + oidx[s]];
     }
     // ---- phase 2: face reconstructions ----
     for (int it = t; it < NXF + NYF; it += NT) {
@@ -638,12 +645,17 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
       } else if (stage == 0) {
         un = uc_own[s] + rhs;
       } else {
-        const Real u0 = own[s] ? Un[cv * n + oidx[s]] : (Real)0;
+        const Real u0 = u0n[s];
         if (stage == 1) un = (Real)(3.0 / 4.0) * u0 + (Real)(1.0 / 4.0) * (uc_own[s] + rhs);
         else un = (Real)(1.0 / 3.0) * u0 + (Real)(2.0 / 3.0) * (uc_own[s] + rhs);
       }
       if (own[s]) Uo[cv * n + oidx[s]] = un;
     }
+    // Drain this variable's memory operations here: loads left pending on
+    // the stage-uniform branches above make the compiler wait for every
+    // outstanding load (the next variable's prefetch included) inside the
+    // face loop.  vmcnt(0), expcnt and lgkmcnt at their maxima (no wait).
+    __builtin_amdgcn_s_waitcnt(0x0F70);
   }
   if (MODE == MODE_RHS) return;
 
