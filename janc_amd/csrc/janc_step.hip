@@ -697,7 +697,7 @@ int jh_ctx_set_tile(jh_ctx* c, int variant) {
 int jh_tile_variants(void) { return kTileVariants; }
 
 int jh_tile_info(int variant, int* rows, int* threads, int* waves) {
-  if (variant < 0 || variant >= 3 || !rows || !threads || !waves) return JH_ERR_ARG;
+  if (variant < 0 || variant >= kTileVariants || !rows || !threads || !waves) return JH_ERR_ARG;
   *rows = kTileRows[variant];
   *threads = 32 * kTileRows[variant];
   *waves = kTileWaves[variant];

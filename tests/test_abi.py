@@ -55,3 +55,18 @@ def test_argument_errors_without_gpu():
     assert lib.jh_ctx_create(8, 8, 1.0, 1.0, bad, None, 0, cp, consts.size, None,
                              ctypes.byref(out)) == _lib.JH_ERR_ARG
     assert lib.jh_advance(None, None, None, None, None, 1.0, 0.0, 1, None) == _lib.JH_ERR_ARG
+
+
+def test_tile_variants_are_described():
+    """Every stage-kernel variant has a description (no GPU call)."""
+    import ctypes
+    from janc_amd import _lib
+    lib = _lib.load()
+    n = lib.jh_tile_variants()
+    assert n >= 3
+    for v in range(n):
+        r, t, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        assert lib.jh_tile_info(v, ctypes.byref(r), ctypes.byref(t), ctypes.byref(w)) == 0
+        assert r.value in (8, 16, 32) and t.value == 32 * r.value and 1 <= w.value <= 8
+    r = ctypes.c_int()
+    assert lib.jh_tile_info(n, ctypes.byref(r), ctypes.byref(r), ctypes.byref(r)) == _lib.JH_ERR_ARG
