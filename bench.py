@@ -32,6 +32,13 @@ FP64_VECTOR_PEAK_TFLOPS = 78.6
 #   chemistry: read U (12) + T (1), write rho*Y (8) + aux (2)      = 23
 WORDS = {"stage": (27 + 39 + 39) / 3.0, "chem": 23.0, "reduce": 13.0, "bc": 0.0}
 
+# Measured HBM bytes per cell per stage-kernel launch of the FP64 headline
+# step (rocprofv3 FETCH_SIZE + WRITE_SIZE passes, each scaled by the same
+# counter's reading for a 256 MiB device-to-device copy; summary under
+# profiles/).  Reads exceed the algorithmic 25 words: each region cell's
+# state is read for its primitives and again per variable.
+STAGE_TRAFFIC_B_PER_CELL_FP64 = 460.6
+
 
 def _run(sim, U, aux, steps, warmup, torch):
     """Warm up, then time `steps` fused steps; returns (seconds, per-kernel)."""
@@ -390,7 +397,8 @@ def main():
                    "health_flags": flags},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": r["achieved_GBs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["frac_hbm"],
-                     "traffic": None},
+                     "traffic": STAGE_TRAFFIC_B_PER_CELL_FP64 * cells if dom == "stage" else None,
+                     "algorithmic_bytes": r["algorithmic_bytes"]},
         "kernels_fp64": {k: {"ms_total": v[0], "launches": v[1]} for k, v in kt.items()},
         "autotune_fp64": tuned,
         "roofline_all_fp64": roof,
