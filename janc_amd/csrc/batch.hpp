@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void reduce_batch_kernel(const CaseDesc<Real>*
 }
 
 template <typename Real>
-__global__ __launch_bounds__(256) void bc_batch_kernel(const CaseDesc<Real>* __restrict__ d,
+__global__ __launch_bounds__(kBcThreads) void bc_batch_kernel(const CaseDesc<Real>* __restrict__ d,
                                                        const int* __restrict__ off, int ncase,
                                                        int stage, int parity, double dt_fixed,
                                                        double cfl) {
@@ -122,9 +122,10 @@ __global__ __launch_bounds__(256) void chem_batch_kernel(const CaseDesc<Real>* _
   const int k = find_case(off, ncase, blockIdx.x);
   const CaseDesc<Real>& cd = d[k];
   const StepArgs<Real> a = case_args(cd, parity, dt_fixed, cfl);
+  __shared__ MixTables mix;
   chem_body<Real, CHEM_STEP>(a, cd.U, cd.aux, nullptr,
                              dt_hist ? dt_hist + (size_t)k * hist_len + s : nullptr,
-                             blockIdx.x - off[k], red, redb);
+                             blockIdx.x - off[k], red, redb, mix);
 }
 
 }  // namespace jh

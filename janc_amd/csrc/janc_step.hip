@@ -160,7 +160,10 @@ void launch_stage(jh_ctx* c, const StepArgs<Real>& a, int stage, const Real* Us,
 
 int tile_rows(const jh_ctx* c) { return kTileRows[c->tile]; }
 
-int bc_blocks(const jh_ctx* c) { return (2 * c->nx + 2 * c->ny + 255) / 256; }
+// bc_kernel: one thread per boundary line in blocks of kBcThreads (small
+// blocks spread the few thousand lines over many CUs: a line of an
+// injection-inlet side runs a 20-step bisection)
+int bc_blocks(const jh_ctx* c) { return (2 * c->nx + 2 * c->ny + kBcThreads - 1) / kBcThreads; }
 
 // Prologue for a state handed in by the caller: zero this parity's slots and
 // reduce the interior maxima (stage-1 LF alpha, CFL).
@@ -189,16 +192,16 @@ int advance_impl(jh_ctx* c, const Real* U, const Real* aux, Real* Uo, Real* auxo
     auto a = make_args<Real>(c, dt, cfl);
     const int bb = bc_blocks(c);
     // stage 1: (Un, An) -> (U1, A1)
-    { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, 256, 0, c->stream>>>(a, Un, An, 0); }
+    { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, Un, An, 0); }
     { Timed t(c, JH_KERNEL_STAGE);
       launch_stage<Real, MODE_RK>(c, a, 0, Un, An, Un, U1, A1); }
     // stage 2: (U1, A1) -> (U2, A2)
-    { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, 256, 0, c->stream>>>(a, U1, A1, 1); }
+    { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, U1, A1, 1); }
     { Timed t(c, JH_KERNEL_STAGE);
       launch_stage<Real, MODE_RK>(c, a, 1, U1, A1, Un, U2, A2); }
     // stage 3: (U2, A2) -> (Uo, auxo); Uo may alias Un (each thread reads
     // Un at its own cell before writing it)
-    { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, 256, 0, c->stream>>>(a, U2, A2, 2); }
+    { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, U2, A2, 2); }
     { Timed t(c, JH_KERNEL_STAGE);
       launch_stage<Real, MODE_RK>(c, a, 2, U2, A2, Un, Uo, auxo); }
     // chemistry + final update_aux in place, next step's maxima
@@ -220,7 +223,7 @@ int rhs_impl(jh_ctx* c, const Real* U, const Real* aux, double dt, Real* out) {
   int rc = prologue<Real>(c, U, aux, dt, 0.0);
   if (rc) return rc;
   auto a = make_args<Real>(c, dt, 0.0);
-  bc_kernel<Real><<<bc_blocks(c), 256, 0, c->stream>>>(a, U, aux, 0);
+  bc_kernel<Real><<<bc_blocks(c), kBcThreads, 0, c->stream>>>(a, U, aux, 0);
   launch_stage<Real, MODE_RHS>(c, a, 0, U, aux, U, out, nullptr);
   return check_launch(c, "rhs kernels");
 }
@@ -346,7 +349,7 @@ int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, cons
   auto a = make_args<double>(c, dt, 0.0);
   const int bb = bc_blocks(c);
   for (int s = 0; s < 3; ++s) {
-    bc_kernel<double><<<bb, 256, 0, st>>>(a, w.U[s], w.A[s], s);
+    bc_kernel<double><<<bb, kBcThreads, 0, st>>>(a, w.U[s], w.A[s], s);
     launch_stage<double, MODE_RK>(c, a, s, w.U[s], w.A[s], U0, w.U[s + 1], w.A[s + 1]);
   }
   HIP_TRY(c, hipMemcpyAsync(w.U[4], w.U[3], NU * n * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -565,7 +568,7 @@ int batch_enqueue(jh_batch* b, hipStream_t st, double dt, double cfl, int nsteps
   for (int s = 0; s < nsteps; ++s) {
     const int parity = s & 1;
     for (int stage = 0; stage < 3; ++stage) {
-      bc_batch_kernel<Real><<<n_bc, 256, 0, st>>>(d, b->off + b->t_bc, nc, stage, parity, dt,
+      bc_batch_kernel<Real><<<n_bc, kBcThreads, 0, st>>>(d, b->off + b->t_bc, nc, stage, parity, dt,
                                                   cfl);
       launch_stage_batch<Real>(b, st, stage, parity, dt, cfl);
     }
@@ -793,13 +796,13 @@ int jh_lf_alpha(jh_ctx* c, const void* U, const void* aux, double* alpha_dev) {
     rc = prologue<double>(c, (const double*)U, (const double*)aux, 1.0, 0.0);
     if (rc) return rc;
     auto a = make_args<double>(c, 1.0, 0.0);
-    bc_kernel<double><<<bc_blocks(c), 256, 0, c->stream>>>(a, (const double*)U,
+    bc_kernel<double><<<bc_blocks(c), kBcThreads, 0, c->stream>>>(a, (const double*)U,
                                                             (const double*)aux, 0);
   } else {
     rc = prologue<float>(c, (const float*)U, (const float*)aux, 1.0, 0.0);
     if (rc) return rc;
     auto a = make_args<float>(c, 1.0, 0.0);
-    bc_kernel<float><<<bc_blocks(c), 256, 0, c->stream>>>(a, (const float*)U,
+    bc_kernel<float><<<bc_blocks(c), kBcThreads, 0, c->stream>>>(a, (const float*)U,
                                                           (const float*)aux, 0);
   }
   copy_scalars_kernel<<<1, 64, 0, c->stream>>>(c->sc, c->parity, 0.0, nullptr, alpha_dev);
@@ -888,13 +891,13 @@ int jh_step_stage(jh_ctx* c, int stage, const void* Us, const void* As, const vo
   if (rc) return rc;
   if (c->precision == JH_PREC_FP64) {
     auto a = make_args<double>(c, dt, cfl);
-    bc_kernel<double><<<bc_blocks(c), 256, 0, c->stream>>>(a, (const double*)Us,
+    bc_kernel<double><<<bc_blocks(c), kBcThreads, 0, c->stream>>>(a, (const double*)Us,
                                                             (const double*)As, stage);
     launch_stage<double, MODE_RK>(c, a, stage, (const double*)Us, (const double*)As,
                                   (const double*)Un, (double*)Uo, (double*)Ao);
   } else {
     auto a = make_args<float>(c, dt, cfl);
-    bc_kernel<float><<<bc_blocks(c), 256, 0, c->stream>>>(a, (const float*)Us,
+    bc_kernel<float><<<bc_blocks(c), kBcThreads, 0, c->stream>>>(a, (const float*)Us,
                                                           (const float*)As, stage);
     launch_stage<float, MODE_RK>(c, a, stage, (const float*)Us, (const float*)As,
                                  (const float*)Un, (float*)Uo, (float*)Ao);
@@ -989,7 +992,8 @@ int jh_batch_create(int ncase, const int* nx, const int* ny, const double* dx, c
       b->h_off[b->t_stage[v] + k + 1] =
           b->h_off[b->t_stage[v] + k] + ((nx[k] + tx - 1) / tx) * ((ny[k] + TY - 1) / TY);
     }
-    b->h_off[b->t_bc + k + 1] = b->h_off[b->t_bc + k] + (2 * nx[k] + 2 * ny[k] + 255) / 256;
+    b->h_off[b->t_bc + k + 1] =
+        b->h_off[b->t_bc + k] + (2 * nx[k] + 2 * ny[k] + kBcThreads - 1) / kBcThreads;
     const int cb = (int)((n + 255) / 256);
     b->h_off[b->t_red + k + 1] = b->h_off[b->t_red + k] + std::min(cb, 64);
     b->h_off[b->t_chem + k + 1] = b->h_off[b->t_chem + k] + cb;

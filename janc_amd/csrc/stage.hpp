@@ -17,6 +17,7 @@
 namespace jh {
 
 constexpr int NU = 4 + (JH_NS - 1);   // conserved variables (12)
+constexpr int kBcThreads = 64;        // bc_kernel block size
 constexpr int NV = NU + 2;            // + aux (gamma, T) in ghost strips
 
 // ---------------------------------------------------------------- helpers
@@ -71,6 +72,18 @@ __device__ __forceinline__ void block_max_bits_atomic(unsigned long long v,
 }
 
 __device__ __forceinline__ double nan_to_zero(double v) { return v == v ? v : 0.0; }
+
+// Block-cooperative copy of MechConst's mixture tables into t (LDS); the
+// caller synchronises the block before reading t.
+__device__ __forceinline__ void load_mix_tables(const MechConst& c, MixTables& t) {
+  constexpr int n = 2 * JH_NS * 11;   // cp[2][NS][5] then h[2][NS][6], contiguous in both
+  static_assert(sizeof(MixTables) == (n + 1) * sizeof(double), "MixTables layout");
+  const double* src = &c.cp[0][0][0];
+  double* dst = &t.cp[0][0][0];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+  if (threadIdx.x == 0) t.Tcr = c.Tcr;
+}
+
 
 // Primitive state of one cell (aux_func.py:39-51), in compute type W.
 template <typename W>
@@ -184,9 +197,10 @@ __device__ __forceinline__ void inj_model(double p, double R, double P0, double 
   for (int i = 0; i < 20; ++i) {
     const double M = 0.5 * (M1 + M2);
     const double f = 1 + (g - 1) / 2 * (M * M);
-    const double P1 = P0 * pow(f, -g / (g - 1));
-    const double V1 = M * pow(f, -0.5) * C0;
-    MFC = A1 * P0 / sqrt(T0) * sqrt(g / R) * M * pow(f, -(g + 1) / 2 / (g - 1));
+    const double lf = log(f);           // the three powers of f from one log
+    const double P1 = P0 * exp(-g / (g - 1) * lf);
+    const double V1 = M * rsqrt(f) * C0;
+    MFC = A1 * P0 / sqrt(T0) * sqrt(g / R) * M * exp(-(g + 1) / 2 / (g - 1) * lf);
     const double A = 0.5;
     const double B = g / (g - 1) * P3 * A3 / MFC;
     const double C = -g / (g - 1) * R * T0;
@@ -361,12 +375,12 @@ __device__ __forceinline__ void bc_body(const StepArgs<Real>& a, const Real* __r
       my = fmax(my, nan_to_zero(fabs(q.v) + q.a));
     }
   }
-  block_max_atomic<256>(mx, &a.sc->amax[a.parity][stage][0], red);
-  block_max_atomic<256>(my, &a.sc->amax[a.parity][stage][1], red);
+  block_max_atomic<kBcThreads>(mx, &a.sc->amax[a.parity][stage][0], red);
+  block_max_atomic<kBcThreads>(my, &a.sc->amax[a.parity][stage][1], red);
 }
 
 template <typename Real>
-__global__ __launch_bounds__(256) void bc_kernel(StepArgs<Real> a, const Real* __restrict__ U,
+__global__ __launch_bounds__(kBcThreads) void bc_kernel(StepArgs<Real> a, const Real* __restrict__ U,
                                                  const Real* __restrict__ aux, int stage) {
   __shared__ double red[4];
   bc_body<Real>(a, U, aux, stage, blockIdx.x, red);
@@ -485,6 +499,7 @@ struct StageLds {
   Real Fx[TX + 1][TY];
   Real Fy[TX][TY + 1];
   double red[Tile<TX>::NW];
+  MixTables mix;              // NASA-7 tables for the Halley epilogue
 };
 
 // One tile (index `tile` in y-fastest order) of one RK stage.
@@ -507,6 +522,7 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
   (void)NW;
 
   const MechConst& c = *a.mc;
+  if (MODE != MODE_RHS) load_mix_tables(c, L.mix);   // read after the phase barriers
   const int nx = a.nx, ny = a.ny, n = nx * ny;
   const int t = threadIdx.x, ly = t & 63, w = t >> 6;
   const int tiles_y = (ny + TY - 1) / TY;
@@ -676,7 +692,7 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
     const double R = fill_Y_get_R(c, Y);
     double T = (double)auxs[n + oidx[s]];
     double gamma;
-    halley_T(c, Y, R, e, T, gamma);
+    halley_T(L.mix, Y, R, e, T, gamma);
     auxo[oidx[s]] = (Real)gamma;
     auxo[n + oidx[s]] = (Real)T;
     if (stage < 2) {
@@ -714,8 +730,12 @@ enum ChemMode : int { CHEM_STEP = 0, CHEM_SOURCE = 1 };
 template <typename Real, int MODE>
 __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real* aux, Real* S,
                                           double* dt_hist, int blk, double* red,
-                                          unsigned long long* redb) {
+                                          unsigned long long* redb, MixTables& mix) {
   const MechConst& c = *a.mc;
+  if (MODE == CHEM_STEP) {
+    load_mix_tables(c, mix);
+    __syncthreads();
+  }
   const int n = a.nx * a.ny;
   const double dt = step_dt(a);
   if (MODE == CHEM_STEP && blk == 0 && threadIdx.x == 0) {
@@ -761,7 +781,7 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
     for (int k = 0; k < JH_NS - 1; ++k) Y[k] = (double)Un[4 + k] * ir;
     const double R = fill_Y_get_R(c, Y);
     double gamma;
-    halley_T(c, Y, R, e, T, gamma);
+    halley_T(mix, Y, R, e, T, gamma);
     aux[i] = (Real)gamma;
     aux[n + i] = (Real)T;
     const unsigned int f = (r2 > 0.0 ? 0u : (unsigned)JH_FLAG_NEGATIVE_RHO) |
@@ -790,7 +810,8 @@ __global__ __launch_bounds__(256) void chem_kernel(StepArgs<Real> a, Real* U, Re
                                                    Real* S, double* dt_hist) {
   __shared__ double red[4];
   __shared__ unsigned long long redb[4];
-  chem_body<Real, MODE>(a, U, aux, S, dt_hist, blockIdx.x, red, redb);
+  __shared__ MixTables mix;
+  chem_body<Real, MODE>(a, U, aux, S, dt_hist, blockIdx.x, red, redb, mix);
 }
 
 // update_aux alone (aux_func.py:21-29), for the unit entry point.

@@ -21,10 +21,22 @@ struct MixPoly {
   double h[6];
 };
 
+// The NASA-7 tables the mixture routines read (cp and h of both ranges and
+// the switch temperature).  MechConst has the same members; kernels that
+// evaluate the mixture with a per-lane range index read a copy in LDS
+// (load_mix_tables), where a per-lane address costs no vector-memory round
+// trip.
+struct MixTables {
+  double cp[2][JH_NS][5];
+  double h[2][JH_NS][6];
+  double Tcr;
+};
+
 // The range index is a runtime (per-lane) value on purpose: with a
 // compile-time index the compiler hoists both ranges' ~200 constants into
 // registers wherever the mixture is evaluated more than once.
-__host__ __device__ __forceinline__ void mix_poly(const MechConst& c, int r,
+template <typename Tab>
+__host__ __device__ __forceinline__ void mix_poly(const Tab& c, int r,
                                                   const double (&Y)[JH_NS],
                                                   MixPoly& m) {
 #pragma unroll
@@ -70,7 +82,8 @@ __host__ __device__ __forceinline__ double fill_Y_get_R(const MechConst& c,
 }
 
 // gamma = cp/(cp - R) of the mixture at T (get_thermo_nasa7).
-__host__ __device__ __forceinline__ double mix_gamma(const MechConst& c,
+template <typename Tab>
+__host__ __device__ __forceinline__ double mix_gamma(const Tab& c,
                                                      const double (&Y)[JH_NS],
                                                      double R, double T,
                                                      double* h_out) {
@@ -86,7 +99,8 @@ __host__ __device__ __forceinline__ double mix_gamma(const MechConst& c,
 // The reference stops when the grid-wide max|res| <= tol; here each cell
 // stops on its own residual.  A converged cell's further Halley updates in
 // the reference move T by O(1e-16) relative, so both forms agree to rounding.
-__host__ __device__ __forceinline__ void halley_T(const MechConst& c,
+template <typename Tab>
+__host__ __device__ __forceinline__ void halley_T(const Tab& c,
                                                   const double (&Y)[JH_NS],
                                                   double R, double e, double& T,
                                                   double& gamma) {
