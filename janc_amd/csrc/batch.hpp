@@ -123,9 +123,10 @@ __global__ __launch_bounds__(256) void chem_batch_kernel(const CaseDesc<Real>* _
   const CaseDesc<Real>& cd = d[k];
   const StepArgs<Real> a = case_args(cd, parity, dt_fixed, cfl);
   __shared__ MixTables mix;
+  __shared__ GibbsTables gt;
   chem_body<Real, CHEM_STEP>(a, cd.U, cd.aux, nullptr,
                              dt_hist ? dt_hist + (size_t)k * hist_len + s : nullptr,
-                             blockIdx.x - off[k], red, redb, mix);
+                             blockIdx.x - off[k], red, redb, mix, gt);
 }
 
 }  // namespace jh

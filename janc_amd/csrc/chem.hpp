@@ -10,8 +10,19 @@
 
 namespace jh {
 
+// The tables gibbs() reads.  MechConst has the same members; the chemistry
+// kernel reads a copy in LDS (the range index is per lane).
+struct GibbsTables {
+  double hc[2][JH_NA][6];
+  double s[2][JH_NA][5];
+  double logc[2][JH_NA];
+  double Tcr;
+  double T0;
+};
+
 // g_k = s_k/R - h_k/(R T) for the active species (get_gibbs, :78-89).
-__host__ __device__ __forceinline__ void gibbs(const MechConst& c, double T, double invT,
+template <typename Tab>
+__host__ __device__ __forceinline__ void gibbs(const Tab& c, double T, double invT,
                                                double (&g)[JH_NA]) {
   const int r = T < c.Tcr ? 0 : 1;
   const double lT = log(c.T0 * T);
@@ -85,8 +96,11 @@ __host__ __device__ __forceinline__ void lu_solve(double (&A)[N][N], double (&b)
 }
 
 // Increment of rho*Y_k for the active species (detailed_reaction, :183-193).
-// rhoY_in: the state's rho*Y_k for the NS-1 transported species.
-__host__ __device__ __forceinline__ void chem_increment(const MechConst& c, double rho,
+// rhoY_in: the state's rho*Y_k for the NS-1 transported species; gt: the
+// Gibbs tables (c itself, or its copy in LDS).
+template <typename GTab>
+__host__ __device__ __forceinline__ void chem_increment(const MechConst& c, const GTab& gt,
+                                                        double rho,
                                                         const double (&rhoY_in)[JH_NS - 1],
                                                         double T, double dt,
                                                         double (&drhoY)[JH_NA]) {
@@ -109,7 +123,7 @@ __host__ __device__ __forceinline__ void chem_increment(const MechConst& c, doub
   }
   const double invT = rcp(T);
   double g[JH_NA];
-  gibbs(c, T, invT, g);
+  gibbs(gt, T, invT, g);
   double eg[JH_NA], ieg[JH_NA];
 #pragma unroll
   for (int k = 0; k < JH_NA; ++k) {
@@ -131,6 +145,13 @@ __host__ __device__ __forceinline__ void chem_increment(const MechConst& c, doub
     dY = fmin(fmax(dY, -Y[k]), 1.0 - Y[k]);
     drhoY[k] = rho * dY;
   }
+}
+
+__host__ __device__ __forceinline__ void chem_increment(const MechConst& c, double rho,
+                                                        const double (&rhoY_in)[JH_NS - 1],
+                                                        double T, double dt,
+                                                        double (&drhoY)[JH_NA]) {
+  chem_increment(c, c, rho, rhoY_in, T, dt, drhoY);
 }
 
 }  // namespace jh

@@ -73,6 +73,20 @@ __device__ __forceinline__ void block_max_bits_atomic(unsigned long long v,
 
 __device__ __forceinline__ double nan_to_zero(double v) { return v == v ? v : 0.0; }
 
+// Block-cooperative copy of MechConst's Gibbs tables into t (LDS); the
+// caller synchronises the block before reading t.
+__device__ __forceinline__ void load_gibbs_tables(const MechConst& c, GibbsTables& t) {
+  constexpr int n = 2 * JH_NA * 12;   // hc[2][NA][6], s[2][NA][5], logc[2][NA], contiguous
+  static_assert(sizeof(GibbsTables) == (n + 2) * sizeof(double), "GibbsTables layout");
+  const double* src = &c.hc[0][0][0];
+  double* dst = &t.hc[0][0][0];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+  if (threadIdx.x == 0) {
+    t.Tcr = c.Tcr;
+    t.T0 = c.T0;
+  }
+}
+
 // Block-cooperative copy of MechConst's mixture tables into t (LDS); the
 // caller synchronises the block before reading t.
 __device__ __forceinline__ void load_mix_tables(const MechConst& c, MixTables& t) {
@@ -730,12 +744,12 @@ enum ChemMode : int { CHEM_STEP = 0, CHEM_SOURCE = 1 };
 template <typename Real, int MODE>
 __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real* aux, Real* S,
                                           double* dt_hist, int blk, double* red,
-                                          unsigned long long* redb, MixTables& mix) {
+                                          unsigned long long* redb, MixTables& mix,
+                                          GibbsTables& gt) {
   const MechConst& c = *a.mc;
-  if (MODE == CHEM_STEP) {
-    load_mix_tables(c, mix);
-    __syncthreads();
-  }
+  if (MODE == CHEM_STEP) load_mix_tables(c, mix);
+  load_gibbs_tables(c, gt);
+  __syncthreads();
   const int n = a.nx * a.ny;
   const double dt = step_dt(a);
   if (MODE == CHEM_STEP && blk == 0 && threadIdx.x == 0) {
@@ -757,7 +771,7 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
 #pragma unroll
     for (int k = 0; k < JH_NS - 1; ++k) rY[k] = Uc[4 + k];
     double d[JH_NA];
-    chem_increment(c, rho, rY, T, dt, d);
+    chem_increment(c, gt, rho, rY, T, dt, d);
     if (MODE == CHEM_SOURCE) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) S[v * n + i] = (Real)0;
@@ -811,7 +825,8 @@ __global__ __launch_bounds__(256) void chem_kernel(StepArgs<Real> a, Real* U, Re
   __shared__ double red[4];
   __shared__ unsigned long long redb[4];
   __shared__ MixTables mix;
-  chem_body<Real, MODE>(a, U, aux, S, dt_hist, blockIdx.x, red, redb, mix);
+  __shared__ GibbsTables gt;
+  chem_body<Real, MODE>(a, U, aux, S, dt_hist, blockIdx.x, red, redb, mix, gt);
 }
 
 // update_aux alone (aux_func.py:21-29), for the unit entry point.

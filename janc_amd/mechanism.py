@@ -435,7 +435,8 @@ def _generate_vjp(m):
         dls = " ".join(dgsum)
         e("    const double dlb = dlk %s + %s * invT;   // d ln kb / dT at fixed X" %
           (dls, _fmt(vs)))
-        e("    const double q = kf - kb;")
+        if r.three_body:
+            e("    const double q = kf - kb;")
         dnu = {}
         for k in range(na):
             d = r.products.get(k, 0.0) - r.reactants.get(k, 0.0)
