@@ -7,12 +7,11 @@
 // routed to its active branch.
 //
 // Data path of the flux VJP (one RK stage): the stage input is padded with
-// its ghost cells (pad_kernel); per x- and y-face, the WENO gradients times
-// the face cotangent are written as ten coefficients per variable
-// (face_vjp_kernel); per padded cell these are gathered into H+-_bar and
-// pushed through the split flux and the primitive map (cell_vjp_kernel); the
-// LF maxima's cotangent goes to the argmax cells (alpha_*); ghost-cell
-// cotangents are folded back onto their source cells (unpad_kernel).
+// its ghost cells (pad_kernel); per padded cell, H+-_bar is gathered from
+// the WENO gradients of the faces that reach it and pushed through the split
+// flux and the primitive map (flux_vjp_tile_kernel); the LF maxima's
+// cotangent goes to the argmax cells (alpha_*); ghost-cell cotangents are
+// folded back onto their source cells (unpad_kernel).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -132,65 +131,6 @@ __device__ __forceinline__ double phys_flux_d(int cv, int dir, const PrimD& q, d
   }
 }
 
-// Face cotangent coefficients.  Faces along DIR (0: x-faces (nx+1) x ny,
-// 1: y-faces nx x (ny+1)); coefficient plane (cv*10 + m): m < 5 left-biased
-// WENO of H+ at stencil slot m, m >= 5 right-biased WENO of H- at slot m-5,
-// each multiplied by the face cotangent F_bar = dt/dh (rbar_f - rbar_{f-1}).
-template <int DIR>
-__global__ __launch_bounds__(256) void face_vjp_kernel(AdjArgs a, const double* __restrict__ Up,
-                                                       const double* __restrict__ ap,
-                                                       const double* __restrict__ rbar,
-                                                       const unsigned long long* alpha_bits,
-                                                       double* __restrict__ coef) {
-  const MechConst& c = *a.mc;
-  const int nx = a.nx, ny = a.ny, n = nx * ny, PY = ny + 6, np = (nx + 6) * PY;
-  const int nf = DIR == 0 ? (nx + 1) * ny : nx * (ny + 1);
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nf) return;
-  int f, row;   // face index along DIR and the cross index
-  if (DIR == 0) {
-    f = t / ny;
-    row = t - f * ny;
-  } else {
-    row = t / (ny + 1);
-    f = t - row * (ny + 1);
-  }
-  const double alpha = bitsd(alpha_bits[DIR]);
-  // six cells of the stencil: interior index f-3 .. f+2 along DIR
-  PrimD q[6];
-  int pt[6];
-#pragma unroll
-  for (int m = 0; m < 6; ++m) {
-    pt[m] = DIR == 0 ? (f + m) * PY + (row + 3) : (row + 3) * PY + (f + m);
-    q[m] = prim_at(c, Up, ap, np, pt[m]);
-  }
-  const double dtdh = a.dt / (DIR == 0 ? a.dx : a.dy);
-  for (int cv = 0; cv < NU; ++cv) {
-    // cotangent of rhs at the cells either side of the face
-    const int cl = DIR == 0 ? (f - 1) * ny + row : row * ny + (f - 1);
-    const int cr = DIR == 0 ? f * ny + row : row * ny + f;
-    const double rl = (f - 1 >= 0) ? rbar[cv * n + cl] : 0.0;
-    const double rr = (f < (DIR == 0 ? nx : ny)) ? rbar[cv * n + cr] : 0.0;
-    const double Fb = dtdh * (rr - rl);
-    double hp[6], hm[6];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
-      const double ucv = Up[cv * np + pt[m]];
-      const double F = phys_flux_d(cv, DIR, q[m], ucv);
-      hp[m] = 0.5 * (F + alpha * ucv);
-      hm[m] = 0.5 * (F - alpha * ucv);
-    }
-    double gL[5], gR[5];
-    weno5_grad(hp[0], hp[1], hp[2], hp[3], hp[4], gL);
-    weno5_grad(hm[5], hm[4], hm[3], hm[2], hm[1], gR);
-#pragma unroll
-    for (int m = 0; m < 5; ++m) {
-      coef[(size_t)(cv * 10 + m) * nf + t] = Fb * gL[m];
-      coef[(size_t)(cv * 10 + 5 + m) * nf + t] = Fb * gR[m];
-    }
-  }
-}
-
 // Cotangents of the primitive map at one cell -> cotangents of U and aux.
 // Inputs: rho_b, u_b, v_b, p_b, a_b (sound speed), Y_b[0:NS-1] and direct
 // U_b contributions; see aux_func.py:39-51 for the forward map.
@@ -224,100 +164,269 @@ __device__ __forceinline__ void prim_to_U_vjp(const MechConst& c, const PrimD& q
   Ub[0] += rhob;
 }
 
-// Per padded cell: gather H+-_bar from the face coefficients, apply the split
-// flux and primitive VJPs; also this cell's share of the LF-alpha cotangent
-// sums (per-block partials, reduced in a fixed order afterwards).
-__global__ __launch_bounds__(256) void cell_vjp_kernel(AdjArgs a, const double* __restrict__ Up,
-                                                       const double* __restrict__ ap,
-                                                       const double* __restrict__ cx,
-                                                       const double* __restrict__ cy,
-                                                       const unsigned long long* alpha_bits,
-                                                       double* __restrict__ Ubp,
-                                                       double* __restrict__ abp,
-                                                       double* __restrict__ part) {
-  __shared__ double red[2][4];
+// ------------------------------------------------------------------
+// Flux VJP of one RK stage, tiled: no face quantities are staged in HBM.
+// Tile = TXA padded rows x 64
+// padded columns, one cell per thread.  Padded cell P receives, per
+// direction and variable, H+_bar = sum_m F_bar(f = P - m) dq_L/dv_m(face f)
+// and H-_bar = sum_m F_bar(f = P - 5 + m) dq_R/dv_m(face f), the WENO
+// gradients recomputed from H+- of the 9-cell neighbourhood held in LDS (the
+// weights of a face are recomputed by each of the 5 cells it reaches, the
+// price of keeping every sum inside the tile); then the split-flux and
+// primitive VJPs of the reference's forward map.
+template <int TXA>
+struct AdjTile {
+  static constexpr int NT = 64 * TXA;          // threads (one padded cell each)
+  static constexpr int NW = TXA;               // waves
+  static constexpr int RX = TXA + 8;           // x-region rows (4-cell halo)
+  static constexpr int RY = TY + 8;            // y-region columns
+  static constexpr int NFX = TXA + 5;          // x-face rows reaching the tile
+  static constexpr int NFY = TY + 5;           // y-face columns reaching the tile
+  static constexpr int NHS = 8 / NW;           // x-halo rows per wave
+};
+
+template <int TXA>
+struct AdjLds {
+  double Hxp[TXA + 8][TY], Hxm[TXA + 8][TY];
+  double Hyp[TXA][TY + 8], Hym[TXA][TY + 8];
+  double Fbx[TXA + 5][TY];
+  double Fby[TXA][TY + 5];
+  double red[2][TXA];
+};
+
+struct PrimL {   // the primitives the physical flux needs
+  double rho, u, v, p;
+};
+
+__device__ __forceinline__ double phys_flux_l(int cv, int dir, const PrimL& q, double ucv) {
+  const double un = dir == 0 ? q.u : q.v;
+  switch (cv) {
+    case 0: return q.rho * un;
+    case 1: return dir == 0 ? q.rho * (q.u * q.u) + q.p : (q.rho * q.u) * q.v;
+    case 2: return dir == 0 ? (q.rho * q.u) * q.v : q.rho * (q.v * q.v) + q.p;
+    case 3: return un * (ucv + q.p);
+    default: return un * ucv;
+  }
+}
+
+template <int TXA>
+__global__ __launch_bounds__(AdjTile<TXA>::NT) void flux_vjp_tile_kernel(
+    AdjArgs a, const double* __restrict__ Up, const double* __restrict__ ap,
+    const double* __restrict__ rbar, const unsigned long long* alpha_bits,
+    double* __restrict__ Ubp, double* __restrict__ abp, double* __restrict__ part) {
+  using G = AdjTile<TXA>;
+  __shared__ AdjLds<TXA> L;
   const MechConst& c = *a.mc;
-  const int nx = a.nx, ny = a.ny, PY = ny + 6, np = (nx + 6) * PY;
-  const int nfx = (nx + 1) * ny, nfy = nx * (ny + 1);
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = a.nx, ny = a.ny, n = nx * ny, PX = nx + 6, PY = ny + 6, np = PX * PY;
+  const int t = threadIdx.x, ly = t & 63, w = t >> 6;
+  const int tiles_y = (PY + TY - 1) / TY;
+  const int X0 = (blockIdx.x / tiles_y) * TXA, Y0 = (blockIdx.x % tiles_y) * TY;
+  const double ax = bitsd(alpha_bits[0]), ay = bitsd(alpha_bits[1]);
+  const double dtdx = a.dt / a.dx, dtdy = a.dt / a.dy;
+
+  // region slots: 0 = own cell (x and y regions), 1..NHS = x-halo rows,
+  // NHS+1 = y-halo cell (t < 8 * TXA)
+  constexpr int NS_ = G::NHS + 2;
+  int px[NS_], py[NS_], xr[NS_], yc[NS_];
+  bool in[NS_];
+  px[0] = X0 + w;
+  py[0] = Y0 + ly;
+  xr[0] = w + 4;
+  yc[0] = ly + 4;
+#pragma unroll
+  for (int h = 0; h < G::NHS; ++h) {
+    const int hr = w + h * G::NW;                  // halo row 0..7
+    const int r = hr < 4 ? hr : TXA + hr;          // region rows 0..3, TXA+4..TXA+7
+    xr[1 + h] = r;
+    px[1 + h] = X0 - 4 + r;
+    py[1 + h] = Y0 + ly;
+    yc[1 + h] = -1;
+  }
+  {
+    constexpr int s = NS_ - 1;
+    const int row = t >> 3, k = t & 7;
+    yc[s] = t < 8 * TXA ? (k < 4 ? k : TY + k) : -1;
+    xr[s] = -1;
+    px[s] = X0 + row;
+    py[s] = Y0 - 4 + (k < 4 ? k : TY + k);
+  }
+  PrimL q[NS_];
+#pragma unroll
+  for (int s = 0; s < NS_; ++s) {
+    in[s] = px[s] >= 0 && px[s] < PX && py[s] >= 0 && py[s] < PY &&
+            (s < NS_ - 1 || t < 8 * TXA);
+    if (in[s]) {
+      const int pt = px[s] * PY + py[s];
+      const PrimD d = prim_at(c, Up, ap, np, pt);
+      q[s] = PrimL{d.rho, d.u, d.v, d.p};
+    } else {
+      q[s] = PrimL{1.0, 0.0, 0.0, 0.0};
+    }
+  }
+  // own cell: which face families reach it, and its source cell
+  const int P = px[0], Q = py[0], i = P - 3, j = Q - 3;
+  const bool own = in[0];
+  const bool xrow = own && j >= 0 && j < ny, ycol = own && i >= 0 && i < nx;
+  int sxx = 0, syy = 0, flip = 0;
+  const bool linear = own && pad_src(a, P, Q, sxx, syy, flip);
+  const bool work = linear && (xrow || ycol);
+  double Ub[NU];
+#pragma unroll
+  for (int v = 0; v < NU; ++v) Ub[v] = 0.0;
+  double rhob = 0.0, ub = 0.0, vb = 0.0, pb = 0.0;
+  double Yb[JH_NS - 1];
+#pragma unroll
+  for (int k = 0; k < JH_NS - 1; ++k) Yb[k] = 0.0;
   double sx_part = 0.0, sy_part = 0.0;
-  if (t < np) {
-    const int P = t / PY, Q = t - P * PY, i = P - 3, j = Q - 3;
-    const bool xrow = j >= 0 && j < ny, ycol = i >= 0 && i < nx;
-    int sxx, syy, flip;
-    const bool linear = pad_src(a, P, Q, sxx, syy, flip);
-    const double ax = bitsd(alpha_bits[0]), ay = bitsd(alpha_bits[1]);
-    double Ub[NU];
+
+  for (int cv = 0; cv < NU; ++cv) {
+    // ---- H+- of the region and the face cotangents ----
+    double ucv_own = 0.0;
 #pragma unroll
-    for (int v = 0; v < NU; ++v) Ub[v] = 0.0;
-    double Tb = 0.0, gb = 0.0;
-    if (linear && (xrow || ycol)) {
-      const PrimD q = prim_at(c, Up, ap, np, t);
-      double rhob = 0.0, ub = 0.0, vb = 0.0, pb = 0.0;
-      double Yb[JH_NS - 1];
+    for (int s = 0; s < NS_; ++s) {
+      double ucv = 0.0;
+      if (in[s]) ucv = Up[(size_t)cv * np + px[s] * PY + py[s]];
+      if (s == 0) ucv_own = ucv;
+      if (xr[s] >= 0) {
+        double hp = 0.0, hm = 0.0;
+        if (in[s]) {
+          const double F = phys_flux_l(cv, 0, q[s], ucv);
+          hp = 0.5 * (F + ax * ucv);
+          hm = 0.5 * (F - ax * ucv);
+        }
+        L.Hxp[xr[s]][ly] = hp;
+        L.Hxm[xr[s]][ly] = hm;
+      }
+      if (yc[s] >= 0) {
+        const int row = s == 0 ? w : (t >> 3);
+        double hp = 0.0, hm = 0.0;
+        if (in[s]) {
+          const double Gf = phys_flux_l(cv, 1, q[s], ucv);
+          hp = 0.5 * (Gf + ay * ucv);
+          hm = 0.5 * (Gf - ay * ucv);
+        }
+        L.Hyp[row][yc[s]] = hp;
+        L.Hym[row][yc[s]] = hm;
+      }
+    }
+    // x faces f = X0-5+k at column Q: F_bar = dt/dx (rbar_f - rbar_{f-1})
+    for (int e = t; e < G::NFX * TY; e += G::NT) {
+      const int k = e >> 6, col = e & 63;
+      const int f = X0 - 5 + k, jj = Y0 + col - 3;
+      double fb = 0.0;
+      if (f >= 0 && f <= nx && jj >= 0 && jj < ny) {
+        const double rl = f - 1 >= 0 ? rbar[(size_t)cv * n + (f - 1) * ny + jj] : 0.0;
+        const double rr = f < nx ? rbar[(size_t)cv * n + f * ny + jj] : 0.0;
+        fb = dtdx * (rr - rl);
+      }
+      L.Fbx[k][col] = fb;
+    }
+    // y faces g = Y0-5+k at row P
+    for (int e = t; e < TXA * G::NFY; e += G::NT) {
+      const int row = e / G::NFY, k = e - row * G::NFY;
+      const int g = Y0 - 5 + k, ii = X0 + row - 3;
+      double fb = 0.0;
+      if (g >= 0 && g <= ny && ii >= 0 && ii < nx) {
+        const double rl = g - 1 >= 0 ? rbar[(size_t)cv * n + ii * ny + (g - 1)] : 0.0;
+        const double rr = g < ny ? rbar[(size_t)cv * n + ii * ny + g] : 0.0;
+        fb = dtdy * (rr - rl);
+      }
+      L.Fby[row][k] = fb;
+    }
+    __syncthreads();
+    // ---- gather H+-_bar of the own cell from the faces that reach it ----
+    if (work) {
+      double hpx = 0.0, hmx = 0.0, hpy = 0.0, hmy = 0.0;
+      double gq[5];
+      if (xrow) {
 #pragma unroll
-      for (int k = 0; k < JH_NS - 1; ++k) Yb[k] = 0.0;
-      for (int cv = 0; cv < NU; ++cv) {
-        double hpx = 0.0, hmx = 0.0, hpy = 0.0, hmy = 0.0;
-        if (xrow) {
-#pragma unroll
-          for (int m = 0; m < 5; ++m) {
-            const int fL = i + 3 - m, fR = i - 2 + m;
-            if (fL >= 0 && fL <= nx) hpx += cx[(size_t)(cv * 10 + m) * nfx + fL * ny + j];
-            if (fR >= 0 && fR <= nx) hmx += cx[(size_t)(cv * 10 + 5 + m) * nfx + fR * ny + j];
+        for (int m = 0; m < 5; ++m) {
+          if (P - m >= 0 && P - m <= nx) {
+            const int r0 = w + 4 - m;
+            weno5_grad(L.Hxp[r0][ly], L.Hxp[r0 + 1][ly], L.Hxp[r0 + 2][ly], L.Hxp[r0 + 3][ly],
+                       L.Hxp[r0 + 4][ly], gq);
+            hpx += L.Fbx[w + 5 - m][ly] * gq[m];
           }
         }
-        if (ycol) {
 #pragma unroll
-          for (int m = 0; m < 5; ++m) {
-            const int fL = j + 3 - m, fR = j - 2 + m;
-            if (fL >= 0 && fL <= ny) hpy += cy[(size_t)(cv * 10 + m) * nfy + i * (ny + 1) + fL];
-            if (fR >= 0 && fR <= ny)
-              hmy += cy[(size_t)(cv * 10 + 5 + m) * nfy + i * (ny + 1) + fR];
+        for (int m = 0; m < 5; ++m) {
+          if (P - 5 + m >= 0 && P - 5 + m <= nx) {
+            const int r0 = w + m;
+            weno5_grad(L.Hxm[r0 + 4][ly], L.Hxm[r0 + 3][ly], L.Hxm[r0 + 2][ly],
+                       L.Hxm[r0 + 1][ly], L.Hxm[r0][ly], gq);
+            hmx += L.Fbx[w + m][ly] * gq[m];
           }
-        }
-        const double ucv = Up[cv * np + t];
-        Ub[cv] += 0.5 * ax * (hpx - hmx) + 0.5 * ay * (hpy - hmy);
-        sx_part += 0.5 * ucv * (hpx - hmx);
-        sy_part += 0.5 * ucv * (hpy - hmy);
-        const double Fx = 0.5 * (hpx + hmx), Fy = 0.5 * (hpy + hmy);
-        // VJP of phys_flux for both directions
-        switch (cv) {
-          case 0:   // rho*u, rho*v
-            rhob += Fx * q.u + Fy * q.v;
-            ub += Fx * q.rho;
-            vb += Fy * q.rho;
-            break;
-          case 1:   // rho*u^2 + p, rho*u*v
-            rhob += Fx * q.u * q.u + Fy * q.u * q.v;
-            ub += Fx * 2.0 * q.rho * q.u + Fy * q.rho * q.v;
-            vb += Fy * q.rho * q.u;
-            pb += Fx;
-            break;
-          case 2:   // rho*u*v, rho*v^2 + p
-            rhob += Fx * q.u * q.v + Fy * q.v * q.v;
-            ub += Fx * q.rho * q.v;
-            vb += Fx * q.rho * q.u + Fy * 2.0 * q.rho * q.v;
-            pb += Fy;
-            break;
-          case 3:   // u (rhoE + p), v (rhoE + p)
-            ub += Fx * (ucv + q.p);
-            vb += Fy * (ucv + q.p);
-            Ub[3] += Fx * q.u + Fy * q.v;
-            pb += Fx * q.u + Fy * q.v;
-            break;
-          default:  // rho*u*Y = u * rhoY
-            ub += Fx * ucv;
-            vb += Fy * ucv;
-            Ub[cv] += Fx * q.u + Fy * q.v;
         }
       }
-      prim_to_U_vjp(c, q, rhob, ub, vb, pb, 0.0, Yb, Ub, Tb, gb);
+      if (ycol) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          if (Q - m >= 0 && Q - m <= ny) {
+            const int c0 = ly + 4 - m;
+            weno5_grad(L.Hyp[w][c0], L.Hyp[w][c0 + 1], L.Hyp[w][c0 + 2], L.Hyp[w][c0 + 3],
+                       L.Hyp[w][c0 + 4], gq);
+            hpy += L.Fby[w][ly + 5 - m] * gq[m];
+          }
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          if (Q - 5 + m >= 0 && Q - 5 + m <= ny) {
+            const int c0 = ly + m;
+            weno5_grad(L.Hym[w][c0 + 4], L.Hym[w][c0 + 3], L.Hym[w][c0 + 2], L.Hym[w][c0 + 1],
+                       L.Hym[w][c0], gq);
+            hmy += L.Fby[w][ly + m] * gq[m];
+          }
+        }
+      }
+      const PrimL& qo = q[0];
+      const double ucv = ucv_own;
+      Ub[cv] += 0.5 * ax * (hpx - hmx) + 0.5 * ay * (hpy - hmy);
+      sx_part += 0.5 * ucv * (hpx - hmx);
+      sy_part += 0.5 * ucv * (hpy - hmy);
+      const double Fx = 0.5 * (hpx + hmx), Fy = 0.5 * (hpy + hmy);
+      switch (cv) {
+        case 0:
+          rhob += Fx * qo.u + Fy * qo.v;
+          ub += Fx * qo.rho;
+          vb += Fy * qo.rho;
+          break;
+        case 1:
+          rhob += Fx * qo.u * qo.u + Fy * qo.u * qo.v;
+          ub += Fx * 2.0 * qo.rho * qo.u + Fy * qo.rho * qo.v;
+          vb += Fy * qo.rho * qo.u;
+          pb += Fx;
+          break;
+        case 2:
+          rhob += Fx * qo.u * qo.v + Fy * qo.v * qo.v;
+          ub += Fx * qo.rho * qo.v;
+          vb += Fx * qo.rho * qo.u + Fy * 2.0 * qo.rho * qo.v;
+          pb += Fy;
+          break;
+        case 3:
+          ub += Fx * (ucv + qo.p);
+          vb += Fy * (ucv + qo.p);
+          Ub[3] += Fx * qo.u + Fy * qo.v;
+          pb += Fx * qo.u + Fy * qo.v;
+          break;
+        default:
+          ub += Fx * ucv;
+          vb += Fy * ucv;
+          Ub[cv] += Fx * qo.u + Fy * qo.v;
+      }
+    }
+    __syncthreads();   // LDS is refilled for the next variable
+  }
+  if (own) {
+    const int pt = P * PY + Q;
+    double Tb = 0.0, gb = 0.0;
+    if (work) {
+      const PrimD qf = prim_at(c, Up, ap, np, pt);
+      prim_to_U_vjp(c, qf, rhob, ub, vb, pb, 0.0, Yb, Ub, Tb, gb);
     }
 #pragma unroll
-    for (int v = 0; v < NU; ++v) Ubp[v * np + t] = Ub[v];
-    abp[t] = gb;
-    abp[np + t] = Tb;
+    for (int v = 0; v < NU; ++v) Ubp[(size_t)v * np + pt] = Ub[v];
+    abp[pt] = gb;
+    abp[np + pt] = Tb;
   }
   // fixed-order block sums of the alpha cotangent
 #pragma unroll
@@ -325,15 +434,20 @@ __global__ __launch_bounds__(256) void cell_vjp_kernel(AdjArgs a, const double* 
     sx_part += __shfl_xor(sx_part, off, 64);
     sy_part += __shfl_xor(sy_part, off, 64);
   }
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) {
-    red[0][wid] = sx_part;
-    red[1][wid] = sy_part;
+  if (ly == 0) {
+    L.red[0][w] = sx_part;
+    L.red[1][w] = sy_part;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-    part[2 * blockIdx.x + 1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+  if (t == 0) {
+    double sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int k = 0; k < TXA; ++k) {
+      sx += L.red[0][k];
+      sy += L.red[1][k];
+    }
+    part[2 * blockIdx.x] = sx;
+    part[2 * blockIdx.x + 1] = sy;
   }
 }
 

@@ -241,6 +241,12 @@ __global__ void zero_u32_kernel(unsigned int* p, int m) {
 
 int blocks_for(size_t m) { return (int)((m + 255) / 256); }
 
+// Tiles of the adjoint flux kernel over the ghost-padded grid.
+constexpr int kAdjRows = 4;
+int adj_tiles(const jh_ctx* c) {
+  return ((c->nx + 6 + kAdjRows - 1) / kAdjRows) * ((c->ny + 6 + TY - 1) / TY);
+}
+
 // JH_BC_USER sides must have been written (jh_set_ghost_strip or through
 // jh_ghost_strip's pointer) since their last use; consumes the marks.
 int take_user_strips(jh_ctx* c) {
@@ -255,7 +261,7 @@ struct AdjWork {
   double *U[5], *A[5];          // forward states of the step: 0 = input, 1-3 RK, 4 = output
   double *Ub, *Uacc, *ab, *rb;  // cotangents
   double *Up, *ap, *Ubp, *abp;  // padded
-  double *cx, *cy, *part, *scal;
+  double *part, *scal;
   unsigned int* cnt;
   unsigned long long* amx;      // recomputed LF maxima (bit patterns)
   int npad, nblk;
@@ -264,10 +270,9 @@ struct AdjWork {
 int adj_work(jh_ctx* c, const double* U0, const double* A0, AdjWork& w) {
   const size_t n = (size_t)c->nx * c->ny;
   const size_t np = (size_t)(c->nx + 6) * (c->ny + 6);
-  const size_t nfx = (size_t)(c->nx + 1) * c->ny, nfy = (size_t)c->nx * (c->ny + 1);
-  const int nblk = blocks_for(np);
+  const int nblk = adj_tiles(c);
   const size_t len = 4 * (NU + 2) * n + (3 * NU + 2) * n + 2 * (NU + 2) * np +
-                     (size_t)NU * 10 * (nfx + nfy) + 2 * nblk + 2 + 2 + 2;
+                     2 * (size_t)nblk + 2 + 2 + 2;
   if (c->adj_len < len) {
     if (c->adj) (void)hipFree(c->adj);
     c->adj = nullptr;
@@ -290,8 +295,6 @@ int adj_work(jh_ctx* c, const double* U0, const double* A0, AdjWork& w) {
   w.ap = p; p += 2 * np;
   w.Ubp = p; p += NU * np;
   w.abp = p; p += 2 * np;
-  w.cx = p; p += NU * 10 * nfx;
-  w.cy = p; p += NU * 10 * nfy;
   w.part = p; p += 2 * nblk;
   w.scal = p; p += 2;
   w.cnt = reinterpret_cast<unsigned int*>(p);
@@ -319,12 +322,10 @@ AdjArgs adj_args(const jh_ctx* c, double dt) {
 void flux_vjp(jh_ctx* c, const AdjArgs& a, AdjWork& w, const double* Us, const double* As,
               const double* rb, const unsigned long long* alpha, double* Ut, double* At) {
   const size_t n = (size_t)c->nx * c->ny;
-  const int nfx = (c->nx + 1) * c->ny, nfy = c->nx * (c->ny + 1);
   hipStream_t st = c->stream;
   pad_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, Us, As, w.Up, w.ap);
-  face_vjp_kernel<0><<<blocks_for(nfx), 256, 0, st>>>(a, w.Up, w.ap, rb, alpha, w.cx);
-  face_vjp_kernel<1><<<blocks_for(nfy), 256, 0, st>>>(a, w.Up, w.ap, rb, alpha, w.cy);
-  cell_vjp_kernel<<<w.nblk, 256, 0, st>>>(a, w.Up, w.ap, w.cx, w.cy, alpha, w.Ubp, w.abp, w.part);
+  flux_vjp_tile_kernel<kAdjRows><<<w.nblk, AdjTile<kAdjRows>::NT, 0, st>>>(
+      a, w.Up, w.ap, rb, alpha, w.Ubp, w.abp, w.part);
   alpha_reduce_kernel<<<1, 256, 0, st>>>(w.part, w.nblk, w.scal);
   zero_u32_kernel<<<1, 64, 0, st>>>(w.cnt, 8);   // counts (+ pad) and the two maxima
   alpha_max_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, w.Up, w.ap, w.amx);
