@@ -14,24 +14,26 @@ namespace jh {
 
 // FP64 form.  w_k = alpha_k / sum(alpha) is evaluated as
 // C_k prod_{j!=k} d_j / sum_m C_m prod_{j!=m} d_j with d_k = (eps+IS_k)^2,
-// which is the same ratio with one division instead of six.
+// the same ratio with one reciprocal instead of six divisions; C_k is folded
+// into the candidate stencils' coefficients (50 FP64 operations per face).
 __host__ __device__ __forceinline__ double weno5(double fm2, double fm1, double f0,
                                                  double fp1, double fp2) {
   const double a1 = fm2 - 4.0 * fm1 + 3.0 * f0, b1 = fm2 - 2.0 * fm1 + f0;
   const double a2 = fm1 - fp1, b2 = fm1 - 2.0 * f0 + fp1;
   const double a3 = 3.0 * f0 - 4.0 * fp1 + fp2, b3 = f0 - 2.0 * fp1 + fp2;
-  const double IS1 = 0.25 * (a1 * a1) + (13.0 / 12.0) * (b1 * b1);
-  const double IS2 = 0.25 * (a2 * a2) + (13.0 / 12.0) * (b2 * b2);
-  const double IS3 = 0.25 * (a3 * a3) + (13.0 / 12.0) * (b3 * b3);
-  double d1 = 1e-6 + IS1, d2 = 1e-6 + IS2, d3 = 1e-6 + IS3;
-  d1 = d1 * d1;
-  d2 = d2 * d2;
-  d3 = d3 * d3;
-  const double p1 = 0.1 * (d2 * d3), p2 = 0.6 * (d1 * d3), p3 = 0.3 * (d1 * d2);
-  const double q1 = (1.0 / 3.0) * fm2 - (7.0 / 6.0) * fm1 + (11.0 / 6.0) * f0;
-  const double q2 = -(1.0 / 6.0) * fm1 + (5.0 / 6.0) * f0 + (1.0 / 3.0) * fp1;
-  const double q3 = (1.0 / 3.0) * f0 + (5.0 / 6.0) * fp1 - (1.0 / 6.0) * fp2;
-  return (p1 * q1 + p2 * q2 + p3 * q3) * rcp(p1 + p2 + p3);
+  // eps + IS_k, IS_k = a_k^2/4 + 13/12 b_k^2
+  const double d1 = fma(0.25 * a1, a1, fma((13.0 / 12.0) * b1, b1, 1e-6));
+  const double d2 = fma(0.25 * a2, a2, fma((13.0 / 12.0) * b2, b2, 1e-6));
+  const double d3 = fma(0.25 * a3, a3, fma((13.0 / 12.0) * b3, b3, 1e-6));
+  const double e1 = d1 * d1, e2 = d2 * d2, e3 = d3 * d3;
+  const double p1 = e2 * e3, p2 = e1 * e3, p3 = e1 * e2;
+  // C_k q_k: (1/3, -7/6, 11/6) * 0.1, (-1/6, 5/6, 1/3) * 0.6, (1/3, 5/6, -1/6) * 0.3
+  const double q1 = (0.1 / 3.0) * fm2 - (0.7 / 6.0) * fm1 + (1.1 / 6.0) * f0;
+  const double q2 = -(0.6 / 6.0) * fm1 + (3.0 / 6.0) * f0 + (0.6 / 3.0) * fp1;
+  const double q3 = (0.3 / 3.0) * f0 + (1.5 / 6.0) * fp1 - (0.3 / 6.0) * fp2;
+  const double num = p1 * q1 + p2 * q2 + p3 * q3;
+  const double den = 0.1 * p1 + 0.6 * p2 + 0.3 * p3;
+  return num * rcp(den);
 }
 
 // FP32 form for the mixed-precision step.  In SI units the split energy flux
