@@ -166,14 +166,15 @@ __device__ __forceinline__ void prim_to_U_vjp(const MechConst& c, const PrimD& q
 
 // ------------------------------------------------------------------
 // Flux VJP of one RK stage, tiled: no face quantities are staged in HBM.
-// Tile = TXA padded rows x 64
-// padded columns, one cell per thread.  Padded cell P receives, per
-// direction and variable, H+_bar = sum_m F_bar(f = P - m) dq_L/dv_m(face f)
-// and H-_bar = sum_m F_bar(f = P - 5 + m) dq_R/dv_m(face f), the WENO
-// gradients recomputed from H+- of the 9-cell neighbourhood held in LDS (the
-// weights of a face are recomputed by each of the 5 cells it reaches, the
-// price of keeping every sum inside the tile); then the split-flux and
-// primitive VJPs of the reference's forward map.
+// Tile = TXA padded rows x 64 padded columns, one cell per thread.  Padded
+// cell P receives, per direction and variable, H+_bar = sum_m F_bar(f = P - m)
+// dq_L/dv_m(face f) and H-_bar = sum_m F_bar(f = P - 5 + m) dq_R/dv_m(face f).
+// Per variable: H+- of the tile and its 4-cell halo and the face cotangents
+// F_bar go to LDS; then, one direction at a time, every face that reaches
+// the tile evaluates its WENO gradient once and stores F_bar * dq/dv_m in LDS,
+// and each cell gathers its five left- and five right-biased terms (in the
+// order m = 0..4); then the split-flux and primitive VJPs of the reference's
+// forward map.
 template <int TXA>
 struct AdjTile {
   static constexpr int NT = 64 * TXA;          // threads (one padded cell each)
@@ -191,6 +192,9 @@ struct AdjLds {
   double Hyp[TXA][TY + 8], Hym[TXA][TY + 8];
   double Fbx[TXA + 5][TY];
   double Fby[TXA][TY + 5];
+  // F_bar * dq/dv_m of the faces reaching the tile: x phase [side][k][m][TY]
+  // (k < TXA+4), y phase [side][row][m][TY+4] in the same storage
+  double C[2 * (TXA + 4) * 5 * TY];
   double red[2][TXA];
 };
 
@@ -210,7 +214,7 @@ __device__ __forceinline__ double phys_flux_l(int cv, int dir, const PrimL& q, d
 }
 
 template <int TXA>
-__global__ __launch_bounds__(AdjTile<TXA>::NT) void flux_vjp_tile_kernel(
+__global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_eu(2))) void flux_vjp_tile_kernel(
     AdjArgs a, const double* __restrict__ Up, const double* __restrict__ ap,
     const double* __restrict__ rbar, const unsigned long long* alpha_bits,
     double* __restrict__ Ubp, double* __restrict__ abp, double* __restrict__ part) {
@@ -334,50 +338,69 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) void flux_vjp_tile_kernel(
       L.Fby[row][k] = fb;
     }
     __syncthreads();
-    // ---- gather H+-_bar of the own cell from the faces that reach it ----
+    // ---- x faces: F_bar * dq/dv_m into LDS, then the own cell's sums ----
+    for (int e = t; e < 2 * (TXA + 4) * TY; e += G::NT) {
+      const int side = e / ((TXA + 4) * TY), rem = e - side * ((TXA + 4) * TY);
+      const int kk = rem >> 6, col = rem & 63;
+      const int f = side == 0 ? X0 - 4 + kk : X0 - 5 + kk;   // face index
+      const int jj = Y0 + col - 3;
+      if (f >= 0 && f <= nx && jj >= 0 && jj < ny) {
+        double g[5];
+        if (side == 0)   // left-biased, cells f..f+4 = region rows kk..kk+4
+          weno5_grad(L.Hxp[kk][col], L.Hxp[kk + 1][col], L.Hxp[kk + 2][col], L.Hxp[kk + 3][col],
+                     L.Hxp[kk + 4][col], g);
+        else             // right-biased, cells f+5..f+1 = region rows kk+4..kk
+          weno5_grad(L.Hxm[kk + 4][col], L.Hxm[kk + 3][col], L.Hxm[kk + 2][col],
+                     L.Hxm[kk + 1][col], L.Hxm[kk][col], g);
+        const double fb = L.Fbx[side == 0 ? kk + 1 : kk][col];
+        double* out = &L.C[((side * (TXA + 4) + kk) * 5) * TY + col];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) out[m * TY] = fb * g[m];
+      }
+    }
+    __syncthreads();
+    double hpx = 0.0, hmx = 0.0, hpy = 0.0, hmy = 0.0;
+    if (work && xrow) {
+#pragma unroll
+      for (int m = 0; m < 5; ++m)
+        if (P - m >= 0 && P - m <= nx) hpx += L.C[((w + 4 - m) * 5 + m) * TY + ly];
+#pragma unroll
+      for (int m = 0; m < 5; ++m)
+        if (P - 5 + m >= 0 && P - 5 + m <= nx)
+          hmx += L.C[(((TXA + 4) + w + m) * 5 + m) * TY + ly];
+    }
+    __syncthreads();
+    // ---- y faces ----
+    for (int e = t; e < 2 * TXA * (TY + 4); e += G::NT) {
+      const int side = e / (TXA * (TY + 4)), rem = e - side * (TXA * (TY + 4));
+      const int row = rem / (TY + 4), kk = rem - row * (TY + 4);
+      const int gf = side == 0 ? Y0 - 4 + kk : Y0 - 5 + kk;
+      const int ii = X0 + row - 3;
+      if (gf >= 0 && gf <= ny && ii >= 0 && ii < nx) {
+        double g[5];
+        if (side == 0)
+          weno5_grad(L.Hyp[row][kk], L.Hyp[row][kk + 1], L.Hyp[row][kk + 2], L.Hyp[row][kk + 3],
+                     L.Hyp[row][kk + 4], g);
+        else
+          weno5_grad(L.Hym[row][kk + 4], L.Hym[row][kk + 3], L.Hym[row][kk + 2],
+                     L.Hym[row][kk + 1], L.Hym[row][kk], g);
+        const double fb = L.Fby[row][side == 0 ? kk + 1 : kk];
+        double* out = &L.C[((side * TXA + row) * 5) * (TY + 4) + kk];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) out[m * (TY + 4)] = fb * g[m];
+      }
+    }
+    __syncthreads();
+    if (work && ycol) {
+#pragma unroll
+      for (int m = 0; m < 5; ++m)
+        if (Q - m >= 0 && Q - m <= ny) hpy += L.C[(w * 5 + m) * (TY + 4) + ly + 4 - m];
+#pragma unroll
+      for (int m = 0; m < 5; ++m)
+        if (Q - 5 + m >= 0 && Q - 5 + m <= ny)
+          hmy += L.C[((TXA + w) * 5 + m) * (TY + 4) + ly + m];
+    }
     if (work) {
-      double hpx = 0.0, hmx = 0.0, hpy = 0.0, hmy = 0.0;
-      double gq[5];
-      if (xrow) {
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-          if (P - m >= 0 && P - m <= nx) {
-            const int r0 = w + 4 - m;
-            weno5_grad(L.Hxp[r0][ly], L.Hxp[r0 + 1][ly], L.Hxp[r0 + 2][ly], L.Hxp[r0 + 3][ly],
-                       L.Hxp[r0 + 4][ly], gq);
-            hpx += L.Fbx[w + 5 - m][ly] * gq[m];
-          }
-        }
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-          if (P - 5 + m >= 0 && P - 5 + m <= nx) {
-            const int r0 = w + m;
-            weno5_grad(L.Hxm[r0 + 4][ly], L.Hxm[r0 + 3][ly], L.Hxm[r0 + 2][ly],
-                       L.Hxm[r0 + 1][ly], L.Hxm[r0][ly], gq);
-            hmx += L.Fbx[w + m][ly] * gq[m];
-          }
-        }
-      }
-      if (ycol) {
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-          if (Q - m >= 0 && Q - m <= ny) {
-            const int c0 = ly + 4 - m;
-            weno5_grad(L.Hyp[w][c0], L.Hyp[w][c0 + 1], L.Hyp[w][c0 + 2], L.Hyp[w][c0 + 3],
-                       L.Hyp[w][c0 + 4], gq);
-            hpy += L.Fby[w][ly + 5 - m] * gq[m];
-          }
-        }
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-          if (Q - 5 + m >= 0 && Q - 5 + m <= ny) {
-            const int c0 = ly + m;
-            weno5_grad(L.Hym[w][c0 + 4], L.Hym[w][c0 + 3], L.Hym[w][c0 + 2], L.Hym[w][c0 + 1],
-                       L.Hym[w][c0], gq);
-            hmy += L.Fby[w][ly + m] * gq[m];
-          }
-        }
-      }
       const PrimL& qo = q[0];
       const double ucv = ucv_own;
       Ub[cv] += 0.5 * ax * (hpx - hmx) + 0.5 * ay * (hpy - hmy);
