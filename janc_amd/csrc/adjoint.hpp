@@ -76,27 +76,14 @@ __device__ __forceinline__ bool pad_src(const AdjArgs& a, int P, int Q, int& sx,
   return true;
 }
 
-// Ghost-padded copy of (U, aux): (12, nx+6, ny+6) and (2, nx+6, ny+6).
+// Ghost-padded copy of (U, aux): (12, nx+6, ny+6) and (2, nx+6, ny+6), and
+// the LF maxima of the padded field (into amx, zeroed beforehand) with the
+// arithmetic of prim_at, which the tie test repeats.
 __global__ __launch_bounds__(256) void pad_kernel(AdjArgs a, const double* __restrict__ U,
                                                   const double* __restrict__ aux,
                                                   double* __restrict__ Up,
-                                                  double* __restrict__ ap) {
-  const int PX = a.nx + 6, PY = a.ny + 6, np = PX * PY, n = a.nx * a.ny;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= np) return;
-  const int P = t / PY, Q = t - P * PY;
-  int sx, sy, flip;
-  if (!pad_src(a, P, Q, sx, sy, flip)) return;
-  const int s = sx * a.ny + sy;
-#pragma unroll
-  for (int v = 0; v < NU; ++v) {
-    double val = U[v * n + s];
-    if ((v == 1 && (flip & 1)) || (v == 2 && (flip & 2))) val = -val;
-    Up[v * np + t] = val;
-  }
-  ap[t] = aux[s];
-  ap[np + t] = aux[n + s];
-}
+                                                  double* __restrict__ ap,
+                                                  unsigned long long* amx);
 
 struct PrimD {
   double rho, ir, u, v, p, R, T, gamma;
@@ -117,6 +104,45 @@ __device__ __forceinline__ PrimD prim_at(const MechConst& c, const double* __res
   q.T = ap[np + t];
   q.p = q.rho * q.R * q.T;
   return q;
+}
+
+__global__ __launch_bounds__(256) void pad_kernel(AdjArgs a, const double* __restrict__ U,
+                                                  const double* __restrict__ aux,
+                                                  double* __restrict__ Up,
+                                                  double* __restrict__ ap,
+                                                  unsigned long long* amx) {
+  __shared__ double red[4];
+  const MechConst& c = *a.mc;
+  const int PX = a.nx + 6, PY = a.ny + 6, np = PX * PY, n = a.nx * a.ny;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double mx = 0.0, my = 0.0;
+  int sx, sy, flip;
+  if (t < np && pad_src(a, t / PY, t % PY, sx, sy, flip)) {
+    const int s = sx * a.ny + sy;
+    double val[NU];
+#pragma unroll
+    for (int v = 0; v < NU; ++v) {
+      val[v] = U[v * n + s];
+      if ((v == 1 && (flip & 1)) || (v == 2 && (flip & 2))) val[v] = -val[v];
+      Up[v * np + t] = val[v];
+    }
+    const double g = aux[s], T = aux[n + s];
+    ap[t] = g;
+    ap[np + t] = T;
+    // prim_at's arithmetic on the values just stored
+    const double rho = val[0], ir = rcp(rho);
+    const double u = val[1] * ir, v = val[2] * ir;
+    double Y[JH_NS];
+#pragma unroll
+    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = val[4 + k] * ir;
+    const double R = fill_Y_get_R(c, Y);
+    const double p = rho * R * T;
+    const double snd = sqrt(g * p * ir);
+    mx = nan_to_zero(fabs(u) + snd);
+    my = nan_to_zero(fabs(v) + snd);
+  }
+  block_max_atomic<256>(mx, &amx[0], red);
+  block_max_atomic<256>(my, &amx[1], red);
 }
 
 // Physical flux of variable cv in direction dir at a cell (LF.py:12).
@@ -474,35 +500,8 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
   }
 }
 
-// alpha_bar = sum of the partials (one block, fixed order), and the number
-// of padded cells attaining each LF maximum.  scal[0..1] = alpha_bar,
-// cnt[0..1] = tie counts.
-// Exact tie test against the maxima alpha_max_kernel reduced from the same
-// arithmetic: JAX's max VJP splits the cotangent among the elements equal to
-// the maximum.
-__device__ __forceinline__ bool is_max(double val, double alpha) { return val == alpha; }
-
-// LF maxima of the padded field recomputed with the arithmetic of
-// alpha_count_kernel / alpha_apply_kernel (NaN ignored, nanmax of LF.py:10-11).
-__global__ __launch_bounds__(256) void alpha_max_kernel(AdjArgs a, const double* __restrict__ Up,
-                                                        const double* __restrict__ ap,
-                                                        unsigned long long* amx) {
-  __shared__ double red[4];
-  const MechConst& c = *a.mc;
-  const int PY = a.ny + 6, np = (a.nx + 6) * PY;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  double mx = 0.0, my = 0.0;
-  int sx, sy, flip;
-  if (t < np && pad_src(a, t / PY, t % PY, sx, sy, flip)) {
-    const PrimD q = prim_at(c, Up, ap, np, t);
-    const double snd = sqrt(q.gamma * q.p * q.ir);
-    mx = nan_to_zero(fabs(q.u) + snd);
-    my = nan_to_zero(fabs(q.v) + snd);
-  }
-  block_max_atomic<256>(mx, &amx[0], red);
-  block_max_atomic<256>(my, &amx[1], red);
-}
-
+// alpha_bar = sum of the per-block partials, in a fixed order (one block):
+// scal[0..1].
 __global__ __launch_bounds__(256) void alpha_reduce_kernel(const double* __restrict__ part,
                                                            int nblk, double* scal) {
   __shared__ double red[2][256];
@@ -526,6 +525,12 @@ __global__ __launch_bounds__(256) void alpha_reduce_kernel(const double* __restr
     scal[1] = red[1][0];
   }
 }
+
+// Exact tie test against the maxima pad_kernel reduced from the same
+// arithmetic: JAX's max VJP splits the cotangent among the elements equal to
+// the maximum.  alpha_count_kernel counts the padded cells attaining each LF
+// maximum (cnt[0..1]); alpha_apply_kernel hands each its share of alpha_bar.
+__device__ __forceinline__ bool is_max(double val, double alpha) { return val == alpha; }
 
 __global__ __launch_bounds__(256) void alpha_count_kernel(AdjArgs a, const double* __restrict__ Up,
                                                           const double* __restrict__ ap,
@@ -705,6 +710,27 @@ __global__ __launch_bounds__(256) void chem_vjp_kernel(AdjArgs a, const double* 
   }
   ab3[t] = 0.0;
   ab3[n + t] = Tb;
+}
+
+// Reverse of one Shu-Osher combine U' = ... + cr (U_s + r(U_s)): with
+// x = Ub (the cotangent of U'), Uacc = aacc*x + bacc*Uacc (bacc = 0
+// overwrites) and, when cr != 0, rb = Ub = cr*x; ab (2 planes) is zeroed.
+__global__ __launch_bounds__(256) void rk_split_kernel(int mU, int mA, double aacc, double bacc,
+                                                       double cr, double* __restrict__ Ub,
+                                                       double* __restrict__ Uacc,
+                                                       double* __restrict__ rb,
+                                                       double* __restrict__ ab) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < mU) {
+    const double x = Ub[t];
+    Uacc[t] = bacc == 0.0 ? aacc * x : aacc * x + bacc * Uacc[t];
+    if (cr != 0.0) {
+      const double r = cr * x;
+      rb[t] = r;
+      Ub[t] = r;
+    }
+  }
+  if (t < mA) ab[t] = 0.0;
 }
 
 // y = a*x + b*y over m doubles (b = 0 overwrites).
