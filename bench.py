@@ -37,7 +37,7 @@ WORDS = {"stage": (27 + 39 + 39) / 3.0, "chem": 23.0, "reduce": 13.0, "bc": 0.0}
 # counter's reading for a 256 MiB device-to-device copy; summary under
 # profiles/).  Reads exceed the algorithmic 25 words: each region cell's
 # state is read for its primitives and again per variable.
-STAGE_TRAFFIC_B_PER_CELL_FP64 = 460.6
+STAGE_TRAFFIC_B_PER_CELL_FP64 = 472.9
 
 
 def _run(sim, U, aux, steps, warmup, torch):
