@@ -313,8 +313,9 @@ def _adjoint_leg(mech, steps, every, ndir, torch):
             "rollout_fwd_plus_adjoint_ms": t_total * 1e3,
             "adjoint_over_fwd_rollout": (t_total - t_roll) / t_fwd,
             "fd_directions": ndir,
-            "fd_max_rel_err": {"%g" % e: max(v) for e, v in errs.items()},
-            "fd_median_rel_err": {"%g" % e: float(np.median(v)) for e, v in errs.items()}}
+            "fd_max_rel_err": {"%g" % e: max(v) for e, v in errs.items()} if ndir else None,
+            "fd_median_rel_err": ({"%g" % e: float(np.median(v)) for e, v in errs.items()}
+                                  if ndir else None)}
 
 
 def main():
