@@ -73,6 +73,14 @@ __device__ __forceinline__ void block_max_bits_atomic(unsigned long long v,
 
 __device__ __forceinline__ double nan_to_zero(double v) { return v == v ? v : 0.0; }
 
+// v[i] of a 4-entry per-side array with a runtime side index, as selects: a
+// kernel that holds its arguments in registers (the batched kernels copy
+// them from memory) would otherwise index them through scratch.
+template <typename T>
+__device__ __forceinline__ T pick4(const T (&v)[4], int i) {
+  return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
+}
+
 // Block-cooperative copy of MechConst's Gibbs tables into t (LDS); the
 // caller synchronises the block before reading t.
 __device__ __forceinline__ void load_gibbs_tables(const MechConst& c, GibbsTables& t) {
@@ -309,8 +317,8 @@ __device__ __forceinline__ void bc_body(const StepArgs<Real>& a, const Real* __r
     const int len = xside ? ny : nx;          // strip row length
     const int N = xside ? nx : ny;            // cells along the normal
     const int gstride = 3 * len;              // plane stride in the strip
-    Real* gs = a.g[side];
-    const int kind = a.bc[side];
+    Real* gs = pick4(a.g, side);
+    const int kind = pick4(a.bc, side);
     // interior cell index (x, y) -> flat
     auto cell = [&](int along) -> int { return xside ? along * ny + pos : pos * ny + along; };
     if (kind != JH_BC_USER) {
@@ -333,7 +341,7 @@ __device__ __forceinline__ void bc_body(const StepArgs<Real>& a, const Real* __r
                         : side == JH_RIGHT ? Uc[1] / Uc[0]
                         : side == JH_BOTTOM ? -(Uc[2] / Uc[0]) : Uc[2] / Uc[0];
         const bool mask = (vn / a_out < 1.0);
-        const double Pb = a.bc_p[side];
+        const double Pb = pick4(a.bc_p, side);
         const double rho_c = mask ? Pb / (p_out / rho) : rho;
         const double p_c = mask ? Pb : p_out;
         const double T_c = mask ? p_c / (rho_c * R) : T_out;
@@ -355,7 +363,7 @@ __device__ __forceinline__ void bc_body(const StepArgs<Real>& a, const Real* __r
       } else if (kind == JH_BC_INJECTION_INLET) {
         const bool low = (side == JH_LEFT || side == JH_BOTTOM);
         double G[NV];
-        inlet_ghost(c, a, U, aux, n, cell(low ? 0 : N - 1), xside, low, a.bc_p[side], G);
+        inlet_ghost(c, a, U, aux, n, cell(low ? 0 : N - 1), xside, low, pick4(a.bc_p, side), G);
         for (int gi = 0; gi < 3; ++gi) {
 #pragma unroll
           for (int v = 0; v < NV; ++v) gs[v * gstride + gi * len + pos] = (Real)G[v];
