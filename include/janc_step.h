@@ -82,10 +82,29 @@ int jh_ctx_set_bc_param(jh_ctx* ctx, int side, double value);
  * of janc_basic_example2.ipynb; the example's nondimensional T0 = 1). */
 int jh_ctx_set_inlet(jh_ctx* ctx, double T0, const double* Y_inj, int n);
 int jh_ctx_set_stream(jh_ctx* ctx, void* stream);
-/* Stage-kernel tile variant: 0 = 64x8 tile / 256 threads / 3 waves per SIMD,
- * 1 = 64x8 / 256 / 4 waves per SIMD (<= 128 VGPRs), 2 = 64x16 / 512 / 4 waves
- * per SIMD.  No counterpart in the reference (XLA picks its own fusion); chosen
- * by the tile autotuner (janc_amd/autotune.py). */
+
+/* Flux scheme: the flux_config of set_flux_solver (solver_2D/flux.py:19-40).
+ *   solver_type    : 'flux_splitting' with split_method 'LF' (flux.py:56-68,
+ *                    flux_splitting/LF.py:4-16) or 'godunov' (flux.py:43-54)
+ *   reconstruction : interface_reconstruction 'WENO5_JS'
+ *                    (reconstruction/WENO5_JS.py:1-117) or 'MUSCL'
+ *                    (reconstruction/MUSCL.py:1-65)
+ *   riemann_solver : 'HLLC' (riemann_solver/HLLC.py:5-77), godunov only.
+ * The reference's other Riemann solver, 'AUSMPW_plus', is not offered: its y
+ * flux averages the y faces' gamma with the x faces' (AUSMPW_plus.py:63,67),
+ * arrays of shapes (nx, ny+1) and (nx+1, ny) that do not broadcast, so the
+ * reference cannot run it on a 2-D grid.  Default: flux splitting, WENO5_JS.
+ * jh_step_vjp supports the default scheme only. */
+enum jh_solver_type { JH_SOLVER_FLUX_SPLITTING = 0, JH_SOLVER_GODUNOV = 1 };
+enum jh_reconstruction { JH_RECON_WENO5_JS = 0, JH_RECON_MUSCL = 1 };
+enum jh_riemann_solver { JH_RIEMANN_HLLC = 0 };
+int jh_ctx_set_flux(jh_ctx* ctx, int solver_type, int reconstruction, int riemann_solver);
+
+/* Stage-kernel tile variant of the flux-splitting kernel (jh_tile_info gives
+ * rows, threads and the waves per SIMD the register budget is sized for).
+ * No counterpart in the reference (XLA picks its own fusion); chosen by the
+ * tile autotuner (janc_amd/autotune.py).  The Godunov kernel has one tile
+ * shape (64 x 7 cells, 512 threads). */
 int jh_ctx_set_tile(jh_ctx* ctx, int variant);
 int jh_ctx_get_tile(const jh_ctx* ctx);
 int jh_tile_variants(void);

@@ -14,6 +14,10 @@ PREC_FP64, PREC_MIXED = 0, 1
 BC_KINDS = {"periodic": 0, "zero_gradient": 1, "slip_wall": 2, "no_slip_wall": 3,
             "pressure_outlet": 4, "user": 5, "injection_inlet": 6}
 SIDES = ("left", "right", "bottom", "top")
+# flux_config values (enum jh_solver_type / jh_reconstruction / jh_riemann_solver)
+SOLVER_TYPES = {"flux_splitting": 0, "godunov": 1}
+RECONSTRUCTIONS = {"WENO5_JS": 0, "MUSCL": 1}
+RIEMANN_SOLVERS = {"HLLC": 0}
 KERNEL_KINDS = ("reduce", "bc", "stage", "chem")
 
 _c_int, _c_double, _vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
@@ -30,6 +34,7 @@ SIGNATURES = {
     "jh_ctx_set_stream": (_c_int, [_vp, _vp]),
     "jh_ctx_set_inlet": (_c_int, [_vp, _c_double, ctypes.POINTER(_c_double), _c_int]),
     "jh_ctx_set_tile": (_c_int, [_vp, _c_int]),
+    "jh_ctx_set_flux": (_c_int, [_vp, _c_int, _c_int, _c_int]),
     "jh_ctx_get_tile": (_c_int, [_vp]),
     "jh_tile_variants": (_c_int, []),
     "jh_tile_info": (_c_int, [_c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),

@@ -224,8 +224,9 @@ def periodic_box(mech, nx=64, ny=32, dx=1e-4, seed=7, amp=0.05):
 
 
 def simulation_config(geom, mech_path=None, cfl=0.3, dt=None, precision="fp64", t_end=1.0,
-                      theta=None):
-    """A reference-style simulation_config dict for janc_amd.Simulator."""
+                      theta=None, flux=None):
+    """A reference-style simulation_config dict for janc_amd.Simulator;
+    ``flux`` updates the flux_config (default: LF flux splitting, WENO5_JS)."""
     from .mechanism import DEFAULT_MECH
     mech_path = mech_path or DEFAULT_MECH
     tc = {"temporal_evolution_scheme": "TVD_RK3", "t_end": t_end}
@@ -238,8 +239,9 @@ def simulation_config(geom, mech_path=None, cfl=0.3, dt=None, precision="fp64", 
         "grid_config": {"dx": geom["dx"], "dy": geom["dy"]},
         "thermo_config": {"thermo_model": "nasa7", "mechanism_directory": mech_path},
         "reaction_config": {"is_detailed_chemistry": True, "mechanism_directory": mech_path},
-        "flux_config": {"solver_type": "flux_splitting", "interface_reconstruction": "WENO5_JS",
-                        "split_method": "LF", "viscosity": "off"},
+        "flux_config": dict({"solver_type": "flux_splitting",
+                             "interface_reconstruction": "WENO5_JS",
+                             "split_method": "LF", "viscosity": "off"}, **(flux or {})),
         "boundary_config": geom["boundary_config"],
         "time_config": tc,
         "computation_config": {"precision": precision},

@@ -12,6 +12,7 @@
 #include "../../include/janc_step.h"
 #include "adjoint.hpp"
 #include "batch.hpp"
+#include "godunov.hpp"
 #include "stage.hpp"
 
 using namespace jh;
@@ -33,6 +34,8 @@ struct jh_ctx {
   int parity = 0;
   bool user_filled[4] = {false, false, false, false};   // JH_BC_USER strips written
   int tile = 0;                   // stage-kernel variant (kTileRows / kTileWaves)
+  int solver = JH_SOLVER_FLUX_SPLITTING;   // jh_ctx_set_flux
+  int recon = JH_RECON_WENO5_JS;
   std::string err;
   // kernel timing with HIP events on the context stream (jh_timing_*)
   bool timing = false;
@@ -129,8 +132,11 @@ int grid_all(const jh_ctx* c) { return (c->nx * c->ny + 255) / 256; }
 
 int tile_rows(const jh_ctx* c);
 
+// Rows of the Godunov stage kernel's tile.
+constexpr int kGodRows = 7;
+
 int stage_blocks(const jh_ctx* c) {
-  const int tx = tile_rows(c);
+  const int tx = c->solver == JH_SOLVER_GODUNOV ? kGodRows : tile_rows(c);
   return ((c->nx + tx - 1) / tx) * ((c->ny + TY - 1) / TY);
 }
 
@@ -141,20 +147,38 @@ constexpr int kTileVariants = 5;
 constexpr int kTileRows[kTileVariants] = {8, 8, 16, 16, 32};
 constexpr int kTileWaves[kTileVariants] = {3, 4, 4, 5, 4};
 
-template <typename Real, int MODE>
-void launch_stage(jh_ctx* c, const StepArgs<Real>& a, int stage, const Real* Us, const Real* As,
-                  const Real* Un, Real* Uo, Real* Ao) {
+template <typename Real, int MODE, int RECON>
+void launch_split_stage(jh_ctx* c, const StepArgs<Real>& a, int stage, const Real* Us,
+                        const Real* As, const Real* Un, Real* Uo, Real* Ao) {
   const int nb = stage_blocks(c);
   switch (c->tile) {
 #define JH_LAUNCH(V, TX, WPE)                                                          \
     case V:                                                                            \
-      stage_kernel<Real, MODE, TX, WPE><<<nb, Tile<TX>::NT, 0, c->stream>>>(a, stage, Us, As, \
-                                                                           Un, Uo, Ao); \
+      stage_kernel<Real, MODE, TX, WPE, RECON><<<nb, Tile<TX>::NT, 0, c->stream>>>(   \
+          a, stage, Us, As, Un, Uo, Ao);                                               \
       break;
     JH_TILE_VARIANTS(JH_LAUNCH)
 #undef JH_LAUNCH
     default:
       break;
+  }
+}
+
+template <typename Real, int MODE>
+void launch_stage(jh_ctx* c, const StepArgs<Real>& a, int stage, const Real* Us, const Real* As,
+                  const Real* Un, Real* Uo, Real* Ao) {
+  const int nb = stage_blocks(c);
+  if (c->solver == JH_SOLVER_GODUNOV) {
+    if (c->recon == JH_RECON_MUSCL)
+      godunov_stage_kernel<Real, MODE, kGodRows, RECON_MUSCL>
+          <<<nb, GTile<kGodRows>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+    else
+      godunov_stage_kernel<Real, MODE, kGodRows, RECON_WENO5>
+          <<<nb, GTile<kGodRows>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+  } else if (c->recon == JH_RECON_MUSCL) {
+    launch_split_stage<Real, MODE, RECON_MUSCL>(c, a, stage, Us, As, Un, Uo, Ao);
+  } else {
+    launch_split_stage<Real, MODE, RECON_WENO5>(c, a, stage, Us, As, Un, Uo, Ao);
   }
 }
 
@@ -697,6 +721,19 @@ int jh_ctx_set_tile(jh_ctx* c, int variant) {
 
 int jh_tile_variants(void) { return kTileVariants; }
 
+int jh_ctx_set_flux(jh_ctx* c, int solver_type, int reconstruction, int riemann_solver) {
+  if (!c) return JH_ERR_ARG;
+  if (solver_type != JH_SOLVER_FLUX_SPLITTING && solver_type != JH_SOLVER_GODUNOV)
+    return fail(c, JH_ERR_ARG, "solver_type must be flux_splitting or godunov");
+  if (reconstruction != JH_RECON_WENO5_JS && reconstruction != JH_RECON_MUSCL)
+    return fail(c, JH_ERR_ARG, "interface_reconstruction must be WENO5_JS or MUSCL");
+  if (riemann_solver != JH_RIEMANN_HLLC)
+    return fail(c, JH_ERR_ARG, "riemann_solver must be HLLC");
+  c->solver = solver_type;
+  c->recon = reconstruction;
+  return JH_OK;
+}
+
 int jh_tile_info(int variant, int* rows, int* threads, int* waves) {
   if (variant < 0 || variant >= kTileVariants || !rows || !threads || !waves) return JH_ERR_ARG;
   *rows = kTileRows[variant];
@@ -815,6 +852,8 @@ int jh_step_vjp(jh_ctx* c, const void* U, const void* aux, double dt, const void
     return fail(c, JH_ERR_ARG, "NULL pointer");
   if (c->precision != JH_PREC_FP64) return fail(c, JH_ERR_ARG, "the adjoint is FP64 only");
   if (!(dt > 0)) return fail(c, JH_ERR_ARG, "the adjoint needs a fixed dt > 0");
+  if (c->solver != JH_SOLVER_FLUX_SPLITTING || c->recon != JH_RECON_WENO5_JS)
+    return fail(c, JH_ERR_ARG, "the adjoint supports flux splitting with WENO5_JS only");
   for (int s = 0; s < 4; ++s)
     if (c->bc[s] == JH_BC_PRESSURE_OUTLET || c->bc[s] == JH_BC_USER ||
         c->bc[s] == JH_BC_INJECTION_INLET)

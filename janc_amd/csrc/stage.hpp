@@ -11,6 +11,7 @@
 
 #include "chem.hpp"
 #include "janc_types.hpp"
+#include "recon.hpp"
 #include "thermo.hpp"
 #include "weno.hpp"
 
@@ -524,8 +525,9 @@ struct StageLds {
   MixTables mix;              // NASA-7 tables for the Halley epilogue
 };
 
-// One tile (index `tile` in y-fastest order) of one RK stage.
-template <typename Real, int MODE, int TX>
+// One tile (index `tile` in y-fastest order) of one RK stage; RECON selects
+// the reconstruction of the split fluxes (recon.hpp).
+template <typename Real, int MODE, int TX, int RECON = RECON_WENO5>
 __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
                                            const Real* __restrict__ Us,
                                            const Real* __restrict__ auxs, const Real* Un,
@@ -654,18 +656,20 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
     for (int it = t; it < NXF + NYF; it += NT) {
       if (it < NXF) {
         const int fr = it >> 6, col = it & 63;
-        const Real L = weno5(sHxp[fr][col], sHxp[fr + 1][col], sHxp[fr + 2][col],
-                             sHxp[fr + 3][col], sHxp[fr + 4][col]);
-        const Real R = weno5(sHxm[fr + 5][col], sHxm[fr + 4][col], sHxm[fr + 3][col],
-                             sHxm[fr + 2][col], sHxm[fr + 1][col]);
+        const Real L = recon_L<RECON, Real>(sHxp[fr][col], sHxp[fr + 1][col], sHxp[fr + 2][col],
+                                            sHxp[fr + 3][col], sHxp[fr + 4][col]);
+        const Real R = recon_R<RECON, Real>(sHxm[fr + 1][col], sHxm[fr + 2][col],
+                                            sHxm[fr + 3][col], sHxm[fr + 4][col],
+                                            sHxm[fr + 5][col]);
         sFx[fr][col] = L + R;
       } else {
         const int j = it - NXF;
         const int xr = j / (TY + 1), fc = j - xr * (TY + 1);
-        const Real L = weno5(sHyp[xr][fc], sHyp[xr][fc + 1], sHyp[xr][fc + 2],
-                             sHyp[xr][fc + 3], sHyp[xr][fc + 4]);
-        const Real R = weno5(sHym[xr][fc + 5], sHym[xr][fc + 4], sHym[xr][fc + 3],
-                             sHym[xr][fc + 2], sHym[xr][fc + 1]);
+        const Real L = recon_L<RECON, Real>(sHyp[xr][fc], sHyp[xr][fc + 1], sHyp[xr][fc + 2],
+                                            sHyp[xr][fc + 3], sHyp[xr][fc + 4]);
+        const Real R = recon_R<RECON, Real>(sHym[xr][fc + 1], sHym[xr][fc + 2],
+                                            sHym[xr][fc + 3], sHym[xr][fc + 4],
+                                            sHym[xr][fc + 5]);
         sFy[xr][fc] = L + R;
       }
     }
@@ -732,13 +736,14 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
   }
 }
 
-template <typename Real, int MODE, int TX, int WPE>
+template <typename Real, int MODE, int TX, int WPE, int RECON>
 __global__ __launch_bounds__(Tile<TX>::NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 void stage_kernel(StepArgs<Real> a, int stage, const Real* __restrict__ Us,
                   const Real* __restrict__ auxs, const Real* Un, Real* Uo,
                   Real* __restrict__ auxo) {
   __shared__ StageLds<Real, TX> L;
-  stage_body<Real, MODE, TX>(a, stage, Us, auxs, Un, Uo, auxo, xcd_tile(blockIdx.x, gridDim.x), L);
+  stage_body<Real, MODE, TX, RECON>(a, stage, Us, auxs, Un, Uo, auxo,
+                                    xcd_tile(blockIdx.x, gridDim.x), L);
 }
 
 // ---------------------------------------------------------- chemistry

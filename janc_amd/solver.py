@@ -132,21 +132,31 @@ class Simulator:
             "You choosed detailed chemistry without specifying the directory of your " \
             "mechanism files"
         self.mech = Mechanism(reaction_config["mechanism_directory"], nondim_config)
-        # flux.py:19-41
+        # flux.py:19-41 (set_flux_solver)
         st = flux_config["solver_type"]
-        if st == "godunov":
-            raise NotImplementedError("the Godunov (HLLC/AUSMPW+) path is not built")
-        if st != "flux_splitting":
+        if st not in ("godunov", "flux_splitting"):
             raise KeyError("JANC only support 'godunov' and 'flux_splitting'")
         rec = flux_config["interface_reconstruction"]
-        if rec != "WENO5_JS":
-            if rec == "MUSCL":
-                raise NotImplementedError("MUSCL reconstruction is not built")
+        if rec not in _lib.RECONSTRUCTIONS:           # reconstruction/__init__.py:3-21
             raise KeyError(rec)
-        if flux_config["split_method"] != "LF":
-            if flux_config["split_method"] == "SW":
-                raise NotImplementedError("Steger-Warming splitting is not built")
-            raise KeyError(flux_config["split_method"])
+        if st == "godunov":
+            rs = flux_config["riemann_solver"]
+            if rs == "AUSMPW_plus":
+                # AUSMPW_plus.py:63,67 averages the y faces' gamma with the x
+                # faces' array; the shapes (nx, ny+1) and (nx+1, ny) do not
+                # broadcast, so the reference cannot run it on a 2-D grid
+                raise NotImplementedError(
+                    "AUSMPW_plus: the reference's 2-D form mixes x- and y-face gamma "
+                    "arrays of different shapes and cannot run")
+            if rs not in _lib.RIEMANN_SOLVERS:        # riemann_solver/__init__.py:3-6
+                raise KeyError(rs)
+        else:
+            rs = "HLLC"
+            if flux_config["split_method"] != "LF":
+                if flux_config["split_method"] == "SW":
+                    raise NotImplementedError("Steger-Warming splitting is not built")
+                raise KeyError(flux_config["split_method"])
+        self.flux = (_lib.SOLVER_TYPES[st], _lib.RECONSTRUCTIONS[rec], _lib.RIEMANN_SOLVERS[rs])
         if flux_config["viscosity"] == "on":
             raise NotImplementedError("viscous fluxes are not built")
         if time_control["temporal_evolution_scheme"] != "TVD_RK3":
@@ -212,6 +222,7 @@ class Simulator:
         if ctx is None:
             ctx = _Context(nx, ny, dx, dy, self.bc_kinds, self._bc_params(theta),
                            self.precision, self.mech)
+            _lib.check(ctx.lib.jh_ctx_set_flux(ctx.h, *self.flux), ctx.h)
             self._ctx[key] = ctx
         else:
             for s, v in enumerate(self._bc_params(theta)):
@@ -387,6 +398,8 @@ class Simulator:
         rules, get_T_nasa7's custom VJP (thermo_model.py:155-190) included."""
         if self.precision != _lib.PREC_FP64:
             raise NotImplementedError("the adjoint is built for the FP64 form")
+        if self.flux != (0, 0, 0):
+            raise NotImplementedError("the adjoint is built for flux splitting with WENO5_JS")
         U, aux = self._state(U, aux)
         Ub, ab = self._state(U_bar, aux_bar)
         ctx = self._context(U, None)

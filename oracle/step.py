@@ -1,6 +1,7 @@
 """Oracle restatement of the 2-D step closure (test infrastructure only).
 
 Follows ``src/janc_v2/simulation/set_simulator.py:44-47`` (flux_func),
+``solver_2D/flux.py:19-40`` (set_flux_solver: flux splitting or Godunov),
 ``:117-122`` (advance_one_step, detailed-chemistry branch), ``:438-447``
 (CFL_2D / advance_func) and ``simulation/time_step/TVD_RK3.py:5-12``.
 """
@@ -17,18 +18,24 @@ class Oracle:
     """One configured solver: mechanism + grid + boundary kinds."""
 
     def __init__(self, mech_path, boundary_config, dx, dy, nondim_config=None,
-                 per_cell_halley=False):
+                 per_cell_halley=False, flux_config=None):
         self.rp, self.th, self.nd = load_mechanism(mech_path, nondim_config)
         self.thermo = Thermo(self.th)
         self.reaction = Reaction(self.rp, self.th, self.nd)
         self.boundary = Boundary(boundary_config, self.thermo)
         self.dx, self.dy = dx, dy
         self.per_cell = per_cell_halley
+        fc = flux_config or {}                     # flux.py:19-40
+        self.solver_type = fc.get("solver_type", "flux_splitting")
+        self.reconstruction = fc.get("interface_reconstruction", "WENO5_JS")
+        self.riemann = fc.get("riemann_solver", "HLLC")
 
     # set_simulator.py:44-47
     def flux_func(self, U, aux, dt, theta=None):
         Ug, ag = self.boundary(U, aux, theta)
-        return dt * (_flux.total_flux(Ug, ag, self.dx, self.dy, self.thermo))
+        return dt * (_flux.scheme_total_flux(Ug, ag, self.dx, self.dy, self.thermo,
+                                             self.solver_type, self.reconstruction,
+                                             self.riemann))
 
     def update_func(self, U, aux):                 # aux_func.py:21-29
         return _flux.update_aux(U, aux, self.thermo, per_cell=self.per_cell)

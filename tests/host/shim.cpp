@@ -5,6 +5,8 @@
 using std::fmax;
 using std::fmin;
 #include "../../janc_amd/csrc/chem.hpp"
+#include "../../janc_amd/csrc/hllc.hpp"
+#include "../../janc_amd/csrc/recon.hpp"
 #include "../../janc_amd/csrc/thermo.hpp"
 #include "../../janc_amd/csrc/weno.hpp"
 
@@ -60,5 +62,32 @@ double host_weno_grad(const double* v, double* g) {
   const double q = jh::weno5_grad(v[0], v[1], v[2], v[3], v[4], gg);
   for (int k = 0; k < 5; ++k) g[k] = gg[k];
   return q;
+}
+
+// HLLC flux per face: qL, qR (12, n) -> F (12, n); dir 0 = x, 1 = y
+void host_hllc(const double* mc, int n, const double* qL, const double* qR, int dir,
+               double* F) {
+  const MechConst& c = *reinterpret_cast<const MechConst*>(mc);
+  for (int i = 0; i < n; ++i) {
+    double l[jh::kNQ], r[jh::kNQ], f[jh::kNQ];
+    for (int v = 0; v < jh::kNQ; ++v) {
+      l[v] = qL[v * n + i];
+      r[v] = qR[v * n + i];
+    }
+    if (dir == 0) jh::hllc_flux<double, 0>(c, c, l, r, f);
+    else jh::hllc_flux<double, 1>(c, c, l, r, f);
+    for (int v = 0; v < jh::kNQ; ++v) F[v * n + i] = f[v];
+  }
+}
+
+// face values of one stencil v0..v5: out = (L, R); recon 0 = WENO5, 1 = MUSCL
+void host_recon(int recon, const double* v, double* out) {
+  if (recon == 1) {
+    out[0] = jh::recon_L<jh::RECON_MUSCL>(v[0], v[1], v[2], v[3], v[4]);
+    out[1] = jh::recon_R<jh::RECON_MUSCL>(v[1], v[2], v[3], v[4], v[5]);
+  } else {
+    out[0] = jh::recon_L<jh::RECON_WENO5>(v[0], v[1], v[2], v[3], v[4]);
+    out[1] = jh::recon_R<jh::RECON_WENO5>(v[1], v[2], v[3], v[4], v[5]);
+  }
 }
 }

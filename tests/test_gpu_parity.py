@@ -50,7 +50,7 @@ def _case(name, mech):
 CASES = ["box", "tube", "outlet", "inlet"]
 
 
-def make_oracle(mech_path, g, theta):
+def make_oracle(mech_path, g, theta, flux_config=None):
     """Oracle for a geometry; injection-inlet sides get the restated example
     boundary function (a user callable in the reference)."""
     bcfg = dict(g["boundary_config"])
@@ -58,7 +58,7 @@ def make_oracle(mech_path, g, theta):
               if bcfg[s + "_boundary"] == "injection_inlet"]
     for s in inlets:
         bcfg[s + "_boundary"] = "slip_wall"
-    o = Oracle(mech_path, bcfg, g["dx"], g["dy"])
+    o = Oracle(mech_path, bcfg, g["dx"], g["dy"], flux_config=flux_config)
     if inlets:
         from oracle.inlet import attach_inlet, stoichiometric_injection_Y
         Y = theta.get("Yinj", stoichiometric_injection_Y(len(o.thermo.Mex)))

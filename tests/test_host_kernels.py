@@ -2,7 +2,8 @@
 compiled for the host with g++ and checked on the CPU: chemistry increment
 against the oracle, its VJP against autograd of the torch twin, the
 temperature-solve VJP against the oracle's get_T_bwd, WENO5 (FP64, FP32)
-against the oracle and the WENO5 gradient against autograd."""
+against the oracle, the WENO5 gradient against autograd, and the Godunov
+path's HLLC flux and MUSCL / WENO5 face values against the oracle."""
 import ctypes
 import os
 import shutil
@@ -142,3 +143,57 @@ def test_weno_gradient(shim):
         g = np.zeros(5)
         shim.host_weno_grad(_p(v), _p(g))
         np.testing.assert_allclose(g, t.grad.numpy(), rtol=1e-9, atol=1e-12)
+
+
+def _face_states(mech, rng, n, vel):
+    T = rng.uniform(300.0, 3000.0, (n, 1))
+    p = rng.uniform(0.3, 30.0, (n, 1)) * 101325.0
+    Y9 = cases.premixed_Y(mech, rng.uniform(0.6, 1.4, (n, 1)))
+    Y9[2:8] += rng.uniform(0.0, 0.02, (6, n, 1))
+    Y9 /= Y9.sum(0, keepdims=True)
+    R = (Y9 / mech.species_M[:, None, None]).sum(0)
+    rho = p / (R * T)
+    u = rng.uniform(-vel, vel, (n, 1))
+    v = rng.uniform(-vel, vel, (n, 1))
+    return np.concatenate([rho[None], u[None], v[None], p[None], Y9[:8]], axis=0)
+
+
+@pytest.mark.parametrize("dir_", [0, 1])
+@pytest.mark.parametrize("vel", [200.0, 3000.0])
+def test_hllc_flux(shim, mech, mech_path, dir_, vel):
+    """Subsonic (all four branches) and supersonic face states."""
+    from oracle.flux import hllc_dir
+    from oracle.step import Oracle
+    th = Oracle(mech_path, {s + "_boundary": "periodic" for s in
+                            ("left", "right", "bottom", "top")}, 1.0, 1.0).thermo
+    rng = np.random.default_rng(10 + dir_)
+    n = 400
+    qL, qR = _face_states(mech, rng, n, vel), _face_states(mech, rng, n, vel)
+    # a few contacts and sonic points: equal normal velocity and pressure
+    qR[1 + dir_, :20] = qL[1 + dir_, :20]
+    qR[3, :20] = qL[3, :20]
+    ref = hllc_dir(qL, qR, th, dir_ + 1).reshape(12, n)
+    F = np.zeros((12, n))
+    shim.host_hllc(_p(mech.const_array()), n, _p(qL.reshape(12, n)), _p(qR.reshape(12, n)),
+                   dir_, _p(F))
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    err = np.abs(F - ref) / scale
+    assert err.max() < 1e-12, err.max(axis=1)
+
+
+@pytest.mark.parametrize("recon", [0, 1])
+def test_face_reconstructions(shim, recon):
+    from oracle import flux as OF
+    rng = np.random.default_rng(20 + recon)
+    # one padded column of 6 cells gives one face; stack many along y
+    m = 300
+    q = rng.standard_normal((1, 6, m + 6)) * 10 ** rng.uniform(-3, 6, (1, 1, m + 6))
+    q[0, :, :50] = np.round(q[0, :, :50])               # ties and flat stretches
+    Lx = OF.RECONSTRUCTION["MUSCL" if recon else "WENO5_JS"][0](q)[0, 0]
+    Rx = OF.RECONSTRUCTION["MUSCL" if recon else "WENO5_JS"][1](q)[0, 0]
+    out = np.zeros(2)
+    for j in range(m):
+        v = np.ascontiguousarray(q[0, :, j + 3])
+        shim.host_recon(recon, _p(v), _p(out))
+        tol = 1e-13 * max(np.abs(v).max(), 1e-300)
+        assert abs(out[0] - Lx[j]) <= tol and abs(out[1] - Rx[j]) <= tol, j
