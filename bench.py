@@ -73,6 +73,46 @@ def _roofline(kt, cells, word_bytes):
 
 CPU_CROP = (400, 400)
 
+# flux_config variants (set_flux_solver, solver_2D/flux.py:19-40); the
+# headline is the first
+SCHEMES = {
+    "lf_weno": {"solver_type": "flux_splitting", "interface_reconstruction": "WENO5_JS",
+                "split_method": "LF"},
+    "hllc_weno": {"solver_type": "godunov", "interface_reconstruction": "WENO5_JS",
+                  "riemann_solver": "HLLC"},
+    "hllc_muscl": {"solver_type": "godunov", "interface_reconstruction": "MUSCL",
+                   "riemann_solver": "HLLC"},
+    "lf_muscl": {"solver_type": "flux_splitting", "interface_reconstruction": "MUSCL",
+                 "split_method": "LF"},
+}
+
+
+def _schemes_leg(U0, aux0, geom, theta, steps, warmup, tune, torch):
+    """The FP64 step of the headline workload with each other flux scheme."""
+    from janc_amd import cases
+    from janc_amd.solver import Simulator
+    cells = U0.shape[1] * U0.shape[2]
+    out = {}
+    for name, fc in SCHEMES.items():
+        if name == "lf_weno":
+            continue
+        sim = Simulator(cases.simulation_config(geom, cfl=0.3, precision="fp64", theta=theta,
+                                                flux=fc))
+        U = torch.as_tensor(U0).cuda()
+        aux = torch.as_tensor(aux0).cuda()
+        tuned = None
+        if tune and fc["solver_type"] == "flux_splitting":
+            tuned = sim.autotune(U, aux, theta=theta)
+        el, kt, dts, flags = _run(sim, U, aux, steps, warmup, torch)
+        out[name] = {"flux_config": fc, "value": cells * steps / el,
+                     "ms_per_step": el / steps * 1e3, "steps": steps,
+                     "kernels": {k: {"ms_total": v[0], "launches": v[1]} for k, v in kt.items()},
+                     "roofline": _roofline(kt, cells, 8), "health_flags": flags,
+                     "autotune": tuned}
+        del U, aux
+        torch.cuda.empty_cache()
+    return out
+
 
 def _cpu_baseline(U, aux, geom, theta, steps):
     """FP64 oracle (numpy, one core) on a 400 x 400 crop of the headline state."""
@@ -341,9 +381,15 @@ def main():
                          "(reference dispatch for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the FP64 headline workload (for profiler runs)")
+    ap.add_argument("--flux", choices=list(SCHEMES), default="lf_weno",
+                    help="flux scheme of the headline workload (default: the reference's "
+                         "LF flux splitting with WENO5_JS)")
+    ap.add_argument("--scheme-steps", type=int, default=50)
+    ap.add_argument("--no-schemes", action="store_true")
     args = ap.parse_args()
     if args.headline_only:
         args.no_mixed = args.no_cpu = args.no_batch = args.no_adjoint = True
+        args.no_schemes = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the single-block step on one GPU; "
               "--gpus N > 1 is not supported (see DESIGN.md for how it would shard)",
@@ -361,10 +407,13 @@ def main():
     # ---- headline: FP64 on the 4000 x 400 annulus ----
     U0, aux0, geom, theta = cases.rdc_annulus(mech)
     nx, ny = U0.shape[1], U0.shape[2]
-    sim = Simulator(cases.simulation_config(geom, cfl=0.3, precision="fp64", theta=theta))
+    sim = Simulator(cases.simulation_config(geom, cfl=0.3, precision="fp64", theta=theta,
+                                            flux=SCHEMES[args.flux]))
     U = torch.as_tensor(U0).cuda()
     aux = torch.as_tensor(aux0).cuda()
-    if args.tile is not None:
+    if args.flux != "lf_weno":
+        tuned = None
+    elif args.tile is not None:
         sim.set_tile(U, args.tile, theta)
         tuned = {"variant": args.tile, "forced": True}
     else:
@@ -398,12 +447,14 @@ def main():
                                "(ZND-like seed 200 columns 30 atm/3000 K, 0.5% T noise), "
                                "9sp-19r H2-air, CFL 0.3; bottom injection inlet (0.2 MPa / "
                                "300 K manifold), top pressure outlet 1 atm",
+                   "flux_config": SCHEMES[args.flux],
                    "nx": nx, "ny": ny, "cells": cells, "precision": "fp64",
                    "dt_first": float(dts[0]), "dt_last": float(dts[-1]),
                    "health_flags": flags},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": r["achieved_GBs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["frac_hbm"],
-                     "traffic": STAGE_TRAFFIC_B_PER_CELL_FP64 * cells if dom == "stage" else None,
+                     "traffic": (STAGE_TRAFFIC_B_PER_CELL_FP64 * cells
+                                 if dom == "stage" and args.flux == "lf_weno" else None),
                      "algorithmic_bytes": r["algorithmic_bytes"]},
         "kernels_fp64": {k: {"ms_total": v[0], "launches": v[1]} for k, v in kt.items()},
         "autotune_fp64": tuned,
@@ -411,6 +462,11 @@ def main():
     }
     del U, aux
     torch.cuda.empty_cache()
+
+    # ---- the other flux schemes on the headline workload ----
+    if not args.no_schemes:
+        result["schemes_fp64"] = _schemes_leg(U0, aux0, geom, theta, args.scheme_steps,
+                                              args.warmup, not args.no_tune, torch)
 
     # ---- mixed vs FP64 on the 8000 x 500 tube ----
     if not args.no_mixed:

@@ -104,7 +104,8 @@ int jh_ctx_set_flux(jh_ctx* ctx, int solver_type, int reconstruction, int rieman
  * rows, threads and the waves per SIMD the register budget is sized for).
  * No counterpart in the reference (XLA picks its own fusion); chosen by the
  * tile autotuner (janc_amd/autotune.py).  The Godunov kernel has one tile
- * shape (64 x 7 cells, 512 threads). */
+ * shape per reconstruction (WENO5_JS: 64 x 3 cells, 256 threads; MUSCL:
+ * 64 x 7 cells, 512 threads). */
 int jh_ctx_set_tile(jh_ctx* ctx, int variant);
 int jh_ctx_get_tile(const jh_ctx* ctx);
 int jh_tile_variants(void);
