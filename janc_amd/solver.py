@@ -111,6 +111,14 @@ class Simulator:
         comp = cfg.get("computation_config", {}) or {}
         if comp.get("is_parallel", False):
             raise NotImplementedError("multi-device x-slab decomposition is not built")
+        # set_simulator.py:370-382: snapshot interval and file of run()
+        out = comp.get("output_settings")
+        if out is not None:
+            self.save_dt = out["save_dt"]
+            self.results_path = out["results_path"]
+        else:
+            self.save_dt = self.t_end
+            self.results_path = "results.h5"
         precision = comp.get("precision", "fp64")
         if precision not in ("fp64", "mixed"):
             raise KeyError("precision must be 'fp64' or 'mixed'")
@@ -336,14 +344,31 @@ class Simulator:
                      int(nsteps), _ptr(dt_hist))
         return U, aux, dt_hist
 
-    def run(self, U_init, aux_init):
-        """Simulator.run (set_simulator.py:457-494) without the HDF5 saver."""
+    def run(self, U_init, aux_init, save_aux=False):
+        """Simulator.run (set_simulator.py:457-494): advance to t_end, saving
+        the state to results_path (janc_amd.h5.H5Saver, groups save_step{k})
+        at t = 0, every save_dt and at the end.  Snapshots are written by a
+        worker thread while the steps continue; ``save_aux`` also stores
+        (gamma, T) for an exact restart (H5Saver.load_state)."""
+        from .h5 import H5Saver
         U, aux = self._state(U_init, aux_init)
         U, aux = U.clone(), aux.clone()
+        saver = H5Saver(self.results_path, "2D", simulator=self)
+        extra = (lambda a: {"aux": a}) if save_aux else (lambda a: {})
         t = 0.0
+        step = 0
+        save_step = 1
+        saver.save_async(0, t, step, u=U, **extra(aux))
+        next_save_time = self.save_dt
         while t < self.t_end:
             U, aux, t_dev, dt = self.advance_func(U, aux, t, self.theta)
             t = float(t_dev)
+            step += 1
+            if t >= next_save_time or t >= self.t_end:
+                saver.save_async(save_step, t, step, u=U, **extra(aux))
+                next_save_time += self.save_dt
+                save_step += 1
+        saver.close()
         return U, aux, t
 
     # ------------------------------------------------------------ unit pieces
