@@ -144,6 +144,54 @@ def _cpu_baseline(U, aux, geom, theta, steps):
                       "%.1f s" % (steps, CPU_CROP[0], CPU_CROP[1], el)}
 
 
+def _snapshot_leg(U0, aux0, geom, theta, steps, every, torch):
+    """HDF5 snapshots of the headline state (H5Saver layout, gzip level 4):
+    `steps` FP64 steps with an asynchronous snapshot every `every` steps vs
+    the same steps without, and one synchronous save timed alone."""
+    import tempfile
+    from janc_amd import cases
+    from janc_amd.h5 import H5Saver
+    from janc_amd.solver import Simulator
+    sim = Simulator(cases.simulation_config(geom, cfl=0.3, precision="fp64", theta=theta))
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        for mode in ("none", "async"):
+            U = torch.as_tensor(U0).cuda()
+            aux = torch.as_tensor(aux0).cuda()
+            sim.advance(U, aux, 2)
+            saver = H5Saver(os.path.join(d, mode + ".h5"), "2D") if mode == "async" else None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for k in range(0, steps, every):
+                sim.advance(U, aux, every)
+                if saver is not None:
+                    saver.save_async(k // every, 0.0, k + every, u=U)
+            torch.cuda.synchronize()
+            t_loop = time.perf_counter() - t0
+            if saver is not None:
+                saver.wait()
+            t_all = time.perf_counter() - t0
+            out[mode] = {"loop_s": t_loop, "until_written_s": t_all}
+            if saver is not None:
+                t0 = time.perf_counter()
+                saver.save(999, 0.0, 0, u=U)
+                out["sync_save_s"] = time.perf_counter() - t0
+                saver.close()
+                size = os.path.getsize(os.path.join(d, "async.h5"))
+                out["file_bytes"] = size
+                out["raw_bytes_per_snapshot"] = U.numel() * 8
+                out["snapshots"] = steps // every + 1
+            del U, aux
+    # stepping slowed by the overlapped saves, and the whole including the
+    # last snapshot's tail (written after the last step)
+    out["loop_overhead_frac"] = out["async"]["loop_s"] / out["none"]["loop_s"] - 1.0
+    out["overhead_frac"] = out["async"]["until_written_s"] / out["none"]["loop_s"] - 1.0
+    out["workload"] = ("%d FP64 steps of the 4000x400 annulus, u (12x4000x400 float64) saved "
+                       "every %d steps with H5Saver.save_async (pinned D2H on a side stream, "
+                       "gzip on 16 host threads)" % (steps, every))
+    return out
+
+
 def _fields(U, aux, inv_M):
     """rho, p, T, Y_k (float64 torch) of a state."""
     import torch
@@ -385,11 +433,13 @@ def main():
                     help="flux scheme of the headline workload (default: the reference's "
                          "LF flux splitting with WENO5_JS)")
     ap.add_argument("--scheme-steps", type=int, default=50)
+    ap.add_argument("--snapshot-steps", type=int, default=600)
+    ap.add_argument("--no-snapshots", action="store_true")
     ap.add_argument("--no-schemes", action="store_true")
     args = ap.parse_args()
     if args.headline_only:
         args.no_mixed = args.no_cpu = args.no_batch = args.no_adjoint = True
-        args.no_schemes = True
+        args.no_schemes = args.no_snapshots = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the single-block step on one GPU; "
               "--gpus N > 1 is not supported (see DESIGN.md for how it would shard)",
@@ -467,6 +517,12 @@ def main():
     if not args.no_schemes:
         result["schemes_fp64"] = _schemes_leg(U0, aux0, geom, theta, args.scheme_steps,
                                               args.warmup, not args.no_tune, torch)
+
+    # ---- HDF5 snapshots of the headline state ----
+    if not args.no_snapshots:
+        result["snapshots"] = _snapshot_leg(U0, aux0, geom, theta, args.snapshot_steps, 200,
+                                            torch)
+        torch.cuda.empty_cache()
 
     # ---- mixed vs FP64 on the 8000 x 500 tube ----
     if not args.no_mixed:

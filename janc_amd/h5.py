@@ -10,6 +10,12 @@ object headers, symbol-table groups, chunked datasets indexed by a version 1
 B-tree, deflate filter) with numpy and zlib.  Files written here open in
 h5py/libhdf5, and files h5py writes with its default settings read here.
 
+Datasets are chunked (~1 MiB chunks of whole rows) and stored through the
+HDF5 shuffle and deflate filters (level 4, h5py's gzip default; the reference
+uses deflate alone, which leaves floating-point data nearly incompressible;
+``shuffle=False`` gives its exact filter set).  Compression runs on a pool
+of host threads.
+
 ``H5Saver`` mirrors the reference's class (``save``, ``list_snapshots``,
 ``load``, ``load_all``, ``close``) and adds ``save_async`` (device-to-host
 copy on a side stream into pinned memory, compression and writing on a
@@ -189,8 +195,10 @@ def _chunk_shape(shape, itemsize, target=1 << 20):
     return tuple(ch)
 
 
-def _compress_chunks(arr, chunks, level, pool):
-    """[(chunk offset tuple, compressed bytes)] in C order of the chunk grid."""
+def _compress_chunks(arr, chunks, level, pool, shuffle):
+    """[(chunk offset tuple, filtered bytes)] in C order of the chunk grid:
+    the HDF5 shuffle filter (bytes grouped by significance, which lets
+    deflate compress floating-point data) when asked, then deflate."""
     shape = arr.shape
     grid = [range(0, max(s, 1), c) for s, c in zip(shape, chunks)]
     offsets = [()]
@@ -204,21 +212,24 @@ def _compress_chunks(arr, chunks, level, pool):
             full = np.zeros(chunks, dtype=arr.dtype)
             full[tuple(slice(0, b) for b in block.shape)] = block
             block = full
-        return off, zlib.compress(np.ascontiguousarray(block).tobytes(), level)
+        raw = np.ascontiguousarray(block)
+        if shuffle and raw.dtype.itemsize > 1:
+            raw = raw.view(np.uint8).reshape(-1, raw.dtype.itemsize).T
+        return off, zlib.compress(np.ascontiguousarray(raw).tobytes(), level)
 
     if pool is None:
         return [one(o) for o in offsets]
     return list(pool.map(one, offsets))
 
 
-def _write_dataset(w, arr, level, pool, chunk_bytes=1 << 20):
+def _write_dataset(w, arr, level, pool, chunk_bytes=1 << 20, shuffle=True):
     arr = np.ascontiguousarray(arr)
     arr = arr.astype(arr.dtype.newbyteorder("<"), copy=False)
     if arr.ndim == 0:
         raise ValueError("scalar datasets are not supported")
     chunks = _chunk_shape(arr.shape, arr.dtype.itemsize, chunk_bytes)
     rank = arr.ndim
-    parts = _compress_chunks(arr, chunks, level, pool)
+    parts = _compress_chunks(arr, chunks, level, pool, shuffle)
     leaves = []
     for off, data in parts:
         addr = w.alloc(data)
@@ -263,8 +274,13 @@ def _write_dataset(w, arr, level, pool, chunk_bytes=1 << 20):
     btree = nodes[0][0]
     layout = struct.pack("<BBBQ", 3, 2, rank + 1, btree) + b"".join(
         struct.pack("<I", c) for c in chunks) + struct.pack("<I", arr.dtype.itemsize)
-    filt = struct.pack("<BB6x", 1, 1) + struct.pack("<HHHH", 1, 0, 1, 1) + \
-        struct.pack("<I4x", level)
+    # filter pipeline (version 1): [shuffle (id 2, element size)], deflate
+    # (id 1, level); both marked optional as libhdf5 marks them
+    filt = b""
+    if shuffle:
+        filt += struct.pack("<HHHH", 2, 0, 1, 1) + struct.pack("<I4x", arr.dtype.itemsize)
+    filt += struct.pack("<HHHH", 1, 0, 1, 1) + struct.pack("<I4x", level)
+    filt = struct.pack("<BB6x", 1, 2 if shuffle else 1) + filt
     msgs = [(_MSG_SPACE, _space_msg(arr.shape), 0),
             (_MSG_DTYPE, _dtype_msg(arr.dtype), 1),
             (_MSG_FILL, struct.pack("<BBBB", 2, 3, 2, 0), 1),
@@ -604,10 +620,12 @@ class H5File:
     """An HDF5 file of top-level groups holding numeric datasets and
     numeric scalar attributes (the structure H5Saver writes)."""
 
-    def __init__(self, path, mode="a", gzip_level=4, threads=16, chunk_bytes=1 << 20):
+    def __init__(self, path, mode="a", gzip_level=4, threads=16, chunk_bytes=1 << 20,
+                 shuffle=True):
         self.path = str(path)
         self.level = gzip_level
         self.chunk_bytes = chunk_bytes
+        self.shuffle = shuffle
         self._pool = ThreadPoolExecutor(max(1, threads)) if threads > 1 else None
         exists = os.path.exists(self.path) and os.path.getsize(self.path) > 0
         if mode == "r":
@@ -660,7 +678,8 @@ class H5File:
             kids = []
             for dname, arr in datasets.items():
                 kids.append((dname, (_write_dataset(w, np.asarray(arr), self.level,
-                                                    self._pool, self.chunk_bytes), None)))
+                                                    self._pool, self.chunk_bytes,
+                                                    self.shuffle), None)))
             oh, bt, hp = _write_group(w, kids, attrs)
             self.top[name] = (oh, (bt, hp))
 
@@ -701,13 +720,13 @@ class H5File:
 class H5Saver:
     """``H5Saver`` of simulation/set_simulator.py:126-318 (2-D form)."""
 
-    def __init__(self, filepath, data_dim="2D", simulator=None, gzip_level=4):
+    def __init__(self, filepath, data_dim="2D", simulator=None, gzip_level=4, shuffle=True):
         if data_dim != "2D":
             raise NotImplementedError("janc_amd implements the 2-D solver only")
         self.filepath = str(filepath)
         self.sim = simulator
         self.level = gzip_level
-        self.file = H5File(self.filepath, "a", gzip_level)
+        self.file = H5File(self.filepath, "a", gzip_level, shuffle=shuffle)
         self._worker = ThreadPoolExecutor(1)
         self._pending = []
 
