@@ -39,6 +39,11 @@ WORDS = {"stage": (27 + 39 + 39) / 3.0, "chem": 23.0, "reduce": 13.0, "bc": 0.0}
 # state is read for its primitives and again per variable.
 STAGE_TRAFFIC_B_PER_CELL_FP64 = 472.9
 
+# Measured FP64 FLOP per cell per launch of the FP64 headline kernels
+# (rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes / cells, FMA = 2; summary
+# under profiles/): the bench turns them into live FP64-vector fractions.
+FP64_FLOP_PER_CELL = {"stage": 5178.0, "chem": 3241.0}
+
 
 def _run(sim, U, aux, steps, warmup, torch):
     """Warm up, then time `steps` fused steps; returns (seconds, per-kernel)."""
@@ -510,6 +515,16 @@ def main():
         "autotune_fp64": tuned,
         "roofline_all_fp64": roof,
     }
+    if args.flux == "lf_weno":
+        fv = {}
+        for kind, fpc in FP64_FLOP_PER_CELL.items():
+            ms, nl = kt.get(kind, (0.0, 0))
+            if nl:
+                tfs = fpc * cells / (ms / nl * 1e-3) / 1e12
+                fv[kind] = {"flop_per_cell": fpc, "avg_ms": ms / nl, "achieved_TFLOPs": tfs,
+                            "peak_TFLOPs": FP64_VECTOR_PEAK_TFLOPS,
+                            "frac": tfs / FP64_VECTOR_PEAK_TFLOPS}
+        result["fp64_vector"] = fv
     del U, aux
     torch.cuda.empty_cache()
 
