@@ -89,6 +89,8 @@ SCHEMES = {
                    "riemann_solver": "HLLC"},
     "lf_muscl": {"solver_type": "flux_splitting", "interface_reconstruction": "MUSCL",
                  "split_method": "LF"},
+    "sw_weno": {"solver_type": "flux_splitting", "interface_reconstruction": "WENO5_JS",
+                "split_method": "SW"},
 }
 
 

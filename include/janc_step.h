@@ -84,8 +84,10 @@ int jh_ctx_set_inlet(jh_ctx* ctx, double T0, const double* Y_inj, int n);
 int jh_ctx_set_stream(jh_ctx* ctx, void* stream);
 
 /* Flux scheme: the flux_config of set_flux_solver (solver_2D/flux.py:19-40).
- *   solver_type    : 'flux_splitting' with split_method 'LF' (flux.py:56-68,
- *                    flux_splitting/LF.py:4-16) or 'godunov' (flux.py:43-54)
+ *   solver_type    : 'flux_splitting' (flux.py:56-68) or 'godunov'
+ *                    (flux.py:43-54)
+ *   split_method   : flux splitting only: 'LF' (flux_splitting/LF.py:4-16)
+ *                    or 'SW' (flux_splitting/SW.py:4-38)
  *   reconstruction : interface_reconstruction 'WENO5_JS'
  *                    (reconstruction/WENO5_JS.py:1-117) or 'MUSCL'
  *                    (reconstruction/MUSCL.py:1-65)
@@ -94,18 +96,21 @@ int jh_ctx_set_stream(jh_ctx* ctx, void* stream);
  * flux averages the y faces' gamma with the x faces' (AUSMPW_plus.py:63,67),
  * arrays of shapes (nx, ny+1) and (nx+1, ny) that do not broadcast, so the
  * reference cannot run it on a 2-D grid.  Default: flux splitting, WENO5_JS.
- * jh_step_vjp supports the default scheme only. */
+ * jh_step_vjp supports the default scheme only (LF, WENO5_JS). */
 enum jh_solver_type { JH_SOLVER_FLUX_SPLITTING = 0, JH_SOLVER_GODUNOV = 1 };
+enum jh_split_method { JH_SPLIT_LF = 0, JH_SPLIT_SW = 1 };
 enum jh_reconstruction { JH_RECON_WENO5_JS = 0, JH_RECON_MUSCL = 1 };
 enum jh_riemann_solver { JH_RIEMANN_HLLC = 0 };
-int jh_ctx_set_flux(jh_ctx* ctx, int solver_type, int reconstruction, int riemann_solver);
+int jh_ctx_set_flux(jh_ctx* ctx, int solver_type, int split_method, int reconstruction,
+                    int riemann_solver);
 
 /* Stage-kernel tile variant of the flux-splitting kernel (jh_tile_info gives
  * rows, threads and the waves per SIMD the register budget is sized for).
  * No counterpart in the reference (XLA picks its own fusion); chosen by the
  * tile autotuner (janc_amd/autotune.py).  The Godunov kernel has one tile
  * shape per reconstruction (WENO5_JS: 64 x 3 cells, 256 threads; MUSCL:
- * 64 x 7 cells, 512 threads). */
+ * 64 x 7 cells, 512 threads), the Steger-Warming kernel one (64 x 16 cells,
+ * 512 threads). */
 int jh_ctx_set_tile(jh_ctx* ctx, int variant);
 int jh_ctx_get_tile(const jh_ctx* ctx);
 int jh_tile_variants(void);

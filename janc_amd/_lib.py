@@ -16,6 +16,7 @@ BC_KINDS = {"periodic": 0, "zero_gradient": 1, "slip_wall": 2, "no_slip_wall": 3
 SIDES = ("left", "right", "bottom", "top")
 # flux_config values (enum jh_solver_type / jh_reconstruction / jh_riemann_solver)
 SOLVER_TYPES = {"flux_splitting": 0, "godunov": 1}
+SPLIT_METHODS = {"LF": 0, "SW": 1}
 RECONSTRUCTIONS = {"WENO5_JS": 0, "MUSCL": 1}
 RIEMANN_SOLVERS = {"HLLC": 0}
 KERNEL_KINDS = ("reduce", "bc", "stage", "chem")
@@ -34,7 +35,7 @@ SIGNATURES = {
     "jh_ctx_set_stream": (_c_int, [_vp, _vp]),
     "jh_ctx_set_inlet": (_c_int, [_vp, _c_double, ctypes.POINTER(_c_double), _c_int]),
     "jh_ctx_set_tile": (_c_int, [_vp, _c_int]),
-    "jh_ctx_set_flux": (_c_int, [_vp, _c_int, _c_int, _c_int]),
+    "jh_ctx_set_flux": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int]),
     "jh_ctx_get_tile": (_c_int, [_vp]),
     "jh_tile_variants": (_c_int, []),
     "jh_tile_info": (_c_int, [_c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),

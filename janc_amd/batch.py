@@ -42,8 +42,8 @@ class BatchSimulator:
                 raise ValueError("all cases must share the mechanism")
             if (s.fixed_dt, s.cfl) != (s0.fixed_dt, s0.cfl):
                 raise ValueError("all cases must share the time control (dt or CFL)")
-        if any(s.flux != (0, 0, 0) for s in sims):
-            raise NotImplementedError("batches run flux splitting with WENO5_JS")
+        if any(s.flux != (0, 0, 0, 0) for s in sims):
+            raise NotImplementedError("batches run LF flux splitting with WENO5_JS")
         self.sims = sims
         self.ncase = len(sims)
         self.precision, self.dtype, self.device = s0.precision, s0.dtype, s0.device

@@ -35,6 +35,7 @@ struct jh_ctx {
   bool user_filled[4] = {false, false, false, false};   // JH_BC_USER strips written
   int tile = 0;                   // stage-kernel variant (kTileRows / kTileWaves)
   int solver = JH_SOLVER_FLUX_SPLITTING;   // jh_ctx_set_flux
+  int split = JH_SPLIT_LF;
   int recon = JH_RECON_WENO5_JS;
   std::string err;
   // kernel timing with HIP events on the context stream (jh_timing_*)
@@ -142,8 +143,14 @@ int god_rows(const jh_ctx* c) {
   return c->recon == JH_RECON_MUSCL ? kGodRowsMuscl : kGodRowsWeno;
 }
 
+// Steger-Warming flux splitting: one tile shape, 64 x 16 at two waves per
+// SIMD (its per-cell eigenvalue splits need ~250 VGPRs).
+constexpr int kSwRows = 16;
+
 int stage_blocks(const jh_ctx* c) {
-  const int tx = c->solver == JH_SOLVER_GODUNOV ? god_rows(c) : tile_rows(c);
+  const int tx = c->solver == JH_SOLVER_GODUNOV ? god_rows(c)
+                 : c->split == JH_SPLIT_SW     ? kSwRows
+                                               : tile_rows(c);
   return ((c->nx + tx - 1) / tx) * ((c->ny + TY - 1) / TY);
 }
 
@@ -182,6 +189,13 @@ void launch_stage(jh_ctx* c, const StepArgs<Real>& a, int stage, const Real* Us,
     else
       godunov_stage_kernel<Real, MODE, kGodRowsWeno, RECON_WENO5>
           <<<nb, GTile<kGodRowsWeno>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+  } else if (c->split == JH_SPLIT_SW) {
+    if (c->recon == JH_RECON_MUSCL)
+      stage_kernel<Real, MODE, kSwRows, 2, RECON_MUSCL, SPLIT_SW>
+          <<<nb, Tile<kSwRows>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+    else
+      stage_kernel<Real, MODE, kSwRows, 2, RECON_WENO5, SPLIT_SW>
+          <<<nb, Tile<kSwRows>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
   } else if (c->recon == JH_RECON_MUSCL) {
     launch_split_stage<Real, MODE, RECON_MUSCL>(c, a, stage, Us, As, Un, Uo, Ao);
   } else {
@@ -728,15 +742,19 @@ int jh_ctx_set_tile(jh_ctx* c, int variant) {
 
 int jh_tile_variants(void) { return kTileVariants; }
 
-int jh_ctx_set_flux(jh_ctx* c, int solver_type, int reconstruction, int riemann_solver) {
+int jh_ctx_set_flux(jh_ctx* c, int solver_type, int split_method, int reconstruction,
+                    int riemann_solver) {
   if (!c) return JH_ERR_ARG;
   if (solver_type != JH_SOLVER_FLUX_SPLITTING && solver_type != JH_SOLVER_GODUNOV)
     return fail(c, JH_ERR_ARG, "solver_type must be flux_splitting or godunov");
+  if (split_method != JH_SPLIT_LF && split_method != JH_SPLIT_SW)
+    return fail(c, JH_ERR_ARG, "split_method must be LF or SW");
   if (reconstruction != JH_RECON_WENO5_JS && reconstruction != JH_RECON_MUSCL)
     return fail(c, JH_ERR_ARG, "interface_reconstruction must be WENO5_JS or MUSCL");
   if (riemann_solver != JH_RIEMANN_HLLC)
     return fail(c, JH_ERR_ARG, "riemann_solver must be HLLC");
   c->solver = solver_type;
+  c->split = split_method;
   c->recon = reconstruction;
   return JH_OK;
 }
@@ -859,8 +877,9 @@ int jh_step_vjp(jh_ctx* c, const void* U, const void* aux, double dt, const void
     return fail(c, JH_ERR_ARG, "NULL pointer");
   if (c->precision != JH_PREC_FP64) return fail(c, JH_ERR_ARG, "the adjoint is FP64 only");
   if (!(dt > 0)) return fail(c, JH_ERR_ARG, "the adjoint needs a fixed dt > 0");
-  if (c->solver != JH_SOLVER_FLUX_SPLITTING || c->recon != JH_RECON_WENO5_JS)
-    return fail(c, JH_ERR_ARG, "the adjoint supports flux splitting with WENO5_JS only");
+  if (c->solver != JH_SOLVER_FLUX_SPLITTING || c->split != JH_SPLIT_LF ||
+      c->recon != JH_RECON_WENO5_JS)
+    return fail(c, JH_ERR_ARG, "the adjoint supports LF flux splitting with WENO5_JS only");
   for (int s = 0; s < 4; ++s)
     if (c->bc[s] == JH_BC_PRESSURE_OUTLET || c->bc[s] == JH_BC_USER ||
         c->bc[s] == JH_BC_INJECTION_INLET)

@@ -111,7 +111,7 @@ __device__ __forceinline__ void load_mix_tables(const MechConst& c, MixTables& t
 // Primitive state of one cell (aux_func.py:39-51), in compute type W.
 template <typename W>
 struct Prim {
-  W rho, u, v, p, a;
+  W rho, u, v, p, a, g;       // g: gamma from aux (set with a)
 };
 
 template <typename W, bool WITH_A, typename Real>
@@ -135,7 +135,11 @@ __device__ __forceinline__ Prim<W> cell_prim(const MechConst& c, const Real* __r
   const W T = (W)ap[stride];
   q.p = (q.rho * R * T);
   q.a = (W)0;
-  if (WITH_A) q.a = sqrt((W)ap[0] * q.p * ir);
+  q.g = (W)0;
+  if (WITH_A) {
+    q.g = (W)ap[0];
+    q.a = sqrt(q.g * q.p * ir);
+  }
   return q;
 }
 
@@ -498,6 +502,51 @@ __device__ __forceinline__ W phys_flux(int cv, int dir, const Prim<W>& q, W ucv)
   }
 }
 
+enum Split : int { SPLIT_LF = 0, SPLIT_SW = 1 };
+
+// Steger-Warming split fluxes of component cv in direction x (dir 0) or y
+// (SW.py:4-38): H+- = sum_k lambda_k+- H_k over the eigenvalues
+// lambda = (un - a, un, un + a), lambda+- = (lambda +- sqrt(lambda^2 + eps^2))/2,
+// eps = 1e-6, with H1, H4 = (1/(2 gamma)) [rho, rho u -+ rho a zx, rho v -+ rho a zy,
+// rho E + p -+ rho a un, rho Y] and H2 = ((gamma-1)/gamma) [rho, rho u, rho v,
+// rho |u|^2 / 2, rho Y]; gamma is the aux value, a = sqrt(gamma p / rho).
+template <typename W>
+__device__ __forceinline__ void sw_split(int cv, int dir, const Prim<W>& q, W ucv, W& hp,
+                                         W& hm) {
+  constexpr W eps2 = (W)(1e-6 * 1e-6);
+  const W un = dir == 0 ? q.u : q.v;
+  const W l1 = un - q.a, l2 = un, l4 = un + q.a;
+  const W s1 = sqrt(l1 * l1 + eps2), s2 = sqrt(l2 * l2 + eps2), s4 = sqrt(l4 * l4 + eps2);
+  const W ig = (W)1 / ((W)2 * q.g), g2 = (q.g - (W)1) / q.g;
+  const W ra = q.rho * q.a;
+  W h1, h2, h4;
+  switch (cv) {
+    case 0:
+      h1 = h2 = h4 = q.rho;
+      break;
+    case 1: {
+      const W ru = q.rho * q.u, d = dir == 0 ? ra : (W)0;
+      h1 = ru - d; h2 = ru; h4 = ru + d;
+      break;
+    }
+    case 2: {
+      const W rv = q.rho * q.v, d = dir == 1 ? ra : (W)0;
+      h1 = rv - d; h2 = rv; h4 = rv + d;
+      break;
+    }
+    case 3:
+      h1 = ucv + q.p - ra * un;
+      h2 = (W)0.5 * q.rho * (q.u * q.u + q.v * q.v);
+      h4 = ucv + q.p + ra * un;
+      break;
+    default:   // rho * Y with Y = rhoY / rho
+      h1 = h2 = h4 = q.rho * (ucv / q.rho);
+  }
+  const W H1 = ig * h1, H2 = g2 * h2, H4 = ig * h4;
+  hp = (W)0.5 * (l1 + s1) * H1 + (W)0.5 * (l2 + s2) * H2 + (W)0.5 * (l4 + s4) * H4;
+  hm = (W)0.5 * (l1 - s1) * H1 + (W)0.5 * (l2 - s2) * H2 + (W)0.5 * (l4 - s4) * H4;
+}
+
 // One SSP-RK3 stage (TVD_RK3.py:5-12) for a 64 x TX tile:
 //   rhs = dt * -(dF/dx + dG/dy)   (set_simulator.py:44-47, flux.py:56-68)
 //   stage 0: U1 = U + rhs; stage 1: U2 = 3/4 U + 1/4 (U1 + rhs);
@@ -526,8 +575,8 @@ struct StageLds {
 };
 
 // One tile (index `tile` in y-fastest order) of one RK stage; RECON selects
-// the reconstruction of the split fluxes (recon.hpp).
-template <typename Real, int MODE, int TX, int RECON = RECON_WENO5>
+// the reconstruction of the split fluxes (recon.hpp), SPLIT the splitting.
+template <typename Real, int MODE, int TX, int RECON = RECON_WENO5, int SPLIT = SPLIT_LF>
 __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
                                            const Real* __restrict__ Us,
                                            const Real* __restrict__ auxs, const Real* Un,
@@ -592,10 +641,11 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
     ref[s] = locate<Real>(a, Us, auxs, gxs[s], gys[s]);
     if (!has[s]) ref[s].ok = false;
     if (ref[s].ok) {
-      q[s] = cell_prim<Real, false>(c, ref[s].u, ref[s].x, ref[s].stride);
+      q[s] = cell_prim<Real, SPLIT == SPLIT_SW>(c, ref[s].u, ref[s].x, ref[s].stride);
       nxt[s] = ref[s].u[0];
     } else {
       q[s].rho = (Real)1; q[s].u = (Real)0; q[s].v = (Real)0; q[s].p = (Real)0; q[s].a = (Real)0;
+      q[s].g = (Real)1;
       nxt[s] = (Real)0;
     }
   }
@@ -619,9 +669,13 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
       if (xreg) {
         Real hp = (Real)0, hm = (Real)0;
         if (ref[s].ok) {
-          const Real F = phys_flux<Real>(cv, 0, q[s], ucv);
-          hp = (Real)0.5 * (F + ax * ucv);
-          hm = (Real)0.5 * (F - ax * ucv);
+          if (SPLIT == SPLIT_SW) {
+            sw_split<Real>(cv, 0, q[s], ucv, hp, hm);
+          } else {
+            const Real F = phys_flux<Real>(cv, 0, q[s], ucv);
+            hp = (Real)0.5 * (F + ax * ucv);
+            hm = (Real)0.5 * (F - ax * ucv);
+          }
         }
         sHxp[rrow[s]][ly] = hp;
         sHxm[rrow[s]][ly] = hm;
@@ -629,9 +683,13 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
       if (yreg) {
         Real hp = (Real)0, hm = (Real)0;
         if (ref[s].ok) {
-          const Real Gf = phys_flux<Real>(cv, 1, q[s], ucv);
-          hp = (Real)0.5 * (Gf + ay * ucv);
-          hm = (Real)0.5 * (Gf - ay * ucv);
+          if (SPLIT == SPLIT_SW) {
+            sw_split<Real>(cv, 1, q[s], ucv, hp, hm);
+          } else {
+            const Real Gf = phys_flux<Real>(cv, 1, q[s], ucv);
+            hp = (Real)0.5 * (Gf + ay * ucv);
+            hm = (Real)0.5 * (Gf - ay * ucv);
+          }
         }
         const int yr = s < 2 ? rrow[s] - 3 : rrow[s];
         sHyp[yr][rcol[s]] = hp;
@@ -736,14 +794,14 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
   }
 }
 
-template <typename Real, int MODE, int TX, int WPE, int RECON>
+template <typename Real, int MODE, int TX, int WPE, int RECON, int SPLIT = SPLIT_LF>
 __global__ __launch_bounds__(Tile<TX>::NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 void stage_kernel(StepArgs<Real> a, int stage, const Real* __restrict__ Us,
                   const Real* __restrict__ auxs, const Real* Un, Real* Uo,
                   Real* __restrict__ auxo) {
   __shared__ StageLds<Real, TX> L;
-  stage_body<Real, MODE, TX, RECON>(a, stage, Us, auxs, Un, Uo, auxo,
-                                    xcd_tile(blockIdx.x, gridDim.x), L);
+  stage_body<Real, MODE, TX, RECON, SPLIT>(a, stage, Us, auxs, Un, Uo, auxo,
+                                           xcd_tile(blockIdx.x, gridDim.x), L);
 }
 
 // ---------------------------------------------------------- chemistry

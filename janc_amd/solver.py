@@ -160,11 +160,12 @@ class Simulator:
                 raise KeyError(rs)
         else:
             rs = "HLLC"
-            if flux_config["split_method"] != "LF":
-                if flux_config["split_method"] == "SW":
-                    raise NotImplementedError("Steger-Warming splitting is not built")
-                raise KeyError(flux_config["split_method"])
-        self.flux = (_lib.SOLVER_TYPES[st], _lib.RECONSTRUCTIONS[rec], _lib.RIEMANN_SOLVERS[rs])
+        # split_method is read for flux splitting only (flux.py:33-35)
+        sm = flux_config["split_method"] if st == "flux_splitting" else "LF"
+        if sm not in _lib.SPLIT_METHODS:              # flux_splitting/__init__.py:3-6
+            raise KeyError(sm)
+        self.flux = (_lib.SOLVER_TYPES[st], _lib.SPLIT_METHODS[sm], _lib.RECONSTRUCTIONS[rec],
+                     _lib.RIEMANN_SOLVERS[rs])
         if flux_config["viscosity"] == "on":
             raise NotImplementedError("viscous fluxes are not built")
         if time_control["temporal_evolution_scheme"] != "TVD_RK3":
@@ -423,8 +424,8 @@ class Simulator:
         rules, get_T_nasa7's custom VJP (thermo_model.py:155-190) included."""
         if self.precision != _lib.PREC_FP64:
             raise NotImplementedError("the adjoint is built for the FP64 form")
-        if self.flux != (0, 0, 0):
-            raise NotImplementedError("the adjoint is built for flux splitting with WENO5_JS")
+        if self.flux != (0, 0, 0, 0):
+            raise NotImplementedError("the adjoint is built for LF flux splitting with WENO5_JS")
         U, aux = self._state(U, aux)
         Ub, ab = self._state(U_bar, aux_bar)
         ctx = self._context(U, None)

@@ -1,7 +1,7 @@
 """Oracle restatement of the 2-D Euler fluxes (test only).
 
 Follows ``src/janc_v2/solver_2D/aux_func.py:21-51``,
-``solver_2D/flux_splitting/LF.py:4-16``,
+``solver_2D/flux_splitting/LF.py:4-16``, ``flux_splitting/SW.py:4-38``,
 ``solver_2D/reconstruction/WENO5_JS.py:1-117``, ``reconstruction/MUSCL.py:1-65``,
 ``solver_2D/riemann_solver/HLLC.py:5-77`` and
 ``solver_2D/flux.py:43-74,123-130`` of the reference.
@@ -55,6 +55,44 @@ def split_flux_lf(ixy, U, aux, thermo):            # LF.py:4-16
     Hplus = 0.5 * (F + theta * U)
     Hminus = 0.5 * (F - theta * U)
     return Hplus, Hminus
+
+
+SW_EPS = 1e-6      # SW.py:26
+
+
+def split_flux_sw(ixy, U, aux, thermo):            # SW.py:4-38
+    """Steger-Warming splitting with the eigenvalue smoothing
+    lambda+- = (lambda +- sqrt(lambda^2 + eps^2)) / 2."""
+    with np.errstate(invalid="ignore"):
+        rho, u, v, Y, p, a = U_to_prim(U, aux, thermo)
+    rhoE = U[3:4]
+    gamma = aux[0:1]
+    zx = (ixy == 1) * 1.0
+    zy = (ixy == 2) * 1.0
+    theta = zx * u + zy * v
+    H1 = (1 / (2 * gamma)) * np.concatenate([rho, rho * u - rho * a * zx, rho * v - rho * a * zy,
+                                             rhoE + p - rho * a * theta, rho * Y], axis=0)
+    H2 = ((gamma - 1) / gamma) * np.concatenate(
+        [rho, rho * u, rho * v, 0.5 * rho * (u ** 2 + v ** 2), rho * Y], axis=0)
+    H4 = (1 / (2 * gamma)) * np.concatenate([rho, rho * u + rho * a * zx, rho * v + rho * a * zy,
+                                             rhoE + p + rho * a * theta, rho * Y], axis=0)
+    lambda1 = theta - a
+    lambda2 = theta
+    lambda4 = theta + a
+    eps = SW_EPS
+    with np.errstate(invalid="ignore"):
+        lap1 = 0.5 * (lambda1 + np.sqrt(np.power(lambda1, 2) + eps ** 2))
+        lam1 = 0.5 * (lambda1 - np.sqrt(np.power(lambda1, 2) + eps ** 2))
+        lap2 = 0.5 * (lambda2 + np.sqrt(np.power(lambda2, 2) + eps ** 2))
+        lam2 = 0.5 * (lambda2 - np.sqrt(np.power(lambda2, 2) + eps ** 2))
+        lap4 = 0.5 * (lambda4 + np.sqrt(np.power(lambda4, 2) + eps ** 2))
+        lam4 = 0.5 * (lambda4 - np.sqrt(np.power(lambda4, 2) + eps ** 2))
+    Hplus = lap1 * H1 + lap2 * H2 + lap4 * H4
+    Hminus = lam1 * H1 + lam2 * H2 + lam4 * H4
+    return Hplus, Hminus
+
+
+SPLIT = {"LF": split_flux_lf, "SW": split_flux_sw}   # flux_splitting/__init__.py:3-6
 
 
 def _weno_core(fjm2, fjm1, fj, fjp1, fjp2):
@@ -244,21 +282,21 @@ def godunov_face_fluxes(U, aux, thermo, reconstruction="WENO5_JS", riemann="HLLC
     return F, G
 
 
-def split_face_fluxes(U, aux, thermo, reconstruction="WENO5_JS"):
-    """(F, G) of flux.py:56-68 with either reconstruction."""
+def split_face_fluxes(U, aux, thermo, reconstruction="WENO5_JS", split="LF"):
+    """(F, G) of flux.py:56-68 with either reconstruction and split."""
     Lx, Rx, Ly, Ry = RECONSTRUCTION[reconstruction]
-    Fplus, Fminus = split_flux_lf(1, U, aux, thermo)
-    Gplus, Gminus = split_flux_lf(2, U, aux, thermo)
+    Fplus, Fminus = SPLIT[split](1, U, aux, thermo)
+    Gplus, Gminus = SPLIT[split](2, U, aux, thermo)
     return Lx(Fplus) + Rx(Fminus), Ly(Gplus) + Ry(Gminus)
 
 
 def scheme_total_flux(U, aux, dx, dy, thermo, solver_type="flux_splitting",
-                      reconstruction="WENO5_JS", riemann="HLLC"):
+                      reconstruction="WENO5_JS", riemann="HLLC", split="LF"):
     """-(dF/dx + dG/dy) for either solver_type (flux.py:43-74)."""
     if solver_type == "godunov":
         F, G = godunov_face_fluxes(U, aux, thermo, reconstruction, riemann)
     else:
-        F, G = split_face_fluxes(U, aux, thermo, reconstruction)
+        F, G = split_face_fluxes(U, aux, thermo, reconstruction, split)
     dF = F[:, 1:, :] - F[:, :-1, :]
     dG = G[:, :, 1:] - G[:, :, :-1]
     net_flux = dF / dx + dG / dy

@@ -29,13 +29,14 @@ class Oracle:
         self.solver_type = fc.get("solver_type", "flux_splitting")
         self.reconstruction = fc.get("interface_reconstruction", "WENO5_JS")
         self.riemann = fc.get("riemann_solver", "HLLC")
+        self.split = fc.get("split_method", "LF")
 
     # set_simulator.py:44-47
     def flux_func(self, U, aux, dt, theta=None):
         Ug, ag = self.boundary(U, aux, theta)
         return dt * (_flux.scheme_total_flux(Ug, ag, self.dx, self.dy, self.thermo,
                                              self.solver_type, self.reconstruction,
-                                             self.riemann))
+                                             self.riemann, self.split))
 
     def update_func(self, U, aux):                 # aux_func.py:21-29
         return _flux.update_aux(U, aux, self.thermo, per_cell=self.per_cell)

@@ -1,6 +1,7 @@
 """GPU parity of the other flux schemes of set_flux_solver (flux.py:19-40):
 the Godunov path (primitive reconstruction + HLLC, flux.py:43-54) with
-WENO5_JS and MUSCL, and flux splitting with MUSCL, against the FP64 oracle.
+WENO5_JS and MUSCL, LF flux splitting with MUSCL, and Steger-Warming flux
+splitting (flux_splitting/SW.py) with both, against the FP64 oracle.
 
 Bar as for the default scheme: FP64 <= 1e-12 relative error per variable;
 mixed form within the mixed-precision bounds of test_gpu_parity.
@@ -25,6 +26,10 @@ SCHEMES = {
                    "riemann_solver": "HLLC"},
     "lf_muscl": {"solver_type": "flux_splitting", "interface_reconstruction": "MUSCL",
                  "split_method": "LF"},
+    "sw_weno": {"solver_type": "flux_splitting", "interface_reconstruction": "WENO5_JS",
+                "split_method": "SW"},
+    "sw_muscl": {"solver_type": "flux_splitting", "interface_reconstruction": "MUSCL",
+                 "split_method": "SW"},
 }
 CASES = ["box", "tube", "outlet", "inlet"]
 
@@ -66,7 +71,7 @@ def test_advance_func_one_step(mech_path, mech, name, scheme):
     assert rel_err(ag, ar) <= FP64_TOL
 
 
-@pytest.mark.parametrize("scheme", ["hllc_weno", "hllc_muscl"])
+@pytest.mark.parametrize("scheme", ["hllc_weno", "hllc_muscl", "sw_weno"])
 @pytest.mark.parametrize("name", ["tube", "inlet"])
 def test_fixed_dt_five_steps(mech_path, mech, name, scheme):
     sim, o, U, aux, theta = _pair(mech_path, mech, name, scheme, dt=4e-9)
@@ -79,7 +84,7 @@ def test_fixed_dt_five_steps(mech_path, mech, name, scheme):
     assert rel_err(ag, ar) <= FP64_TOL
 
 
-@pytest.mark.parametrize("scheme", ["hllc_weno", "lf_muscl"])
+@pytest.mark.parametrize("scheme", ["hllc_weno", "lf_muscl", "sw_weno"])
 @pytest.mark.parametrize("name", ["tube", "inlet"])
 def test_mixed(mech_path, mech, name, scheme):
     sim, o, U, aux, theta = _pair(mech_path, mech, name, scheme, precision="mixed")

@@ -252,3 +252,65 @@ def test_godunov_and_splitting_converge_together(mech_path, mech):
         s = _smooth_rhs(mech_path, mech, nx, dict(GOD, solver_type="flux_splitting"))
         d.append(np.abs(g - s).max() / np.abs(s).max())
     assert d[1] < 1e-3 and d[0] / d[1] > 16.0, d
+
+
+# ------------------------------------------------------------------ Steger-Warming
+@pytest.mark.parametrize("ixy", [1, 2])
+def test_sw_split_sums_to_the_flux_of_a_perfect_gas(mech, thermo, ixy):
+    """F+ + F- = lambda1 H1 + lambda2 H2 + lambda4 H4 = F exactly when
+    rho E = p/(gamma-1) + rho|u|^2/2 (the Steger-Warming identity): pins every
+    entry of H1, H2, H4 and the eigenvalues."""
+    rng = np.random.default_rng(8)
+    q = _states(mech, rng, 80)
+    rho, u, v, p, Y = q[0:1], q[1:2], q[2:3], q[3:4], q[4:]
+    gamma = rng.uniform(1.2, 1.6, size=rho.shape)
+    E = p / (gamma - 1) + 0.5 * rho * (u ** 2 + v ** 2)
+    U = np.concatenate([rho, rho * u, rho * v, E, rho * Y], axis=0)
+    T = p / (thermo.get_R(Y) * rho)
+    aux = np.concatenate([gamma, T], axis=0)
+    Hp, Hm = F.split_flux_sw(ixy, U, aux, thermo)
+    Hl, Hr = F.split_flux_lf(ixy, U, aux, thermo)
+    Fphys = Hl + Hr
+    scale = np.abs(Fphys).max(axis=(1, 2), keepdims=True) + np.abs(U).max(axis=(1, 2),
+                                                                       keepdims=True) * 1e3
+    assert np.all(np.abs(Hp + Hm - Fphys) <= 1e-12 * scale)
+    # the eigenvalue split: H+ carries the non-negative, H- the non-positive
+    # characteristic speeds; a supersonic state has H- ~ 0
+    qs = q.copy()
+    qs[ixy] = 3000.0
+    rho, u, v, p, Y = qs[0:1], qs[1:2], qs[2:3], qs[3:4], qs[4:]
+    E = p / (gamma - 1) + 0.5 * rho * (u ** 2 + v ** 2)
+    U = np.concatenate([rho, rho * u, rho * v, E, rho * Y], axis=0)
+    Hp, Hm = F.split_flux_sw(ixy, U, aux, thermo)
+    assert np.all(np.abs(Hm) <= 1e-14 * np.abs(Hp).max(axis=(1, 2), keepdims=True))
+
+
+def test_sw_split_mirror_symmetry(mech, thermo):
+    rng = np.random.default_rng(9)
+    q = _states(mech, rng, 50)
+    U, gamma, T = _conserved(q, thermo)
+    aux = np.concatenate([gamma, T], axis=0)
+    Hp, Hm = F.split_flux_sw(1, U, aux, thermo)
+    Um = U.copy()
+    Um[1] *= -1
+    Mp, Mm = F.split_flux_sw(1, Um, aux, thermo)
+    sgn = -np.ones((12, 1, 1))
+    sgn[1] = 1
+    scale = np.abs(Hp).max(axis=(1, 2), keepdims=True)
+    assert np.all(np.abs(Mp - sgn * Hm) <= 1e-12 * scale)
+    assert np.all(np.abs(Mm - sgn * Hp) <= 1e-12 * scale)
+
+
+def test_sw_divergence_telescopes_and_preserves_uniform_flow(mech_path, mech):
+    U, aux, geo = cases.periodic_box(mech, nx=20, ny=14)
+    o = Oracle(mech_path, PER, geo["dx"], geo["dy"],
+               flux_config={"solver_type": "flux_splitting", "split_method": "SW",
+                            "interface_reconstruction": "WENO5_JS"})
+    rhs = o.flux_func(U, aux, 1.0)
+    tot = rhs[:, :-1, :-1].sum(axis=(1, 2))
+    assert np.all(np.abs(tot) <= 1e-11 * np.abs(rhs).sum(axis=(1, 2)))
+    T = np.full((10, 9), 900.0)
+    Y9 = cases.premixed_Y(mech, np.ones((10, 9)))
+    U, aux = cases.build_state(mech, T, np.full((10, 9), 2e5), Y9, np.full((10, 9), 120.0),
+                               np.full((10, 9), -40.0))
+    assert np.abs(o.flux_func(U, aux, 1e-7)).max() <= 1e-12 * np.abs(U).max()
