@@ -218,9 +218,10 @@ struct AdjLds {
   double Hyp[TXA][TY + 8], Hym[TXA][TY + 8];
   double Fbx[TXA + 5][TY];
   double Fby[TXA][TY + 5];
-  // F_bar * dq/dv_m of the faces reaching the tile: x phase [side][k][m][TY]
-  // (k < TXA+4), y phase [side][row][m][TY+4] in the same storage
-  double C[2 * (TXA + 4) * 5 * TY];
+  // F_bar * dq/dv_m of the faces reaching the tile: x faces
+  // [side][k][m][TY] (k < TXA+4), y faces [side][row][m][TY+4]
+  double Cx[2 * (TXA + 4) * 5 * TY];
+  double Cy[2 * TXA * 5 * (TY + 4)];
   double red[2][TXA];
 };
 
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
       L.Fby[row][k] = fb;
     }
     __syncthreads();
-    // ---- x faces: F_bar * dq/dv_m into LDS, then the own cell's sums ----
+    // ---- x and y faces: F_bar * dq/dv_m into LDS ----
     for (int e = t; e < 2 * (TXA + 4) * TY; e += G::NT) {
       const int side = e / ((TXA + 4) * TY), rem = e - side * ((TXA + 4) * TY);
       const int kk = rem >> 6, col = rem & 63;
@@ -379,24 +380,11 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
           weno5_grad(L.Hxm[kk + 4][col], L.Hxm[kk + 3][col], L.Hxm[kk + 2][col],
                      L.Hxm[kk + 1][col], L.Hxm[kk][col], g);
         const double fb = L.Fbx[side == 0 ? kk + 1 : kk][col];
-        double* out = &L.C[((side * (TXA + 4) + kk) * 5) * TY + col];
+        double* out = &L.Cx[((side * (TXA + 4) + kk) * 5) * TY + col];
 #pragma unroll
         for (int m = 0; m < 5; ++m) out[m * TY] = fb * g[m];
       }
     }
-    __syncthreads();
-    double hpx = 0.0, hmx = 0.0, hpy = 0.0, hmy = 0.0;
-    if (work && xrow) {
-#pragma unroll
-      for (int m = 0; m < 5; ++m)
-        if (P - m >= 0 && P - m <= nx) hpx += L.C[((w + 4 - m) * 5 + m) * TY + ly];
-#pragma unroll
-      for (int m = 0; m < 5; ++m)
-        if (P - 5 + m >= 0 && P - 5 + m <= nx)
-          hmx += L.C[(((TXA + 4) + w + m) * 5 + m) * TY + ly];
-    }
-    __syncthreads();
-    // ---- y faces ----
     for (int e = t; e < 2 * TXA * (TY + 4); e += G::NT) {
       const int side = e / (TXA * (TY + 4)), rem = e - side * (TXA * (TY + 4));
       const int row = rem / (TY + 4), kk = rem - row * (TY + 4);
@@ -411,20 +399,31 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
           weno5_grad(L.Hym[row][kk + 4], L.Hym[row][kk + 3], L.Hym[row][kk + 2],
                      L.Hym[row][kk + 1], L.Hym[row][kk], g);
         const double fb = L.Fby[row][side == 0 ? kk + 1 : kk];
-        double* out = &L.C[((side * TXA + row) * 5) * (TY + 4) + kk];
+        double* out = &L.Cy[((side * TXA + row) * 5) * (TY + 4) + kk];
 #pragma unroll
         for (int m = 0; m < 5; ++m) out[m * (TY + 4)] = fb * g[m];
       }
     }
     __syncthreads();
+    // ---- the own cell's sums over the faces that reach it ----
+    double hpx = 0.0, hmx = 0.0, hpy = 0.0, hmy = 0.0;
+    if (work && xrow) {
+#pragma unroll
+      for (int m = 0; m < 5; ++m)
+        if (P - m >= 0 && P - m <= nx) hpx += L.Cx[((w + 4 - m) * 5 + m) * TY + ly];
+#pragma unroll
+      for (int m = 0; m < 5; ++m)
+        if (P - 5 + m >= 0 && P - 5 + m <= nx)
+          hmx += L.Cx[(((TXA + 4) + w + m) * 5 + m) * TY + ly];
+    }
     if (work && ycol) {
 #pragma unroll
       for (int m = 0; m < 5; ++m)
-        if (Q - m >= 0 && Q - m <= ny) hpy += L.C[(w * 5 + m) * (TY + 4) + ly + 4 - m];
+        if (Q - m >= 0 && Q - m <= ny) hpy += L.Cy[(w * 5 + m) * (TY + 4) + ly + 4 - m];
 #pragma unroll
       for (int m = 0; m < 5; ++m)
         if (Q - 5 + m >= 0 && Q - 5 + m <= ny)
-          hmy += L.C[((TXA + w) * 5 + m) * (TY + 4) + ly + m];
+          hmy += L.Cy[((TXA + w) * 5 + m) * (TY + 4) + ly + m];
     }
     if (work) {
       const PrimL& qo = q[0];
@@ -463,7 +462,10 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
           Ub[cv] += Fx * qo.u + Fy * qo.v;
       }
     }
-    __syncthreads();   // LDS is refilled for the next variable
+    // No barrier before the next variable: its H and F_bar writes only
+    // conflict with this variable's face loops (finished before the barrier
+    // above), and its face loops' Cx/Cy writes come after its own first
+    // barrier, which every thread reaches after these reads.
   }
   if (own) {
     const int pt = P * PY + Q;
