@@ -157,6 +157,13 @@ int jh_lf_alpha(jh_ctx* ctx, const void* U, const void* aux, double* alpha_dev);
 int jh_step_vjp(jh_ctx* ctx, const void* U, const void* aux, double dt, const void* U_bar,
                 const void* aux_bar, void* U_in_bar, void* aux_in_bar, void* U_out,
                 void* aux_out);
+/* As jh_step_vjp for a step whose output (U_next, aux_next) the caller
+ * already holds (a rollout's next state): the RK stages are recomputed, the
+ * chemistry's forward pass is not.  No counterpart of its own in the
+ * reference (the same reverse-mode derivative). */
+int jh_step_vjp_known(jh_ctx* ctx, const void* U, const void* aux, double dt,
+                      const void* U_next, const void* aux_next, const void* U_bar,
+                      const void* aux_bar, void* U_in_bar, void* aux_in_bar);
 
 /* Device pointer and element count of one side's ghost strip (14 planes of
  * 3 x len: vars 0-11 = U, 12-13 = aux; left/right strips are (14, 3, ny) with

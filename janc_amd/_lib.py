@@ -48,6 +48,7 @@ SIGNATURES = {
     "jh_cfl_dt": (_c_int, [_vp, _vp, _vp, _c_double, _vp]),
     "jh_lf_alpha": (_c_int, [_vp, _vp, _vp, _vp]),
     "jh_step_vjp": (_c_int, [_vp, _vp, _vp, _c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "jh_step_vjp_known": (_c_int, [_vp, _vp, _vp, _c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jh_ghost_strip": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_int)]),
     "jh_set_ghost_strip": (_c_int, [_vp, _c_int, _vp]),
     "jh_step_begin": (_c_int, [_vp, _vp, _vp, _c_double, _c_double]),

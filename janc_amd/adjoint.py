@@ -5,7 +5,9 @@
 kernels).  ``rollout(sim, U, aux, dt, nsteps, checkpoint_every)`` runs many
 steps keeping only every ``checkpoint_every``-th state; its backward replays
 each segment from its checkpoint (the RK stages of every step are recomputed
-inside ``jh_step_vjp``) and sweeps the step VJPs in reverse.
+inside the step VJP) and sweeps the step VJPs in reverse.  Each step's VJP is
+given the step's output, which the forward pass or the replay already holds,
+so the chemistry's forward pass is not repeated (``jh_step_vjp_known``).
 
 These replace JAX's reverse mode over the reference's ``advance_one_step``
 (set_simulator.py:117-122), the custom VJP of ``get_T_nasa7``
@@ -14,24 +16,24 @@ These replace JAX's reverse mode over the reference's ``advance_one_step``
 import torch
 
 
-def _vjp(sim, U, aux, dt, Ub, ab):
-    return sim.step_vjp(U, aux, dt, Ub.contiguous(), ab.contiguous())
+def _vjp(sim, U, aux, dt, Ub, ab, out=None):
+    return sim.step_vjp(U, aux, dt, Ub.contiguous(), ab.contiguous(), known_output=out)
 
 
 class _Step(torch.autograd.Function):
     @staticmethod
     def forward(ctx, U, aux, sim, dt):
         Uo, ao = sim.advance_one_step(U.detach(), aux.detach(), sim.dx, sim.dy, dt)
-        ctx.save_for_backward(U.detach(), aux.detach())
+        ctx.save_for_backward(U.detach(), aux.detach(), Uo.detach(), ao.detach())
         ctx.sim, ctx.dt = sim, dt
         return Uo, ao
 
     @staticmethod
     def backward(ctx, Ub, ab):
-        U, aux = ctx.saved_tensors
+        U, aux, Uo, ao = ctx.saved_tensors
         Ub = torch.zeros_like(U) if Ub is None else Ub
         ab = torch.zeros_like(aux) if ab is None else ab
-        gU, ga = _vjp(ctx.sim, U, aux, ctx.dt, Ub, ab)
+        gU, ga = _vjp(ctx.sim, U, aux, ctx.dt, Ub, ab, out=(Uo, ao))
         return gU, ga, None, None
 
 
@@ -52,6 +54,7 @@ class _Rollout(torch.autograd.Function):
             U, aux = sim.advance_one_step(U, aux, sim.dx, sim.dy, dt)
         ctx.sim, ctx.dt, ctx.nsteps, ctx.every = sim, dt, nsteps, every
         ctx.ckpt = ckpt
+        ctx.final = (U.clone(), aux.clone())
         return U, aux
 
     @staticmethod
@@ -59,6 +62,7 @@ class _Rollout(torch.autograd.Function):
         sim, dt, nsteps, every = ctx.sim, ctx.dt, ctx.nsteps, ctx.every
         Ub = Ub.contiguous().clone()
         ab = (torch.zeros_like(ctx.ckpt[0][1]) if ab is None else ab).contiguous().clone()
+        nxt = ctx.final                            # output of the segment's last step
         for seg in reversed(range(len(ctx.ckpt))):
             U, aux = ctx.ckpt[seg]
             first = seg * every
@@ -67,9 +71,11 @@ class _Rollout(torch.autograd.Function):
             for _ in range(first, last - 1):       # replay the segment
                 U, aux = sim.advance_one_step(U, aux, sim.dx, sim.dy, dt)
                 states.append((U, aux))
-            for Us, As in reversed(states):
-                Ub, ab = _vjp(sim, Us, As, dt, Ub, ab)
-        ctx.ckpt = None
+            outs = states[1:] + [nxt]
+            for (Us, As), out in zip(reversed(states), reversed(outs)):
+                Ub, ab = _vjp(sim, Us, As, dt, Ub, ab, out=out)
+            nxt = ctx.ckpt[seg]
+        ctx.ckpt = ctx.final = None
         return Ub, ab, None, None, None, None
 
 

@@ -379,8 +379,12 @@ void flux_vjp(jh_ctx* c, const AdjArgs& a, AdjWork& w, const double* Us, const d
   unpad_kernel<<<blocks_for(n), 256, 0, st>>>(a, w.Ubp, w.abp, Ut, At);
 }
 
+// Uk/Ak: the step's output when the caller already has it (the RK stages are
+// still recomputed; only the chemistry's forward pass, whose output the
+// reverse sweep starts from, is skipped).
 int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, const double* Ubar,
-                  const double* Abar, double* Ub0, double* Ab0, double* Uo, double* Ao) {
+                  const double* Abar, double* Ub0, double* Ab0, double* Uo, double* Ao,
+                  const double* Uk = nullptr, const double* Ak = nullptr) {
   AdjWork w;
   int rc = adj_work(c, U0, A0, w);
   if (rc) return rc;
@@ -397,9 +401,17 @@ int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, cons
     bc_kernel<double><<<bb, kBcThreads, 0, st>>>(a, w.U[s], w.A[s], s);
     launch_stage<double, MODE_RK>(c, a, s, w.U[s], w.A[s], U0, w.U[s + 1], w.A[s + 1]);
   }
-  HIP_TRY(c, hipMemcpyAsync(w.U[4], w.U[3], NU * n * sizeof(double), hipMemcpyDeviceToDevice, st));
-  HIP_TRY(c, hipMemcpyAsync(w.A[4], w.A[3], 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
-  chem_kernel<double, CHEM_STEP><<<grid_all(c), 256, 0, st>>>(a, w.U[4], w.A[4], nullptr, nullptr);
+  if (Uk && Ak) {
+    w.U[4] = const_cast<double*>(Uk);
+    w.A[4] = const_cast<double*>(Ak);
+  } else {
+    HIP_TRY(c, hipMemcpyAsync(w.U[4], w.U[3], NU * n * sizeof(double), hipMemcpyDeviceToDevice,
+                              st));
+    HIP_TRY(c, hipMemcpyAsync(w.A[4], w.A[3], 2 * n * sizeof(double), hipMemcpyDeviceToDevice,
+                              st));
+    chem_kernel<double, CHEM_STEP><<<grid_all(c), 256, 0, st>>>(a, w.U[4], w.A[4], nullptr,
+                                                                nullptr);
+  }
   rc = check_launch(c, "vjp forward");
   if (rc) return rc;
   c->parity ^= 1;
@@ -869,10 +881,9 @@ int jh_lf_alpha(jh_ctx* c, const void* U, const void* aux, double* alpha_dev) {
   return check_launch(c, "lf_alpha kernels");
 }
 
-int jh_step_vjp(jh_ctx* c, const void* U, const void* aux, double dt, const void* U_bar,
-                const void* aux_bar, void* U_in_bar, void* aux_in_bar, void* U_out,
-                void* aux_out) {
-  if (!c) return JH_ERR_ARG;
+namespace {
+int vjp_checks(jh_ctx* c, const void* U, const void* aux, double dt, const void* U_bar,
+               const void* aux_bar, const void* U_in_bar, const void* aux_in_bar) {
   if (!U || !aux || !U_bar || !aux_bar || !U_in_bar || !aux_in_bar)
     return fail(c, JH_ERR_ARG, "NULL pointer");
   if (c->precision != JH_PREC_FP64) return fail(c, JH_ERR_ARG, "the adjoint is FP64 only");
@@ -885,9 +896,31 @@ int jh_step_vjp(jh_ctx* c, const void* U, const void* aux, double dt, const void
         c->bc[s] == JH_BC_INJECTION_INLET)
       return fail(c, JH_ERR_ARG, "the adjoint supports periodic, zero_gradient, slip_wall and "
                                  "no_slip_wall boundaries");
+  return JH_OK;
+}
+}  // namespace
+
+int jh_step_vjp(jh_ctx* c, const void* U, const void* aux, double dt, const void* U_bar,
+                const void* aux_bar, void* U_in_bar, void* aux_in_bar, void* U_out,
+                void* aux_out) {
+  if (!c) return JH_ERR_ARG;
+  int rc = vjp_checks(c, U, aux, dt, U_bar, aux_bar, U_in_bar, aux_in_bar);
+  if (rc) return rc;
   return step_vjp_impl(c, (const double*)U, (const double*)aux, dt, (const double*)U_bar,
                        (const double*)aux_bar, (double*)U_in_bar, (double*)aux_in_bar,
                        (double*)U_out, (double*)aux_out);
+}
+
+int jh_step_vjp_known(jh_ctx* c, const void* U, const void* aux, double dt, const void* U_next,
+                      const void* aux_next, const void* U_bar, const void* aux_bar,
+                      void* U_in_bar, void* aux_in_bar) {
+  if (!c) return JH_ERR_ARG;
+  if (!U_next || !aux_next) return fail(c, JH_ERR_ARG, "NULL pointer");
+  int rc = vjp_checks(c, U, aux, dt, U_bar, aux_bar, U_in_bar, aux_in_bar);
+  if (rc) return rc;
+  return step_vjp_impl(c, (const double*)U, (const double*)aux, dt, (const double*)U_bar,
+                       (const double*)aux_bar, (double*)U_in_bar, (double*)aux_in_bar, nullptr,
+                       nullptr, (const double*)U_next, (const double*)aux_next);
 }
 
 int jh_timing_enable(jh_ctx* c, int on) {

@@ -416,12 +416,14 @@ class Simulator:
         ctx.call("jh_lf_alpha", _ptr(U), _ptr(aux), _ptr(out))
         return out
 
-    def step_vjp(self, U, aux, dt, U_bar, aux_bar, with_output=False):
+    def step_vjp(self, U, aux, dt, U_bar, aux_bar, with_output=False, known_output=None):
         """Reverse-mode derivative of advance_one_step at fixed dt (FP64).
 
         Returns (U_in_bar, aux_in_bar), plus the step's output (U, aux) when
         ``with_output``: the VJP of set_simulator.py:117-122 under JAX's
-        rules, get_T_nasa7's custom VJP (thermo_model.py:155-190) included."""
+        rules, get_T_nasa7's custom VJP (thermo_model.py:155-190) included.
+        ``known_output`` = the step's output (U, aux) when the caller has it
+        (jh_step_vjp_known: the chemistry's forward pass is not repeated)."""
         if self.precision != _lib.PREC_FP64:
             raise NotImplementedError("the adjoint is built for the FP64 form")
         if self.flux != (0, 0, 0, 0):
@@ -430,6 +432,11 @@ class Simulator:
         Ub, ab = self._state(U_bar, aux_bar)
         ctx = self._context(U, None)
         gU, ga = torch.empty_like(U), torch.empty_like(aux)
+        if known_output is not None and not with_output:
+            Un, an = self._state(*known_output)
+            ctx.call("jh_step_vjp_known", _ptr(U), _ptr(aux), float(dt), _ptr(Un), _ptr(an),
+                     _ptr(Ub), _ptr(ab), _ptr(gU), _ptr(ga))
+            return gU, ga
         Uo = torch.empty_like(U) if with_output else None
         ao = torch.empty_like(aux) if with_output else None
         ctx.call("jh_step_vjp", _ptr(U), _ptr(aux), float(dt), _ptr(Ub), _ptr(ab), _ptr(gU),

@@ -129,6 +129,16 @@ def test_step_vjp_linear_in_cotangent(mech_path, mech):
     assert rel_err(a3, 2.0 * a1 - a2).max() <= 1e-12
 
 
+def test_vjp_with_known_output_is_identical(mech_path, mech):
+    """jh_step_vjp_known (the step's output supplied, no chemistry forward
+    pass) gives the same cotangents bit for bit as jh_step_vjp."""
+    sim, g, U, aux = _setup(mech_path, mech, "tube")
+    W, Wa = _cotangents(U, aux, 7)
+    gU, ga, Uo, ao = sim.step_vjp(U, aux, DT, W, Wa, with_output=True)
+    kU, ka = sim.step_vjp(U, aux, DT, W, Wa, known_output=(Uo, ao))
+    assert torch.equal(gU, kU) and torch.equal(ga, ka)
+
+
 def test_rollout_checkpointing_matches_chained_steps(mech_path, mech):
     from janc_amd import adjoint
     sim, g, U, aux = _setup(mech_path, mech, "tube")
