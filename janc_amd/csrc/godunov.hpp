@@ -167,38 +167,42 @@ __device__ __forceinline__ void godunov_stage_body(const StepArgs<Real>& a, int 
   const bool own = gx < nx && gy < ny;
   const int o = gx * ny + gy;
   const double dt = step_dt(a);
-  const Real idx = (Real)(1.0 / a.dx), idy = (Real)(1.0 / a.dy);
-  Real un[NU];
+  const double idx = 1.0 / a.dx, idy = 1.0 / a.dy;
+  double un[NU];       // FP64 whatever Real (see stage_body)
 #pragma unroll
   for (int cv = 0; cv < NU; ++cv) {
     Real Gn = __shfl_down(Fl[cv], 1);
     if (ly == TY - 1) Gn = L.Ge[cv][w];
-    const Real dF = L.Fx[cv][w + 1][ly] - L.Fx[cv][w][ly];
-    const Real dG = Gn - Fl[cv];
-    const Real rhs = (Real)dt * (-(dF * idx + dG * idy));
+    const double dF = (double)L.Fx[cv][w + 1][ly] - (double)L.Fx[cv][w][ly];
+    const double dG = (double)Gn - (double)Fl[cv];
+    const double rhs = dt * (-(dF * idx + dG * idy));
     if (MODE == MODE_RHS) {
       un[cv] = rhs;
     } else {
-      const Real uc = own ? Us[cv * n + o] : (Real)0;
+      const double uc = own ? (double)Us[cv * n + o] : 0.0;
       if (stage == 0) {
         un[cv] = uc + rhs;
       } else {
-        const Real u0 = own ? Un[cv * n + o] : (Real)0;
-        if (stage == 1) un[cv] = (Real)(3.0 / 4.0) * u0 + (Real)(1.0 / 4.0) * (uc + rhs);
-        else un[cv] = (Real)(1.0 / 3.0) * u0 + (Real)(2.0 / 3.0) * (uc + rhs);
+        const double u0 = own ? (double)Un[cv * n + o] : 0.0;
+        if (stage == 1) un[cv] = (3.0 / 4.0) * u0 + (1.0 / 4.0) * (uc + rhs);
+        else un[cv] = (1.0 / 3.0) * u0 + (2.0 / 3.0) * (uc + rhs);
       }
     }
-    if (own) Uo[cv * n + o] = un[cv];
+    if (own) Uo[cv * n + o] = (Real)un[cv];
   }
   if (MODE == MODE_RHS || !own) return;
 
   // ---------------- update_aux (aux_func.py:21-29) of the new state
-  const double rho = (double)un[0], ir = rcp(rho);
-  const double u = (double)un[1] * ir, v = (double)un[2] * ir;
-  const double e = (double)un[3] * ir - 0.5 * (u * u + v * v);
+  // the Halley solve sees the stored (rounded) state, as the next stage does
+  double us[NU];
+#pragma unroll
+  for (int cv = 0; cv < NU; ++cv) us[cv] = (double)(Real)un[cv];
+  const double rho = us[0], ir = rcp(rho);
+  const double u = us[1] * ir, v = us[2] * ir;
+  const double e = us[3] * ir - 0.5 * (u * u + v * v);
   double Y[JH_NS];
 #pragma unroll
-  for (int k = 0; k < JH_NS - 1; ++k) Y[k] = (double)un[4 + k] * ir;
+  for (int k = 0; k < JH_NS - 1; ++k) Y[k] = us[4 + k] * ir;
   const double R = fill_Y_get_R(c, Y);
   double T = (double)auxs[n + o];
   double gamma;

@@ -575,13 +575,16 @@ struct StageLds {
 };
 
 // One tile (index `tile` in y-fastest order) of one RK stage; RECON selects
-// the reconstruction of the split fluxes (recon.hpp), SPLIT the splitting.
-template <typename Real, int MODE, int TX, int RECON = RECON_WENO5, int SPLIT = SPLIT_LF>
+// the reconstruction of the split fluxes (recon.hpp), SPLIT the splitting,
+// W the arithmetic type of the split fluxes, reconstructions, divergence and
+// RK combine (Real, the storage type, unless stated).
+template <typename Real, int MODE, int TX, int RECON = RECON_WENO5, int SPLIT = SPLIT_LF,
+          typename W = Real>
 __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
                                            const Real* __restrict__ Us,
                                            const Real* __restrict__ auxs, const Real* Un,
                                            Real* Uo, Real* __restrict__ auxo, int tile,
-                                           StageLds<Real, TX>& L) {
+                                           StageLds<W, TX>& L) {
   using G = Tile<TX>;
   constexpr int NT = G::NT, NW = G::NW;
   constexpr int NXF = G::NXF, NYF = G::NYF, NHS = G::NHS, NSL = G::NSL;
@@ -602,9 +605,9 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
   const int x0 = (tile / tiles_y) * TX;
   const int y0 = (tile % tiles_y) * TY;
   const double dt = step_dt(a);
-  const Real idx = (Real)(1.0 / a.dx), idy = (Real)(1.0 / a.dy);
-  const Real ax = (Real)bitsd(a.sc->amax[a.parity][stage][0]);
-  const Real ay = (Real)bitsd(a.sc->amax[a.parity][stage][1]);
+  const double idx = 1.0 / a.dx, idy = 1.0 / a.dy;
+  const W ax = (W)(Real)bitsd(a.sc->amax[a.parity][stage][0]);
+  const W ay = (W)(Real)bitsd(a.sc->amax[a.parity][stage][1]);
 
   int rrow[NSL], rcol[NSL];       // LDS coordinates: x-region row / y-region col
   int gxs[NSL], gys[NSL];
@@ -634,18 +637,18 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
     gys[s] = y0 - 3 + col;
   }
   CellRef<Real> ref[NSL];
-  Prim<Real> q[NSL];
+  Prim<W> q[NSL];
   Real nxt[NSL];
 #pragma unroll
   for (int s = 0; s < NSL; ++s) {
     ref[s] = locate<Real>(a, Us, auxs, gxs[s], gys[s]);
     if (!has[s]) ref[s].ok = false;
     if (ref[s].ok) {
-      q[s] = cell_prim<Real, SPLIT == SPLIT_SW>(c, ref[s].u, ref[s].x, ref[s].stride);
+      q[s] = cell_prim<W, SPLIT == SPLIT_SW, Real>(c, ref[s].u, ref[s].x, ref[s].stride);
       nxt[s] = ref[s].u[0];
     } else {
-      q[s].rho = (Real)1; q[s].u = (Real)0; q[s].v = (Real)0; q[s].p = (Real)0; q[s].a = (Real)0;
-      q[s].g = (Real)1;
+      q[s].rho = (W)1; q[s].u = (W)0; q[s].v = (W)0; q[s].p = (W)0; q[s].a = (W)0;
+      q[s].g = (W)1;
       nxt[s] = (Real)0;
     }
   }
@@ -659,36 +662,36 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
 
   for (int cv = 0; cv < NU; ++cv) {
     // ---- phase 1: split fluxes into LDS ----
-    Real uc_own[2];
+    W uc_own[2];
 #pragma unroll
     for (int s = 0; s < NSL; ++s) {
       if (!has[s]) continue;
-      const Real ucv = nxt[s];
+      const W ucv = (W)nxt[s];
       if (s < 2) uc_own[s] = ucv;
       const bool xreg = s < NSL - 1, yreg = (s < 2) || (s == NSL - 1);
       if (xreg) {
-        Real hp = (Real)0, hm = (Real)0;
+        W hp = (W)0, hm = (W)0;
         if (ref[s].ok) {
           if (SPLIT == SPLIT_SW) {
-            sw_split<Real>(cv, 0, q[s], ucv, hp, hm);
+            sw_split<W>(cv, 0, q[s], ucv, hp, hm);
           } else {
-            const Real F = phys_flux<Real>(cv, 0, q[s], ucv);
-            hp = (Real)0.5 * (F + ax * ucv);
-            hm = (Real)0.5 * (F - ax * ucv);
+            const W F = phys_flux<W>(cv, 0, q[s], ucv);
+            hp = (W)0.5 * (F + ax * ucv);
+            hm = (W)0.5 * (F - ax * ucv);
           }
         }
         sHxp[rrow[s]][ly] = hp;
         sHxm[rrow[s]][ly] = hm;
       }
       if (yreg) {
-        Real hp = (Real)0, hm = (Real)0;
+        W hp = (W)0, hm = (W)0;
         if (ref[s].ok) {
           if (SPLIT == SPLIT_SW) {
-            sw_split<Real>(cv, 1, q[s], ucv, hp, hm);
+            sw_split<W>(cv, 1, q[s], ucv, hp, hm);
           } else {
-            const Real Gf = phys_flux<Real>(cv, 1, q[s], ucv);
-            hp = (Real)0.5 * (Gf + ay * ucv);
-            hm = (Real)0.5 * (Gf - ay * ucv);
+            const W Gf = phys_flux<W>(cv, 1, q[s], ucv);
+            hp = (W)0.5 * (Gf + ay * ucv);
+            hm = (W)0.5 * (Gf - ay * ucv);
           }
         }
         const int yr = s < 2 ? rrow[s] - 3 : rrow[s];
@@ -704,28 +707,28 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
       for (int s = 0; s < NSL; ++s)
         if (ref[s].ok) nxt[s] = ref[s].u[(cv + 1) * ref[s].stride];
     }
-    Real u0n[2] = {(Real)0, (Real)0};
+    W u0n[2] = {(W)0, (W)0};
     if (MODE != MODE_RHS && stage != 0) {
 #pragma unroll
       for (int s = 0; s < 2; ++s)
-        if (own[s]) u0n[s] = Un[cv * n + oidx[s]];
+        if (own[s]) u0n[s] = (W)Un[cv * n + oidx[s]];
     }
     // ---- phase 2: face reconstructions ----
     for (int it = t; it < NXF + NYF; it += NT) {
       if (it < NXF) {
         const int fr = it >> 6, col = it & 63;
-        const Real L = recon_L<RECON, Real>(sHxp[fr][col], sHxp[fr + 1][col], sHxp[fr + 2][col],
+        const W L = recon_L<RECON, W>(sHxp[fr][col], sHxp[fr + 1][col], sHxp[fr + 2][col],
                                             sHxp[fr + 3][col], sHxp[fr + 4][col]);
-        const Real R = recon_R<RECON, Real>(sHxm[fr + 1][col], sHxm[fr + 2][col],
+        const W R = recon_R<RECON, W>(sHxm[fr + 1][col], sHxm[fr + 2][col],
                                             sHxm[fr + 3][col], sHxm[fr + 4][col],
                                             sHxm[fr + 5][col]);
         sFx[fr][col] = L + R;
       } else {
         const int j = it - NXF;
         const int xr = j / (TY + 1), fc = j - xr * (TY + 1);
-        const Real L = recon_L<RECON, Real>(sHyp[xr][fc], sHyp[xr][fc + 1], sHyp[xr][fc + 2],
+        const W L = recon_L<RECON, W>(sHyp[xr][fc], sHyp[xr][fc + 1], sHyp[xr][fc + 2],
                                             sHyp[xr][fc + 3], sHyp[xr][fc + 4]);
-        const Real R = recon_R<RECON, Real>(sHym[xr][fc + 1], sHym[xr][fc + 2],
+        const W R = recon_R<RECON, W>(sHym[xr][fc + 1], sHym[xr][fc + 2],
                                             sHym[xr][fc + 3], sHym[xr][fc + 4],
                                             sHym[xr][fc + 5]);
         sFy[xr][fc] = L + R;
@@ -736,20 +739,23 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       const int ir = rrow[s] - 3;
-      const Real dF = sFx[ir + 1][ly] - sFx[ir][ly];
-      const Real dG = sFy[ir][ly + 1] - sFy[ir][ly];
-      const Real rhs = (Real)dt * (-(dF * idx + dG * idy));
-      Real un;
+      // FP64 whatever W: in FP32 the stage increment, often below half an
+      // ulp of the state, would be lost to each intermediate rounding and
+      // the mixed form's error would grow systematically
+      const double dF = (double)sFx[ir + 1][ly] - (double)sFx[ir][ly];
+      const double dG = (double)sFy[ir][ly + 1] - (double)sFy[ir][ly];
+      const double rhs = dt * (-(dF * idx + dG * idy));
+      double un;
       if (MODE == MODE_RHS) {
         un = rhs;
       } else if (stage == 0) {
-        un = uc_own[s] + rhs;
+        un = (double)uc_own[s] + rhs;
       } else {
-        const Real u0 = u0n[s];
-        if (stage == 1) un = (Real)(3.0 / 4.0) * u0 + (Real)(1.0 / 4.0) * (uc_own[s] + rhs);
-        else un = (Real)(1.0 / 3.0) * u0 + (Real)(2.0 / 3.0) * (uc_own[s] + rhs);
+        const double u0 = (double)u0n[s], uc = (double)uc_own[s];
+        if (stage == 1) un = (3.0 / 4.0) * u0 + (1.0 / 4.0) * (uc + rhs);
+        else un = (1.0 / 3.0) * u0 + (2.0 / 3.0) * (uc + rhs);
       }
-      if (own[s]) Uo[cv * n + oidx[s]] = un;
+      if (own[s]) Uo[cv * n + oidx[s]] = (Real)un;
     }
     // Drain this variable's memory operations here: loads left pending on
     // the stage-uniform branches above make the compiler wait for every
@@ -794,14 +800,15 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
   }
 }
 
-template <typename Real, int MODE, int TX, int WPE, int RECON, int SPLIT = SPLIT_LF>
+template <typename Real, int MODE, int TX, int WPE, int RECON, int SPLIT = SPLIT_LF,
+          typename W = Real>
 __global__ __launch_bounds__(Tile<TX>::NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 void stage_kernel(StepArgs<Real> a, int stage, const Real* __restrict__ Us,
                   const Real* __restrict__ auxs, const Real* Un, Real* Uo,
                   Real* __restrict__ auxo) {
-  __shared__ StageLds<Real, TX> L;
-  stage_body<Real, MODE, TX, RECON, SPLIT>(a, stage, Us, auxs, Un, Uo, auxo,
-                                           xcd_tile(blockIdx.x, gridDim.x), L);
+  __shared__ StageLds<W, TX> L;
+  stage_body<Real, MODE, TX, RECON, SPLIT, W>(a, stage, Us, auxs, Un, Uo, auxo,
+                                              xcd_tile(blockIdx.x, gridDim.x), L);
 }
 
 // ---------------------------------------------------------- chemistry
