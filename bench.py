@@ -458,9 +458,8 @@ def main():
         args.no_mixed = args.no_cpu = args.no_batch = args.no_adjoint = True
         args.no_schemes = args.no_snapshots = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs the single-block step on one GPU; "
-              "--gpus N > 1 is not supported (see DESIGN.md for how it would shard)",
-              file=sys.stderr)
+        print("bench.py: the single-block step runs on one GPU; --gpus N > 1 is not "
+              "supported (the x-slab decomposition is not built)", file=sys.stderr)
         sys.exit(2)
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
