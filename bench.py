@@ -42,7 +42,7 @@ STAGE_TRAFFIC_B_PER_CELL_FP64 = 472.3
 # Measured FP64 FLOP per cell per launch of the FP64 headline kernels
 # (rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes / cells, FMA = 2; summary
 # under profiles/): the bench turns them into live FP64-vector fractions.
-FP64_FLOP_PER_CELL = {"stage": 5210.0, "chem": 3241.0}
+FP64_FLOP_PER_CELL = {"stage": 5210.0, "chem": 3326.0}
 
 
 def _run(sim, U, aux, steps, warmup, torch):

@@ -113,7 +113,7 @@ void stage_batch_kernel(const CaseDesc<Real>* __restrict__ d, const int* __restr
 // Chemistry + final update_aux for every case; dt_hist (optional) is
 // (ncase, hist_len) row-major, step s written at column s.
 template <typename Real>
-__global__ __launch_bounds__(256) void chem_batch_kernel(const CaseDesc<Real>* __restrict__ d,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void chem_batch_kernel(const CaseDesc<Real>* __restrict__ d,
                                                          const int* __restrict__ off, int ncase,
                                                          int parity, double dt_fixed, double cfl,
                                                          double* dt_hist, int hist_len, int s) {

@@ -253,7 +253,8 @@ def _generate(m):
     e("// model/reaction_model.py:91-130 (reactionConstant_i summed over i).")
     e("// X: unclipped rho*Y/M (all species); Xc: clip(X, 1e-50) for active species;")
     e("// invMX[n] = 1/(M_n*Xc_n); eg/ieg: exp(+-g) of the Gibbs s/R - h/(RT) of the active")
-    e("// species, so exp(-sum_k dnu_k g_k) = prod_k (eg_k or ieg_k)^|dnu_k|.")
+    e("// species, so exp(-sum_k dnu_k g_k) = prod_k (eg_k or ieg_k)^|dnu_k| (formed here as")
+    e("// a quotient of eg products; ieg is read by the VJP).")
     e("__host__ __device__ __forceinline__ void mech_rates_jacobian(")
     e("    const MechConst& __restrict__ c, const double T, const double invT, const double logT,")
     e("    const double (&X)[JH_NS], const double (&Xc)[JH_NA], const double (&invMX)[JH_NA],")
@@ -266,7 +267,17 @@ def _generate(m):
     e("  for (int k = 0; k < JH_NA; ++k) { w[k] = 0.0;")
     e("#pragma unroll")
     e("    for (int n = 0; n < JH_NA; ++n) J[k][n] = 0.0; }")
+    # third-body concentrations first: X is not needed after them
     for i, r in enumerate(m.reactions):
+        if r.three_body:
+            terms = " + ".join("c.eff[%d][%d] * X[%d]" % (i, k, k) for k in range(ns))
+            e("  const double cM%d = %s;" % (i, terms))
+    for i, r in enumerate(m.reactions):
+        # one reaction at a time: interleaving them for ILP raises the live
+        # register count past what two resident waves per SIMD allow
+        e("#if defined(__HIP_DEVICE_COMPILE__)")
+        e("  __builtin_amdgcn_sched_barrier(0);")
+        e("#endif")
         e("  { // R%d: %s" % (i, r.equation))
         # kf_i = A * T^B * exp(-Ea/R / T)
         if r.b == 0.0:
@@ -290,13 +301,19 @@ def _generate(m):
         pf = " * ".join(_power_expr(k, n) for k, n in sorted(r.reactants.items()))
         pb = " * ".join(_power_expr(k, n) for k, n in sorted(r.products.items()))
         e("    const double kf = kfi * (%s);" % pf)
-        # exp(-sum_k dnu_k g_k) as a product of per-species factors
-        fac = []
+        # exp(-sum_k dnu_k g_k) = prod(eg of net reactants) / prod(eg of net
+        # products): one reciprocal per reaction instead of a live ieg table
+        num, den = [], []
         for k in range(na):
             d = r.products.get(k, 0.0) - r.reactants.get(k, 0.0)
             if d != 0.0:
-                fac += [("ieg[%d]" if d > 0 else "eg[%d]") % k] * int(abs(d))
-        eqf = " * ".join(fac) if fac else "1.0"
+                (den if d > 0 else num).extend(["eg[%d]" % k] * int(abs(d)))
+        if num and den:
+            eqf = "(%s) * jh::rcp(%s)" % (" * ".join(num), " * ".join(den))
+        elif den:
+            eqf = "jh::rcp(%s)" % " * ".join(den)
+        else:
+            eqf = " * ".join(num) if num else "1.0"
         vs = sum(r.products.values()) - sum(r.reactants.values())
         # kb = kf_i / (exp(sum dnu g) Pf^vsum) * prod X^nu_b (reaction_model.py:113)
         if vs == 0.0:
@@ -309,8 +326,7 @@ def _generate(m):
             eq = "(%s * pow(Pf, %s))" % (eqf, _fmt(-vs))
         e("    const double kb = kfi * %s * (%s);" % (eq, pb))
         if r.three_body:
-            terms = " + ".join("c.eff[%d][%d] * X[%d]" % (i, k, k) for k in range(ns))
-            e("    const double cM = %s;" % terms)
+            e("    const double cM = cM%d;" % i)
         e("    const double q = kf - kb;")
         cq = "cM * q" if r.three_body else "q"
         # w_k += dnu_k * cM * q
