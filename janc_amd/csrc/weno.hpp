@@ -12,28 +12,33 @@
 
 namespace jh {
 
-// FP64 form.  w_k = alpha_k / sum(alpha) is evaluated as
-// C_k prod_{j!=k} d_j / sum_m C_m prod_{j!=m} d_j with d_k = (eps+IS_k)^2,
-// the same ratio with one reciprocal instead of six divisions; C_k is folded
-// into the candidate stencils' coefficients (50 FP64 operations per face).
+// FP64 form.  With the first differences d and the second differences b
+// around f0 (d0 = f0 - fm1, d1 = fp1 - f0, b_k centred on fm1, f0, fp1), the
+// WENO5_JS.py:17-33 smoothness indicators and candidates are
+//   IS_1 = (b_-1/2 + d0)^2 + 13/12 b_-1^2,   q_1 = f0 + b_-1/3 + d0/2
+//   IS_2 = ((d0 + d1)/2)^2 + 13/12 b_0^2,    q_2 = f0 - b_0/6 + d1/2
+//   IS_3 = (b_+1/2 - d1)^2 + 13/12 b_+1^2,   q_3 = f0 - b_+1/6 + d1/2
+// (a_1 = b_-1 + 2 d0, a_2 = -(d0 + d1), a_3 = b_+1 - 2 d1 in the reference's
+// terms).  w_k = alpha_k / sum(alpha) is evaluated as
+// C_k prod_{j!=k} e_j / sum_m C_m prod_{j!=m} e_j with e_k = (eps+IS_k)^2,
+// the same ratio with one reciprocal instead of six divisions, and the value
+// as f0 + sum_k w_k (q_k - f0).  44 FP64 operations per face.
 __host__ __device__ __forceinline__ double weno5(double fm2, double fm1, double f0,
                                                  double fp1, double fp2) {
-  const double a1 = fm2 - 4.0 * fm1 + 3.0 * f0, b1 = fm2 - 2.0 * fm1 + f0;
-  const double a2 = fm1 - fp1, b2 = fm1 - 2.0 * f0 + fp1;
-  const double a3 = 3.0 * f0 - 4.0 * fp1 + fp2, b3 = f0 - 2.0 * fp1 + fp2;
-  // eps + IS_k, IS_k = a_k^2/4 + 13/12 b_k^2
-  const double d1 = fma(0.25 * a1, a1, fma((13.0 / 12.0) * b1, b1, 1e-6));
-  const double d2 = fma(0.25 * a2, a2, fma((13.0 / 12.0) * b2, b2, 1e-6));
-  const double d3 = fma(0.25 * a3, a3, fma((13.0 / 12.0) * b3, b3, 1e-6));
-  const double e1 = d1 * d1, e2 = d2 * d2, e3 = d3 * d3;
-  const double p1 = e2 * e3, p2 = e1 * e3, p3 = e1 * e2;
-  // C_k q_k: (1/3, -7/6, 11/6) * 0.1, (-1/6, 5/6, 1/3) * 0.6, (1/3, 5/6, -1/6) * 0.3
-  const double q1 = (0.1 / 3.0) * fm2 - (0.7 / 6.0) * fm1 + (1.1 / 6.0) * f0;
-  const double q2 = -(0.6 / 6.0) * fm1 + (3.0 / 6.0) * f0 + (0.6 / 3.0) * fp1;
-  const double q3 = (0.3 / 3.0) * f0 + (1.5 / 6.0) * fp1 - (0.3 / 6.0) * fp2;
-  const double num = p1 * q1 + p2 * q2 + p3 * q3;
-  const double den = 0.1 * p1 + 0.6 * p2 + 0.3 * p3;
-  return num * rcp(den);
+  const double dA = fm1 - fm2, dB = f0 - fm1, dC = fp1 - f0, dD = fp2 - fp1;
+  const double bA = dB - dA, bB = dC - dB, bC = dD - dC;
+  const double tA = fma((13.0 / 12.0) * bA, bA, 1e-6);
+  const double tB = fma((13.0 / 12.0) * bB, bB, 1e-6);
+  const double tC = fma((13.0 / 12.0) * bC, bC, 1e-6);
+  const double hd0 = 0.5 * dB, hd1 = 0.5 * dC;
+  const double A1 = fma(0.5, bA, dB), A2 = hd0 + hd1, A3 = fma(0.5, bC, -dC);
+  const double e1 = fma(A1, A1, tA), e2 = fma(A2, A2, tB), e3 = fma(A3, A3, tC);
+  const double s1 = e1 * e1, s2 = e2 * e2, s3 = e3 * e3;
+  const double P1 = 0.1 * (s2 * s3), P2 = 0.6 * (s1 * s3), P3 = 0.3 * (s1 * s2);
+  const double c1 = fma(1.0 / 3.0, bA, hd0);
+  const double c2 = fma(-1.0 / 6.0, bB, hd1);
+  const double c3 = fma(-1.0 / 6.0, bC, hd1);
+  return fma(fma(P1, c1, fma(P2, c2, P3 * c3)), rcp(P1 + P2 + P3), f0);
 }
 
 // FP32 form for the mixed-precision step.  In SI units the split energy flux

@@ -124,6 +124,35 @@ def test_weno(shim):
         assert abs(got32 - _weno_core(*v32.astype(np.float64))) <= 2e-6 * np.abs(v32).max()
 
 
+def test_weno_stencil_kinds(shim):
+    """The FP64 form (differences around f0, candidates as f0 plus a
+    correction) against the oracle's WENO5_JS formulas on smooth, rough,
+    discontinuous, constant and sign-alternating stencils over 20 decades of
+    scale, in both orientations."""
+    from oracle.flux import _weno_core
+    rng = np.random.default_rng(5)
+    x = np.arange(5.0)
+    stencils = [np.zeros(5)]
+    for _ in range(2000):
+        kind = rng.integers(5)
+        if kind == 0:
+            f = np.sin(0.3 * x + rng.uniform(0, 6)) + rng.uniform(-2, 2)
+        elif kind == 1:
+            f = rng.standard_normal(5)
+        elif kind == 2:
+            f = np.where(x < rng.integers(1, 5), rng.standard_normal(), rng.standard_normal())
+        elif kind == 3:
+            f = np.full(5, rng.standard_normal())
+        else:
+            f = (-1.0) ** x * rng.uniform(0.5, 1.5, 5)
+        stencils.append(f * 10 ** rng.uniform(-6, 14))
+    for f in stencils:
+        tol = 2e-15 * max(np.abs(f).max(), 1e-300)
+        for g in (f, f[::-1]):
+            g = np.ascontiguousarray(g)
+            assert abs(shim.host_weno(_p(g)) - _weno_core(*g)) <= tol
+
+
 def test_weno_gradient(shim):
     rng = np.random.default_rng(1)
     for _ in range(200):
