@@ -35,21 +35,28 @@ struct MixTables {
 // The range index is a runtime (per-lane) value on purpose: with a
 // compile-time index the compiler hoists both ranges' ~200 constants into
 // registers wherever the mixture is evaluated more than once.
+// The h polynomial's coefficients are the cp polynomial's divided by the
+// power they multiply (h_j = cp_j / (5 - j), j < 5: the tables are built so),
+// plus the constant term; the mixture's h_j are formed the same way from its
+// cp_j, and only the constant term is summed over the species.
 template <typename Tab>
 __host__ __device__ __forceinline__ void mix_poly(const Tab& c, int r,
                                                   const double (&Y)[JH_NS],
                                                   MixPoly& m) {
 #pragma unroll
   for (int j = 0; j < 5; ++j) m.cp[j] = 0.0;
-#pragma unroll
-  for (int j = 0; j < 6; ++j) m.h[j] = 0.0;
+  m.h[5] = 0.0;
 #pragma unroll
   for (int k = 0; k < JH_NS; ++k) {
 #pragma unroll
     for (int j = 0; j < 5; ++j) m.cp[j] += Y[k] * c.cp[r][k][j];
-#pragma unroll
-    for (int j = 0; j < 6; ++j) m.h[j] += Y[k] * c.h[r][k][j];
+    m.h[5] += Y[k] * c.h[r][k][5];
   }
+  m.h[0] = m.cp[0] * (1.0 / 5.0);
+  m.h[1] = m.cp[1] * (1.0 / 4.0);
+  m.h[2] = m.cp[2] * (1.0 / 3.0);
+  m.h[3] = m.cp[3] * (1.0 / 2.0);
+  m.h[4] = m.cp[4];
 }
 
 // cp, dcp/dT and h of the mixture at T (thermo_model.py:115-126).

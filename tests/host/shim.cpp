@@ -55,6 +55,21 @@ void host_getT_vjp(const double* mc, int n, const double* Y8, const double* T, c
   }
 }
 
+// Halley solve per cell (update_aux): Y8 (8, n), e, initial T -> gamma, T
+void host_halley(const double* mc, int n, const double* Y8, const double* e, const double* T0,
+                 double* gamma, double* T) {
+  const MechConst& c = *reinterpret_cast<const MechConst*>(mc);
+  for (int i = 0; i < n; ++i) {
+    double Y[JH_NS];
+    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = Y8[k * n + i];
+    const double R = jh::fill_Y_get_R(c, Y);
+    double t = T0[i], g;
+    jh::halley_T(c, Y, R, e[i], t, g);
+    gamma[i] = g;
+    T[i] = t;
+  }
+}
+
 double host_weno(const double* v) { return jh::weno5(v[0], v[1], v[2], v[3], v[4]); }
 float host_weno_f32(const float* v) { return jh::weno5(v[0], v[1], v[2], v[3], v[4]); }
 double host_weno_grad(const double* v, double* g) {

@@ -111,6 +111,28 @@ def test_temperature_vjp(shim, mech, mech_path):
     assert _rel(Yb, dY) < 1e-12
 
 
+def test_temperature_solve(shim, mech, mech_path):
+    """halley_T (mixture polynomials from the cp tables, per-cell stop)
+    against the oracle's get_T with the per-cell stop, from initial guesses
+    10 % off, on both sides of the 1000 K range switch."""
+    from oracle.mech import load_mechanism
+    from oracle.thermo import Thermo
+    U, aux = _cells(mech)
+    n = U.shape[1] * U.shape[2]
+    rho = U[0].ravel()
+    Y = (U[4:] / U[0]).reshape(8, -1)
+    u, v = U[1].ravel() / rho, U[2].ravel() / rho
+    e = U[3].ravel() / rho - 0.5 * (u * u + v * v)
+    T0 = aux[1].ravel() * np.random.default_rng(4).uniform(0.9, 1.1, n)
+    g, T = np.zeros(n), np.zeros(n)
+    shim.host_halley(_p(mech.const_array()), n, _p(Y), _p(e), _p(T0), _p(g), _p(T))
+    _, th, _ = load_mechanism(mech_path)
+    ref = Thermo(th).get_T(e[None], Y, T0[None], per_cell=True)
+    assert _rel(T, ref[1]) < 1e-12
+    assert _rel(g, ref[0]) < 1e-12
+    assert np.abs(T - aux[1].ravel()).max() < 1e-6 * aux[1].max()
+
+
 def test_weno(shim):
     from oracle.flux import _weno_core
     rng = np.random.default_rng(0)
