@@ -37,12 +37,12 @@ WORDS = {"stage": (27 + 39 + 39) / 3.0, "chem": 23.0, "reduce": 13.0, "bc": 0.0}
 # counter's reading for a 256 MiB device-to-device copy; summary under
 # profiles/).  Reads exceed the algorithmic 25 words: each region cell's
 # state is read for its primitives and again per variable.
-STAGE_TRAFFIC_B_PER_CELL_FP64 = 472.3
+STAGE_TRAFFIC_B_PER_CELL_FP64 = 459.8
 
 # Measured FP64 FLOP per cell per launch of the FP64 headline kernels
 # (rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes / cells, FMA = 2; summary
 # under profiles/): the bench turns them into live FP64-vector fractions.
-FP64_FLOP_PER_CELL = {"stage": 5210.0, "chem": 3326.0}
+FP64_FLOP_PER_CELL = {"stage": 4627.0, "chem": 3239.0}
 
 
 def _run(sim, U, aux, steps, warmup, torch):

@@ -32,6 +32,10 @@ __device__ __forceinline__ double bitsd(unsigned long long b) {
 
 // Wave-then-block max of non-negative doubles (NaN handling by the caller),
 // then one 64-bit atomicMax per block.  Every thread of the block must call.
+// A slot only grows while kernels fold into it, so the atomic is skipped when
+// a relaxed load already shows a value at least as large (a stale load is a
+// lower bound): blocks that finish together would otherwise queue their
+// atomics on the one address.
 template <int NT>
 __device__ __forceinline__ void block_max_atomic(double v, unsigned long long* dst,
                                                  double* red_lds) {
@@ -45,7 +49,9 @@ __device__ __forceinline__ void block_max_atomic(double v, unsigned long long* d
     double m = red_lds[0];
 #pragma unroll
     for (int i = 1; i < NT / 64; ++i) m = fmax(m, red_lds[i]);
-    atomicMax(dst, dbits(m));
+    const unsigned long long mb = dbits(m);
+    if (mb > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(dst, mb);
   }
 }
 
@@ -68,7 +74,8 @@ __device__ __forceinline__ void block_max_bits_atomic(unsigned long long v,
     unsigned long long m = red_lds[0];
 #pragma unroll
     for (int i = 1; i < NT / 64; ++i) m = red_lds[i] > m ? red_lds[i] : m;
-    atomicMax(dst, m);
+    if (m > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(dst, m);
   }
 }
 
