@@ -41,35 +41,29 @@ __host__ __device__ __forceinline__ double weno5(double fm2, double fm1, double 
   return fma(fma(P1, c1, fma(P2, c2, P3 * c3)), rcp(P1 + P2 + P3), f0);
 }
 
-// FP32 form for the mixed-precision step.  In SI units the split energy flux
-// jumps by ~1e11 across a detonation front, so (eps+IS)^2 ~ 1e44 overflows
-// FP32.  The stencil is scaled by s = max|f|: IS' = IS/s^2 and
-// eps' = eps/s^2 give the same weights (the common factor s^4 cancels).
-// eps' is held in [1e-15, 1e15] so every square stays inside FP32's range;
-// the lower clamp acts only where IS/s^2 < 1e-15, i.e. below FP32 rounding of
-// the normalised stencil, and the upper one where the weights are already the
-// linear weights.
+// FP32 form for the mixed-precision step: the FP64 form's differences and
+// candidates, with the weights normalised by the smallest e_k = eps + IS_k,
+// alpha_k = C_k (e_min / e_k)^2 (ratios <= 1, the smallest weight's ratio 1).
+// e_k itself is never squared: in SI units the split energy flux jumps by
+// ~1e11 across a detonation front, so e_k reaches ~1e23 and e_k^2 would
+// overflow FP32, while a ratio that underflows belongs to a weight that is
+// zero at FP32 precision anyway.  ~45 FP32 operations per face.
 __host__ __device__ __forceinline__ float weno5(float fm2, float fm1, float f0, float fp1,
                                                 float fp2) {
-  float s = fmaxf(fmaxf(fabsf(fm2), fabsf(fm1)), fmaxf(fmaxf(fabsf(f0), fabsf(fp1)), fabsf(fp2)));
-  const float is = rcp(fmaxf(s, 1e-20f));
-  const float eps = fminf(fmaxf(1e-6f * is * is, 1e-15f), 1e15f);
-  const float g0 = fm2 * is, g1 = fm1 * is, g2 = f0 * is, g3 = fp1 * is, g4 = fp2 * is;
-  const float a1 = g0 - 4.0f * g1 + 3.0f * g2, b1 = g0 - 2.0f * g1 + g2;
-  const float a2 = g1 - g3, b2 = g1 - 2.0f * g2 + g3;
-  const float a3 = 3.0f * g2 - 4.0f * g3 + g4, b3 = g2 - 2.0f * g3 + g4;
-  const float IS1 = 0.25f * (a1 * a1) + (13.0f / 12.0f) * (b1 * b1);
-  const float IS2 = 0.25f * (a2 * a2) + (13.0f / 12.0f) * (b2 * b2);
-  const float IS3 = 0.25f * (a3 * a3) + (13.0f / 12.0f) * (b3 * b3);
-  float d1 = eps + IS1, d2 = eps + IS2, d3 = eps + IS3;
-  const float al1 = 0.1f * rcp(d1 * d1), al2 = 0.6f * rcp(d2 * d2), al3 = 0.3f * rcp(d3 * d3);
-  const float q1 = (1.0f / 3.0f) * fm2 - (7.0f / 6.0f) * fm1 + (11.0f / 6.0f) * f0;
-  const float q2 = -(1.0f / 6.0f) * fm1 + (5.0f / 6.0f) * f0 + (1.0f / 3.0f) * fp1;
-  const float q3 = (1.0f / 3.0f) * f0 + (5.0f / 6.0f) * fp1 - (1.0f / 6.0f) * fp2;
-  // alpha_k reaches 6e29 (eps' = 1e-15): normalise before multiplying by the
-  // unscaled candidates, whose magnitude may be ~1e10
-  const float inv = rcp(al1 + al2 + al3);
-  return (al1 * inv) * q1 + (al2 * inv) * q2 + (al3 * inv) * q3;
+  const float dA = fm1 - fm2, dB = f0 - fm1, dC = fp1 - f0, dD = fp2 - fp1;
+  const float bA = dB - dA, bB = dC - dB, bC = dD - dC;
+  const float hB = 0.5f * dB, hC = 0.5f * dC;
+  const float A1 = fmaf(0.5f, bA, dB), A2 = hB + hC, A3 = fmaf(0.5f, bC, -dC);
+  const float e1 = fmaf(A1, A1, fmaf((13.0f / 12.0f) * bA, bA, 1e-6f));
+  const float e2 = fmaf(A2, A2, fmaf((13.0f / 12.0f) * bB, bB, 1e-6f));
+  const float e3 = fmaf(A3, A3, fmaf((13.0f / 12.0f) * bC, bC, 1e-6f));
+  const float em = fminf(e1, fminf(e2, e3));
+  const float r1 = em * rcp(e1), r2 = em * rcp(e2), r3 = em * rcp(e3);
+  const float a1 = 0.1f * (r1 * r1), a2 = 0.6f * (r2 * r2), a3 = 0.3f * (r3 * r3);
+  const float c1 = fmaf(1.0f / 3.0f, bA, hB);
+  const float c2 = fmaf(-1.0f / 6.0f, bB, hC);
+  const float c3 = fmaf(-1.0f / 6.0f, bC, hC);
+  return fmaf(fmaf(a1, c1, fmaf(a2, c2, a3 * c3)), rcp(a1 + a2 + a3), f0);
 }
 
 }  // namespace jh

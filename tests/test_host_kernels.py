@@ -148,9 +148,9 @@ def test_weno(shim):
 
 def test_weno_stencil_kinds(shim):
     """The FP64 form (differences around f0, candidates as f0 plus a
-    correction) against the oracle's WENO5_JS formulas on smooth, rough,
-    discontinuous, constant and sign-alternating stencils over 20 decades of
-    scale, in both orientations."""
+    correction) and the FP32 form against the oracle's WENO5_JS formulas on
+    smooth, rough, discontinuous, constant and sign-alternating stencils over
+    20 decades of scale, in both orientations."""
     from oracle.flux import _weno_core
     rng = np.random.default_rng(5)
     x = np.arange(5.0)
@@ -168,11 +168,20 @@ def test_weno_stencil_kinds(shim):
         else:
             f = (-1.0) ** x * rng.uniform(0.5, 1.5, 5)
         stencils.append(f * 10 ** rng.uniform(-6, 14))
+    F32 = ctypes.POINTER(ctypes.c_float)
     for f in stencils:
         tol = 2e-15 * max(np.abs(f).max(), 1e-300)
         for g in (f, f[::-1]):
             g = np.ascontiguousarray(g)
             assert abs(shim.host_weno(_p(g)) - _weno_core(*g)) <= tol
+            # FP32 form (mixed precision): normalised weights, no overflow up
+            # to detonation-front fluxes; checked against the FP64 formula on
+            # the FP32-rounded stencil
+            if np.abs(g).max() < 1e13:
+                g32 = g.astype(np.float32)
+                got = shim.host_weno_f32(g32.ctypes.data_as(F32))
+                ref = _weno_core(*g32.astype(np.float64))
+                assert abs(got - ref) <= 1e-6 * max(np.abs(g32).max(), 1e-30)
 
 
 def test_weno_gradient(shim):
