@@ -72,38 +72,46 @@ namespace jh {
 
 // Value and gradient of the FP64 weno5 with respect to its five arguments
 // (the reverse mode of WENO5_JS.py:17-33 as JAX differentiates it):
-//   dq/dv_j = sum_k w_k dq_k/dv_j + sum_k (q_k - q)(-2 w_k/(eps+IS_k)) dIS_k/dv_j.
+//   dq/dv_j = sum_k w_k dq_k/dv_j + sum_k kappa_k de_k/dv_j,
+//   kappa_k = -2 w_k (q_k - q) / e_k,  e_k = eps + IS_k,
+// in weno5's difference form (q_k - q = c_k - sum_m w_m c_m), with the three
+// 1/e_k from one reciprocal of e_1 e_2 e_3.  With A_k the halved first-
+// derivative terms (IS_k = A_k^2 + 13/12 b_k^2), de_k/dv = 2 A_k dA_k/dv +
+// 13/6 b_k db_k/dv, where dA_1 = (1/2, -2, 3/2, 0, 0), dA_2 = (0, -1/2, 0,
+// 1/2, 0), dA_3 = (0, 0, 3/2, -2, 1/2) and db_1 = (1, -2, 1, 0, 0), db_2 =
+// (0, 1, -2, 1, 0), db_3 = (0, 0, 1, -2, 1).
 __host__ __device__ __forceinline__ double weno5_grad(double v0, double v1, double v2, double v3,
                                                       double v4, double (&g)[5]) {
-  const double a1 = v0 - 4.0 * v1 + 3.0 * v2, b1 = v0 - 2.0 * v1 + v2;
-  const double a2 = v1 - v3, b2 = v1 - 2.0 * v2 + v3;
-  const double a3 = 3.0 * v2 - 4.0 * v3 + v4, b3 = v2 - 2.0 * v3 + v4;
-  const double e1 = 1e-6 + (0.25 * (a1 * a1) + (13.0 / 12.0) * (b1 * b1));
-  const double e2 = 1e-6 + (0.25 * (a2 * a2) + (13.0 / 12.0) * (b2 * b2));
-  const double e3 = 1e-6 + (0.25 * (a3 * a3) + (13.0 / 12.0) * (b3 * b3));
-  const double d1 = e1 * e1, d2 = e2 * e2, d3 = e3 * e3;
-  const double p1 = 0.1 * (d2 * d3), p2 = 0.6 * (d1 * d3), p3 = 0.3 * (d1 * d2);
-  const double is = rcp(p1 + p2 + p3);
-  const double w1 = p1 * is, w2 = p2 * is, w3 = p3 * is;
-  const double q1 = (1.0 / 3.0) * v0 - (7.0 / 6.0) * v1 + (11.0 / 6.0) * v2;
-  const double q2 = -(1.0 / 6.0) * v1 + (5.0 / 6.0) * v2 + (1.0 / 3.0) * v3;
-  const double q3 = (1.0 / 3.0) * v2 + (5.0 / 6.0) * v3 - (1.0 / 6.0) * v4;
-  const double q = w1 * q1 + w2 * q2 + w3 * q3;
-  // coefficient of dIS_k/dv in dq/dv
-  const double c1 = (q1 - q) * (-2.0 * w1) * rcp(e1);
-  const double c2 = (q2 - q) * (-2.0 * w2) * rcp(e2);
-  const double c3 = (q3 - q) * (-2.0 * w3) * rcp(e3);
-  // dIS_k/dv = 0.5 a_k da_k/dv + (13/6) b_k db_k/dv
-  const double A1 = 0.5 * a1, B1 = (13.0 / 6.0) * b1;
-  const double A2 = 0.5 * a2, B2 = (13.0 / 6.0) * b2;
-  const double A3 = 0.5 * a3, B3 = (13.0 / 6.0) * b3;
-  g[0] = w1 * (1.0 / 3.0) + c1 * (A1 + B1);
-  g[1] = w1 * (-7.0 / 6.0) + w2 * (-1.0 / 6.0) + c1 * (-4.0 * A1 - 2.0 * B1) + c2 * (A2 + B2);
-  g[2] = w1 * (11.0 / 6.0) + w2 * (5.0 / 6.0) + w3 * (1.0 / 3.0) + c1 * (3.0 * A1 + B1) +
-         c2 * (-2.0 * B2) + c3 * (3.0 * A3 + B3);
-  g[3] = w2 * (1.0 / 3.0) + w3 * (5.0 / 6.0) + c2 * (-A2 + B2) + c3 * (-4.0 * A3 - 2.0 * B3);
-  g[4] = w3 * (-1.0 / 6.0) + c3 * (A3 + B3);
-  return q;
+  const double dA = v1 - v0, dB = v2 - v1, dC = v3 - v2, dD = v4 - v3;
+  const double bA = dB - dA, bB = dC - dB, bC = dD - dC;
+  const double hB = 0.5 * dB, hC = 0.5 * dC;
+  const double A1 = fma(0.5, bA, dB), A2 = hB + hC, A3 = fma(0.5, bC, -dC);
+  const double e1 = fma(A1, A1, fma((13.0 / 12.0) * bA, bA, 1e-6));
+  const double e2 = fma(A2, A2, fma((13.0 / 12.0) * bB, bB, 1e-6));
+  const double e3 = fma(A3, A3, fma((13.0 / 12.0) * bC, bC, 1e-6));
+  const double E12 = e1 * e2, E13 = e1 * e3, E23 = e2 * e3;
+  const double P1 = 0.1 * (E23 * E23), P2 = 0.6 * (E13 * E13), P3 = 0.3 * (E12 * E12);
+  const double R = rcp(P1 + P2 + P3);
+  const double w1 = P1 * R, w2 = P2 * R, w3 = P3 * R;
+  const double c1 = fma(1.0 / 3.0, bA, hB);
+  const double c2 = fma(-1.0 / 6.0, bB, hC);
+  const double c3 = fma(-1.0 / 6.0, bC, hC);
+  const double cb = fma(w1, c1, fma(w2, c2, w3 * c3));
+  const double iE = rcp(E12 * e3);
+  // 2 kappa_k A_k and 13/6 kappa_k b_k
+  const double m1 = -2.0 * (w1 * (E23 * iE)) * (c1 - cb);
+  const double m2 = -2.0 * (w2 * (E13 * iE)) * (c2 - cb);
+  const double m3 = -2.0 * (w3 * (E12 * iE)) * (c3 - cb);
+  const double a1 = 2.0 * m1 * A1, a2 = 2.0 * m2 * A2, a3 = 2.0 * m3 * A3;
+  const double b1 = (13.0 / 6.0) * m1 * bA, b2 = (13.0 / 6.0) * m2 * bB;
+  const double b3 = (13.0 / 6.0) * m3 * bC;
+  g[0] = w1 * (1.0 / 3.0) + 0.5 * a1 + b1;
+  g[1] = w1 * (-7.0 / 6.0) + w2 * (-1.0 / 6.0) - 2.0 * (a1 + b1) - 0.5 * a2 + b2;
+  g[2] = w1 * (11.0 / 6.0) + w2 * (5.0 / 6.0) + w3 * (1.0 / 3.0) + 1.5 * (a1 + a3) + b1 -
+         2.0 * b2 + b3;
+  g[3] = w2 * (1.0 / 3.0) + w3 * (5.0 / 6.0) + 0.5 * a2 + b2 - 2.0 * (a3 + b3);
+  g[4] = w3 * (-1.0 / 6.0) + 0.5 * a3 + b3;
+  return v2 + cb;
 }
 
 }  // namespace jh

@@ -186,8 +186,10 @@ def test_weno_stencil_kinds(shim):
 
 def test_weno_gradient(shim):
     rng = np.random.default_rng(1)
-    for _ in range(200):
+    for it in range(300):
         v = rng.standard_normal(5) * 10 ** rng.uniform(-4, 4)
+        if it % 3 == 0:   # a jump inside the stencil
+            v = np.where(np.arange(5) < rng.integers(1, 5), v[0], v[4])
         t = torch.tensor(v, requires_grad=True)
         f = t
         IS1 = 0.25 * (f[0] - 4 * f[1] + 3 * f[2]) ** 2 + 13 / 12 * (f[0] - 2 * f[1] + f[2]) ** 2
@@ -199,10 +201,12 @@ def test_weno_gradient(shim):
         q = torch.stack([f[0] / 3 - 7 * f[1] / 6 + 11 * f[2] / 6,
                          -f[1] / 6 + 5 * f[2] / 6 + f[3] / 3,
                          f[2] / 3 + 5 * f[3] / 6 - f[4] / 6])
-        (w * q).sum().backward()
+        val = (w * q).sum()
+        val.backward()
         g = np.zeros(5)
-        shim.host_weno_grad(_p(v), _p(g))
+        got = shim.host_weno_grad(_p(v), _p(g))
         np.testing.assert_allclose(g, t.grad.numpy(), rtol=1e-9, atol=1e-12)
+        assert abs(got - float(val.detach())) <= 1e-13 * np.abs(v).max()
 
 
 def _face_states(mech, rng, n, vel):
