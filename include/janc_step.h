@@ -205,9 +205,12 @@ enum jh_kernel_kind {
   JH_KERNEL_KINDS = 4
 };
 
-/* Time every launch of jh_advance with HIP events on the context stream
- * (no counterpart in the reference, which has no profiling hooks). */
-int jh_timing_enable(jh_ctx* ctx, int on);
+/* Time launches with HIP events on the context stream (no counterpart in the
+ * reference, which has no profiling hooks).  every = 0: off; 1: every
+ * launch; k > 1: every launch outside jh_advance, and inside it the launches
+ * of every k-th step (counted from this call), which keeps the events' own
+ * cost (~5 % of a step when every launch is bracketed) out of the rest. */
+int jh_timing_enable(jh_ctx* ctx, int every);
 /* Sum of event-measured milliseconds and launch counts per jh_kernel_kind
  * since the last read (ms[4], counts[4]); synchronises the stream. */
 int jh_timing_read(jh_ctx* ctx, double* ms, int* counts);

@@ -460,10 +460,12 @@ class Simulator:
 
     def timing(self, U, enable=None):
         """Enable/disable per-launch HIP-event timing, or read (and reset)
-        {kind: (total_ms, launches)} when ``enable`` is None."""
+        {kind: (total_ms, launches)} when ``enable`` is None.  ``enable`` may
+        be a sampling period k: fused advance() calls then time the launches
+        of every k-th step only."""
         ctx = self._context(U, None)
         if enable is not None:
-            _lib.check(ctx.lib.jh_timing_enable(ctx.h, int(bool(enable))), ctx.h)
+            _lib.check(ctx.lib.jh_timing_enable(ctx.h, int(enable)), ctx.h)
             return None
         ms = (ctypes.c_double * 4)()
         cnt = (ctypes.c_int * 4)()
