@@ -207,10 +207,15 @@ void launch_stage(jh_ctx* c, const StepArgs<Real>& a, int stage, const Real* Us,
 
 int tile_rows(const jh_ctx* c) { return kTileRows[c->tile]; }
 
-// bc_kernel: one thread per boundary line in blocks of kBcThreads (small
-// blocks spread the few thousand lines over many CUs: a line of an
-// injection-inlet side runs a 20-step bisection)
-int bc_blocks(const jh_ctx* c) { return (2 * c->nx + 2 * c->ny + kBcThreads - 1) / kBcThreads; }
+// bc_kernel: one thread per boundary line in blocks of kBcThreads, then the
+// lines of injection-inlet sides, kInletTeam lanes each, for their bisection
+int bc_blocks(const jh_ctx* c) {
+  int lines = 0;
+  for (int s = 0; s < 4; ++s)
+    if (c->bc[s] == JH_BC_INJECTION_INLET) lines += s <= JH_RIGHT ? c->ny : c->nx;
+  constexpr int per_block = kBcThreads / kInletTeam;
+  return bc_line_blocks(c->nx, c->ny) + (lines + per_block - 1) / per_block;
+}
 
 // Prologue for a state handed in by the caller: zero this parity's slots and
 // reduce the interior maxima (stage-1 LF alpha, CFL).

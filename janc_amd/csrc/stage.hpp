@@ -19,6 +19,7 @@ namespace jh {
 
 constexpr int NU = 4 + (JH_NS - 1);   // conserved variables (12)
 constexpr int kBcThreads = 64;        // bc_kernel block size
+constexpr int kInletTeam = 16;        // bc_kernel lanes per injection-inlet line
 constexpr int NV = NU + 2;            // + aux (gamma, T) in ghost strips
 
 // ---------------------------------------------------------------- helpers
@@ -247,11 +248,71 @@ __device__ __forceinline__ void inj_model(double p, double R, double P0, double 
   rho = MFC / V / A3;
 }
 
+// inj_model for a team of TL lanes (a power of two, at most 64) sharing one
+// boundary line.  The bisection's midpoints are dyadic rationals (exact in
+// FP64), so the TL - 1 nodes of the next log2(TL) levels of the bisection
+// tree below the current interval are evaluated at once, one per lane (team
+// lane j = heap node j: level l = floor(log2 j), midpoint
+// M1 + (M2 - M1)(2k + 1)/2^(l+1), k = j - 2^l), and the sequential decisions
+// are replayed from the lanes' P2 >= P1 bits: the same midpoints and
+// decisions as inj_model in 20 / log2(TL) rounds instead of 20 steps.  V and
+// MFC come from the lane that evaluated the last midpoint.  Every lane of a
+// team calls it with the same arguments; other teams of the wave may not.
+template <int TL>
+__device__ __forceinline__ void inj_model_team(double p, double R, double P0, double T0,
+                                               double& V3, double& rho) {
+  static_assert(TL >= 2 && TL <= 64 && (TL & (TL - 1)) == 0, "team of 2..64 lanes");
+  constexpr int LV = TL == 2 ? 1 : TL == 4 ? 2 : TL == 8 ? 3 : TL == 16 ? 4 : TL == 32 ? 5 : 6;
+  constexpr double g = 1.29, A1 = 1.0, A3 = 5.0, A2 = A3 - A1;
+  const double C0 = sqrt(g * R * T0);
+  const double P3 = p;
+  const int lane = threadIdx.x & 63, tl = lane & (TL - 1), base = lane - tl;
+  double M1 = 0.0, M2 = 1.0, MFC = 0.0, V = 0.0;
+  for (int done = 0; done < 20;) {
+    const int lv = 20 - done < LV ? 20 - done : LV;   // levels this round
+    bool ge = false;
+    double Vl = 0.0, MFCl = 0.0;
+    if (tl >= 1 && tl < (1 << lv)) {
+      const int l = 31 - __clz(tl), k = tl - (1 << l);
+      const double M = M1 + (M2 - M1) * ((double)(2 * k + 1) * ldexp(1.0, -(l + 1)));
+      const double f = 1 + (g - 1) / 2 * (M * M);
+      const double lf = log(f);
+      const double P1 = P0 * exp(-g / (g - 1) * lf);
+      const double V1 = M * rsqrt(f) * C0;
+      MFCl = A1 * P0 / sqrt(T0) * sqrt(g / R) * M * exp(-(g + 1) / 2 / (g - 1) * lf);
+      const double A = 0.5;
+      const double B = g / (g - 1) * P3 * A3 / MFCl;
+      const double C = -g / (g - 1) * R * T0;
+      Vl = (-B + sqrt(B * B - 4 * A * C)) / (2 * A);
+      const double P2 = (MFCl * (Vl - V1) - P1 * A1 + P3 * A3) / A2;
+      ge = P2 >= P1;
+    }
+    const unsigned long long bits = __ballot(ge) >> base;
+    int node = 1, last = 1;
+    for (int l = 0; l < lv; ++l) {
+      const double M = 0.5 * (M1 + M2);
+      last = node;
+      if ((bits >> node) & 1ull) {
+        M2 = M;
+        node = 2 * node;
+      } else {
+        M1 = M;
+        node = 2 * node + 1;
+      }
+    }
+    V = __shfl(Vl, base + last);
+    MFC = __shfl(MFCl, base + last);
+    done += lv;
+  }
+  V3 = V;
+  rho = MFC / V / A3;
+}
+
 // Ghost state of an injection-inlet side from its wall cell ci
 // (bottom_boundary of the same example): injected stream where the wall
 // cell's pressure is below P0, else the wall cell with its normal velocity
 // negated; the pressure is the wall cell's in both cases.
-template <typename Real>
+template <typename Real, bool TEAM = false>
 __device__ __forceinline__ void inlet_ghost(const MechConst& c, const StepArgs<Real>& a,
                                             const Real* __restrict__ U,
                                             const Real* __restrict__ aux, int n, int ci,
@@ -282,7 +343,8 @@ __device__ __forceinline__ void inlet_ghost(const MechConst& c, const StepArgs<R
     (void)mix_gamma(c, Y, R, T_in, &h);
   } else {
     double V3, rho_m;
-    inj_model(p, Ri, P0, a.inj_T0, V3, rho_m);
+    if (TEAM) inj_model_team<kInletTeam>(p, Ri, P0, a.inj_T0, V3, rho_m);
+    else inj_model(p, Ri, P0, a.inj_T0, V3, rho_m);
     T_c = p / (Ri * rho_m);
     gam = mix_gamma(c, Yi, Ri, T_c, &h);
     rho_c = p / (Ri * T_c);
@@ -300,12 +362,35 @@ __device__ __forceinline__ void inlet_ghost(const MechConst& c, const StepArgs<R
   G[NU + 1] = T_c;
 }
 
+// LF maxima of a computed ghost state G (U, gamma, T) as the strip stores it
+// (the arithmetic of cell_prim on the stored values).
+template <typename Real>
+__device__ __forceinline__ void ghost_maxima(const MechConst& c, const double (&G)[NV],
+                                             double& mx, double& my) {
+  Real Gr[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) Gr[v] = (Real)G[v];
+  const Prim<double> q = cell_prim<double, true>(c, Gr, Gr + NU, 1);
+  mx = fmax(mx, nan_to_zero(fabs(q.u) + q.a));
+  my = fmax(my, nan_to_zero(fabs(q.v) + q.a));
+}
+
 // Ghost strips for one RK stage (boundary.py:126-132 with the BC kinds of
 // boundary_conditions_2D/*.py and the periodic pre-pad of
 // boundary_padding.py:20-25), plus the ghost cells' share of the LF maxima.
 // One thread per boundary-normal line: threads [0, ny) left, [ny, 2ny) right,
 // [2ny, 2ny+nx) bottom, [2ny+nx, 2ny+2nx) top.  blk = block index.
-template <typename Real>
+// TEAM_INLET: the lines of injection-inlet sides are left to the blocks from
+// bc_line_blocks(nx, ny) on, kInletTeam lanes per line (inj_model_team),
+// lines in side order left, right, bottom, top.
+// Only computed ghost states (injection inlet, pressure outlet, callable
+// strips) add to the maxima: a copied ghost (periodic, walls, zero gradient)
+// has |u| + a and |v| + a of an interior cell, already in them.
+__host__ __device__ __forceinline__ int bc_line_blocks(int nx, int ny) {
+  return (2 * nx + 2 * ny + kBcThreads - 1) / kBcThreads;
+}
+
+template <typename Real, bool TEAM_INLET = false>
 __device__ __forceinline__ void bc_body(const StepArgs<Real>& a, const Real* __restrict__ U,
                                         const Real* __restrict__ aux, int stage, int blk,
                                         double* red) {
@@ -320,10 +405,29 @@ __device__ __forceinline__ void bc_body(const StepArgs<Real>& a, const Real* __r
   const int tid = blk * blockDim.x + threadIdx.x;
   double mx = 0.0, my = 0.0;
   int side = -1, pos = 0;
-  if (tid < ny) { side = JH_LEFT; pos = tid; }
-  else if (tid < 2 * ny) { side = JH_RIGHT; pos = tid - ny; }
-  else if (tid < 2 * ny + nx) { side = JH_BOTTOM; pos = tid - 2 * ny; }
-  else if (tid < 2 * ny + 2 * nx) { side = JH_TOP; pos = tid - 2 * ny - nx; }
+  const int line_blocks = bc_line_blocks(nx, ny);
+  const bool team = TEAM_INLET && blk >= line_blocks;
+  if (team) {
+    int j = (blk - line_blocks) * (kBcThreads / kInletTeam) + (int)threadIdx.x / kInletTeam;
+#pragma unroll
+    for (int sd = 0; sd < 4; ++sd) {        // compile-time side index
+      if (side < 0 && a.bc[sd] == JH_BC_INJECTION_INLET) {
+        const int len = sd <= JH_RIGHT ? ny : nx;
+        if (j < len) {
+          side = sd;
+          pos = j;
+        } else {
+          j -= len;
+        }
+      }
+    }
+  } else {
+    if (tid < ny) { side = JH_LEFT; pos = tid; }
+    else if (tid < 2 * ny) { side = JH_RIGHT; pos = tid - ny; }
+    else if (tid < 2 * ny + nx) { side = JH_BOTTOM; pos = tid - 2 * ny; }
+    else if (tid < 2 * ny + 2 * nx) { side = JH_TOP; pos = tid - 2 * ny - nx; }
+    if (TEAM_INLET && side >= 0 && pick4(a.bc, side) == JH_BC_INJECTION_INLET) side = -1;
+  }
   if (side >= 0) {
     const bool xside = side <= JH_RIGHT;
     const int len = xside ? ny : nx;          // strip row length
@@ -372,14 +476,20 @@ __device__ __forceinline__ void bc_body(const StepArgs<Real>& a, const Real* __r
 #pragma unroll
           for (int v = 0; v < NV; ++v) gs[v * gstride + gi * len + pos] = (Real)G[v];
         }
+        ghost_maxima<Real>(c, G, mx, my);
       } else if (kind == JH_BC_INJECTION_INLET) {
         const bool low = (side == JH_LEFT || side == JH_BOTTOM);
         double G[NV];
-        inlet_ghost(c, a, U, aux, n, cell(low ? 0 : N - 1), xside, low, pick4(a.bc_p, side), G);
-        for (int gi = 0; gi < 3; ++gi) {
+        // team: the kInletTeam lanes of this line compute G together
+        inlet_ghost<Real, TEAM_INLET>(c, a, U, aux, n, cell(low ? 0 : N - 1), xside, low,
+                                      pick4(a.bc_p, side), G);
+        if (!team || (threadIdx.x & (kInletTeam - 1)) == 0) {
+          for (int gi = 0; gi < 3; ++gi) {
 #pragma unroll
-          for (int v = 0; v < NV; ++v) gs[v * gstride + gi * len + pos] = (Real)G[v];
+            for (int v = 0; v < NV; ++v) gs[v * gstride + gi * len + pos] = (Real)G[v];
+          }
         }
+        ghost_maxima<Real>(c, G, mx, my);
       } else {
         for (int gi = 0; gi < 3; ++gi) {
           // source cell along the normal for ghost layer gi (gi = 0 is the
@@ -402,11 +512,13 @@ __device__ __forceinline__ void bc_body(const StepArgs<Real>& a, const Real* __r
         }
       }
     }
-    for (int gi = 0; gi < 3; ++gi) {
-      const Real* gp = gs + gi * len + pos;
-      const Prim<double> q = cell_prim<double, true>(c, gp, gp + NU * gstride, gstride);
-      mx = fmax(mx, nan_to_zero(fabs(q.u) + q.a));
-      my = fmax(my, nan_to_zero(fabs(q.v) + q.a));
+    if (kind == JH_BC_USER) {
+      for (int gi = 0; gi < 3; ++gi) {
+        const Real* gp = gs + gi * len + pos;
+        const Prim<double> q = cell_prim<double, true>(c, gp, gp + NU * gstride, gstride);
+        mx = fmax(mx, nan_to_zero(fabs(q.u) + q.a));
+        my = fmax(my, nan_to_zero(fabs(q.v) + q.a));
+      }
     }
   }
   block_max_atomic<kBcThreads>(mx, &a.sc->amax[a.parity][stage][0], red);
@@ -417,14 +529,14 @@ template <typename Real>
 __global__ __launch_bounds__(kBcThreads) void bc_kernel(StepArgs<Real> a, const Real* __restrict__ U,
                                                  const Real* __restrict__ aux, int stage) {
   __shared__ double red[4];
-  bc_body<Real>(a, U, aux, stage, blockIdx.x, red);
+  bc_body<Real, true>(a, U, aux, stage, blockIdx.x, red);
 }
 
 // ------------------------------------------------------------ stage kernel
 
 constexpr int TY = 64;            // tile width along y (one wave)
 
-// Tile geometry for TX rows along x (TX = 8 or 16; 32*TX threads).  The
+// Tile geometry for TX rows along x (TX = 8, 16 or 32; 32*TX threads).  The
 // kernel's WPE template argument is the occupancy (waves per SIMD) the
 // register allocation must allow: at most 512/WPE VGPRs per lane.
 template <int TX>
