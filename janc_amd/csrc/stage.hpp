@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <type_traits>
+
 #include "chem.hpp"
 #include "janc_types.hpp"
 #include "recon.hpp"
@@ -715,6 +717,10 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
   auto& sFy = L.Fy;
   double* red = L.red;
   (void)NW;
+  // face runs for the FP32 WENO5 (the FP64 form's carried differences would
+  // push the kernel past its register budget)
+  constexpr bool kRuns = RECON == RECON_WENO5 && std::is_same<W, float>::value;
+  static_assert(!kRuns || (TX % 4 == 0 && TX <= 64), "face runs need TX a multiple of 4");
 
   const MechConst& c = *a.mc;
   if (MODE != MODE_RHS) load_mix_tables(c, L.mix);   // read after the phase barriers
@@ -833,6 +839,35 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
         if (own[s]) u0n[s] = (W)Un[cv * n + oidx[s]];
     }
     // ---- phase 2: face reconstructions ----
+    if constexpr (kRuns) {
+      // FP32 WENO5: runs of four consecutive faces (weno5_run).  The first
+      // half of the block takes the x faces: lane = column, wave = run
+      // (TX / 4 runs per column; the last also takes face row TX).  The
+      // second half the y faces: 16 runs per row, run = u % 16, row = u / 16;
+      // run 15 also takes face column 64.
+      constexpr int NXT = NT / 2;
+      if (t < NXT) {
+        const int col = t & 63, fr0 = (t >> 6) * 4;
+        weno5_run<4, false, false>(&sHxp[fr0][col], TY, &sFx[fr0][col], TY);
+        weno5_run<4, true, true>(&sHxm[fr0 + 1][col], TY, &sFx[fr0][col], TY);
+        if (fr0 + 4 == TX) {
+          sFx[TX][col] = weno5(sHxp[TX][col], sHxp[TX + 1][col], sHxp[TX + 2][col],
+                               sHxp[TX + 3][col], sHxp[TX + 4][col]) +
+                         weno5(sHxm[TX + 5][col], sHxm[TX + 4][col], sHxm[TX + 3][col],
+                               sHxm[TX + 2][col], sHxm[TX + 1][col]);
+        }
+      } else {
+        const int u = t - NXT, row = u >> 4, fc0 = (u & 15) * 4;
+        weno5_run<4, false, false>(&sHyp[row][fc0], 1, &sFy[row][fc0], 1);
+        weno5_run<4, true, true>(&sHym[row][fc0 + 1], 1, &sFy[row][fc0], 1);
+        if (fc0 + 4 == TY) {
+          sFy[row][TY] = weno5(sHyp[row][TY], sHyp[row][TY + 1], sHyp[row][TY + 2],
+                               sHyp[row][TY + 3], sHyp[row][TY + 4]) +
+                         weno5(sHym[row][TY + 5], sHym[row][TY + 4], sHym[row][TY + 3],
+                               sHym[row][TY + 2], sHym[row][TY + 1]);
+        }
+      }
+    } else
     for (int it = t; it < NXF + NYF; it += NT) {
       if (it < NXF) {
         const int fr = it >> 6, col = it & 63;

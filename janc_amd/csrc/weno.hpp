@@ -66,6 +66,53 @@ __host__ __device__ __forceinline__ float weno5(float fm2, float fm1, float f0, 
   return fmaf(fmaf(a1, c1, fmaf(a2, c2, a3 * c3)), rcp(a1 + a2 + a3), f0);
 }
 
+// The FP32 values at K consecutive faces of one line, read from memory (LDS)
+// one value per face (f(j) = p[j * s]) and written to (ACC: added into)
+// q[i * qs].  Neighbouring faces share the first and second differences and
+// the 13/12 b^2 + eps terms, which the run carries; every operation is the
+// one the FP32 weno5 performs for that face, so the values equal per-face
+// calls bit for bit.  MIRROR = false: out_i = weno5(f(i), .., f(i+4)) (the
+// face between f(i+2) and f(i+3)); MIRROR = true: out_i = weno5(f(i+4), ..,
+// f(i)) (the face between f(i+1) and f(i+2)).
+template <int K, bool MIRROR, bool ACC>
+__host__ __device__ __forceinline__ void weno5_run(const float* p, int s, float* q, int qs) {
+  // forward differences d_j = f(j+1) - f(j) and b_j = d_j - d_(j-1); the
+  // mirrored stencil's differences are their exact negations
+  float dB = p[2 * s] - p[s], dC = p[3 * s] - p[2 * s];
+  float bA = dB - (p[s] - p[0]), bB = dC - dB;
+  float tA = fmaf((13.0f / 12.0f) * bA, bA, 1e-6f), tB = fmaf((13.0f / 12.0f) * bB, bB, 1e-6f);
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const float dD = p[(i + 4) * s] - p[(i + 3) * s];
+    const float bC = dD - dC, tC = fmaf((13.0f / 12.0f) * bC, bC, 1e-6f);
+    float A1, A2, A3, c1, c2, c3, t1, t3, f0;
+    if (!MIRROR) {   // f0 = f(i+2): d0 = dB, d1 = dC; b_-1, b_0, b_+1 = bA, bB, bC
+      const float hB = 0.5f * dB, hC = 0.5f * dC;
+      A1 = fmaf(0.5f, bA, dB); A2 = hB + hC; A3 = fmaf(0.5f, bC, -dC);
+      c1 = fmaf(1.0f / 3.0f, bA, hB); c2 = fmaf(-1.0f / 6.0f, bB, hC);
+      c3 = fmaf(-1.0f / 6.0f, bC, hC);
+      t1 = tA; t3 = tC;
+      f0 = p[(i + 2) * s];
+    } else {         // f0 = f(i+2): d0 = -dC, d1 = -dB; b_-1, b_0, b_+1 = bC, bB, bA
+      const float hB = 0.5f * -dC, hC = 0.5f * -dB;
+      A1 = fmaf(0.5f, bC, -dC); A2 = hB + hC; A3 = fmaf(0.5f, bA, dB);
+      c1 = fmaf(1.0f / 3.0f, bC, hB); c2 = fmaf(-1.0f / 6.0f, bB, hC);
+      c3 = fmaf(-1.0f / 6.0f, bA, hC);
+      t1 = tC; t3 = tA;
+      f0 = p[(i + 2) * s];
+    }
+    const float e1 = fmaf(A1, A1, t1), e2 = fmaf(A2, A2, tB), e3 = fmaf(A3, A3, t3);
+    const float em = fminf(e1, fminf(e2, e3));
+    const float r1 = em * rcp(e1), r2 = em * rcp(e2), r3 = em * rcp(e3);
+    const float a1 = 0.1f * (r1 * r1), a2 = 0.6f * (r2 * r2), a3 = 0.3f * (r3 * r3);
+    const float v = fmaf(fmaf(a1, c1, fmaf(a2, c2, a3 * c3)), rcp(a1 + a2 + a3), f0);
+    if (ACC) q[i * qs] += v;
+    else q[i * qs] = v;
+    dB = dC; dC = dD;
+    bA = bB; tA = tB; bB = bC; tB = tC;
+  }
+}
+
 }  // namespace jh
 
 namespace jh {

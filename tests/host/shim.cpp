@@ -70,6 +70,12 @@ void host_halley(const double* mc, int n, const double* Y8, const double* e, con
   }
 }
 
+// K = 4 FP32 faces from one 8-value window: L[i] left-, R[i] right-biased
+void host_weno_run_f32(const float* f, float* L, float* R) {
+  jh::weno5_run<4, false, false>(f, 1, L, 1);
+  jh::weno5_run<4, true, false>(f, 1, R, 1);
+}
+
 double host_weno(const double* v) { return jh::weno5(v[0], v[1], v[2], v[3], v[4]); }
 float host_weno_f32(const float* v) { return jh::weno5(v[0], v[1], v[2], v[3], v[4]); }
 double host_weno_grad(const double* v, double* g) {

@@ -184,6 +184,26 @@ def test_weno_stencil_kinds(shim):
                 assert abs(got - ref) <= 1e-6 * max(np.abs(g32).max(), 1e-30)
 
 
+def test_weno_run_f32_equals_per_face(shim):
+    """The FP32 run form (four neighbouring faces sharing differences) gives
+    the per-face FP32 weno5 values bit for bit, both orientations."""
+    F32 = ctypes.POINTER(ctypes.c_float)
+    shim.host_weno_f32.restype = ctypes.c_float
+    rng = np.random.default_rng(8)
+    for it in range(2000):
+        f = (rng.standard_normal(8) * 10 ** rng.uniform(-6, 12)).astype(np.float32)
+        if it % 4 == 0:
+            f = np.where(np.arange(8) < rng.integers(1, 8), f[0], f[7]).astype(np.float32)
+        L, R = np.zeros(4, np.float32), np.zeros(4, np.float32)
+        shim.host_weno_run_f32(f.ctypes.data_as(F32), L.ctypes.data_as(F32),
+                               R.ctypes.data_as(F32))
+        for i in range(4):
+            w = np.ascontiguousarray(f[i:i + 5])
+            assert L[i] == shim.host_weno_f32(w.ctypes.data_as(F32))
+            w = np.ascontiguousarray(f[i:i + 5][::-1])
+            assert R[i] == shim.host_weno_f32(w.ctypes.data_as(F32))
+
+
 def test_weno_gradient(shim):
     rng = np.random.default_rng(1)
     for it in range(300):
