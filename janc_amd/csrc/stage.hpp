@@ -271,7 +271,7 @@ __device__ __forceinline__ void inj_model_team(double p, double R, double P0, do
   const int lane = threadIdx.x & 63, tl = lane & (TL - 1), base = lane - tl;
   double M1 = 0.0, M2 = 1.0, MFC = 0.0, V = 0.0;
   for (int done = 0; done < 20;) {
-    const int lv = 20 - done < LV ? 20 - done : LV;   // levels this round
+    const int lv = 20 - done < LV ? 20 - done : LV;   // levels in this pass
     bool ge = false;
     double Vl = 0.0, MFCl = 0.0;
     if (tl >= 1 && tl < (1 << lv)) {
