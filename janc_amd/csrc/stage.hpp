@@ -257,7 +257,7 @@ __device__ __forceinline__ void inj_model(double p, double R, double P0, double 
 // lane j = heap node j: level l = floor(log2 j), midpoint
 // M1 + (M2 - M1)(2k + 1)/2^(l+1), k = j - 2^l), and the sequential decisions
 // are replayed from the lanes' P2 >= P1 bits: the same midpoints and
-// decisions as inj_model in 20 / log2(TL) rounds instead of 20 steps.  V and
+// decisions as inj_model in 20 / log2(TL) passes instead of 20 steps.  V and
 // MFC come from the lane that evaluated the last midpoint.  Every lane of a
 // team calls it with the same arguments; other teams of the wave may not.
 template <int TL>
