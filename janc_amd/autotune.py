@@ -48,7 +48,7 @@ def _save_file(table):
             json.dump(table, fh, indent=1, sort_keys=True)
 
 
-def tune(sim, U, aux, steps=4, theta=None, force=False):
+def tune(sim, U, aux, steps=8, theta=None, force=False):
     """Pick the fastest stage-kernel variant for sim's context of U's shape.
 
     Returns {"variant": best, "ms_per_step": {variant: ms}, "cached": bool}.
@@ -62,17 +62,23 @@ def tune(sim, U, aux, steps=4, theta=None, force=False):
         best = table[key]["variant"]
         sim.set_tile(U, best, theta)
         return dict(table[key], cached=True)
+    # every variant is timed in each of `rounds` interleaved passes and keeps
+    # its best: one short sample per variant cannot separate variants within
+    # a few per cent of each other
     times = {}
-    for v in variants():
-        sim.set_tile(U, v["variant"], theta)
-        Ut, at = U.clone(), aux.clone()
-        sim.advance(Ut, at, 1, theta)                   # warm-up / first-launch costs
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sim.advance(Ut, at, steps, theta)
-        torch.cuda.synchronize()
-        times[v["variant"]] = (time.perf_counter() - t0) / steps * 1e3
-        del Ut, at
+    rounds = 2
+    for r in range(rounds):
+        for v in variants():
+            sim.set_tile(U, v["variant"], theta)
+            Ut, at = U.clone(), aux.clone()
+            sim.advance(Ut, at, 1, theta)               # warm-up / first-launch costs
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sim.advance(Ut, at, steps, theta)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / steps * 1e3
+            times[v["variant"]] = min(ms, times.get(v["variant"], ms))
+            del Ut, at
     best = min(times, key=times.get)
     sim.set_tile(U, best, theta)
     entry = {"variant": best, "ms_per_step": {str(k): v for k, v in times.items()}}

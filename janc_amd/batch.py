@@ -158,22 +158,25 @@ class BatchSimulator:
 
     def autotune(self, steps=5, graph=True):
         """Time every stage-kernel variant on copies of the bound states (one
-        warm-up advance, then one timed advance of ``steps``) and keep the
-        fastest.  Returns {"variant": best, "ms_per_step": {variant: ms}}."""
+        warm-up advance, then one timed advance of ``steps``) in two
+        interleaved passes and keep the fastest, each variant at its best
+        pass.  Returns {"variant": best, "ms_per_step": {variant: ms}}."""
         from .autotune import variants
         if self._states is None:
             raise RuntimeError("bind() the states first")
         Us, auxs = self._states
         times = {}
-        for v in variants():
-            self.set_tile(v["variant"])
-            self.bind([U.clone() for U in Us], [a.clone() for a in auxs])
-            self.advance(steps, graph=graph)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            self.advance(steps, graph=graph)
-            torch.cuda.synchronize()
-            times[v["variant"]] = (time.perf_counter() - t0) / steps * 1e3
+        for r in range(2):
+            for v in variants():
+                self.set_tile(v["variant"])
+                self.bind([U.clone() for U in Us], [a.clone() for a in auxs])
+                self.advance(steps, graph=graph)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                self.advance(steps, graph=graph)
+                torch.cuda.synchronize()
+                ms = (time.perf_counter() - t0) / steps * 1e3
+                times[v["variant"]] = min(ms, times.get(v["variant"], ms))
         best = min(times, key=times.get)
         self.set_tile(best)
         self.bind(Us, auxs)

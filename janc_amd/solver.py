@@ -453,7 +453,7 @@ class Simulator:
         ctx = self._context(U, theta)
         return int(ctx.lib.jh_ctx_get_tile(ctx.h))
 
-    def autotune(self, U, aux, steps=4, theta=None, force=False):
+    def autotune(self, U, aux, steps=8, theta=None, force=False):
         """Time every stage-kernel variant on a copy of (U, aux); keep the fastest."""
         from .autotune import tune
         return tune(self, U, aux, steps=steps, theta=theta, force=force)
