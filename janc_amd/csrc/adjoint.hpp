@@ -6,12 +6,15 @@
 // evenly among tied maximisers of the padded field, the clip's cotangent
 // routed to its active branch.
 //
-// Data path of the flux VJP (one RK stage): the stage input is padded with
-// its ghost cells (pad_kernel); per padded cell, H+-_bar is gathered from
-// the WENO gradients of the faces that reach it and pushed through the split
-// flux and the primitive map (flux_vjp_tile_kernel); the LF maxima's
-// cotangent goes to the argmax cells (alpha_*); ghost-cell cotangents are
-// folded back onto their source cells (unpad_kernel).
+// Data path of the flux VJP (one RK stage), over the reference's ghost-padded
+// field without storing it: a padded cell reads its source interior cell
+// through the boundary map (pad_src, prim_src).  Per padded cell, H+-_bar is
+// gathered from the WENO gradients of the faces that reach it and pushed
+// through the split flux and the primitive map (flux_vjp_tile_kernel); an
+// interior cell's cotangent is added into the output, a ghost cell's is kept
+// in a padded buffer; the LF maxima's cotangent goes to the argmax cells
+// (alpha_*); the ghost-cell cotangents are then folded back onto their source
+// cells (fold_ghosts_kernel).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -26,14 +29,14 @@ struct AdjArgs {
   int bc[4];
 };
 
-__device__ __forceinline__ int wrap_x(int i, int n) { return i < 0 ? n - 1 + i : 1 + (i - n); }
+__host__ __device__ inline int wrap_x(int i, int n) { return i < 0 ? n - 1 + i : 1 + (i - n); }
 
 // Source interior cell of padded cell (P, Q) and the sign flips of the two
 // momenta (bit 0: rho*u, bit 1: rho*v) -- boundary_padding.py:20-25 then the
 // BC kinds of boundary_conditions_2D/*.py.  Returns false for a kind without
 // a linear map (pressure outlet, user strips).
-__device__ __forceinline__ bool pad_src(const AdjArgs& a, int P, int Q, int& sx, int& sy,
-                                        int& flip) {
+__host__ __device__ inline bool pad_src(const AdjArgs& a, int P, int Q, int& sx, int& sy,
+                                    int& flip) {
   const int nx = a.nx, ny = a.ny, i = P - 3, j = Q - 3;
   const bool xin = i >= 0 && i < nx, yin = j >= 0 && j < ny;
   flip = 0;
@@ -76,73 +79,42 @@ __device__ __forceinline__ bool pad_src(const AdjArgs& a, int P, int Q, int& sx,
   return true;
 }
 
-// Ghost-padded copy of (U, aux): (12, nx+6, ny+6) and (2, nx+6, ny+6), and
-// the LF maxima of the padded field (into amx, zeroed beforehand) with the
-// arithmetic of prim_at, which the tie test repeats.
-__global__ __launch_bounds__(256) void pad_kernel(AdjArgs a, const double* __restrict__ U,
-                                                  const double* __restrict__ aux,
-                                                  double* __restrict__ Up,
-                                                  double* __restrict__ ap,
-                                                  unsigned long long* amx);
-
 struct PrimD {
   double rho, ir, u, v, p, R, T, gamma;
   double Y[JH_NS];
 };
 
-__device__ __forceinline__ PrimD prim_at(const MechConst& c, const double* __restrict__ Up,
-                                         const double* __restrict__ ap, int np, int t) {
+// Primitives of a padded cell, read from its source interior cell s with the
+// momenta's signs flipped per flip (pad_src).  The flips are applied to the
+// quotients: -(m * ir) equals (-m) * ir exactly, so these are the values the
+// reference computes on its ghost-padded copy.
+__device__ __forceinline__ PrimD prim_src(const MechConst& c, const double* __restrict__ U,
+                                          const double* __restrict__ aux, int n, int s,
+                                          int flip) {
   PrimD q;
-  q.rho = Up[t];
+  q.rho = U[s];
   q.ir = rcp(q.rho);
-  q.u = Up[np + t] * q.ir;
-  q.v = Up[2 * np + t] * q.ir;
+  q.u = U[n + s] * q.ir;
+  q.v = U[2 * n + s] * q.ir;
+  if (flip & 1) q.u = -q.u;
+  if (flip & 2) q.v = -q.v;
 #pragma unroll
-  for (int k = 0; k < JH_NS - 1; ++k) q.Y[k] = Up[(4 + k) * np + t] * q.ir;
+  for (int k = 0; k < JH_NS - 1; ++k) q.Y[k] = U[(4 + k) * n + s] * q.ir;
   q.R = fill_Y_get_R(c, q.Y);
-  q.gamma = ap[t];
-  q.T = ap[np + t];
+  q.gamma = aux[s];
+  q.T = aux[n + s];
   q.p = q.rho * q.R * q.T;
   return q;
 }
 
-__global__ __launch_bounds__(256) void pad_kernel(AdjArgs a, const double* __restrict__ U,
-                                                  const double* __restrict__ aux,
-                                                  double* __restrict__ Up,
-                                                  double* __restrict__ ap,
-                                                  unsigned long long* amx) {
-  __shared__ double red[4];
-  const MechConst& c = *a.mc;
-  const int PX = a.nx + 6, PY = a.ny + 6, np = PX * PY, n = a.nx * a.ny;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  double mx = 0.0, my = 0.0;
-  int sx, sy, flip;
-  if (t < np && pad_src(a, t / PY, t % PY, sx, sy, flip)) {
-    const int s = sx * a.ny + sy;
-    double val[NU];
-#pragma unroll
-    for (int v = 0; v < NU; ++v) {
-      val[v] = U[v * n + s];
-      if ((v == 1 && (flip & 1)) || (v == 2 && (flip & 2))) val[v] = -val[v];
-      Up[v * np + t] = val[v];
-    }
-    const double g = aux[s], T = aux[n + s];
-    ap[t] = g;
-    ap[np + t] = T;
-    // prim_at's arithmetic on the values just stored
-    const double rho = val[0], ir = rcp(rho);
-    const double u = val[1] * ir, v = val[2] * ir;
-    double Y[JH_NS];
-#pragma unroll
-    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = val[4 + k] * ir;
-    const double R = fill_Y_get_R(c, Y);
-    const double p = rho * R * T;
-    const double snd = sqrt(g * p * ir);
-    mx = nan_to_zero(fabs(u) + snd);
-    my = nan_to_zero(fabs(v) + snd);
-  }
-  block_max_atomic<256>(mx, &amx[0], red);
-  block_max_atomic<256>(my, &amx[1], red);
+// Value of variable v at a padded cell whose source cell holds val.
+__device__ __forceinline__ double flip_var(int v, int flip, double val) {
+  return (v == 1 && (flip & 1)) || (v == 2 && (flip & 2)) ? -val : val;
+}
+
+// The padded cell (P, Q) is an interior cell (its own source, no flips).
+__host__ __device__ inline bool pad_interior(const AdjArgs& a, int P, int Q) {
+  return P >= 3 && P < a.nx + 3 && Q >= 3 && Q < a.ny + 3;
 }
 
 // Physical flux of variable cv in direction dir at a cell (LF.py:12).
@@ -222,7 +194,7 @@ struct AdjLds {
   // [side][k][m][TY] (k < TXA+4), y faces [side][row][m][TY+4]
   double Cx[2 * (TXA + 4) * 5 * TY];
   double Cy[2 * TXA * 5 * (TY + 4)];
-  double red[2][TXA];
+  double red[4][TXA];
 };
 
 struct PrimL {   // the primitives the physical flux needs
@@ -242,9 +214,10 @@ __device__ __forceinline__ double phys_flux_l(int cv, int dir, const PrimL& q, d
 
 template <int TXA>
 __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_eu(2))) void flux_vjp_tile_kernel(
-    AdjArgs a, const double* __restrict__ Up, const double* __restrict__ ap,
+    AdjArgs a, const double* __restrict__ U, const double* __restrict__ aux,
     const double* __restrict__ rbar, const unsigned long long* alpha_bits,
-    double* __restrict__ Ubp, double* __restrict__ abp, double* __restrict__ part) {
+    double* __restrict__ Ut, double* __restrict__ At, double* __restrict__ Ubp,
+    double* __restrict__ abp, double* __restrict__ part) {
   using G = AdjTile<TXA>;
   __shared__ AdjLds<TXA> L;
   const MechConst& c = *a.mc;
@@ -282,25 +255,28 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
     py[s] = Y0 - 4 + (k < 4 ? k : TY + k);
   }
   PrimL q[NS_];
+  int src[NS_], fl[NS_];   // source interior cell and momentum flips (pad_src)
 #pragma unroll
   for (int s = 0; s < NS_; ++s) {
     in[s] = px[s] >= 0 && px[s] < PX && py[s] >= 0 && py[s] < PY &&
             (s < NS_ - 1 || t < 8 * TXA);
+    int sx = 0, sy = 0, f = 0;
+    // the adjoint admits linear boundary kinds only, so pad_src holds here
+    if (in[s]) in[s] = pad_src(a, px[s], py[s], sx, sy, f);
+    src[s] = sx * ny + sy;
+    fl[s] = f;
     if (in[s]) {
-      const int pt = px[s] * PY + py[s];
-      const PrimD d = prim_at(c, Up, ap, np, pt);
+      const PrimD d = prim_src(c, U, aux, n, src[s], f);
       q[s] = PrimL{d.rho, d.u, d.v, d.p};
     } else {
       q[s] = PrimL{1.0, 0.0, 0.0, 0.0};
     }
   }
-  // own cell: which face families reach it, and its source cell
+  // own cell: which face families reach it
   const int P = px[0], Q = py[0], i = P - 3, j = Q - 3;
   const bool own = in[0];
   const bool xrow = own && j >= 0 && j < ny, ycol = own && i >= 0 && i < nx;
-  int sxx = 0, syy = 0, flip = 0;
-  const bool linear = own && pad_src(a, P, Q, sxx, syy, flip);
-  const bool work = linear && (xrow || ycol);
+  const bool work = own && (xrow || ycol);
   double Ub[NU];
 #pragma unroll
   for (int v = 0; v < NU; ++v) Ub[v] = 0.0;
@@ -316,7 +292,7 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
 #pragma unroll
     for (int s = 0; s < NS_; ++s) {
       double ucv = 0.0;
-      if (in[s]) ucv = Up[(size_t)cv * np + px[s] * PY + py[s]];
+      if (in[s]) ucv = flip_var(cv, fl[s], U[(size_t)cv * n + src[s]]);
       if (s == 0) ucv_own = ucv;
       if (xr[s] >= 0) {
         double hp = 0.0, hm = 0.0;
@@ -467,75 +443,110 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
     // above), and its face loops' Cx/Cy writes come after its own first
     // barrier, which every thread reaches after these reads.
   }
+  // the own cell's |u| + a and |v| + a: the padded field's LF maxima are
+  // reduced from these (per block here, then alpha_reduce_kernel), with the
+  // arithmetic that alpha_count_kernel's tie test repeats
+  double mxo = 0.0, myo = 0.0;
   if (own) {
-    const int pt = P * PY + Q;
     double Tb = 0.0, gb = 0.0;
-    if (work) {
-      const PrimD qf = prim_at(c, Up, ap, np, pt);
-      prim_to_U_vjp(c, qf, rhob, ub, vb, pb, 0.0, Yb, Ub, Tb, gb);
-    }
+    const PrimD qf = prim_src(c, U, aux, n, src[0], fl[0]);
+    const double snd = sqrt(qf.gamma * qf.p * qf.ir);
+    mxo = nan_to_zero(fabs(qf.u) + snd);
+    myo = nan_to_zero(fabs(qf.v) + snd);
+    if (work) prim_to_U_vjp(c, qf, rhob, ub, vb, pb, 0.0, Yb, Ub, Tb, gb);
+    if (pad_interior(a, P, Q)) {   // its own source: straight into the output
+      const int s = src[0];
 #pragma unroll
-    for (int v = 0; v < NU; ++v) Ubp[(size_t)v * np + pt] = Ub[v];
-    abp[pt] = gb;
-    abp[np + pt] = Tb;
+      for (int v = 0; v < NU; ++v) Ut[(size_t)v * n + s] += Ub[v];
+      At[s] += gb;
+      At[n + s] += Tb;
+    } else {                       // a ghost: kept for fold_ghosts_kernel
+      const int pt = P * PY + Q;
+#pragma unroll
+      for (int v = 0; v < NU; ++v) Ubp[(size_t)v * np + pt] = Ub[v];
+      abp[pt] = gb;
+      abp[np + pt] = Tb;
+    }
   }
-  // fixed-order block sums of the alpha cotangent
+  // fixed-order block sums of the alpha cotangent, and the block maxima
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     sx_part += __shfl_xor(sx_part, off, 64);
     sy_part += __shfl_xor(sy_part, off, 64);
+    mxo = fmax(mxo, __shfl_xor(mxo, off, 64));
+    myo = fmax(myo, __shfl_xor(myo, off, 64));
   }
   if (ly == 0) {
     L.red[0][w] = sx_part;
     L.red[1][w] = sy_part;
+    L.red[2][w] = mxo;
+    L.red[3][w] = myo;
   }
   __syncthreads();
   if (t == 0) {
-    double sx = 0.0, sy = 0.0;
+    double sx = 0.0, sy = 0.0, mx = 0.0, my = 0.0;
 #pragma unroll
     for (int k = 0; k < TXA; ++k) {
       sx += L.red[0][k];
       sy += L.red[1][k];
+      mx = fmax(mx, L.red[2][k]);
+      my = fmax(my, L.red[3][k]);
     }
-    part[2 * blockIdx.x] = sx;
-    part[2 * blockIdx.x + 1] = sy;
+    part[4 * blockIdx.x] = sx;
+    part[4 * blockIdx.x + 1] = sy;
+    part[4 * blockIdx.x + 2] = mx;
+    part[4 * blockIdx.x + 3] = my;
   }
 }
 
-// alpha_bar = sum of the per-block partials, in a fixed order (one block):
-// scal[0..1].
+// alpha_bar = sum of the per-block partials, in a fixed order, and the LF
+// maxima of the padded field = max of the block maxima (one block):
+// scal[0..1] and amx[0..1] (bit patterns); clears the tie counts cnt[0..1]
+// for alpha_count_kernel.
 __global__ __launch_bounds__(256) void alpha_reduce_kernel(const double* __restrict__ part,
-                                                           int nblk, double* scal) {
-  __shared__ double red[2][256];
-  double sx = 0.0, sy = 0.0;
+                                                           int nblk, double* scal,
+                                                           unsigned long long* amx,
+                                                           unsigned int* cnt) {
+  __shared__ double red[4][256];
+  double sx = 0.0, sy = 0.0, mx = 0.0, my = 0.0;
   for (int b = threadIdx.x; b < nblk; b += 256) {
-    sx += part[2 * b];
-    sy += part[2 * b + 1];
+    sx += part[4 * b];
+    sy += part[4 * b + 1];
+    mx = fmax(mx, part[4 * b + 2]);
+    my = fmax(my, part[4 * b + 3]);
   }
   red[0][threadIdx.x] = sx;
   red[1][threadIdx.x] = sy;
+  red[2][threadIdx.x] = mx;
+  red[3][threadIdx.x] = my;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) {
       red[0][threadIdx.x] += red[0][threadIdx.x + s];
       red[1][threadIdx.x] += red[1][threadIdx.x + s];
+      red[2][threadIdx.x] = fmax(red[2][threadIdx.x], red[2][threadIdx.x + s]);
+      red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + s]);
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     scal[0] = red[0][0];
     scal[1] = red[1][0];
+    amx[0] = dbits(red[2][0]);
+    amx[1] = dbits(red[3][0]);
+    cnt[0] = 0u;
+    cnt[1] = 0u;
   }
 }
 
-// Exact tie test against the maxima pad_kernel reduced from the same
+// Exact tie test against the maxima alpha_reduce_kernel reduced from the same
 // arithmetic: JAX's max VJP splits the cotangent among the elements equal to
 // the maximum.  alpha_count_kernel counts the padded cells attaining each LF
 // maximum (cnt[0..1]); alpha_apply_kernel hands each its share of alpha_bar.
 __device__ __forceinline__ bool is_max(double val, double alpha) { return val == alpha; }
 
-__global__ __launch_bounds__(256) void alpha_count_kernel(AdjArgs a, const double* __restrict__ Up,
-                                                          const double* __restrict__ ap,
+__global__ __launch_bounds__(256) void alpha_count_kernel(AdjArgs a, const double* __restrict__ U,
+                                                          const double* __restrict__ aux,
                                                           const unsigned long long* alpha_bits,
                                                           unsigned int* cnt) {
   const MechConst& c = *a.mc;
@@ -545,17 +556,19 @@ __global__ __launch_bounds__(256) void alpha_count_kernel(AdjArgs a, const doubl
   int sx, sy, flip;
   const int P = t / PY, Q = t - P * PY;
   if (!pad_src(a, P, Q, sx, sy, flip)) return;
-  const PrimD q = prim_at(c, Up, ap, np, t);
+  const PrimD q = prim_src(c, U, aux, a.nx * a.ny, sx * a.ny + sy, flip);
   const double snd = sqrt(q.gamma * q.p * q.ir);
   if (is_max(fabs(q.u) + snd, bitsd(alpha_bits[0]))) atomicAdd(&cnt[0], 1u);
   if (is_max(fabs(q.v) + snd, bitsd(alpha_bits[1]))) atomicAdd(&cnt[1], 1u);
 }
 
-__global__ __launch_bounds__(256) void alpha_apply_kernel(AdjArgs a, const double* __restrict__ Up,
-                                                          const double* __restrict__ ap,
+__global__ __launch_bounds__(256) void alpha_apply_kernel(AdjArgs a, const double* __restrict__ U,
+                                                          const double* __restrict__ aux,
                                                           const unsigned long long* alpha_bits,
                                                           const double* scal,
                                                           const unsigned int* cnt,
+                                                          double* __restrict__ Ut,
+                                                          double* __restrict__ At,
                                                           double* __restrict__ Ubp,
                                                           double* __restrict__ abp) {
   const MechConst& c = *a.mc;
@@ -565,7 +578,7 @@ __global__ __launch_bounds__(256) void alpha_apply_kernel(AdjArgs a, const doubl
   int sx, sy, flip;
   const int P = t / PY, Q = t - P * PY;
   if (!pad_src(a, P, Q, sx, sy, flip)) return;
-  const PrimD q = prim_at(c, Up, ap, np, t);
+  const PrimD q = prim_src(c, U, aux, a.nx * a.ny, sx * a.ny + sy, flip);
   const double snd = sqrt(q.gamma * q.p * q.ir);
   const bool mx = cnt[0] > 0 && is_max(fabs(q.u) + snd, bitsd(alpha_bits[0]));
   const bool my = cnt[1] > 0 && is_max(fabs(q.v) + snd, bitsd(alpha_bits[1]));
@@ -582,60 +595,50 @@ __global__ __launch_bounds__(256) void alpha_apply_kernel(AdjArgs a, const doubl
   for (int k = 0; k < JH_NS - 1; ++k) Yb[k] = 0.0;
   double Tb = 0.0, gb = 0.0;
   prim_to_U_vjp(c, q, 0.0, wx * su, wy * sv, 0.0, wx + wy, Yb, Ub, Tb, gb);
+  if (pad_interior(a, P, Q)) {     // as in flux_vjp_tile_kernel
+    const int n = a.nx * a.ny, s = sx * a.ny + sy;
 #pragma unroll
-  for (int v = 0; v < NU; ++v) Ubp[v * np + t] += Ub[v];
-  abp[t] += gb;
-  abp[np + t] += Tb;
+    for (int v = 0; v < NU; ++v) Ut[v * n + s] += Ub[v];
+    At[s] += gb;
+    At[n + s] += Tb;
+  } else {
+#pragma unroll
+    for (int v = 0; v < NU; ++v) Ubp[v * np + t] += Ub[v];
+    abp[t] += gb;
+    abp[np + t] += Tb;
+  }
 }
 
-// Fold padded cotangents back onto the interior cells they were copied from
-// (transpose of pad_kernel), added into Ub / ab.
-__global__ __launch_bounds__(256) void unpad_kernel(AdjArgs a, const double* __restrict__ Ubp,
-                                                    const double* __restrict__ abp,
-                                                    double* __restrict__ Ub,
-                                                    double* __restrict__ ab) {
-  const int nx = a.nx, ny = a.ny, n = nx * ny, PY = ny + 6, np = (nx + 6) * PY;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int x = t / ny, y = t - x * ny;
-  int Pc[7], Qc[7], npc = 0, nqc = 0;
-  Pc[npc++] = x + 3;
-  if (x <= 3 || x >= nx - 4) {
-    for (int g = 0; g < 3; ++g) {
-      Pc[npc++] = g;
-      Pc[npc++] = nx + 3 + g;
-    }
-  }
-  Qc[nqc++] = y + 3;
-  if (y <= 3 || y >= ny - 4) {
-    for (int g = 0; g < 3; ++g) {
-      Qc[nqc++] = g;
-      Qc[nqc++] = ny + 3 + g;
-    }
-  }
+// Fold the ghost cells' cotangents back onto the interior cells they were
+// copied from (the transpose of pad_src on the ghosts), added into Ut / At.
+// Target k is interior cell tgt[k]; its ghost copies are ent[beg[k] ..
+// beg[k+1]), each the padded position * 4 + the momentum flips.
+__global__ __launch_bounds__(256) void fold_ghosts_kernel(AdjArgs a, const int* __restrict__ tgt,
+                                                          const int* __restrict__ beg,
+                                                          const int* __restrict__ ent, int m,
+                                                          const double* __restrict__ Ubp,
+                                                          const double* __restrict__ abp,
+                                                          double* __restrict__ Ut,
+                                                          double* __restrict__ At) {
+  const int n = a.nx * a.ny, np = (a.nx + 6) * (a.ny + 6);
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
   double acc[NU + 2];
 #pragma unroll
   for (int v = 0; v < NU + 2; ++v) acc[v] = 0.0;
-  for (int ip = 0; ip < npc; ++ip) {
-    for (int iq = 0; iq < nqc; ++iq) {
-      int sx, sy, flip;
-      if (!pad_src(a, Pc[ip], Qc[iq], sx, sy, flip)) continue;
-      if (sx != x || sy != y) continue;
-      const int pt = Pc[ip] * PY + Qc[iq];
+  const int e1 = beg[k + 1];
+  for (int e = beg[k]; e < e1; ++e) {
+    const int pt = ent[e] >> 2, flip = ent[e] & 3;
 #pragma unroll
-      for (int v = 0; v < NU; ++v) {
-        double val = Ubp[v * np + pt];
-        if ((v == 1 && (flip & 1)) || (v == 2 && (flip & 2))) val = -val;
-        acc[v] += val;
-      }
-      acc[NU] += abp[pt];
-      acc[NU + 1] += abp[np + pt];
-    }
+    for (int v = 0; v < NU; ++v) acc[v] += flip_var(v, flip, Ubp[(size_t)v * np + pt]);
+    acc[NU] += abp[pt];
+    acc[NU + 1] += abp[np + pt];
   }
+  const int t = tgt[k];
 #pragma unroll
-  for (int v = 0; v < NU; ++v) Ub[v * n + t] += acc[v];
-  ab[t] += acc[NU];
-  ab[n + t] += acc[NU + 1];
+  for (int v = 0; v < NU; ++v) Ut[(size_t)v * n + t] += acc[v];
+  At[t] += acc[NU];
+  At[n + t] += acc[NU + 1];
 }
 
 // Cotangent of (e, Y) -> U: e = U3/rho - (u^2+v^2)/2, Y = U_k/rho.
@@ -659,14 +662,12 @@ __device__ __forceinline__ void eY_to_U(const double (&Uc)[NU], double eb,
 }
 
 // Ub += VJP of aux = get_T(U) at the converged aux (gamma_bar, T_bar = ab).
-__global__ __launch_bounds__(256) void getT_vjp_kernel(AdjArgs a, const double* __restrict__ U,
-                                                       const double* __restrict__ aux,
-                                                       const double* __restrict__ ab,
-                                                       double* __restrict__ Ub) {
-  const MechConst& c = *a.mc;
-  const int n = a.nx * a.ny;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+// The U cotangent that (gamma_bar, T_bar) = (ab[t], ab[n + t]) of cell t
+// sends through get_T's VJP and (e, Y)(U).
+__device__ __forceinline__ void getT_vjp_cell(const MechConst& c, const double* __restrict__ U,
+                                              const double* __restrict__ aux,
+                                              const double* __restrict__ ab, int n, int t,
+                                              double (&Ubc)[NU]) {
   double Uc[NU];
 #pragma unroll
   for (int v = 0; v < NU; ++v) Uc[v] = U[v * n + t];
@@ -677,10 +678,20 @@ __global__ __launch_bounds__(256) void getT_vjp_kernel(AdjArgs a, const double* 
   const double R = fill_Y_get_R(c, Y);
   double eb, Yb[JH_NS - 1];
   getT_vjp(c, Y, R, aux[n + t], ab[t], ab[n + t], eb, Yb);
-  double Ubc[NU];
 #pragma unroll
   for (int v = 0; v < NU; ++v) Ubc[v] = 0.0;
   eY_to_U(Uc, eb, Yb, Ubc);
+}
+
+__global__ __launch_bounds__(256) void getT_vjp_kernel(AdjArgs a, const double* __restrict__ U,
+                                                       const double* __restrict__ aux,
+                                                       const double* __restrict__ ab,
+                                                       double* __restrict__ Ub) {
+  const int n = a.nx * a.ny;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double Ubc[NU];
+  getT_vjp_cell(*a.mc, U, aux, ab, n, t, Ubc);
 #pragma unroll
   for (int v = 0; v < NU; ++v) Ub[v * n + t] += Ubc[v];
 }
@@ -714,25 +725,38 @@ __global__ __launch_bounds__(256) void chem_vjp_kernel(AdjArgs a, const double* 
   ab3[n + t] = Tb;
 }
 
-// Reverse of one Shu-Osher combine U' = ... + cr (U_s + r(U_s)): with
-// x = Ub (the cotangent of U'), Uacc = aacc*x + bacc*Uacc (bacc = 0
-// overwrites) and, when cr != 0, rb = Ub = cr*x; ab (2 planes) is zeroed.
-__global__ __launch_bounds__(256) void rk_split_kernel(int mU, int mA, double aacc, double bacc,
-                                                       double cr, double* __restrict__ Ub,
-                                                       double* __restrict__ Uacc,
-                                                       double* __restrict__ rb,
-                                                       double* __restrict__ ab) {
+// getT_vjp_kernel followed by the reverse of one Shu-Osher combine
+// U' = ... + cr (U_s + r(U_s)), in one pass over the cells: with x = Ub +
+// get_T's VJP of ab (the value getT_vjp_kernel would store), Uacc =
+// aacc*x + bacc*Uacc (bacc = 0 overwrites), Ub = x (cr = 0) or
+// rb = Ub = cr*x, and ab zeroed.
+__global__ __launch_bounds__(256) void getT_split_kernel(AdjArgs a, const double* __restrict__ U,
+                                                         const double* __restrict__ aux,
+                                                         double aacc, double bacc, double cr,
+                                                         double* __restrict__ Ub,
+                                                         double* __restrict__ Uacc,
+                                                         double* __restrict__ rb,
+                                                         double* __restrict__ ab) {
+  const int n = a.nx * a.ny;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < mU) {
-    const double x = Ub[t];
-    Uacc[t] = bacc == 0.0 ? aacc * x : aacc * x + bacc * Uacc[t];
+  if (t >= n) return;
+  double Ubc[NU];
+  getT_vjp_cell(*a.mc, U, aux, ab, n, t, Ubc);
+#pragma unroll
+  for (int v = 0; v < NU; ++v) {
+    const size_t i = (size_t)v * n + t;
+    const double x = Ub[i] + Ubc[v];
+    Uacc[i] = bacc == 0.0 ? aacc * x : aacc * x + bacc * Uacc[i];
     if (cr != 0.0) {
       const double r = cr * x;
-      rb[t] = r;
-      Ub[t] = r;
+      rb[i] = r;
+      Ub[i] = r;
+    } else {
+      Ub[i] = x;
     }
   }
-  if (t < mA) ab[t] = 0.0;
+  ab[t] = 0.0;
+  ab[n + t] = 0.0;
 }
 
 // y = a*x + b*y over m doubles (b = 0 overwrites).

@@ -48,6 +48,9 @@ struct jh_ctx {
   // adjoint workspace (allocated on the first jh_step_vjp call)
   double* adj = nullptr;
   size_t adj_len = 0;
+  // its ghost-fold lists were built for these boundary kinds (-1: not built)
+  int adj_fold_bc[4] = {-1, -1, -1, -1};
+  int adj_fold_m = 0;              // interior cells with ghost copies
 };
 
 namespace {
@@ -320,10 +323,11 @@ int take_user_strips(jh_ctx* c) {
 struct AdjWork {
   double *U[5], *A[5];          // forward states of the step: 0 = input, 1-3 RK, 4 = output
   double *Ub, *Uacc, *ab, *rb;  // cotangents
-  double *Up, *ap, *Ubp, *abp;  // padded
+  double *Ubp, *abp;            // ghost cells' cotangents, on the padded grid
   double *part, *scal;
   unsigned int* cnt;
   unsigned long long* amx;      // recomputed LF maxima (bit patterns)
+  int* fold;                    // ghost-fold lists: tgt[m], beg[m + 1], ent[np - n]
   int npad, nblk;
 };
 
@@ -331,12 +335,14 @@ int adj_work(jh_ctx* c, const double* U0, const double* A0, AdjWork& w) {
   const size_t n = (size_t)c->nx * c->ny;
   const size_t np = (size_t)(c->nx + 6) * (c->ny + 6);
   const int nblk = adj_tiles(c);
-  const size_t len = 4 * (NU + 2) * n + (3 * NU + 2) * n + 2 * (NU + 2) * np +
-                     2 * (size_t)nblk + 2 + 2 + 2;
+  const size_t ng = np - n;      // ghost cells
+  const size_t len = 4 * (NU + 2) * n + (3 * NU + 2) * n + (NU + 2) * np +
+                     4 * (size_t)nblk + 2 + 2 + 2 + (3 * ng + 2) / 2;
   if (c->adj_len < len) {
     if (c->adj) (void)hipFree(c->adj);
     c->adj = nullptr;
     c->adj_len = 0;
+    c->adj_fold_bc[0] = -1;
     HIP_TRY(c, hipMalloc(&c->adj, len * sizeof(double)));
     c->adj_len = len;
   }
@@ -351,15 +357,15 @@ int adj_work(jh_ctx* c, const double* U0, const double* A0, AdjWork& w) {
   w.Uacc = p; p += NU * n;
   w.rb = p; p += NU * n;
   w.ab = p; p += 2 * n;
-  w.Up = p; p += NU * np;
-  w.ap = p; p += 2 * np;
   w.Ubp = p; p += NU * np;
   w.abp = p; p += 2 * np;
-  w.part = p; p += 2 * nblk;
+  w.part = p; p += 4 * nblk;
   w.scal = p; p += 2;
   w.cnt = reinterpret_cast<unsigned int*>(p);
   p += 2;
   w.amx = reinterpret_cast<unsigned long long*>(p);
+  p += 2;
+  w.fold = reinterpret_cast<int*>(p);
   w.npad = (int)np;
   w.nblk = nblk;
   return JH_OK;
@@ -377,21 +383,66 @@ AdjArgs adj_args(const jh_ctx* c, double dt) {
   return a;
 }
 
+// Ghost-fold lists of fold_ghosts_kernel for the context's grid and
+// boundary kinds: the ghost cells grouped by source cell, in padded order
+// within a group.  Built on the host and copied once per set of kinds.
+int adj_fold(jh_ctx* c, const AdjArgs& a, AdjWork& w) {
+  if (std::equal(c->bc, c->bc + 4, c->adj_fold_bc)) return JH_OK;
+  const int nx = c->nx, ny = c->ny, PX = nx + 6, PY = ny + 6;
+  if ((long long)PX * PY > (1LL << 29))   // entries hold position * 4 in an int
+    return fail(c, JH_ERR_ARG, "grid too large for the adjoint");
+  std::vector<std::pair<int, int>> g;   // (source cell, padded position * 4 + flips)
+  g.reserve((size_t)PX * PY - (size_t)nx * ny);
+  for (int P = 0; P < PX; ++P)
+    for (int Q = 0; Q < PY; ++Q) {
+      if (pad_interior(a, P, Q)) continue;
+      int sx, sy, flip;
+      if (!pad_src(a, P, Q, sx, sy, flip)) continue;
+      g.emplace_back(sx * ny + sy, (P * PY + Q) * 4 + flip);
+    }
+  std::stable_sort(g.begin(), g.end(),
+                   [](const std::pair<int, int>& l, const std::pair<int, int>& r) {
+                     return l.first < r.first;
+                   });
+  std::vector<int> tgt, beg, ent;
+  for (size_t e = 0; e < g.size(); ++e) {
+    if (e == 0 || g[e].first != g[e - 1].first) {
+      tgt.push_back(g[e].first);
+      beg.push_back((int)e);
+    }
+    ent.push_back(g[e].second);
+  }
+  beg.push_back((int)g.size());
+  const int m = (int)tgt.size();
+  std::vector<int> buf;
+  buf.reserve(tgt.size() + beg.size() + ent.size());
+  buf.insert(buf.end(), tgt.begin(), tgt.end());
+  buf.insert(buf.end(), beg.begin(), beg.end());
+  buf.insert(buf.end(), ent.begin(), ent.end());
+  HIP_TRY(c, hipMemcpyAsync(w.fold, buf.data(), buf.size() * sizeof(int), hipMemcpyHostToDevice,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));   // buf is freed on return
+  std::copy(c->bc, c->bc + 4, c->adj_fold_bc);
+  c->adj_fold_m = m;
+  return JH_OK;
+}
+
 // Flux VJP of one RK stage with input (Us, As) and rhs cotangent rb; the
 // results are added into (Ut, At).
 void flux_vjp(jh_ctx* c, const AdjArgs& a, AdjWork& w, const double* Us, const double* As,
               const double* rb, const unsigned long long* alpha, double* Ut, double* At) {
-  const size_t n = (size_t)c->nx * c->ny;
   hipStream_t st = c->stream;
-  zero_u32_kernel<<<1, 64, 0, st>>>(w.cnt, 8);   // counts (+ pad) and the two maxima
-  pad_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, Us, As, w.Up, w.ap, w.amx);
   flux_vjp_tile_kernel<kAdjRows><<<w.nblk, AdjTile<kAdjRows>::NT, 0, st>>>(
-      a, w.Up, w.ap, rb, alpha, w.Ubp, w.abp, w.part);
-  alpha_reduce_kernel<<<1, 256, 0, st>>>(w.part, w.nblk, w.scal);
-  alpha_count_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, w.Up, w.ap, w.amx, w.cnt);
-  alpha_apply_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, w.Up, w.ap, w.amx, w.scal, w.cnt,
-                                                         w.Ubp, w.abp);
-  unpad_kernel<<<blocks_for(n), 256, 0, st>>>(a, w.Ubp, w.abp, Ut, At);
+      a, Us, As, rb, alpha, Ut, At, w.Ubp, w.abp, w.part);
+  alpha_reduce_kernel<<<1, 256, 0, st>>>(w.part, w.nblk, w.scal, w.amx, w.cnt);
+  alpha_count_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, Us, As, w.amx, w.cnt);
+  alpha_apply_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, Us, As, w.amx, w.scal, w.cnt, Ut,
+                                                         At, w.Ubp, w.abp);
+  const int m = c->adj_fold_m;
+  const int* fl = w.fold;
+  if (m > 0)
+    fold_ghosts_kernel<<<blocks_for(m), 256, 0, st>>>(a, fl, fl + m, fl + 2 * m + 1, m, w.Ubp,
+                                                      w.abp, Ut, At);
 }
 
 // Uk/Ak: the step's output when the caller already has it (the RK stages are
@@ -433,6 +484,8 @@ int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, cons
   if (Uo) HIP_TRY(c, hipMemcpyAsync(Uo, w.U[4], NU * n * sizeof(double), hipMemcpyDeviceToDevice, st));
   if (Ao) HIP_TRY(c, hipMemcpyAsync(Ao, w.A[4], 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
   const AdjArgs aa = adj_args(c, dt);
+  rc = adj_fold(c, aa, w);
+  if (rc) return rc;
   const unsigned long long* alpha[3] = {&c->sc->amax[P][0][0], &c->sc->amax[P][1][0],
                                         &c->sc->amax[P][2][0]};
   // ---- reverse sweep ----
@@ -440,19 +493,19 @@ int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, cons
   axpby_kernel<<<bn, 256, 0, st>>>(NU * n, 1.0, Ubar, 0.0, w.Ub);
   getT_vjp_kernel<<<bc1, 256, 0, st>>>(aa, w.U[4], w.A[4], Abar, w.Ub);
   chem_vjp_kernel<<<bc1, 256, 0, st>>>(aa, w.U[3], w.A[3], w.Ub, w.Ub, w.ab);
-  getT_vjp_kernel<<<bc1, 256, 0, st>>>(aa, w.U[3], w.A[3], w.ab, w.Ub);
+  // each getT_split_kernel: get_T's VJP at the stage state, then the
+  // reverse of the stage's combine
   // stage 3: U3 = 1/3 U + 2/3 (U2 + r(U2))
-  rk_split_kernel<<<bn, 256, 0, st>>>(NU * n, 2 * n, 1.0 / 3.0, 0.0, 2.0 / 3.0, w.Ub, w.Uacc,
-                                      w.rb, w.ab);
+  getT_split_kernel<<<bc1, 256, 0, st>>>(aa, w.U[3], w.A[3], 1.0 / 3.0, 0.0, 2.0 / 3.0, w.Ub,
+                                         w.Uacc, w.rb, w.ab);
   flux_vjp(c, aa, w, w.U[2], w.A[2], w.rb, alpha[2], w.Ub, w.ab);
-  getT_vjp_kernel<<<bc1, 256, 0, st>>>(aa, w.U[2], w.A[2], w.ab, w.Ub);
   // stage 2: U2 = 3/4 U + 1/4 (U1 + r(U1))
-  rk_split_kernel<<<bn, 256, 0, st>>>(NU * n, 2 * n, 3.0 / 4.0, 1.0, 1.0 / 4.0, w.Ub, w.Uacc,
-                                      w.rb, w.ab);
+  getT_split_kernel<<<bc1, 256, 0, st>>>(aa, w.U[2], w.A[2], 3.0 / 4.0, 1.0, 1.0 / 4.0, w.Ub,
+                                         w.Uacc, w.rb, w.ab);
   flux_vjp(c, aa, w, w.U[1], w.A[1], w.rb, alpha[1], w.Ub, w.ab);
-  getT_vjp_kernel<<<bc1, 256, 0, st>>>(aa, w.U[1], w.A[1], w.ab, w.Ub);
   // stage 1: U1 = U + r(U)
-  rk_split_kernel<<<bn, 256, 0, st>>>(NU * n, 2 * n, 1.0, 1.0, 0.0, w.Ub, w.Uacc, nullptr, w.ab);
+  getT_split_kernel<<<bc1, 256, 0, st>>>(aa, w.U[1], w.A[1], 1.0, 1.0, 0.0, w.Ub, w.Uacc,
+                                         nullptr, w.ab);
   flux_vjp(c, aa, w, w.U[0], w.A[0], w.Ub, alpha[0], w.Uacc, w.ab);
   HIP_TRY(c, hipMemcpyAsync(Ub0, w.Uacc, NU * n * sizeof(double), hipMemcpyDeviceToDevice, st));
   HIP_TRY(c, hipMemcpyAsync(Ab0, w.ab, 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
