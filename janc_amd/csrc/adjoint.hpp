@@ -683,9 +683,12 @@ __device__ __forceinline__ void getT_vjp_cell(const MechConst& c, const double* 
   eY_to_U(Uc, eb, Yb, Ubc);
 }
 
+// Ub = Ubar + get_T's VJP of ab at the state (U, aux): the start of the
+// reverse sweep, at the step's output.
 __global__ __launch_bounds__(256) void getT_vjp_kernel(AdjArgs a, const double* __restrict__ U,
                                                        const double* __restrict__ aux,
                                                        const double* __restrict__ ab,
+                                                       const double* __restrict__ Ubar,
                                                        double* __restrict__ Ub) {
   const int n = a.nx * a.ny;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -693,7 +696,7 @@ __global__ __launch_bounds__(256) void getT_vjp_kernel(AdjArgs a, const double* 
   double Ubc[NU];
   getT_vjp_cell(*a.mc, U, aux, ab, n, t, Ubc);
 #pragma unroll
-  for (int v = 0; v < NU; ++v) Ub[v * n + t] += Ubc[v];
+  for (int v = 0; v < NU; ++v) Ub[v * n + t] = Ubar[v * n + t] + Ubc[v];
 }
 
 // U4 = U3 + S(U3, T3): Ub3 = Ub4 + S_vjp, Tb3 = S_vjp_T (model/reaction_model.py:183-193).
@@ -757,14 +760,6 @@ __global__ __launch_bounds__(256) void getT_split_kernel(AdjArgs a, const double
   }
   ab[t] = 0.0;
   ab[n + t] = 0.0;
-}
-
-// y = a*x + b*y over m doubles (b = 0 overwrites).
-__global__ __launch_bounds__(256) void axpby_kernel(int m, double a, const double* __restrict__ x,
-                                                    double b, double* __restrict__ y) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= m) return;
-  y[t] = b == 0.0 ? a * x[t] : a * x[t] + b * y[t];
 }
 
 }  // namespace jh

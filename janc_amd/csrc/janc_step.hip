@@ -455,7 +455,7 @@ int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, cons
   int rc = adj_work(c, U0, A0, w);
   if (rc) return rc;
   const size_t n = (size_t)c->nx * c->ny;
-  const int bn = blocks_for(NU * n), bc1 = blocks_for(n);
+  const int bc1 = blocks_for(n);
   hipStream_t st = c->stream;
   // ---- forward recompute, keeping every stage ----
   rc = prologue<double>(c, U0, A0, dt, 0.0);
@@ -490,8 +490,7 @@ int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, cons
                                         &c->sc->amax[P][2][0]};
   // ---- reverse sweep ----
   // final update_aux and chemistry: Ub = U4_bar -> U3_bar
-  axpby_kernel<<<bn, 256, 0, st>>>(NU * n, 1.0, Ubar, 0.0, w.Ub);
-  getT_vjp_kernel<<<bc1, 256, 0, st>>>(aa, w.U[4], w.A[4], Abar, w.Ub);
+  getT_vjp_kernel<<<bc1, 256, 0, st>>>(aa, w.U[4], w.A[4], Abar, Ubar, w.Ub);
   chem_vjp_kernel<<<bc1, 256, 0, st>>>(aa, w.U[3], w.A[3], w.Ub, w.Ub, w.ab);
   // each getT_split_kernel: get_T's VJP at the stage state, then the
   // reverse of the stage's combine

@@ -129,6 +129,18 @@ def test_step_vjp_linear_in_cotangent(mech_path, mech):
     assert rel_err(a3, 2.0 * a1 - a2).max() <= 1e-12
 
 
+@pytest.mark.parametrize("name", ["walls", "box_ties"])
+def test_step_vjp_is_deterministic(mech_path, mech, name):
+    """Two VJPs of the same inputs agree bit for bit: the cotangents reach
+    their cells through fixed-order gathers (ghost folds, block sums), not
+    through atomics."""
+    sim, g, U, aux = _setup(mech_path, mech, name)
+    W, Wa = _cotangents(U, aux, 9)
+    g1, a1 = sim.step_vjp(U, aux, DT, W, Wa)
+    g2, a2 = sim.step_vjp(U, aux, DT, W, Wa)
+    assert torch.equal(g1, g2) and torch.equal(a1, a2)
+
+
 def test_vjp_with_known_output_is_identical(mech_path, mech):
     """jh_step_vjp_known (the step's output supplied, no chemistry forward
     pass) gives the same cotangents bit for bit as jh_step_vjp."""
