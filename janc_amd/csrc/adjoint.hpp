@@ -195,7 +195,12 @@ struct AdjLds {
   double Cx[2 * (TXA + 4) * 5 * TY];
   double Cy[2 * TXA * 5 * (TY + 4)];
   double red[4][TXA];
+  unsigned int ties[2][TXA];
 };
+
+// Exact tie test of the LF maxima: JAX's max VJP splits the cotangent
+// among the elements equal to the maximum.
+__device__ __forceinline__ bool is_max(double val, double alpha) { return val == alpha; }
 
 struct PrimL {   // the primitives the physical flux needs
   double rho, u, v, p;
@@ -445,14 +450,14 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
   }
   // the own cell's |u| + a and |v| + a: the padded field's LF maxima are
   // reduced from these (per block here, then alpha_reduce_kernel), with the
-  // arithmetic that alpha_count_kernel's tie test repeats
-  double mxo = 0.0, myo = 0.0;
+  // arithmetic that alpha_apply_kernel's tie test repeats
+  double rxo = 0.0, ryo = 0.0;   // raw values, for the tie test
   if (own) {
     double Tb = 0.0, gb = 0.0;
     const PrimD qf = prim_src(c, U, aux, n, src[0], fl[0]);
     const double snd = sqrt(qf.gamma * qf.p * qf.ir);
-    mxo = nan_to_zero(fabs(qf.u) + snd);
-    myo = nan_to_zero(fabs(qf.v) + snd);
+    rxo = fabs(qf.u) + snd;
+    ryo = fabs(qf.v) + snd;
     if (work) prim_to_U_vjp(c, qf, rhob, ub, vb, pb, 0.0, Yb, Ub, Tb, gb);
     if (pad_interior(a, P, Q)) {   // its own source: straight into the output
       const int s = src[0];
@@ -468,7 +473,10 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
       abp[np + pt] = Tb;
     }
   }
-  // fixed-order block sums of the alpha cotangent, and the block maxima
+  // fixed-order block sums of the alpha cotangent, the block maxima, and
+  // the block's count of cells attaining them (exact: a block whose maximum
+  // is the field's holds all of its cells that attain it)
+  double mxo = nan_to_zero(rxo), myo = nan_to_zero(ryo);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     sx_part += __shfl_xor(sx_part, off, 64);
@@ -483,83 +491,98 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
     L.red[3][w] = myo;
   }
   __syncthreads();
+  double mx = 0.0, my = 0.0;
+#pragma unroll
+  for (int k = 0; k < TXA; ++k) {
+    mx = fmax(mx, L.red[2][k]);
+    my = fmax(my, L.red[3][k]);
+  }
+  const unsigned long long tx = __ballot(own && is_max(rxo, mx));
+  const unsigned long long ty = __ballot(own && is_max(ryo, my));
+  if (ly == 0) {
+    L.ties[0][w] = (unsigned int)__popcll(tx);
+    L.ties[1][w] = (unsigned int)__popcll(ty);
+  }
+  __syncthreads();
   if (t == 0) {
-    double sx = 0.0, sy = 0.0, mx = 0.0, my = 0.0;
+    double sx = 0.0, sy = 0.0;
+    unsigned int cx = 0, cy = 0;
 #pragma unroll
     for (int k = 0; k < TXA; ++k) {
       sx += L.red[0][k];
       sy += L.red[1][k];
-      mx = fmax(mx, L.red[2][k]);
-      my = fmax(my, L.red[3][k]);
+      cx += L.ties[0][k];
+      cy += L.ties[1][k];
     }
-    part[4 * blockIdx.x] = sx;
-    part[4 * blockIdx.x + 1] = sy;
-    part[4 * blockIdx.x + 2] = mx;
-    part[4 * blockIdx.x + 3] = my;
+    double* o = part + 6 * blockIdx.x;
+    o[0] = sx;
+    o[1] = sy;
+    o[2] = mx;
+    o[3] = my;
+    o[4] = (double)cx;
+    o[5] = (double)cy;
   }
 }
 
-// alpha_bar = sum of the per-block partials, in a fixed order, and the LF
-// maxima of the padded field = max of the block maxima (one block):
-// scal[0..1] and amx[0..1] (bit patterns); clears the tie counts cnt[0..1]
-// for alpha_count_kernel.
+// alpha_bar = sum of the per-block partials, in a fixed order; the LF maxima
+// of the padded field = max of the block maxima; the tie counts = sum of the
+// counts of the blocks whose maximum is the field's (one block): scal[0..1],
+// amx[0..1] (bit patterns) and cnt[0..1] for alpha_apply_kernel, which hands
+// each padded cell attaining a maximum its share of alpha_bar.
 __global__ __launch_bounds__(256) void alpha_reduce_kernel(const double* __restrict__ part,
                                                            int nblk, double* scal,
                                                            unsigned long long* amx,
                                                            unsigned int* cnt) {
   __shared__ double red[4][256];
+  const int t = threadIdx.x;
   double sx = 0.0, sy = 0.0, mx = 0.0, my = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += 256) {
-    sx += part[4 * b];
-    sy += part[4 * b + 1];
-    mx = fmax(mx, part[4 * b + 2]);
-    my = fmax(my, part[4 * b + 3]);
+  for (int b = t; b < nblk; b += 256) {
+    sx += part[6 * b];
+    sy += part[6 * b + 1];
+    mx = fmax(mx, part[6 * b + 2]);
+    my = fmax(my, part[6 * b + 3]);
   }
-  red[0][threadIdx.x] = sx;
-  red[1][threadIdx.x] = sy;
-  red[2][threadIdx.x] = mx;
-  red[3][threadIdx.x] = my;
+  red[0][t] = sx;
+  red[1][t] = sy;
+  red[2][t] = mx;
+  red[3][t] = my;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + s];
-      red[1][threadIdx.x] += red[1][threadIdx.x + s];
-      red[2][threadIdx.x] = fmax(red[2][threadIdx.x], red[2][threadIdx.x + s]);
-      red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + s]);
+    if (t < s) {
+      red[0][t] += red[0][t + s];
+      red[1][t] += red[1][t + s];
+      red[2][t] = fmax(red[2][t], red[2][t + s]);
+      red[3][t] = fmax(red[3][t], red[3][t + s]);
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
+  const double gx = red[2][0], gy = red[3][0];
+  if (t == 0) {
     scal[0] = red[0][0];
     scal[1] = red[1][0];
-    amx[0] = dbits(red[2][0]);
-    amx[1] = dbits(red[3][0]);
-    cnt[0] = 0u;
-    cnt[1] = 0u;
+    amx[0] = dbits(gx);
+    amx[1] = dbits(gy);
   }
-}
-
-// Exact tie test against the maxima alpha_reduce_kernel reduced from the same
-// arithmetic: JAX's max VJP splits the cotangent among the elements equal to
-// the maximum.  alpha_count_kernel counts the padded cells attaining each LF
-// maximum (cnt[0..1]); alpha_apply_kernel hands each its share of alpha_bar.
-__device__ __forceinline__ bool is_max(double val, double alpha) { return val == alpha; }
-
-__global__ __launch_bounds__(256) void alpha_count_kernel(AdjArgs a, const double* __restrict__ U,
-                                                          const double* __restrict__ aux,
-                                                          const unsigned long long* alpha_bits,
-                                                          unsigned int* cnt) {
-  const MechConst& c = *a.mc;
-  const int PY = a.ny + 6, np = (a.nx + 6) * PY;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= np) return;
-  int sx, sy, flip;
-  const int P = t / PY, Q = t - P * PY;
-  if (!pad_src(a, P, Q, sx, sy, flip)) return;
-  const PrimD q = prim_src(c, U, aux, a.nx * a.ny, sx * a.ny + sy, flip);
-  const double snd = sqrt(q.gamma * q.p * q.ir);
-  if (is_max(fabs(q.u) + snd, bitsd(alpha_bits[0]))) atomicAdd(&cnt[0], 1u);
-  if (is_max(fabs(q.v) + snd, bitsd(alpha_bits[1]))) atomicAdd(&cnt[1], 1u);
+  __syncthreads();                 // red[0..1][0] read before reuse
+  double cx = 0.0, cy = 0.0;
+  for (int b = t; b < nblk; b += 256) {
+    if (part[6 * b + 2] == gx) cx += part[6 * b + 4];
+    if (part[6 * b + 3] == gy) cy += part[6 * b + 5];
+  }
+  red[0][t] = cx;
+  red[1][t] = cy;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      red[0][t] += red[0][t + s];
+      red[1][t] += red[1][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    cnt[0] = (unsigned int)red[0][0];
+    cnt[1] = (unsigned int)red[1][0];
+  }
 }
 
 __global__ __launch_bounds__(256) void alpha_apply_kernel(AdjArgs a, const double* __restrict__ U,

@@ -337,7 +337,7 @@ int adj_work(jh_ctx* c, const double* U0, const double* A0, AdjWork& w) {
   const int nblk = adj_tiles(c);
   const size_t ng = np - n;      // ghost cells
   const size_t len = 4 * (NU + 2) * n + (3 * NU + 2) * n + (NU + 2) * np +
-                     4 * (size_t)nblk + 2 + 2 + 2 + (3 * ng + 2) / 2;
+                     6 * (size_t)nblk + 2 + 2 + 2 + (3 * ng + 2) / 2;
   if (c->adj_len < len) {
     if (c->adj) (void)hipFree(c->adj);
     c->adj = nullptr;
@@ -359,7 +359,7 @@ int adj_work(jh_ctx* c, const double* U0, const double* A0, AdjWork& w) {
   w.ab = p; p += 2 * n;
   w.Ubp = p; p += NU * np;
   w.abp = p; p += 2 * np;
-  w.part = p; p += 4 * nblk;
+  w.part = p; p += 6 * nblk;
   w.scal = p; p += 2;
   w.cnt = reinterpret_cast<unsigned int*>(p);
   p += 2;
@@ -435,7 +435,6 @@ void flux_vjp(jh_ctx* c, const AdjArgs& a, AdjWork& w, const double* Us, const d
   flux_vjp_tile_kernel<kAdjRows><<<w.nblk, AdjTile<kAdjRows>::NT, 0, st>>>(
       a, Us, As, rb, alpha, Ut, At, w.Ubp, w.abp, w.part);
   alpha_reduce_kernel<<<1, 256, 0, st>>>(w.part, w.nblk, w.scal, w.amx, w.cnt);
-  alpha_count_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, Us, As, w.amx, w.cnt);
   alpha_apply_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, Us, As, w.amx, w.scal, w.cnt, Ut,
                                                          At, w.Ubp, w.abp);
   const int m = c->adj_fold_m;
