@@ -6,14 +6,18 @@ CFL dt, three SSP-RK3 stages of WENO5 + global Lax-Friedrichs flux
 divergence with Halley temperature updates, point-implicit 9-species /
 19-reaction H2-air chemistry and the final temperature solve.
 
-Headline (``value``): the FP64 plain step on the 4000 x 400 (1.6M-cell)
-periodic-in-x detonation annulus, inputs resident in HBM, K steps timed
-between two device synchronisations.  Also reported: the mixed-precision and
-FP64 steps on the 8000 x 500 (4M-cell) tube, the per-kernel HIP-event times
-with the roofline of the dominant kernel, and the FP64 CPU oracle timed on a
-200 x 80 crop of the headline workload on this host.
+Headline (``value``): the mixed-precision step (FP32 state, split fluxes and
+WENO; FP64 divergence, RK combine, chemistry and temperature solve) on the
+8000 x 500 (4M-cell) tube with a smooth equivalence-ratio field and a
+40-column 20 atm / 2800 K ignition slab; inputs resident in HBM, K steps
+timed between two device synchronisations.  Beside it on the same workload:
+the FP64 step, the mixed form's error after 1000 steps (against the FP64
+step on the full tube and against the FP64 CPU oracle on a crop), the
+adjoint's cost per forward step, the batch and annulus legs, the roofline of
+the dominant kernel and the CPU oracle timed on this host's cores.
 
-Prints ONE JSON line.  Single GPU only.
+Every leg prints one JSON line of its own (``{"leg": ...}``) as it finishes;
+the LAST line is the compact summary (a few KB).  Single GPU only.
 """
 import argparse
 import json
@@ -32,26 +36,32 @@ FP64_VECTOR_PEAK_TFLOPS = 78.6
 #   chemistry: read U (12) + T (1), write rho*Y (8) + aux (2)      = 23
 WORDS = {"stage": (27 + 39 + 39) / 3.0, "chem": 23.0, "reduce": 13.0, "bc": 0.0}
 
-# Measured HBM bytes per cell per stage-kernel launch of the FP64 headline
-# step (rocprofv3 FETCH_SIZE + WRITE_SIZE passes, each scaled by the same
-# counter's reading for a 256 MiB device-to-device copy; summary under
-# profiles/).  Reads exceed the algorithmic 25 words: each region cell's
-# state is read for its primitives and again per variable.
-STAGE_TRAFFIC_B_PER_CELL_FP64 = 459.8
+# Measured HBM bytes per cell per stage-kernel launch (rocprofv3 FETCH_SIZE +
+# WRITE_SIZE passes, each scaled by the same counter's reading for a 256 MiB
+# device-to-device copy; summaries under profiles/): FP64 = the 4000 x 400
+# annulus step (tile variant 3), mixed = the 8000 x 500 tube (variant 1).
+# Reads exceed the algorithmic words: each region cell's state is read for
+# its primitives and again per variable.
+STAGE_TRAFFIC_B_PER_CELL = {"fp64": 459.8, "mixed": 211.9}
 
-# Measured FP64 FLOP per cell per launch of the FP64 headline kernels
-# (rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes / cells, FMA = 2; summary
-# under profiles/): the bench turns them into live FP64-vector fractions.
-FP64_FLOP_PER_CELL = {"stage": 4627.0, "chem": 3239.0}
-
+# Measured FP64 FLOP per cell per chemistry launch (rocprofv3
+# SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes / cells, FMA = 2; summary under
+# profiles/).  The chemistry is FP64 in both forms.
+CHEM_FP64_FLOP_PER_CELL = 3239.0
 
 # Per-kernel HIP events bracket the launches of every KERNEL_TIMING_EVERY-th
 # step of the timed region: events around every launch add ~5 % to a step.
 KERNEL_TIMING_EVERY = 5
 
 
+def emit(leg, obj):
+    """One detail line (the summary line comes last)."""
+    print(json.dumps(dict({"leg": leg}, **obj), default=float), flush=True)
+
+
 def _run(sim, U, aux, steps, warmup, torch):
-    """Warm up, then time `steps` fused steps; returns (seconds, per-kernel)."""
+    """Warm up, then time `steps` fused steps; returns (seconds, per-kernel,
+    dt history, health flags)."""
     if warmup:
         sim.advance(U, aux, warmup)
     torch.cuda.synchronize()
@@ -81,13 +91,247 @@ def _roofline(kt, cells, word_bytes):
     return out
 
 
-CPU_CROP = (400, 400)
+def _kernels(kt):
+    return {k: {"ms_total": v[0], "launches": v[1]} for k, v in kt.items()}
 
-# flux_config variants (set_flux_solver, solver_2D/flux.py:19-40); the
-# headline is the first
+
+# ------------------------------------------------------------------ CPU legs
+# The CPU work runs in worker processes forked before this process touches
+# the GPU, so no worker inherits a GPU handle.
+
+_CPU = {}      # state shared with the forked workers (set before the fork)
+
+# FP64 oracle crop of the tube for the mixed form's accuracy check: the
+# ignition slab and the phi field's variation are inside it.
+ORACLE_CROP = (200, 50)
+# CPU baseline: each worker advances its own crop of the tube.
+CPU_CROP = (250, 125)
+
+
+def _tube_bc():
+    """Boundary kinds of the tube configurations (janc_amd.cases.detonation_tube)."""
+    return {"left_boundary": "slip_wall", "right_boundary": "zero_gradient",
+            "bottom_boundary": "slip_wall", "top_boundary": "slip_wall"}
+
+
+def _cpu_worker(job):
+    """Time `steps` CFL steps of the FP64 numpy oracle on one crop."""
+    x0, y0, nxc, nyc, steps = job
+    from threadpoolctl import threadpool_limits
+    from oracle.step import Oracle
+    from janc_amd.mechanism import DEFAULT_MECH
+    with threadpool_limits(1):
+        U = _CPU["U"][:, x0:x0 + nxc, y0:y0 + nyc].copy()
+        aux = _CPU["aux"][:, x0:x0 + nxc, y0:y0 + nyc].copy()
+        o = Oracle(DEFAULT_MECH, _tube_bc(), _CPU["dx"], _CPU["dx"])
+        t = 0.0
+        U, aux, t, _ = o.advance_func(U, aux, t, 0.3)          # warm caches
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            U, aux, t, _ = o.advance_func(U, aux, t, 0.3)
+        t1 = time.perf_counter()
+    return nxc * nyc * steps, t0, t1
+
+
+def _oracle_crop_worker(job):
+    """FP64 oracle on the accuracy crop at a fixed dt; states every `every` steps."""
+    nxc, nyc, dt, steps, every = job
+    from threadpoolctl import threadpool_limits
+    from oracle.step import Oracle
+    from janc_amd.mechanism import DEFAULT_MECH
+    import numpy as np
+    with threadpool_limits(1):
+        U = _CPU["U"][:, :nxc, :nyc].copy()
+        aux = _CPU["aux"][:, :nxc, :nyc].copy()
+        o = Oracle(DEFAULT_MECH, _tube_bc(), _CPU["dx"], _CPU["dx"])
+        out = []
+        t0 = time.perf_counter()
+        for s in range(1, steps + 1):
+            U, aux = o.advance_one_step(U, aux, dt)
+            if s % every == 0:
+                out.append((s, np.array(U), np.array(aux)))
+    return out, time.perf_counter() - t0
+
+
+def _cpu_baseline(pool, nworkers, steps):
+    jobs = []
+    for w in range(nworkers):
+        jobs.append(((w // 4) * CPU_CROP[0], (w % 4) * CPU_CROP[1], CPU_CROP[0], CPU_CROP[1],
+                     steps))
+    res = pool.map(_cpu_worker, jobs, chunksize=1)
+    work = sum(r[0] for r in res)
+    wall = max(r[2] for r in res) - min(r[1] for r in res)
+    per = [r[0] / (r[2] - r[1]) for r in res]
+    return {"value": work / wall, "unit": "cell-updates/s", "cores": nworkers, "kind": "port",
+            "per_process_value": sum(per) / len(per),
+            "sample": "%d processes (one core each), each %d CFL steps of the FP64 numpy oracle "
+                      "on its own %dx%d crop of the 8000x500 tube state (crops tile x<%d, "
+                      "y<500; slab in crop 0); value = all crops' cell-updates / wall time "
+                      "from the first start to the last end (%.1f s)"
+                      % (nworkers, steps, CPU_CROP[0], CPU_CROP[1],
+                         CPU_CROP[0] * ((nworkers + 3) // 4), wall)}
+
+
+# ------------------------------------------------------------------ GPU legs
+
+def _fields(U, aux, inv_M):
+    """rho, p, T, Y_k (float64 torch) of a state."""
+    import torch
+    U = torch.as_tensor(U).double()
+    aux = torch.as_tensor(aux).double()
+    inv_M = torch.as_tensor(inv_M, device=U.device)
+    rho = U[0]
+    Y = U[4:] / rho
+    Y9 = torch.cat([Y, (1.0 - Y.sum(0))[None]])
+    R = (inv_M[:, None, None] * Y9).sum(0)
+    return {"rho": rho, "p": rho * R * aux[1], "T": aux[1], "Y": Y}
+
+
+def _errors(fm, fr, species):
+    """Relative L2 and Linf of rho, p, T and every Y_k (fm against fr)."""
+    row = {}
+    for k in ("rho", "p", "T"):
+        d = fm[k] - fr[k]
+        row[k] = {"L2": float(d.norm() / fr[k].norm()),
+                  "Linf": float(d.abs().max() / fr[k].abs().max())}
+    ys = {}
+    for i, name in enumerate(species):
+        d = fm["Y"][i] - fr["Y"][i]
+        ys[name] = {"L2": float(d.norm() / max(float(fr["Y"][i].norm()), 1e-300)),
+                    "Linf": float(d.abs().max() / max(float(fr["Y"][i].abs().max()), 1e-300)),
+                    "max": float(fr["Y"][i].abs().max())}
+    row["Y"] = ys
+    return row
+
+
+def _worst(row):
+    """Largest relative L2 of T and over the Y_k of one error row."""
+    yk = max(row["Y"], key=lambda s: row["Y"][s]["L2"])
+    return {"T_L2": row["T"]["L2"], "rho_L2": row["rho"]["L2"], "p_L2": row["p"]["L2"],
+            "Y_L2_max": row["Y"][yk]["L2"], "Y_L2_max_species": yk,
+            "T_Linf": row["T"]["Linf"],
+            "Y_Linf_max": max(v["Linf"] for v in row["Y"].values())}
+
+
+def _headline(mech, U3, aux3, g3, steps, warmup, tune, torch, tile=None):
+    """Mixed and FP64 steps on the 8000 x 500 tube, `steps` CFL steps each."""
+    from janc_amd import cases
+    from janc_amd.solver import Simulator
+    n3 = U3.shape[1] * U3.shape[2]
+    out = {}
+    for prec, dtype in (("mixed", torch.float32), ("fp64", torch.float64)):
+        s3 = Simulator(cases.simulation_config(g3, cfl=0.3, precision=prec))
+        Ut = torch.as_tensor(U3, dtype=dtype).cuda()
+        at = torch.as_tensor(aux3, dtype=dtype).cuda()
+        if tile is not None:
+            s3.set_tile(Ut, tile)
+            t3 = {"variant": tile, "forced": True}
+        else:
+            t3 = s3.autotune(Ut, at) if tune else None
+        e3, k3, d3, f3 = _run(s3, Ut, at, steps, warmup, torch)
+        out[prec] = {"value": n3 * steps / e3, "ms_per_step": e3 / steps * 1e3,
+                     "kernels": _kernels(k3), "kt": k3,
+                     "roofline": _roofline(k3, n3, 4 if prec == "mixed" else 8),
+                     "health_flags": f3, "dt_first": float(d3[0]), "dt_last": float(d3[-1]),
+                     "autotune": t3}
+        del Ut, at, s3
+        torch.cuda.empty_cache()
+    return out
+
+
+def _accuracy(mech, U3, aux3, g3, steps, every, torch):
+    """Mixed vs FP64 step on the 8000 x 500 tube at one fixed dt (the initial
+    CFL-0.3 dt): relative L2 and Linf of rho, p, T, Y_k every `every` steps."""
+    from janc_amd import cases
+    from janc_amd.solver import Simulator
+    probe = Simulator(cases.simulation_config(g3, cfl=0.3))
+    dt = float(probe.CFL(U3, aux3, 0.3))
+    del probe
+    inv_M = 1.0 / mech.species_M
+    sims, states = {}, {}
+    for prec, dtype in (("mixed", torch.float32), ("fp64", torch.float64)):
+        sims[prec] = Simulator(cases.simulation_config(g3, dt=dt, precision=prec))
+        states[prec] = (torch.as_tensor(U3, dtype=dtype).cuda(),
+                        torch.as_tensor(aux3, dtype=dtype).cuda())
+    report = []
+    done = 0
+    while done < steps:
+        n = min(every, steps - done)
+        for prec in sims:
+            sims[prec].advance(*states[prec], n)
+        done += n
+        row = _errors(_fields(*states["mixed"], inv_M), _fields(*states["fp64"], inv_M),
+                      mech.species[:-1])
+        row["step"] = done
+        report.append(row)
+    flags = {p: sims[p].health(states[p][0]) for p in sims}
+    return {"workload": "8000x500 tube (dx 1.25e-5 m), fixed dt %.4g s, mixed vs FP64 GPU "
+                        "step (the FP64 step is parity-checked against the oracle)" % dt,
+            "dt": dt, "steps": steps, "health_flags": flags, "every": report}
+
+
+def _accuracy_crop_gpu(mech, U3, aux3, g3, dt, steps, every, torch):
+    """The mixed step on the oracle's crop at the same fixed dt; host copies
+    of the state every `every` steps."""
+    from janc_amd import cases
+    from janc_amd.solver import Simulator
+    nxc, nyc = ORACLE_CROP
+    geo = dict(g3)
+    sim = Simulator(cases.simulation_config(geo, dt=dt, precision="mixed"))
+    U = torch.as_tensor(U3[:, :nxc, :nyc].copy(), dtype=torch.float32).cuda()
+    a = torch.as_tensor(aux3[:, :nxc, :nyc].copy(), dtype=torch.float32).cuda()
+    out = {}
+    done = 0
+    while done < steps:
+        sim.advance(U, a, every)
+        done += every
+        out[done] = (U.double().cpu(), a.double().cpu())
+    return out, sim.health(U)
+
+
+def _accuracy_crop_compare(mech, gpu_states, oracle_states, steps, dt, el):
+    inv_M = 1.0 / mech.species_M
+    rows = []
+    for s, Uo, ao in oracle_states:
+        Ug, ag = gpu_states[s]
+        row = _errors(_fields(Ug, ag, inv_M), _fields(Uo, ao, inv_M), mech.species[:-1])
+        row["step"] = s
+        rows.append(row)
+    return {"workload": "%dx%d corner crop of the 8000x500 tube (slab and phi field "
+                        "included), fixed dt %.4g s, %d steps: mixed GPU step vs the FP64 "
+                        "numpy oracle (%.1f s on one core)" % (ORACLE_CROP + (dt, steps, el)),
+            "steps": steps, "every": rows}
+
+
+def _annulus_leg(mech, steps, warmup, tune, torch):
+    """FP64 step on the 4000 x 400 periodic-in-x detonation annulus."""
+    from janc_amd import cases
+    from janc_amd.solver import Simulator
+    U0, aux0, geom, theta = cases.rdc_annulus(mech)
+    nx, ny = U0.shape[1], U0.shape[2]
+    sim = Simulator(cases.simulation_config(geom, cfl=0.3, precision="fp64", theta=theta))
+    U = torch.as_tensor(U0).cuda()
+    aux = torch.as_tensor(aux0).cuda()
+    tuned = sim.autotune(U, aux, theta=theta) if tune else None
+    el, kt, dts, flags = _run(sim, U, aux, steps, warmup, torch)
+    cells = nx * ny
+    roof = _roofline(kt, cells, 8)
+    out = {"workload": "FP64 plain step, 4000x400 periodic-in-x detonation annulus (ZND-like "
+                       "seed 200 columns 30 atm/3000 K, 0.5% T noise), CFL 0.3; bottom "
+                       "injection inlet (0.2 MPa / 300 K manifold), top pressure outlet 1 atm",
+           "value": cells * steps / el, "ms_per_step": el / steps * 1e3, "steps": steps,
+           "cells": cells, "kernels": _kernels(kt), "roofline": roof, "health_flags": flags,
+           "dt_first": float(dts[0]), "dt_last": float(dts[-1]), "autotune": tuned}
+    tr = STAGE_TRAFFIC_B_PER_CELL["fp64"]
+    if "stage" in roof and tr:
+        out["stage_traffic_bytes"] = tr * cells
+    del U, aux
+    torch.cuda.empty_cache()
+    return out, (U0, aux0, geom, theta)
+
+
+# flux_config variants (set_flux_solver, solver_2D/flux.py:19-40)
 SCHEMES = {
-    "lf_weno": {"solver_type": "flux_splitting", "interface_reconstruction": "WENO5_JS",
-                "split_method": "LF"},
     "hllc_weno": {"solver_type": "godunov", "interface_reconstruction": "WENO5_JS",
                   "riemann_solver": "HLLC"},
     "hllc_muscl": {"solver_type": "godunov", "interface_reconstruction": "MUSCL",
@@ -100,14 +344,12 @@ SCHEMES = {
 
 
 def _schemes_leg(U0, aux0, geom, theta, steps, warmup, tune, torch):
-    """The FP64 step of the headline workload with each other flux scheme."""
+    """The FP64 annulus step with each other flux scheme."""
     from janc_amd import cases
     from janc_amd.solver import Simulator
     cells = U0.shape[1] * U0.shape[2]
     out = {}
     for name, fc in SCHEMES.items():
-        if name == "lf_weno":
-            continue
         sim = Simulator(cases.simulation_config(geom, cfl=0.3, precision="fp64", theta=theta,
                                                 flux=fc))
         U = torch.as_tensor(U0).cuda()
@@ -118,44 +360,11 @@ def _schemes_leg(U0, aux0, geom, theta, steps, warmup, tune, torch):
         el, kt, dts, flags = _run(sim, U, aux, steps, warmup, torch)
         out[name] = {"flux_config": fc, "value": cells * steps / el,
                      "ms_per_step": el / steps * 1e3, "steps": steps,
-                     "kernels": {k: {"ms_total": v[0], "launches": v[1]} for k, v in kt.items()},
-                     "roofline": _roofline(kt, cells, 8), "health_flags": flags,
-                     "autotune": tuned}
+                     "kernels": _kernels(kt), "roofline": _roofline(kt, cells, 8),
+                     "health_flags": flags, "autotune": tuned}
         del U, aux
         torch.cuda.empty_cache()
     return out
-
-
-def _cpu_baseline(U, aux, geom, theta, steps):
-    """FP64 oracle (numpy, one core) on a 400 x 400 crop of the headline state."""
-    from oracle.step import Oracle
-    from janc_amd.mechanism import DEFAULT_MECH
-    crop = (slice(None), slice(0, CPU_CROP[0]), slice(0, CPU_CROP[1]))
-    Uc, ac = U[crop].copy(), aux[crop].copy()
-    bcfg = dict(geom["boundary_config"])
-    inlets = [k for k, v in bcfg.items() if v == "injection_inlet"]
-    for k in inlets:
-        bcfg[k] = "slip_wall"
-    o = Oracle(DEFAULT_MECH, bcfg, geom["dx"], geom["dy"])
-    if inlets:
-        from oracle.inlet import attach_inlet
-        for k in inlets:
-            attach_inlet(o, k.replace("_boundary", ""), theta["P0_inj"], theta["T0_inj"],
-                         theta["Yinj"])
-    t = 0.0
-    o.advance_func(Uc, ac, t, 0.3, theta)          # warm caches
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        Uc, ac, t, _ = o.advance_func(Uc, ac, t, 0.3, theta)
-    el = time.perf_counter() - t0
-    cells = Uc.shape[1] * Uc.shape[2]
-    return {"value": cells * steps / el, "unit": "cell-updates/s", "cores": 1,
-            "kind": "port",
-            "sample": "%d CFL steps of the FP64 numpy oracle on the %dx%d corner crop "
-                      "(seed columns included) of the 4000x400 headline state, same BC kinds; "
-                      "%.1f s" % (steps, CPU_CROP[0], CPU_CROP[1], el)}
-
-
 def _snapshot_leg(U0, aux0, geom, theta, steps, every, torch):
     """HDF5 snapshots of the headline state (H5Saver layout, gzip level 4):
     `steps` FP64 steps with an asynchronous snapshot every `every` steps vs
@@ -202,61 +411,6 @@ def _snapshot_leg(U0, aux0, geom, theta, steps, every, torch):
                        "every %d steps with H5Saver.save_async (pinned D2H on a side stream, "
                        "gzip on 16 host threads)" % (steps, every))
     return out
-
-
-def _fields(U, aux, inv_M):
-    """rho, p, T, Y_k (float64 torch) of a state."""
-    import torch
-    U = U.double()
-    aux = aux.double()
-    rho = U[0]
-    Y = U[4:] / rho
-    Y9 = torch.cat([Y, (1.0 - Y.sum(0))[None]])
-    R = (inv_M[:, None, None] * Y9).sum(0)
-    return {"rho": rho, "p": rho * R * aux[1], "T": aux[1], "Y": Y}
-
-
-def _accuracy(mech, steps, every, torch):
-    """Mixed vs FP64 step on the 8000 x 500 tube at one fixed dt (the initial
-    CFL-0.3 dt): relative L2 and Linf of rho, p, T, Y_k every `every` steps."""
-    from janc_amd import cases
-    from janc_amd.solver import Simulator
-    U3, aux3, g3 = cases.mixed_tube(mech)
-    probe = Simulator(cases.simulation_config(g3, cfl=0.3))
-    dt = float(probe.CFL(U3, aux3, 0.3))
-    del probe
-    inv_M = torch.as_tensor(1.0 / mech.species_M).cuda()
-    sims, states = {}, {}
-    for prec, dtype in (("mixed", torch.float32), ("fp64", torch.float64)):
-        sims[prec] = Simulator(cases.simulation_config(g3, dt=dt, precision=prec))
-        states[prec] = (torch.as_tensor(U3, dtype=dtype).cuda(),
-                        torch.as_tensor(aux3, dtype=dtype).cuda())
-    report = []
-    done = 0
-    while done < steps:
-        n = min(every, steps - done)
-        for prec in sims:
-            sims[prec].advance(*states[prec], n)
-        done += n
-        fm = _fields(*states["mixed"], inv_M)
-        fd = _fields(*states["fp64"], inv_M)
-        row = {"step": done}
-        for k in ("rho", "p", "T"):
-            d = (fm[k] - fd[k])
-            row[k] = {"L2": float(d.norm() / fd[k].norm()),
-                      "Linf": float(d.abs().max() / fd[k].abs().max())}
-        ys = {}
-        for i, name in enumerate(mech.species[:-1]):
-            d = fm["Y"][i] - fd["Y"][i]
-            ys[name] = {"L2": float(d.norm() / max(float(fd["Y"][i].norm()), 1e-300)),
-                        "Linf": float(d.abs().max() / max(float(fd["Y"][i].abs().max()), 1e-300)),
-                        "max": float(fd["Y"][i].abs().max())}
-        row["Y"] = ys
-        report.append(row)
-    flags = {p: sims[p].health(states[p][0]) for p in sims}
-    return {"workload": "8000x500 tube (dx 1.25e-5 m), fixed dt %.4g s, mixed vs FP64 GPU "
-                        "step (the FP64 step is parity-checked against the oracle)" % dt,
-            "dt": dt, "steps": steps, "health_flags": flags, "every": report}
 
 
 def _batch_leg(mech, steps, tune, torch):
@@ -435,19 +589,31 @@ def _adjoint_leg(mech, steps, every, ndir, torch):
                                   if ndir else None)}
 
 
+
+
+def _summary_adjoint(adj):
+    keys = ("adjoint_over_fwd_step", "adjoint_over_fwd_rollout", "step_fwd_ms", "step_vjp_ms",
+            "fd_directions", "fd_max_rel_err", "fd_median_rel_err")
+    return {k: adj[k] for k in keys if k in adj}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--mixed-steps", type=int, default=100)
-    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--accuracy-steps", type=int, default=1000)
+    ap.add_argument("--crop-steps", type=int, default=100,
+                    help="steps of the mixed-vs-oracle comparison on the tube crop")
+    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="CPU-baseline processes (default: min(15, usable cores))")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     ap.add_argument("--tile", type=int, default=None,
-                    help="force the headline stage-kernel variant (skips its autotuning)")
+                    help="force the stage-kernel variant of the tube steps (no autotuning)")
+    ap.add_argument("--annulus-steps", type=int, default=100)
+    ap.add_argument("--no-annulus", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=100)
@@ -457,150 +623,206 @@ def main():
                     help="one 256 MiB device-to-device copy before the timed region "
                          "(reference dispatch for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--headline-only", action="store_true",
-                    help="only the FP64 headline workload (for profiler runs)")
-    ap.add_argument("--flux", choices=list(SCHEMES), default="lf_weno",
-                    help="flux scheme of the headline workload (default: the reference's "
-                         "LF flux splitting with WENO5_JS)")
+                    help="only the mixed and FP64 steps on the tube (for profiler runs)")
+    ap.add_argument("--annulus-only", action="store_true",
+                    help="only the FP64 annulus step (for profiler runs)")
     ap.add_argument("--scheme-steps", type=int, default=50)
     ap.add_argument("--snapshot-steps", type=int, default=600)
     ap.add_argument("--no-snapshots", action="store_true")
     ap.add_argument("--no-schemes", action="store_true")
     args = ap.parse_args()
-    if args.headline_only:
-        args.no_mixed = args.no_cpu = args.no_batch = args.no_adjoint = True
-        args.no_schemes = args.no_snapshots = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: the single-block step runs on one GPU; --gpus N > 1 is not "
               "supported (the x-slab decomposition is not built)", file=sys.stderr)
         sys.exit(2)
+    only = args.headline_only or args.annulus_only
+    if only:
+        args.no_cpu = args.no_batch = args.no_adjoint = True
+        args.no_schemes = args.no_snapshots = True
+        args.accuracy_steps = args.crop_steps = 0
+        args.no_annulus = args.headline_only
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    import numpy as np
-    import torch
     from janc_amd import cases
     from janc_amd.mechanism import Mechanism
-    from janc_amd.solver import Simulator
 
     mech = Mechanism()
-    # ---- headline: FP64 on the 4000 x 400 annulus ----
-    U0, aux0, geom, theta = cases.rdc_annulus(mech)
-    nx, ny = U0.shape[1], U0.shape[2]
-    sim = Simulator(cases.simulation_config(geom, cfl=0.3, precision="fp64", theta=theta,
-                                            flux=SCHEMES[args.flux]))
-    U = torch.as_tensor(U0).cuda()
-    aux = torch.as_tensor(aux0).cuda()
-    if args.flux != "lf_weno":
-        tuned = None
-    elif args.tile is not None:
-        sim.set_tile(U, args.tile, theta)
-        tuned = {"variant": args.tile, "forced": True}
-    else:
-        tuned = sim.autotune(U, aux, theta=theta) if not args.no_tune else None
+    U3, aux3, g3 = cases.mixed_tube(mech)
+    nx, ny = U3.shape[1], U3.shape[2]
+    cells = nx * ny
+
+    # ---- CPU work: worker processes forked before the GPU is touched ----
+    pool = None
+    cpu = None
+    nproc = 0
+    if not args.no_cpu:
+        import multiprocessing as mp
+        _CPU.update(U=U3, aux=aux3, dx=g3["dx"])
+        nproc = args.cpu_procs or min(15, len(os.sched_getaffinity(0)))
+        pool = mp.get_context("fork").Pool(nproc)
+    try:
+        if pool is not None:
+            cpu = _cpu_baseline(pool, nproc, args.cpu_steps)
+            emit("cpu_baseline", cpu)
+        _gpu_legs(args, mech, U3, aux3, g3, cells, pool, cpu, only)
+    finally:
+        if pool is not None:
+            pool.close()
+            pool.join()
+
+
+def _gpu_legs(args, mech, U3, aux3, g3, cells, pool, cpu, only):
+    import torch
+    from janc_amd.solver import Simulator
+    tune = not args.no_tune
     if args.calibrate_copy:
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    el, kt, dts, flags = _run(sim, U, aux, args.steps, args.warmup, torch)
-    cells = nx * ny
-    value = cells * args.steps / el
-    roof = _roofline(kt, cells, 8)
-    dom = max((k for k in roof), key=lambda k: kt[k][0])
+
+    # ---- the oracle's crop run for the mixed form's accuracy (cpu_baseline
+    # leg: the oracle is the checker) starts now, beside the GPU legs ----
+    crop_job = None
+    dt_fixed = None
+    if pool is not None and args.crop_steps > 0:
+        from janc_amd import cases
+        probe = Simulator(cases.simulation_config(g3, cfl=0.3))
+        dt_fixed = float(probe.CFL(U3, aux3, 0.3))
+        del probe
+        every = max(1, args.crop_steps // 5)
+        crop_job = pool.apply_async(_oracle_crop_worker,
+                                    ((ORACLE_CROP[0], ORACLE_CROP[1], dt_fixed,
+                                      args.crop_steps, every),))
+
+    summary = {}
+    if args.annulus_only:
+        ann, _ = _annulus_leg(mech, args.steps, args.warmup, tune, torch)
+        emit("fp64_annulus", ann)
+        print(json.dumps({"metric": METRIC, "value": ann["value"], "unit": "cell-updates/s",
+                          "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": ann["ms_per_step"], "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                          "data": "synthetic",
+                          "config": {"workload": ann["workload"], "cells": ann["cells"]}}),
+              flush=True)
+        return
+
+    # ---- headline: mixed and FP64 on the 8000 x 500 tube ----
+    head = _headline(mech, U3, aux3, g3, args.steps, args.warmup, tune, torch, args.tile)
+    mixed, fp64 = head["mixed"], head["fp64"]
+    emit("headline_tube", {k: {kk: vv for kk, vv in v.items() if kk != "kt"}
+                           for k, v in head.items()})
+    roof = mixed["roofline"]
+    kt = mixed["kt"]
+    dom = max(roof, key=lambda k: kt[k][0])
     r = roof[dom]
-    result = {
+    tr = STAGE_TRAFFIC_B_PER_CELL["mixed"] if dom == "stage" else None
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": r["achieved_GBs"],
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["frac_hbm"],
+                "traffic": tr * cells if tr else None,
+                "algorithmic_bytes": r["algorithmic_bytes"], "avg_ms": r["avg_ms"]}
+    chem_fv = None
+    if "chem" in kt and kt["chem"][1]:
+        ms, nl = kt["chem"]
+        tfs = CHEM_FP64_FLOP_PER_CELL * cells / (ms / nl * 1e-3) / 1e12
+        chem_fv = {"bound": "fp64-valu", "achieved": tfs, "peak": FP64_VECTOR_PEAK_TFLOPS,
+                   "unit": "TFLOP/s", "frac": tfs / FP64_VECTOR_PEAK_TFLOPS,
+                   "avg_ms": ms / nl, "flop_per_cell": CHEM_FP64_FLOP_PER_CELL}
+
+    # ---- mixed accuracy: 1000 steps against the FP64 step ----
+    acc = None
+    if args.accuracy_steps > 0:
+        acc = _accuracy(mech, U3, aux3, g3, args.accuracy_steps, 100, torch)
+        emit("mixed_accuracy", acc)
+        torch.cuda.empty_cache()
+    crop_gpu = None
+    if crop_job is not None:
+        every = max(1, args.crop_steps // 5)
+        crop_gpu, crop_flags = _accuracy_crop_gpu(mech, U3, aux3, g3, dt_fixed,
+                                                  args.crop_steps, every, torch)
+
+    ann = None
+    if not args.no_annulus:
+        ann, ann_state = _annulus_leg(mech, args.annulus_steps, args.warmup, tune, torch)
+        emit("fp64_annulus", ann)
+        if not args.no_schemes:
+            emit("schemes_fp64_annulus", _schemes_leg(*ann_state, args.scheme_steps,
+                                                      args.warmup, tune, torch))
+        if not args.no_snapshots:
+            emit("snapshots", _snapshot_leg(*ann_state, args.snapshot_steps, 200, torch))
+            torch.cuda.empty_cache()
+    bat = None
+    if not args.no_batch:
+        bat = _batch_leg(mech, args.batch_steps, tune, torch)
+        emit("batch", bat)
+        torch.cuda.empty_cache()
+    adj = None
+    if not args.no_adjoint:
+        adj = _adjoint_leg(mech, args.adjoint_steps, 10, args.fd_directions, torch)
+        emit("adjoint", adj)
+        torch.cuda.empty_cache()
+    crop = None
+    if crop_job is not None:
+        states, el = crop_job.get()
+        crop = _accuracy_crop_compare(mech, crop_gpu, states, args.crop_steps, dt_fixed, el)
+        crop["health_flags"] = crop_flags
+        emit("mixed_vs_oracle_crop", crop)
+
+    # ---- summary (last line) ----
+    summary.update({
         "metric": METRIC,
-        "value": value,
+        "value": mixed["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": el / args.steps * 1e3,
+        "ms_per_step": mixed["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "FP64 plain step, 4000x400 periodic-in-x detonation annulus "
-                               "(ZND-like seed 200 columns 30 atm/3000 K, 0.5% T noise), "
-                               "9sp-19r H2-air, CFL 0.3; bottom injection inlet (0.2 MPa / "
-                               "300 K manifold), top pressure outlet 1 atm",
-                   "flux_config": SCHEMES[args.flux],
-                   "nx": nx, "ny": ny, "cells": cells, "precision": "fp64",
-                   "dt_first": float(dts[0]), "dt_last": float(dts[-1]),
-                   "health_flags": flags},
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": r["achieved_GBs"],
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["frac_hbm"],
-                     "traffic": (STAGE_TRAFFIC_B_PER_CELL_FP64 * cells
-                                 if dom == "stage" and args.flux == "lf_weno" else None),
-                     "algorithmic_bytes": r["algorithmic_bytes"]},
-        "kernels_fp64": {k: {"ms_total": v[0], "launches": v[1]} for k, v in kt.items()},
-        "kernel_timing": ("HIP events on the context stream around the launches of every "
-                          "%dth step of the timed region" % KERNEL_TIMING_EVERY),
-        "autotune_fp64": tuned,
-        "roofline_all_fp64": roof,
-    }
-    if args.flux == "lf_weno":
-        fv = {}
-        for kind, fpc in FP64_FLOP_PER_CELL.items():
-            ms, nl = kt.get(kind, (0.0, 0))
-            if nl:
-                tfs = fpc * cells / (ms / nl * 1e-3) / 1e12
-                fv[kind] = {"flop_per_cell": fpc, "avg_ms": ms / nl, "achieved_TFLOPs": tfs,
-                            "peak_TFLOPs": FP64_VECTOR_PEAK_TFLOPS,
-                            "frac": tfs / FP64_VECTOR_PEAK_TFLOPS}
-        result["fp64_vector"] = fv
-    del U, aux
-    torch.cuda.empty_cache()
-
-    # ---- the other flux schemes on the headline workload ----
-    if not args.no_schemes:
-        result["schemes_fp64"] = _schemes_leg(U0, aux0, geom, theta, args.scheme_steps,
-                                              args.warmup, not args.no_tune, torch)
-
-    # ---- HDF5 snapshots of the headline state ----
-    if not args.no_snapshots:
-        result["snapshots"] = _snapshot_leg(U0, aux0, geom, theta, args.snapshot_steps, 200,
-                                            torch)
-        torch.cuda.empty_cache()
-
-    # ---- mixed vs FP64 on the 8000 x 500 tube ----
-    if not args.no_mixed:
-        U3, aux3, g3 = cases.mixed_tube(mech)
-        n3 = U3.shape[1] * U3.shape[2]
-        out = {}
-        for prec, dtype in (("mixed", torch.float32), ("fp64", torch.float64)):
-            s3 = Simulator(cases.simulation_config(g3, cfl=0.3, precision=prec))
-            Ut = torch.as_tensor(U3, dtype=dtype).cuda()
-            at = torch.as_tensor(aux3, dtype=dtype).cuda()
-            t3 = s3.autotune(Ut, at) if not args.no_tune else None
-            e3, k3, d3, f3 = _run(s3, Ut, at, args.mixed_steps, args.warmup, torch)
-            out[prec] = {"value": n3 * args.mixed_steps / e3, "ms_per_step": e3 / args.mixed_steps * 1e3,
-                         "kernels": {k: {"ms_total": v[0], "launches": v[1]} for k, v in k3.items()},
-                         "roofline": _roofline(k3, n3, 4 if prec == "mixed" else 8),
-                         "health_flags": f3, "autotune": t3}
-            del Ut, at
-            torch.cuda.empty_cache()
-        out["speedup_mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
-        out["workload"] = ("8000x500 tube (dx 1.25e-5 m), phi in [0.8,1.2] smooth field, "
-                           "40-column 20 atm/2800 K ignition slab, %d CFL steps"
-                           % args.mixed_steps)
-        result["mixed_tube"] = out
-
-    if not args.no_mixed and args.accuracy_steps > 0:
-        result["mixed_accuracy"] = _accuracy(mech, args.accuracy_steps, 100, torch)
-
-    if not args.no_batch:
-        result["batch"] = _batch_leg(mech, args.batch_steps, not args.no_tune, torch)
-        torch.cuda.empty_cache()
-    if not args.no_adjoint:
-        result["adjoint"] = _adjoint_leg(mech, args.adjoint_steps, 10, args.fd_directions, torch)
-        torch.cuda.empty_cache()
-
-    if not args.no_cpu:
-        result["cpu_baseline"] = _cpu_baseline(U0, aux0, geom, theta, args.cpu_steps)
-        result["speedup_vs_cpu"] = value / result["cpu_baseline"]["value"]
-    print(json.dumps(result), flush=True)
+        "config": {"workload": "mixed-precision step (FP32 state, split fluxes, WENO5; FP64 "
+                               "divergence, RK combine, chemistry, T solve), 8000x500 tube, "
+                               "dx 1.25e-5 m, phi in [0.8,1.2] smooth field, 40-column 20 atm/"
+                               "2800 K ignition slab, slip/zero-gradient/slip/slip, CFL 0.3, "
+                               "9sp-19r H2-air, LF + WENO5_JS",
+                   "nx": U3.shape[1], "ny": U3.shape[2], "cells": cells,
+                   "precision": "mixed"},
+        "fp64_value": fp64["value"],
+        "fp64_ms_per_step": fp64["ms_per_step"],
+        "speedup_mixed_over_fp64": mixed["value"] / fp64["value"],
+        "kernels_mixed_ms": {k: v["ms_total"] / v["launches"]
+                             for k, v in mixed["kernels"].items() if v["launches"]},
+        "kernels_fp64_ms": {k: v["ms_total"] / v["launches"]
+                            for k, v in fp64["kernels"].items() if v["launches"]},
+        "health_flags": {"mixed": mixed["health_flags"], "fp64": fp64["health_flags"]},
+        "roofline": roofline,
+        "roofline_chem": chem_fv,
+    })
+    if acc is not None:
+        summary["mixed_err_vs_fp64_gpu"] = dict(_worst(acc["every"][-1]),
+                                                steps=acc["every"][-1]["step"])
+    if crop is not None:
+        summary["mixed_err_vs_oracle_crop"] = dict(_worst(crop["every"][-1]),
+                                                   steps=crop["every"][-1]["step"],
+                                                   crop="%dx%d" % ORACLE_CROP)
+    if adj is not None:
+        summary["adjoint"] = _summary_adjoint(adj)
+        summary["adjoint_over_fwd_step"] = adj["adjoint_over_fwd_step"]
+    if bat is not None:
+        summary["batch"] = {"value": bat["value"], "speedup_vs_sequential":
+                            bat["speedup_vs_sequential"],
+                            "bit_identical_to_sequential": bat["bit_identical_to_sequential"]}
+    if ann is not None:
+        summary["fp64_annulus_4000x400"] = {"value": ann["value"],
+                                            "ms_per_step": ann["ms_per_step"]}
+    if cpu is not None:
+        summary["cpu_baseline"] = cpu
+        summary["speedup_vs_cpu"] = mixed["value"] / cpu["value"]
+    print(json.dumps(summary), flush=True)
 
 
 if __name__ == "__main__":

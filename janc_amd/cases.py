@@ -188,11 +188,21 @@ def smooth_field(rng, nx, ny, modes=4, kmax=3):
     return 2.0 * s / span - 1.0 if span > 0 else s
 
 
-def mixed_tube(mech, nx=8000, ny=500, dx=TUBE_DX, seed=3):
+def mixed_tube(mech, nx=8000, ny=500, dx=TUBE_DX, seed=3, crop=None):
     """Config 3: phi from a seeded smooth field in [0.8, 1.2]; a 40-column
-    ignition slab at 20 atm / 2800 K; BCs as config 1."""
+    ignition slab at 20 atm / 2800 K; BCs as config 1.
+
+    ``crop=(cx, cy)`` returns the state of cells [0, cx) x [0, cy) of the
+    nx x ny tube, equal to cropping the full state (every value is a
+    per-cell function of the full grid's phi field) without building it."""
     rng = np.random.default_rng(seed)
     phi = 1.0 + 0.2 * smooth_field(rng, nx, ny)
+    if crop is not None:
+        cx, cy = crop
+        if not (0 < cx <= nx and 0 < cy <= ny):
+            raise ValueError("crop must lie inside the %dx%d tube" % (nx, ny))
+        nx, ny = cx, cy
+        phi = phi[:cx, :cy]
     return detonation_tube(mech, nx=nx, ny=ny, dx=dx, seed=seed, driver=40,
                            p_drv=20 * P_ATM, T_drv=2800.0, phi=phi, rho_noise=0.0)
 

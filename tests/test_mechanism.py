@@ -62,3 +62,12 @@ def test_config_errors_follow_reference(tmp_path):
         Mechanism(str(tmp_path / "mech.cti"))
     with pytest.raises(FileNotFoundError):
         Mechanism(str(tmp_path / "missing.yaml"))
+
+
+def test_mixed_tube_crop_equals_cropped_state(mech_path):
+    from janc_amd import cases
+    mech = Mechanism(mech_path)
+    U, aux, g = cases.mixed_tube(mech, nx=300, ny=60)
+    Uc, ac, gc = cases.mixed_tube(mech, nx=300, ny=60, crop=(90, 25))
+    assert np.array_equal(Uc, U[:, :90, :25]) and np.array_equal(ac, aux[:, :90, :25])
+    assert gc == g
