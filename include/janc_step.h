@@ -113,6 +113,22 @@ int jh_ctx_set_flux(jh_ctx* ctx, int solver_type, int split_method, int reconstr
  * 512 threads). */
 int jh_ctx_set_tile(jh_ctx* ctx, int variant);
 int jh_ctx_get_tile(const jh_ctx* ctx);
+
+/* Stop rule of the Halley temperature solve after every RK stage, after the
+ * chemistry and in jh_update_aux (get_T_nasa7, model/thermo_model.py:
+ * 135-153).  JH_HALLEY_GRID is the reference's: every cell makes N* updates,
+ * N* = the first iteration at which max|res| over the grid is <= 5e-9
+ * (at most 5); each cell's own count and its state after one more update are
+ * kept, and a fix-up pass applies N* (one extra light launch per solve; the
+ * first call allocates two planes and a byte per cell).  JH_HALLEY_PER_CELL
+ * (default) stops each cell on its own residual: equal to the grid-wide stop
+ * to rounding in SI units (a converged cell's further updates move T by
+ * ~1e-16 relative), ~1e-9 apart with nondimensional energy scales, where the
+ * absolute tolerance is a larger fraction of e.  The batch API stops per
+ * cell. */
+enum jh_halley_stop { JH_HALLEY_PER_CELL = 0, JH_HALLEY_GRID = 1 };
+int jh_ctx_set_halley_stop(jh_ctx* ctx, int mode);
+int jh_ctx_get_halley_stop(const jh_ctx* ctx);
 int jh_tile_variants(void);
 int jh_tile_info(int variant, int* rows, int* threads, int* waves);
 const char* jh_last_error(const jh_ctx* ctx);
@@ -218,7 +234,20 @@ int jh_timing_enable(jh_ctx* ctx, int every);
 int jh_timing_read(jh_ctx* ctx, double* ms, int* counts);
 
 /* Health flags accumulated on the device since the last call (JH_FLAG_*
- * bits: 1 = non-finite value, 2 = rho <= 0); synchronises the stream. */
+ * bits); synchronises the stream.  The reference has no run-time error
+ * channel (NaNs propagate, the temperature solve and the chemistry solve
+ * stop or return silently). */
+enum jh_health_flag {
+  JH_HEALTH_NONFINITE = 1,        /* a non-finite (gamma, T) or state value */
+  JH_HEALTH_NEGATIVE_RHO = 2,     /* rho <= 0 in an updated cell */
+  JH_HEALTH_HALLEY_NOCONV = 4,    /* a temperature solve ended at its 5-iteration cap
+                                     with |res| > 5e-9 (thermo_model.py:140-152) */
+  JH_HEALTH_SINGULAR_PIVOT = 8,   /* a zero or non-finite pivot in a chemistry solve
+                                     (reaction_model.py:157) */
+  JH_HEALTH_HALLEY_REGRESS = 16,  /* JH_HALLEY_GRID: a converged cell's next update left
+                                     the tolerance (the fix-up assumes it does not) */
+  JH_HEALTH_NONPOSITIVE_T = 32    /* T <= 0 after a temperature solve */
+};
 int jh_health(jh_ctx* ctx, unsigned int* flags);
 
 /* ---------------------------------------------------------------- batches

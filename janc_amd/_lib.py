@@ -20,6 +20,16 @@ SPLIT_METHODS = {"LF": 0, "SW": 1}
 RECONSTRUCTIONS = {"WENO5_JS": 0, "MUSCL": 1}
 RIEMANN_SOLVERS = {"HLLC": 0}
 KERNEL_KINDS = ("reduce", "bc", "stage", "chem")
+# temperature-solve stop (enum jh_halley_stop)
+HALLEY_STOP = {"per_cell": 0, "grid": 1}
+# health flag bits (enum jh_health_flag)
+HEALTH_FLAGS = {"nonfinite": 1, "negative_rho": 2, "halley_noconv": 4, "singular_pivot": 8,
+                "halley_regress": 16, "nonpositive_T": 32}
+
+
+def health_names(flags):
+    """Names of the health flag bits set in ``flags``."""
+    return [k for k, b in HEALTH_FLAGS.items() if flags & b]
 
 _c_int, _c_double, _vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
 # name -> (restype, argtypes); every symbol the public header declares
@@ -37,6 +47,8 @@ SIGNATURES = {
     "jh_ctx_set_tile": (_c_int, [_vp, _c_int]),
     "jh_ctx_set_flux": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int]),
     "jh_ctx_get_tile": (_c_int, [_vp]),
+    "jh_ctx_set_halley_stop": (_c_int, [_vp, _c_int]),
+    "jh_ctx_get_halley_stop": (_c_int, [_vp]),
     "jh_tile_variants": (_c_int, []),
     "jh_tile_info": (_c_int, [_c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),
                               ctypes.POINTER(_c_int)]),

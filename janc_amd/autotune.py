@@ -27,10 +27,24 @@ def variants():
 
 
 def _key(sim, U):
+    """Cache key: GPU, precision, grid, flux scheme and boundary kinds (the
+    boundary kernel's cost and the stage kernel's halo sources differ with
+    them)."""
     import torch
     dev = torch.cuda.get_device_name(U.device)
-    return "%s|%s|%dx%d" % (dev, "fp64" if sim.precision == _lib.PREC_FP64 else "mixed",
-                           U.shape[1], U.shape[2])
+    return "%s|%s|%dx%d|flux=%s|bc=%s" % (
+        dev, "fp64" if sim.precision == _lib.PREC_FP64 else "mixed", U.shape[1], U.shape[2],
+        ",".join(str(v) for v in sim.flux), ",".join(str(k) for k in sim.bc_kinds))
+
+
+def tunable(sim):
+    """Whether the context's stage kernel has variants: the tile variant
+    selects the LF flux-splitting kernel (WENO5_JS or MUSCL) with the
+    per-cell temperature-solve stop; the Godunov and Steger-Warming kernels
+    and the grid-wide-stop kernel have one tile shape each."""
+    solver, split = sim.flux[0], sim.flux[1]
+    return (solver == _lib.SOLVER_TYPES["flux_splitting"] and split == _lib.SPLIT_METHODS["LF"]
+            and sim.halley_stop == "per_cell")
 
 
 def _load_file():
@@ -55,6 +69,9 @@ def tune(sim, U, aux, steps=8, theta=None, force=False):
     The caller's U and aux are not modified."""
     import torch
     U, aux = sim._state(U, aux)
+    if not tunable(sim):
+        return {"variant": sim.get_tile(U, theta), "tuned": False,
+                "reason": "this flux path / temperature-solve stop has one tile shape"}
     key = _key(sim, U)
     table = dict(_load_file())
     table.update(_MEM)

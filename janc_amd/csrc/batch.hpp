@@ -122,11 +122,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
   const int k = find_case(off, ncase, blockIdx.x);
   const CaseDesc<Real>& cd = d[k];
   const StepArgs<Real> a = case_args(cd, parity, dt_fixed, cfl);
+  __shared__ unsigned long long redg[kGridSlotsChem * 4];
   __shared__ MixTables mix;
   __shared__ GibbsTables gt;
   chem_body<Real, CHEM_STEP>(a, cd.U, cd.aux, nullptr,
                              dt_hist ? dt_hist + (size_t)k * hist_len + s : nullptr,
-                             blockIdx.x - off[k], red, redb, mix, gt);
+                             blockIdx.x - off[k], red, redb, redg, mix, gt);
 }
 
 }  // namespace jh

@@ -42,8 +42,11 @@ __host__ __device__ __forceinline__ void gibbs(const Tab& c, double T, double in
 // Solve A x = b in place (b <- x) by LU with partial pivoting, the
 // factorisation jnp.linalg.solve uses (LAPACK getrf/getrs).  Row swaps are
 // predicated selects so every index stays compile-time and A lives in VGPRs.
+// Returns false when a pivot is zero or not finite (getrf's info > 0; the
+// reference's solve returns non-finite values then, and so does this one).
 template <int N>
-__host__ __device__ __forceinline__ void lu_solve(double (&A)[N][N], double (&b)[N]) {
+__host__ __device__ __forceinline__ bool lu_solve(double (&A)[N][N], double (&b)[N]) {
+  bool ok = true;
 #pragma unroll
   for (int col = 0; col < N; ++col) {
     int p = col;
@@ -77,6 +80,7 @@ __host__ __device__ __forceinline__ void lu_solve(double (&A)[N][N], double (&b)
         b[r] = s ? t : b[r];
       }
     }
+    ok = ok && (amax > 0.0) && (amax < INFINITY);
     const double inv = rcp(A[col][col]);
 #pragma unroll
     for (int r = col + 1; r < N; ++r) {
@@ -93,13 +97,15 @@ __host__ __device__ __forceinline__ void lu_solve(double (&A)[N][N], double (&b)
     for (int j = r + 1; j < N; ++j) s -= A[r][j] * b[j];
     b[r] = s * rcp(A[r][r]);
   }
+  return ok;
 }
 
 // Increment of rho*Y_k for the active species (detailed_reaction, :183-193).
 // rhoY_in: the state's rho*Y_k for the NS-1 transported species; gt: the
-// Gibbs tables (c itself, or its copy in LDS).
+// Gibbs tables (c itself, or its copy in LDS).  Returns lu_solve's pivot
+// check.
 template <typename GTab>
-__host__ __device__ __forceinline__ void chem_increment(const MechConst& c, const GTab& gt,
+__host__ __device__ __forceinline__ bool chem_increment(const MechConst& c, const GTab& gt,
                                                         double rho,
                                                         const double (&rhoY_in)[JH_NS - 1],
                                                         double T, double dt,
@@ -138,20 +144,21 @@ __host__ __device__ __forceinline__ void chem_increment(const MechConst& c, cons
 #pragma unroll
     for (int n = 0; n < JH_NA; ++n) J[k][n] = ((k == n) ? idt : 0.0) - J[k][n];
   }
-  lu_solve<JH_NA>(J, w);
+  const bool ok = lu_solve<JH_NA>(J, w);
 #pragma unroll
   for (int k = 0; k < JH_NA; ++k) {
     double dY = w[k] * irho;
     dY = fmin(fmax(dY, -Y[k]), 1.0 - Y[k]);
     drhoY[k] = rho * dY;
   }
+  return ok;
 }
 
-__host__ __device__ __forceinline__ void chem_increment(const MechConst& c, double rho,
+__host__ __device__ __forceinline__ bool chem_increment(const MechConst& c, double rho,
                                                         const double (&rhoY_in)[JH_NS - 1],
                                                         double T, double dt,
                                                         double (&drhoY)[JH_NA]) {
-  chem_increment(c, c, rho, rhoY_in, T, dt, drhoY);
+  return chem_increment(c, c, rho, rhoY_in, T, dt, drhoY);
 }
 
 }  // namespace jh

@@ -190,25 +190,54 @@ __device__ __forceinline__ void godunov_stage_body(const StepArgs<Real>& a, int 
     }
     if (own) Uo[cv * n + o] = (Real)un[cv];
   }
-  if (MODE == MODE_RHS || !own) return;
+  if (MODE == MODE_RHS) return;
 
   // ---------------- update_aux (aux_func.py:21-29) of the new state
   // the Halley solve sees the stored (rounded) state, as the next stage does
-  double us[NU];
+  unsigned long long nb = 0ull;     // grid-wide stop: this lane's update count
+  unsigned int fl = 0u;
+  if (own) {
+    double us[NU];
 #pragma unroll
-  for (int cv = 0; cv < NU; ++cv) us[cv] = (double)(Real)un[cv];
-  const double rho = us[0], ir = rcp(rho);
-  const double u = us[1] * ir, v = us[2] * ir;
-  const double e = us[3] * ir - 0.5 * (u * u + v * v);
-  double Y[JH_NS];
+    for (int cv = 0; cv < NU; ++cv) us[cv] = (double)(Real)un[cv];
+    const double rho = us[0], ir = rcp(rho);
+    const double u = us[1] * ir, v = us[2] * ir;
+    const double e = us[3] * ir - 0.5 * (u * u + v * v);
+    double Y[JH_NS];
 #pragma unroll
-  for (int k = 0; k < JH_NS - 1; ++k) Y[k] = us[4 + k] * ir;
-  const double R = fill_Y_get_R(c, Y);
-  double T = (double)auxs[n + o];
-  double gamma;
-  halley_T(L.mix, Y, R, e, T, gamma);
-  auxo[o] = (Real)gamma;
-  auxo[n + o] = (Real)T;
+    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = us[4 + k] * ir;
+    const double R = fill_Y_get_R(c, Y);
+    double T = (double)auxs[n + o];
+    double gamma;
+    bool conv;
+    if (a.hside) {
+      HalleyMore hm;
+      conv = halley_T<MixTables, true>(L.mix, Y, R, e, T, gamma, &hm);
+      a.hside[o] = (Real)hm.g1;
+      a.hside[n + o] = (Real)hm.T1;
+      a.hnit[o] = (unsigned char)hm.n;
+      if (hm.n < kHalleyMaxIter && !hm.ok1) fl |= JH_FLAG_HALLEY_REGRESS;
+      nb = (unsigned long long)hm.n;
+    } else {
+      conv = halley_T(L.mix, Y, R, e, T, gamma);
+    }
+    auxo[o] = (Real)gamma;
+    auxo[n + o] = (Real)T;
+    fl |= halley_flags(rho, T, gamma, conv);
+  }
+  // every lane of the wave reaches here (waves w >= TX left above)
+  raise_flags(a.sc, fl);
+  if (a.hside) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long x = __shfl_xor(nb, off, 64);
+      nb = x > nb ? x : nb;
+    }
+    unsigned long long* dst = &a.sc->gs[a.parity][stage].nstar;
+    if ((threadIdx.x & 63) == 0 &&
+        nb > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(dst, nb);
+  }
 }
 
 template <typename Real, int MODE, int TX, int RECON>

@@ -37,6 +37,11 @@ struct jh_ctx {
   int solver = JH_SOLVER_FLUX_SPLITTING;   // jh_ctx_set_flux
   int split = JH_SPLIT_LF;
   int recon = JH_RECON_WENO5_JS;
+  // temperature-solve stop (jh_ctx_set_halley_stop); the grid-wide form's
+  // per-cell buffers: (gamma, T) after one more update, update counts
+  int halley = JH_HALLEY_PER_CELL;
+  void* hside = nullptr;
+  unsigned char* hnit = nullptr;
   std::string err;
   // kernel timing with HIP events on the context stream (jh_timing_*)
   bool timing = false;             // record events around the current launches
@@ -124,7 +129,25 @@ StepArgs<Real> make_args(jh_ctx* c, double dt_fixed, double cfl) {
   }
   a.inj_T0 = c->inj_T0;
   for (int k = 0; k < JH_NS - 1; ++k) a.inj_Y[k] = c->inj_Y[k];
+  const bool grid = c->halley == JH_HALLEY_GRID;
+  a.hside = grid ? static_cast<Real*>(c->hside) : nullptr;
+  a.hnit = grid ? c->hnit : nullptr;
   return a;
+}
+
+// Grid-wide temperature-solve stop: after a producer of (gamma, T) written
+// to aux, apply N* and fold the producer's maxima for N* into the LF slot
+// (lf_parity, lf_stage) (lf_stage < 0: none) and, with with_cfl, the CFL
+// slot of lf_parity.  No launch in the per-cell form.
+template <typename Real>
+void halley_fix(jh_ctx* c, int prod, int lf_parity, int lf_stage, int with_cfl, Real* aux) {
+  if (c->halley != JH_HALLEY_GRID) return;
+  const int n = c->nx * c->ny;
+  const int nb = std::min((n + 255) / 256, 2048);
+  halley_fix_kernel<Real><<<nb, 256, 0, c->stream>>>(c->sc, c->parity, prod, lf_parity, lf_stage,
+                                                     with_cfl, aux,
+                                                     static_cast<const Real*>(c->hside), c->hnit,
+                                                     n);
 }
 
 int grid_cells(const jh_ctx* c) {
@@ -166,9 +189,20 @@ constexpr int kTileVariants = 5;
 constexpr int kTileRows[kTileVariants] = {8, 8, 16, 16, 32};
 constexpr int kTileWaves[kTileVariants] = {3, 4, 4, 5, 4};
 
+// With the grid-wide temperature-solve stop the flux-splitting stage kernel
+// runs one tile shape (its epilogue carries the per-candidate maxima): 16
+// rows at a 4-wave register budget, whatever the context's variant.
+constexpr int kGridRows = 16;
+
 template <typename Real, int MODE, int RECON>
 void launch_split_stage(jh_ctx* c, const StepArgs<Real>& a, int stage, const Real* Us,
                         const Real* As, const Real* Un, Real* Uo, Real* Ao) {
+  if (MODE == MODE_RK && c->halley == JH_HALLEY_GRID) {
+    const int nbg = ((c->nx + kGridRows - 1) / kGridRows) * ((c->ny + TY - 1) / TY);
+    stage_kernel<Real, MODE, kGridRows, 4, RECON, SPLIT_LF, Real, true>
+        <<<nbg, Tile<kGridRows>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+    return;
+  }
   const int nb = stage_blocks(c);
   switch (c->tile) {
 #define JH_LAUNCH(V, TX, WPE)                                                          \
@@ -195,12 +229,22 @@ void launch_stage(jh_ctx* c, const StepArgs<Real>& a, int stage, const Real* Us,
       godunov_stage_kernel<Real, MODE, kGodRowsWeno, RECON_WENO5>
           <<<nb, GTile<kGodRowsWeno>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
   } else if (c->split == JH_SPLIT_SW) {
-    if (c->recon == JH_RECON_MUSCL)
-      stage_kernel<Real, MODE, kSwRows, 2, RECON_MUSCL, SPLIT_SW>
-          <<<nb, Tile<kSwRows>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
-    else
-      stage_kernel<Real, MODE, kSwRows, 2, RECON_WENO5, SPLIT_SW>
-          <<<nb, Tile<kSwRows>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+    const bool grid = MODE == MODE_RK && c->halley == JH_HALLEY_GRID;
+    if (c->recon == JH_RECON_MUSCL) {
+      if (grid)
+        stage_kernel<Real, MODE, kSwRows, 2, RECON_MUSCL, SPLIT_SW, Real, true>
+            <<<nb, Tile<kSwRows>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+      else
+        stage_kernel<Real, MODE, kSwRows, 2, RECON_MUSCL, SPLIT_SW>
+            <<<nb, Tile<kSwRows>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+    } else {
+      if (grid)
+        stage_kernel<Real, MODE, kSwRows, 2, RECON_WENO5, SPLIT_SW, Real, true>
+            <<<nb, Tile<kSwRows>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+      else
+        stage_kernel<Real, MODE, kSwRows, 2, RECON_WENO5, SPLIT_SW>
+            <<<nb, Tile<kSwRows>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);
+    }
   } else if (c->recon == JH_RECON_MUSCL) {
     launch_split_stage<Real, MODE, RECON_MUSCL>(c, a, stage, Us, As, Un, Uo, Ao);
   } else {
@@ -253,20 +297,24 @@ int advance_impl(jh_ctx* c, const Real* U, const Real* aux, Real* Uo, Real* auxo
     // stage 1: (Un, An) -> (U1, A1)
     { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, Un, An, 0); }
     { Timed t(c, JH_KERNEL_STAGE);
-      launch_stage<Real, MODE_RK>(c, a, 0, Un, An, Un, U1, A1); }
+      launch_stage<Real, MODE_RK>(c, a, 0, Un, An, Un, U1, A1);
+      halley_fix<Real>(c, 0, c->parity, 1, 0, A1); }
     // stage 2: (U1, A1) -> (U2, A2)
     { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, U1, A1, 1); }
     { Timed t(c, JH_KERNEL_STAGE);
-      launch_stage<Real, MODE_RK>(c, a, 1, U1, A1, Un, U2, A2); }
+      launch_stage<Real, MODE_RK>(c, a, 1, U1, A1, Un, U2, A2);
+      halley_fix<Real>(c, 1, c->parity, 2, 0, A2); }
     // stage 3: (U2, A2) -> (Uo, auxo); Uo may alias Un (each thread reads
     // Un at its own cell before writing it)
     { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, U2, A2, 2); }
     { Timed t(c, JH_KERNEL_STAGE);
-      launch_stage<Real, MODE_RK>(c, a, 2, U2, A2, Un, Uo, auxo); }
+      launch_stage<Real, MODE_RK>(c, a, 2, U2, A2, Un, Uo, auxo);
+      halley_fix<Real>(c, 2, c->parity, -1, 0, auxo); }
     // chemistry + final update_aux in place, next step's maxima
     { Timed t(c, JH_KERNEL_CHEM);
       chem_kernel<Real, CHEM_STEP><<<grid_all(c), 256, 0, c->stream>>>(
-          a, Uo, auxo, nullptr, dt_hist ? dt_hist + s : nullptr); }
+          a, Uo, auxo, nullptr, dt_hist ? dt_hist + s : nullptr);
+      halley_fix<Real>(c, JH_PROD_CHEM, c->parity ^ 1, 0, 1, auxo); }
     rc = check_launch(c, "step kernels");
     if (rc) {
       c->timing = timing;
@@ -465,6 +513,7 @@ int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, cons
   for (int s = 0; s < 3; ++s) {
     bc_kernel<double><<<bb, kBcThreads, 0, st>>>(a, w.U[s], w.A[s], s);
     launch_stage<double, MODE_RK>(c, a, s, w.U[s], w.A[s], U0, w.U[s + 1], w.A[s + 1]);
+    halley_fix<double>(c, s, P, s < 2 ? s + 1 : -1, 0, w.A[s + 1]);
   }
   if (Uk && Ak) {
     w.U[4] = const_cast<double*>(Uk);
@@ -476,6 +525,7 @@ int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, cons
                               st));
     chem_kernel<double, CHEM_STEP><<<grid_all(c), 256, 0, st>>>(a, w.U[4], w.A[4], nullptr,
                                                                 nullptr);
+    halley_fix<double>(c, JH_PROD_CHEM, P ^ 1, 0, 1, w.A[4]);
   }
   rc = check_launch(c, "vjp forward");
   if (rc) return rc;
@@ -785,6 +835,8 @@ int jh_ctx_destroy(jh_ctx* c) {
   }
   for (auto e : c->ev_free) (void)hipEventDestroy(e);
   if (c->adj) (void)hipFree(c->adj);
+  if (c->hside) (void)hipFree(c->hside);
+  if (c->hnit) (void)hipFree(c->hnit);
   (void)hipFree(c->mc);
   (void)hipFree(c->sc);
   for (int s = 0; s < 4; ++s) (void)hipFree(c->ghost[s]);
@@ -820,6 +872,21 @@ int jh_ctx_set_tile(jh_ctx* c, int variant) {
 }
 
 int jh_tile_variants(void) { return kTileVariants; }
+
+int jh_ctx_set_halley_stop(jh_ctx* c, int mode) {
+  if (!c) return JH_ERR_ARG;
+  if (mode != JH_HALLEY_PER_CELL && mode != JH_HALLEY_GRID)
+    return fail(c, JH_ERR_ARG, "unknown temperature-solve stop mode");
+  if (mode == JH_HALLEY_GRID && !c->hside) {
+    const size_t n = (size_t)c->nx * c->ny;
+    HIP_TRY(c, hipMalloc(&c->hside, elem_size(c) * 2 * n));
+    HIP_TRY(c, hipMalloc(&c->hnit, n));
+  }
+  c->halley = mode;
+  return JH_OK;
+}
+
+int jh_ctx_get_halley_stop(const jh_ctx* c) { return c ? c->halley : -1; }
 
 int jh_ctx_set_flux(jh_ctx* c, int solver_type, int split_method, int reconstruction,
                     int riemann_solver) {
@@ -887,14 +954,18 @@ int jh_rhs(jh_ctx* c, const void* U, const void* aux, double dt, void* out) {
 int jh_update_aux(jh_ctx* c, const void* U, const void* aux_in, void* aux_out) {
   if (!c) return JH_ERR_ARG;
   if (!U || !aux_in || !aux_out) return fail(c, JH_ERR_ARG, "NULL pointer");
+  if (c->halley == JH_HALLEY_GRID)
+    reset_grid_slot_kernel<<<1, 64, 0, c->stream>>>(c->sc, c->parity, JH_PROD_UPDATE);
   if (c->precision == JH_PREC_FP64) {
     auto a = make_args<double>(c, 1.0, 0.0);
     update_aux_kernel<double><<<grid_all(c), 256, 0, c->stream>>>(
         a, (const double*)U, (const double*)aux_in, (double*)aux_out);
+    halley_fix<double>(c, JH_PROD_UPDATE, c->parity, -1, 0, (double*)aux_out);
   } else {
     auto a = make_args<float>(c, 1.0, 0.0);
     update_aux_kernel<float><<<grid_all(c), 256, 0, c->stream>>>(
         a, (const float*)U, (const float*)aux_in, (float*)aux_out);
+    halley_fix<float>(c, JH_PROD_UPDATE, c->parity, -1, 0, (float*)aux_out);
   }
   return check_launch(c, "update_aux_kernel");
 }
@@ -1061,12 +1132,14 @@ int jh_step_stage(jh_ctx* c, int stage, const void* Us, const void* As, const vo
                                                             (const double*)As, stage);
     launch_stage<double, MODE_RK>(c, a, stage, (const double*)Us, (const double*)As,
                                   (const double*)Un, (double*)Uo, (double*)Ao);
+    halley_fix<double>(c, stage, c->parity, stage < 2 ? stage + 1 : -1, 0, (double*)Ao);
   } else {
     auto a = make_args<float>(c, dt, cfl);
     bc_kernel<float><<<bc_blocks(c), kBcThreads, 0, c->stream>>>(a, (const float*)Us,
                                                           (const float*)As, stage);
     launch_stage<float, MODE_RK>(c, a, stage, (const float*)Us, (const float*)As,
                                  (const float*)Un, (float*)Uo, (float*)Ao);
+    halley_fix<float>(c, stage, c->parity, stage < 2 ? stage + 1 : -1, 0, (float*)Ao);
   }
   return check_launch(c, "step stage kernels");
 }
@@ -1081,10 +1154,12 @@ int jh_step_finish(jh_ctx* c, void* U, void* aux, double dt, double cfl, double*
     auto a = make_args<double>(c, dt, cfl);
     chem_kernel<double, CHEM_STEP><<<grid_all(c), 256, 0, c->stream>>>(
         a, (double*)U, (double*)aux, nullptr, dt_dev);
+    halley_fix<double>(c, JH_PROD_CHEM, c->parity ^ 1, 0, 1, (double*)aux);
   } else {
     auto a = make_args<float>(c, dt, cfl);
     chem_kernel<float, CHEM_STEP><<<grid_all(c), 256, 0, c->stream>>>(
         a, (float*)U, (float*)aux, nullptr, dt_dev);
+    halley_fix<float>(c, JH_PROD_CHEM, c->parity ^ 1, 0, 1, (float*)aux);
   }
   const int rc = check_launch(c, "chem_kernel");
   c->parity ^= 1;

@@ -37,6 +37,26 @@ struct MechConst {
 enum JhSide : int { JH_LEFT = JH_SIDE_LEFT, JH_RIGHT = JH_SIDE_RIGHT, JH_BOTTOM = JH_SIDE_BOTTOM,
                     JH_TOP = JH_SIDE_TOP };
 
+// Number of values N* = max over cells of the Halley update count can take.
+constexpr int kNstar = 6;
+
+// Producers of (gamma, T) whose Halley solves are tracked for the grid-wide
+// stop: the three RK stage kernels, the chemistry kernel, update_aux alone.
+enum : int { JH_PROD_CHEM = 3, JH_PROD_UPDATE = 4, JH_PRODUCERS = 5 };
+
+// Grid-wide Halley stop, one producer launch: N* and, for every candidate
+// value of N*, the maxima the producer's epilogue reduces (the LF maxima of
+// the next RK stage and, after the chemistry, the CFL maximum), all as IEEE
+// bit patterns / integers for a 64-bit atomicMax.  Contiguous so one block
+// reduction writes the first 1 + 2 kNstar (stages) or all (chemistry).
+struct GridSlots {
+  unsigned long long nstar;
+  unsigned long long lf[2][kNstar];   // [x/y][N*]
+  unsigned long long cfl[kNstar];     // [N*]
+};
+constexpr int kGridSlotsStage = 1 + 2 * kNstar;
+constexpr int kGridSlotsChem = 1 + 3 * kNstar;
+
 // Device-resident step scalars.  Maxima of non-negative doubles are kept as
 // their IEEE bit patterns so a 64-bit unsigned atomicMax orders them; a NaN
 // (all exponent bits set) compares above every finite value.
@@ -46,9 +66,15 @@ struct StepScalars {
   double dt_last;                     // dt of the last completed step
   unsigned int flags;                 // health flags (JH_FLAG_*)
   unsigned int pad;
+  GridSlots gs[2][JH_PRODUCERS];      // [step parity][producer]
 };
 
+// Health flags (the public header's jh_health_flag bits).
 enum : unsigned int {
-  JH_FLAG_NONFINITE = 1u,      // a non-finite value reached a reduction
-  JH_FLAG_NEGATIVE_RHO = 2u,   // rho <= 0 in an updated cell
+  JH_FLAG_NONFINITE = JH_HEALTH_NONFINITE,
+  JH_FLAG_NEGATIVE_RHO = JH_HEALTH_NEGATIVE_RHO,
+  JH_FLAG_HALLEY_NOCONV = JH_HEALTH_HALLEY_NOCONV,
+  JH_FLAG_SINGULAR_PIVOT = JH_HEALTH_SINGULAR_PIVOT,
+  JH_FLAG_HALLEY_REGRESS = JH_HEALTH_HALLEY_REGRESS,
+  JH_FLAG_NONPOSITIVE_T = JH_HEALTH_NONPOSITIVE_T,
 };

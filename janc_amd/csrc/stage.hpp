@@ -82,7 +82,73 @@ __device__ __forceinline__ void block_max_bits_atomic(unsigned long long v,
   }
 }
 
+// K maxima at once (bit patterns / counts), one atomicMax per value and
+// block into dst[0..K): the grid-wide-stop slots of one producer.  red holds
+// K * NT/64 values.  Every thread of the block must call.
+template <int NT, int K>
+__device__ __forceinline__ void block_max_bits_multi(unsigned long long (&v)[K],
+                                                     unsigned long long* dst,
+                                                     unsigned long long* red) {
+  constexpr int NW = NT / 64;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(v[k], off, 64);
+      v[k] = o > v[k] ? o : v[k];
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[k * NW + wid] = v[k];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < K) {
+    const int k = threadIdx.x;
+    unsigned long long m = red[k * NW];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) m = red[k * NW + i] > m ? red[k * NW + i] : m;
+    if (m > __hip_atomic_load(dst + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(dst + k, m);
+  }
+}
+
 __device__ __forceinline__ double nan_to_zero(double v) { return v == v ? v : 0.0; }
+
+// Health flags of one temperature solve's result (JH_FLAG_*).
+__device__ __forceinline__ unsigned int halley_flags(double rho, double T, double gamma,
+                                                     bool conv) {
+  unsigned int f = 0u;
+  if (rho <= 0.0) f |= JH_FLAG_NEGATIVE_RHO;
+  if (!(isfinite(T) && isfinite(gamma))) f |= JH_FLAG_NONFINITE;
+  else if (!(T > 0.0)) f |= JH_FLAG_NONPOSITIVE_T;
+  if (!conv) f |= JH_FLAG_HALLEY_NOCONV;
+  return f;
+}
+
+// One atomicOr per wave that has flags to raise (flags are rare).
+__device__ __forceinline__ void raise_flags(StepScalars* sc, unsigned int f) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) f |= __shfl_xor(f, off, 64);
+  if (f && (threadIdx.x & 63) == 0) atomicOr(&sc->flags, f);
+}
+
+// Per-candidate contribution of one cell to the grid-wide-stop maxima
+// v[off .. off + kNstar): the cell's value at its own stop (x, for N* = n) or
+// after one more update (x1, for N* > n); nothing for N* < n (N* >= n for
+// every cell).  off is a compile-time constant at every call site.
+template <int K>
+__device__ __forceinline__ void cand_max(unsigned long long (&v)[K], int off, int n, double x,
+                                         double x1) {
+  const unsigned long long b = dbits(x), b1 = dbits(x1);
+#pragma unroll
+  for (int j = 0; j < kNstar; ++j) {
+    const unsigned long long c = j == n ? b : j > n ? b1 : 0ull;
+    v[off + j] = c > v[off + j] ? c : v[off + j];
+  }
+}
 
 // v[i] of a 4-entry per-side array with a runtime side index, as selects: a
 // kernel that holds its arguments in registers (the batched kernels copy
@@ -168,6 +234,13 @@ struct StepArgs {
   double bc_p[4];             // pressure_outlet back pressure / inlet P0 per side
   double inj_T0;              // injection inlet: manifold total temperature
   double inj_Y[JH_NS - 1];    // injection inlet: injected mass fractions
+  // Grid-wide Halley stop (null: each cell stops on its own residual).  A
+  // producer of (gamma, T) writes, per cell, its update count (hnit) and the
+  // (gamma, T) after one more update (hside, two planes), and reduces its N*
+  // and per-candidate maxima into StepScalars::gs; halley_fix_kernel then
+  // applies N*.
+  Real* hside;
+  unsigned char* hnit;
 };
 
 template <typename Real>
@@ -178,11 +251,21 @@ __device__ __forceinline__ double step_dt(const StepArgs<Real>& a) {
 
 // ----------------------------------------------------- reset / reductions
 
-// Zero the reduction slots of one step parity.
-__global__ void reset_kernel(StepScalars* sc, int parity) {
+// Zero the reduction slots of one step parity (blockDim.x >= 7).
+__device__ __forceinline__ void zero_slots(StepScalars* sc, int parity) {
   const int t = threadIdx.x;
   if (t < 6) (&sc->amax[parity][0][0])[t] = 0ull;
   if (t == 6) sc->cmax[parity] = 0ull;
+  constexpr int ng = (int)(sizeof(GridSlots) / 8) * JH_PRODUCERS;
+  for (int i = t; i < ng; i += blockDim.x) (&sc->gs[parity][0].nstar)[i] = 0ull;
+}
+
+__global__ void reset_kernel(StepScalars* sc, int parity) { zero_slots(sc, parity); }
+
+// Zero one producer's grid-wide-stop slots (update_aux alone).
+__global__ void reset_grid_slot_kernel(StepScalars* sc, int parity, int prod) {
+  constexpr int m = (int)(sizeof(GridSlots) / 8);
+  if ((int)threadIdx.x < m) (&sc->gs[parity][prod].nstar)[threadIdx.x] = 0ull;
 }
 
 // Interior part of the stage-1 LF maxima and the CFL maximum of a state
@@ -398,11 +481,9 @@ __device__ __forceinline__ void bc_body(const StepArgs<Real>& a, const Real* __r
                                         double* red) {
   const MechConst& c = *a.mc;
   const int nx = a.nx, ny = a.ny, n = nx * ny;
-  if (stage == 0 && blk == 0 && threadIdx.x < 7) {
+  if (stage == 0 && blk == 0) {
     // the other parity's slots are written from here on in this step
-    const int o = a.parity ^ 1;
-    if (threadIdx.x < 6) (&a.sc->amax[o][0][0])[threadIdx.x] = 0ull;
-    else a.sc->cmax[o] = 0ull;
+    zero_slots(a.sc, a.parity ^ 1);
   }
   const int tid = blk * blockDim.x + threadIdx.x;
   double mx = 0.0, my = 0.0;
@@ -698,9 +779,11 @@ struct StageLds {
 // One tile (index `tile` in y-fastest order) of one RK stage; RECON selects
 // the reconstruction of the split fluxes (recon.hpp), SPLIT the splitting,
 // W the arithmetic type of the split fluxes, reconstructions, divergence and
-// RK combine (Real, the storage type, unless stated).
+// RK combine (Real, the storage type, unless stated); GRID the grid-wide
+// temperature-solve stop (StepArgs::hside set), compiled separately so the
+// per-cell kernels keep their register budget.
 template <typename Real, int MODE, int TX, int RECON = RECON_WENO5, int SPLIT = SPLIT_LF,
-          typename W = Real>
+          typename W = Real, bool GRID = false>
 __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
                                            const Real* __restrict__ Us,
                                            const Real* __restrict__ auxs, const Real* Un,
@@ -922,7 +1005,12 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
   // ---- update_aux (aux_func.py:21-29) + next stage's LF maxima ----
   // The new state is read back from this thread's own stores (a runtime
   // component index into a register array would live in scratch).
+  constexpr bool grid = GRID;
   double mx = 0.0, my = 0.0;
+  unsigned long long gv[kGridSlotsStage];   // grid-wide stop: N*, x and y candidates
+#pragma unroll
+  for (int k = 0; k < kGridSlotsStage; ++k) gv[k] = 0ull;
+  unsigned int fl = 0u;
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     if (!own[s]) continue;
@@ -936,33 +1024,60 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
     const double R = fill_Y_get_R(c, Y);
     double T = (double)auxs[n + oidx[s]];
     double gamma;
-    halley_T(L.mix, Y, R, e, T, gamma);
+    HalleyMore hm{0, 0.0, 0.0, true, true};
+    bool conv;
+    if constexpr (grid) {
+      conv = halley_T<MixTables, true>(L.mix, Y, R, e, T, gamma, &hm);
+      a.hside[oidx[s]] = (Real)hm.g1;
+      a.hside[n + oidx[s]] = (Real)hm.T1;
+      a.hnit[oidx[s]] = (unsigned char)hm.n;
+      if (hm.n < kHalleyMaxIter && !hm.ok1) fl |= JH_FLAG_HALLEY_REGRESS;
+      gv[0] = (unsigned long long)hm.n > gv[0] ? (unsigned long long)hm.n : gv[0];
+    } else {
+      conv = halley_T(L.mix, Y, R, e, T, gamma);
+    }
     auxo[oidx[s]] = (Real)gamma;
     auxo[n + oidx[s]] = (Real)T;
+    fl |= halley_flags(rho, T, gamma, conv);
     if (stage < 2) {
       // next stage's alpha from the stored (rounded) values
       const double gs = (double)(Real)gamma, Ts = (double)(Real)T;
       const double p = rho * R * Ts;
       const double snd = sqrt(gs * p * ir);
-      mx = fmax(mx, nan_to_zero(fabs(u) + snd));
-      my = fmax(my, nan_to_zero(fabs(v) + snd));
+      const double sx = nan_to_zero(fabs(u) + snd), sy = nan_to_zero(fabs(v) + snd);
+      if constexpr (grid) {
+        const double g1 = (double)(Real)hm.g1, T1 = (double)(Real)hm.T1;
+        const double p1 = rho * R * T1;
+        const double snd1 = sqrt(g1 * p1 * ir);
+        cand_max(gv, 1, hm.n, sx, nan_to_zero(fabs(u) + snd1));
+        cand_max(gv, 1 + kNstar, hm.n, sy, nan_to_zero(fabs(v) + snd1));
+      } else {
+        mx = fmax(mx, sx);
+        my = fmax(my, sy);
+      }
     }
   }
-  if (stage < 2) {
+  raise_flags(a.sc, fl);
+  if constexpr (grid) {
+    // the H arrays are free once every thread has passed the last phase 3
+    // (the reduction's first barrier)
+    block_max_bits_multi<NT, kGridSlotsStage>(gv, &a.sc->gs[a.parity][stage].nstar,
+                                              reinterpret_cast<unsigned long long*>(&L.Hxp[0][0]));
+  } else if (stage < 2) {
     block_max_atomic<NT>(mx, &a.sc->amax[a.parity][stage + 1][0], red);
     block_max_atomic<NT>(my, &a.sc->amax[a.parity][stage + 1][1], red);
   }
 }
 
 template <typename Real, int MODE, int TX, int WPE, int RECON, int SPLIT = SPLIT_LF,
-          typename W = Real>
+          typename W = Real, bool GRID = false>
 __global__ __launch_bounds__(Tile<TX>::NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 void stage_kernel(StepArgs<Real> a, int stage, const Real* __restrict__ Us,
                   const Real* __restrict__ auxs, const Real* Un, Real* Uo,
                   Real* __restrict__ auxo) {
   __shared__ StageLds<W, TX> L;
-  stage_body<Real, MODE, TX, RECON, SPLIT, W>(a, stage, Us, auxs, Un, Uo, auxo,
-                                              xcd_tile(blockIdx.x, gridDim.x), L);
+  stage_body<Real, MODE, TX, RECON, SPLIT, W, GRID>(a, stage, Us, auxs, Un, Uo, auxo,
+                                                    xcd_tile(blockIdx.x, gridDim.x), L);
 }
 
 // ---------------------------------------------------------- chemistry
@@ -976,8 +1091,8 @@ enum ChemMode : int { CHEM_STEP = 0, CHEM_SOURCE = 1 };
 template <typename Real, int MODE>
 __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real* aux, Real* S,
                                           double* dt_hist, int blk, double* red,
-                                          unsigned long long* redb, MixTables& mix,
-                                          GibbsTables& gt) {
+                                          unsigned long long* redb, unsigned long long* redg,
+                                          MixTables& mix, GibbsTables& gt) {
   const MechConst& c = *a.mc;
   if (MODE == CHEM_STEP) load_mix_tables(c, mix);
   load_gibbs_tables(c, gt);
@@ -988,8 +1103,13 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
     a.sc->dt_last = dt;
     if (dt_hist) *dt_hist = dt;
   }
+  const bool grid = a.hside != nullptr;
   double mx = 0.0, my = 0.0;
   unsigned long long cm = 0ull;
+  unsigned long long gv[kGridSlotsChem];   // grid-wide stop: N*, x, y and CFL candidates
+#pragma unroll
+  for (int k = 0; k < kGridSlotsChem; ++k) gv[k] = 0ull;
+  unsigned int fl = 0u;
   // one cell per thread: a grid-stride loop would let the compiler keep every
   // mechanism constant in registers across iterations
   const int i = blk * blockDim.x + threadIdx.x;
@@ -1003,14 +1123,16 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
 #pragma unroll
     for (int k = 0; k < JH_NS - 1; ++k) rY[k] = Uc[4 + k];
     double d[JH_NA];
-    chem_increment(c, gt, rho, rY, T, dt, d);
+    const bool ok = chem_increment(c, gt, rho, rY, T, dt, d);
     if (MODE == CHEM_SOURCE) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) S[v * n + i] = (Real)0;
 #pragma unroll
       for (int k = 0; k < JH_NA; ++k) S[(4 + k) * n + i] = (Real)d[k];
+      if (!ok) atomicOr(&a.sc->flags, (unsigned)JH_FLAG_SINGULAR_PIVOT);
       return;
     }
+    if (!ok) fl |= JH_FLAG_SINGULAR_PIVOT;
     Real Un[NU];
 #pragma unroll
     for (int v = 0; v < 4; ++v) Un[v] = U[v * n + i];
@@ -1027,27 +1149,52 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
     for (int k = 0; k < JH_NS - 1; ++k) Y[k] = (double)Un[4 + k] * ir;
     const double R = fill_Y_get_R(c, Y);
     double gamma;
-    halley_T(mix, Y, R, e, T, gamma);
+    HalleyMore hm{0, 0.0, 0.0, true, true};
+    bool conv;
+    if (grid) {
+      conv = halley_T<MixTables, true>(mix, Y, R, e, T, gamma, &hm);
+      a.hside[i] = (Real)hm.g1;
+      a.hside[n + i] = (Real)hm.T1;
+      a.hnit[i] = (unsigned char)hm.n;
+      if (hm.n < kHalleyMaxIter && !hm.ok1) fl |= JH_FLAG_HALLEY_REGRESS;
+      gv[0] = (unsigned long long)hm.n;
+    } else {
+      conv = halley_T(mix, Y, R, e, T, gamma);
+    }
     aux[i] = (Real)gamma;
     aux[n + i] = (Real)T;
-    const unsigned int f = (r2 > 0.0 ? 0u : (unsigned)JH_FLAG_NEGATIVE_RHO) |
-                           (isfinite(T) && isfinite(gamma) ? 0u : (unsigned)JH_FLAG_NONFINITE);
-    if (f) atomicOr(&a.sc->flags, f);
+    fl |= halley_flags(r2, T, gamma, conv);
     // next step: CFL_2D and stage-1 LF maxima from the stored values
     const double gs = (double)(Real)gamma, Ts = (double)(Real)T;
     const double p = r2 * R * Ts;
     const double snd = sqrt(gs * p * ir);
     const double sx = fabs(u) + snd, sy = fabs(v) + snd;
-    mx = fmax(mx, nan_to_zero(sx));
-    my = fmax(my, nan_to_zero(sy));
-    const unsigned long long b = dbits(sx / a.dx + sy / a.dy);
-    cm = b > cm ? b : cm;
+    if (grid) {
+      const double g1 = (double)(Real)hm.g1, T1 = (double)(Real)hm.T1;
+      const double p1 = r2 * R * T1;
+      const double snd1 = sqrt(g1 * p1 * ir);
+      const double sx1 = fabs(u) + snd1, sy1 = fabs(v) + snd1;
+      cand_max(gv, 1, hm.n, nan_to_zero(sx), nan_to_zero(sx1));
+      cand_max(gv, 1 + kNstar, hm.n, nan_to_zero(sy), nan_to_zero(sy1));
+      cand_max(gv, 1 + 2 * kNstar, hm.n, sx / a.dx + sy / a.dy, sx1 / a.dx + sy1 / a.dy);
+    } else {
+      mx = fmax(mx, nan_to_zero(sx));
+      my = fmax(my, nan_to_zero(sy));
+      const unsigned long long b = dbits(sx / a.dx + sy / a.dy);
+      cm = b > cm ? b : cm;
+    }
   }
   if (MODE == CHEM_STEP) {
+    raise_flags(a.sc, fl);
     const int o = a.parity ^ 1;
-    block_max_atomic<256>(mx, &a.sc->amax[o][0][0], red);
-    block_max_atomic<256>(my, &a.sc->amax[o][0][1], red);
-    block_max_bits_atomic<256>(cm, &a.sc->cmax[o], redb);
+    if (grid) {
+      block_max_bits_multi<256, kGridSlotsChem>(gv, &a.sc->gs[a.parity][JH_PROD_CHEM].nstar,
+                                                redg);
+    } else {
+      block_max_atomic<256>(mx, &a.sc->amax[o][0][0], red);
+      block_max_atomic<256>(my, &a.sc->amax[o][0][1], red);
+      block_max_bits_atomic<256>(cm, &a.sc->cmax[o], redb);
+    }
   }
 }
 
@@ -1059,19 +1206,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                                                    Real* S, double* dt_hist) {
   __shared__ double red[4];
   __shared__ unsigned long long redb[4];
+  __shared__ unsigned long long redg[kGridSlotsChem * 4];
   __shared__ MixTables mix;
   __shared__ GibbsTables gt;
-  chem_body<Real, MODE>(a, U, aux, S, dt_hist, blockIdx.x, red, redb, mix, gt);
+  chem_body<Real, MODE>(a, U, aux, S, dt_hist, blockIdx.x, red, redb, redg, mix, gt);
 }
 
-// update_aux alone (aux_func.py:21-29), for the unit entry point.
+// update_aux alone (aux_func.py:21-29), for the unit entry point; in grid
+// mode followed by halley_fix_kernel (producer JH_PROD_UPDATE).
 template <typename Real>
 __global__ __launch_bounds__(256) void update_aux_kernel(StepArgs<Real> a, const Real* __restrict__ U,
                                                          const Real* __restrict__ aux_in,
                                                          Real* __restrict__ aux_out) {
   const MechConst& c = *a.mc;
+  __shared__ unsigned long long redg[4];
   const int n = a.nx * a.ny;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long nb[1] = {0ull};
+  unsigned int fl = 0u;
   if (i < n) {
     const double rho = (double)U[i], ir = rcp(rho);
     const double u = (double)U[n + i] * ir, v = (double)U[2 * n + i] * ir;
@@ -1082,9 +1234,54 @@ __global__ __launch_bounds__(256) void update_aux_kernel(StepArgs<Real> a, const
     const double R = fill_Y_get_R(c, Y);
     double T = (double)aux_in[n + i];
     double gamma;
-    halley_T(c, Y, R, e, T, gamma);
+    bool conv;
+    if (a.hside) {
+      HalleyMore hm;
+      conv = halley_T<MechConst, true>(c, Y, R, e, T, gamma, &hm);
+      a.hside[i] = (Real)hm.g1;
+      a.hside[n + i] = (Real)hm.T1;
+      a.hnit[i] = (unsigned char)hm.n;
+      if (hm.n < kHalleyMaxIter && !hm.ok1) fl |= JH_FLAG_HALLEY_REGRESS;
+      nb[0] = (unsigned long long)hm.n;
+    } else {
+      conv = halley_T(c, Y, R, e, T, gamma);
+    }
     aux_out[i] = (Real)gamma;
     aux_out[n + i] = (Real)T;
+    fl |= halley_flags(rho, T, gamma, conv);
+  }
+  raise_flags(a.sc, fl);
+  if (a.hside)
+    block_max_bits_multi<256, 1>(nb, &a.sc->gs[a.parity][JH_PROD_UPDATE].nstar, redg);
+}
+
+// Grid-wide Halley stop (thermo_model.py:140-142), after a producer of
+// (gamma, T) with StepArgs::hside set: N* = max over cells of the per-cell
+// update count n; a cell with n < N* takes its state after one more update
+// (halley_T's HalleyMore).  Block 0 also folds the producer's maxima for N*
+// into the slots the next kernels read: the LF maxima of stage lf_stage of
+// parity lf_parity (lf_stage < 0: none) and, with with_cfl, that parity's CFL
+// maximum.  Grid-stride over the cells.
+template <typename Real>
+__global__ __launch_bounds__(256) void halley_fix_kernel(StepScalars* sc, int parity, int prod,
+                                                         int lf_parity, int lf_stage,
+                                                         int with_cfl, Real* __restrict__ aux,
+                                                         const Real* __restrict__ side,
+                                                         const unsigned char* __restrict__ nit,
+                                                         int n) {
+  const GridSlots& g = sc->gs[parity][prod];
+  const unsigned long long ns64 = g.nstar;
+  const int ns = ns64 < (unsigned long long)kNstar ? (int)ns64 : kNstar - 1;
+  if (blockIdx.x == 0) {
+    const int t = threadIdx.x;
+    if (t < 2 && lf_stage >= 0) atomicMax(&sc->amax[lf_parity][lf_stage][t], g.lf[t][ns]);
+    if (t == 2 && with_cfl) atomicMax(&sc->cmax[lf_parity], g.cfl[ns]);
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if ((int)nit[i] < ns) {
+      aux[i] = side[i];
+      aux[n + i] = side[n + i];
+    }
   }
 }
 

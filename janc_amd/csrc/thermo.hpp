@@ -102,15 +102,33 @@ __host__ __device__ __forceinline__ double mix_gamma(const Tab& c,
   return cp * rcp(cp - R);
 }
 
-// Halley iteration for T from e (get_T_nasa7, thermo_model.py:135-153).
-// The reference stops when the grid-wide max|res| <= tol; here each cell
-// stops on its own residual.  A converged cell's further Halley updates in
-// the reference move T by O(1e-16) relative, so both forms agree to rounding.
-template <typename Tab>
-__host__ __device__ __forceinline__ void halley_T(const Tab& c,
+// What a Halley solve reports besides (T, gamma) when the grid-wide stop is
+// reproduced: n = the updates this cell made before its
+// own residual was within tol (or kHalleyMaxIter); T1, g1 = the state after
+// one more update (n < kHalleyMaxIter); ok1 = that update's residual is
+// within tol too (a cell that converged stays converged); conv = the cell's
+// final residual is within tol.
+struct HalleyMore {
+  int n;
+  double T1, g1;
+  bool ok1, conv;
+};
+
+// Halley iteration for T from e (get_T_nasa7, thermo_model.py:135-153),
+// stopping per cell: each cell iterates while its own |res| > tol, at most
+// kHalleyMaxIter times.  The reference stops when the grid-wide max|res| <=
+// tol, so every cell makes N* = max over cells of n updates.  In SI units a
+// converged cell's further updates move T by O(1e-16) relative; with
+// nondimensional scales (tol is absolute in the energy unit) they can move it
+// by ~1e-9, and the grid-wide stop is reproduced from HalleyMore: a cell with
+// n < N* takes the state after one more update (the updates after that move
+// it by rounding only, the residual being ~tol^3 after one Halley step).
+// Returns whether the final residual is within tol.
+template <typename Tab, bool MORE = false>
+__host__ __device__ __forceinline__ bool halley_T(const Tab& c,
                                                   const double (&Y)[JH_NS],
                                                   double R, double e, double& T,
-                                                  double& gamma) {
+                                                  double& gamma, HalleyMore* more = nullptr) {
   int r = T < c.Tcr ? 0 : 1;
   MixPoly m;
   mix_poly(c, r, Y, m);
@@ -119,7 +137,8 @@ __host__ __device__ __forceinline__ void halley_T(const Tab& c,
   double res = (h - R * T) - e;
   double d1 = cp - R;
   gamma = cp * rcp(cp - R);
-  for (int i = 0; i < kHalleyMaxIter && fabs(res) > kHalleyTol; ++i) {
+  int i = 0;
+  for (; i < kHalleyMaxIter && fabs(res) > kHalleyTol; ++i) {
     const double dT = -2.0 * res * d1 * rcp(2.0 * (d1 * d1) - res * dcp);
     T = T + dT;
     const int rn = T < c.Tcr ? 0 : 1;
@@ -132,6 +151,25 @@ __host__ __device__ __forceinline__ void halley_T(const Tab& c,
     d1 = cp - R;
     gamma = cp * rcp(cp - R);
   }
+  if (MORE) {
+    more->n = i;
+    more->conv = !(fabs(res) > kHalleyTol);
+    more->T1 = T;
+    more->g1 = gamma;
+    more->ok1 = more->conv;
+    if (i < kHalleyMaxIter) {
+      const double dT = -2.0 * res * d1 * rcp(2.0 * (d1 * d1) - res * dcp);
+      const double T1 = T + dT;
+      const int rn = T1 < c.Tcr ? 0 : 1;
+      if (rn != r) mix_poly(c, rn, Y, m);
+      mix_eval(m, T1, cp, dcp, h);
+      const double res1 = (h - R * T1) - e;
+      more->T1 = T1;
+      more->g1 = cp * rcp(cp - R);
+      more->ok1 = !(fabs(res1) > kHalleyTol);
+    }
+  }
+  return !(fabs(res) > kHalleyTol);
 }
 
 }  // namespace jh

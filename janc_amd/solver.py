@@ -119,6 +119,18 @@ class Simulator:
         else:
             self.save_dt = self.t_end
             self.results_path = "results.h5"
+        # stop rule of the Halley temperature solve (get_T_nasa7,
+        # thermo_model.py:135-153): 'grid' reproduces the reference's
+        # grid-wide max|res| test, 'per_cell' stops each cell on its own
+        # residual (equal to rounding in SI units); 'auto' (default) takes the
+        # grid-wide rule for nondimensional configurations, where the absolute
+        # tolerance is a larger fraction of the energy
+        stop = comp.get("halley_stop", "auto")
+        if stop not in ("auto", "grid", "per_cell"):
+            raise KeyError("halley_stop must be 'auto', 'grid' or 'per_cell'")
+        if stop == "auto":
+            stop = "grid" if nondim_config is not None else "per_cell"
+        self.halley_stop = stop
         precision = comp.get("precision", "fp64")
         if precision not in ("fp64", "mixed"):
             raise KeyError("precision must be 'fp64' or 'mixed'")
@@ -232,6 +244,8 @@ class Simulator:
             ctx = _Context(nx, ny, dx, dy, self.bc_kinds, self._bc_params(theta),
                            self.precision, self.mech)
             _lib.check(ctx.lib.jh_ctx_set_flux(ctx.h, *self.flux), ctx.h)
+            _lib.check(ctx.lib.jh_ctx_set_halley_stop(ctx.h, _lib.HALLEY_STOP[self.halley_stop]),
+                       ctx.h)
             self._ctx[key] = ctx
         else:
             for s, v in enumerate(self._bc_params(theta)):
@@ -473,7 +487,8 @@ class Simulator:
         return {k: (ms[i], cnt[i]) for i, k in enumerate(_lib.KERNEL_KINDS)}
 
     def health(self, U):
-        """Device health flags since the last query (1 non-finite, 2 rho <= 0)."""
+        """Device health flags since the last query (bits of
+        ``_lib.HEALTH_FLAGS``; ``_lib.health_names`` spells them out)."""
         ctx = self._context(U, None)
         f = ctypes.c_uint()
         _lib.check(ctx.lib.jh_health(ctx.h, ctypes.byref(f)), ctx.h)

@@ -70,6 +70,42 @@ void host_halley(const double* mc, int n, const double* Y8, const double* e, con
   }
 }
 
+// Halley solve per cell with the grid-wide-stop record: the per-cell (gamma,
+// T), the update count n, the state after one more update (g1, T1) and the
+// flags conv / ok1 (HalleyMore)
+void host_halley_more(const double* mc, int n, const double* Y8, const double* e,
+                      const double* T0, double* gamma, double* T, int* nit, double* g1,
+                      double* T1, int* conv, int* ok1) {
+  const MechConst& c = *reinterpret_cast<const MechConst*>(mc);
+  for (int i = 0; i < n; ++i) {
+    double Y[JH_NS];
+    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = Y8[k * n + i];
+    const double R = jh::fill_Y_get_R(c, Y);
+    double t = T0[i], g;
+    jh::HalleyMore hm;
+    const bool cv = jh::halley_T<MechConst, true>(c, Y, R, e[i], t, g, &hm);
+    gamma[i] = g;
+    T[i] = t;
+    nit[i] = hm.n;
+    g1[i] = hm.g1;
+    T1[i] = hm.T1;
+    conv[i] = cv && hm.conv;
+    ok1[i] = hm.ok1;
+  }
+}
+
+// lu_solve on one N x N system (row-major A, b <- x); returns the pivot check
+int host_lu_solve8(double* A, double* b) {
+  double M[JH_NA][JH_NA], v[JH_NA];
+  for (int r = 0; r < JH_NA; ++r) {
+    v[r] = b[r];
+    for (int j = 0; j < JH_NA; ++j) M[r][j] = A[r * JH_NA + j];
+  }
+  const bool ok = jh::lu_solve<JH_NA>(M, v);
+  for (int r = 0; r < JH_NA; ++r) b[r] = v[r];
+  return ok ? 1 : 0;
+}
+
 // K = 4 FP32 faces from one 8-value window: L[i] left-, R[i] right-biased
 void host_weno_run_f32(const float* f, float* L, float* R) {
   jh::weno5_run<4, false, false>(f, 1, L, 1);

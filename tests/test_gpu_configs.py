@@ -15,7 +15,7 @@ seconds.
 import numpy as np
 import pytest
 
-from janc_amd import cases
+from janc_amd import _lib, cases
 from janc_amd.mechanism import Mechanism
 from oracle.step import Oracle
 
@@ -69,6 +69,8 @@ def test_c1_literal_parity_then_negative_T(mech_path, mech):
             break
     assert Tmin[-1] < 0, Tmin
     assert min(Tmin[:-1]) > 290.0
+    # the device flag names it
+    assert sim.health(Ug) & _lib.HEALTH_FLAGS["nonpositive_T"]
     assert 25 <= 10 + len(Tmin) <= 35
 
 

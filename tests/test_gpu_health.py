@@ -1,0 +1,103 @@
+"""Device health flags (jh_health): each bit raised by plain input data that
+causes it, and none by a clean state.
+
+The reference has no run-time error channel: NaNs propagate, get_T_nasa7
+stops after 5 iterations whatever the residual (thermo_model.py:140-152) and
+jnp.linalg.solve returns non-finite values for a singular system
+(reaction_model.py:157)."""
+import numpy as np
+import pytest
+
+from janc_amd import _lib, cases
+from janc_amd.mechanism import Mechanism
+
+torch = pytest.importorskip("torch")
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not torch.cuda.is_available(), reason="needs a GPU")]
+
+F = _lib.HEALTH_FLAGS
+
+
+@pytest.fixture(scope="module")
+def tube(mech_path):
+    from janc_amd.solver import Simulator
+    mech = Mechanism(mech_path)
+    U, aux, g = cases.detonation_tube(mech, nx=45, ny=22, driver=8)
+    sim = Simulator(cases.simulation_config(g, mech_path, dt=3e-9))
+    return sim, U, aux
+
+
+def _flags(sim, U):
+    return sim.health(torch.as_tensor(U))
+
+
+def test_clean_state_raises_nothing(tube):
+    sim, U, aux = tube
+    sim.health(torch.as_tensor(U))                    # clear
+    Ug, ag = sim.advance_one_step(U, aux, sim.dx, sim.dy, 3e-9)
+    assert sim.health(Ug) == 0
+
+
+def test_nan_cell(tube):
+    sim, U, aux = tube
+    sim.health(torch.as_tensor(U))
+    U1 = U.copy()
+    U1[3, 30, 10] = np.nan                            # rho*E of one cell
+    print("health: NaN cell", flush=True)
+    sim.advance_one_step(U1, aux, sim.dx, sim.dy, 3e-9)
+    f = _flags(sim, U1)
+    assert f & F["nonfinite"], _lib.health_names(f)
+    assert f & F["singular_pivot"], _lib.health_names(f)   # its chemistry solve
+    assert not f & F["negative_rho"], _lib.health_names(f)
+
+
+def test_negative_density_cell(tube):
+    sim, U, aux = tube
+    sim.health(torch.as_tensor(U))
+    U1 = U.copy()
+    U1[:, 40, 5] *= -1.0                              # every conserved value of one cell
+    print("health: rho < 0 cell", flush=True)
+    sim.advance_one_step(U1, aux, sim.dx, sim.dy, 3e-9)
+    f = _flags(sim, U1)
+    assert f & F["negative_rho"], _lib.health_names(f)
+
+
+def test_nan_and_negative_density_together(tube):
+    sim, U, aux = tube
+    sim.health(torch.as_tensor(U))
+    U1 = U.copy()
+    U1[0, 12, 3] = np.nan
+    U1[:, 40, 17] *= -1.0
+    sim.advance_one_step(U1, aux, sim.dx, sim.dy, 3e-9)
+    f = _flags(sim, U1)
+    assert f & F["nonfinite"] and f & F["negative_rho"], _lib.health_names(f)
+
+
+def test_temperature_solve_flags(tube):
+    """update_aux from temperature guesses far from the root: 30000 K for a
+    300 K cell does not converge in 5 iterations; 10000 K converges to the
+    mixture polynomial's root below zero."""
+    sim, U, aux = tube
+    sim.health(torch.as_tensor(U))
+    a1 = aux.copy()
+    assert abs(a1[1, 44, 20] - 300.0) < 5.0
+    a1[1, 44, 20] = 30000.0
+    sim.update_func(U, a1)
+    f = _flags(sim, U)
+    assert f == F["halley_noconv"], _lib.health_names(f)
+    a2 = aux.copy()
+    a2[1, 44, 20] = 10000.0
+    out = sim.update_func(U, a2)
+    f = _flags(sim, U)
+    assert float(out[1, 44, 20]) < 0.0
+    assert f == F["nonpositive_T"], _lib.health_names(f)
+
+
+def test_reaction_source_pivot_flag(tube):
+    sim, U, aux = tube
+    sim.health(torch.as_tensor(U))
+    U1 = U.copy()
+    U1[5, 7, 7] = np.nan                              # rho*Y_O2 of one cell
+    sim.reaction_source_terms(U1, aux, 3e-9)
+    f = _flags(sim, U1)
+    assert f == F["singular_pivot"], _lib.health_names(f)

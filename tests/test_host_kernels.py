@@ -281,3 +281,73 @@ def test_face_reconstructions(shim, recon):
         shim.host_recon(recon, _p(v), _p(out))
         tol = 1e-13 * max(np.abs(v).max(), 1e-300)
         assert abs(out[0] - Lx[j]) <= tol and abs(out[1] - Rx[j]) <= tol, j
+
+
+def _halley_inputs(mech_si, nondim=None, seed=6, spread=0.1):
+    """Cells of _cells (built with the SI mechanism) in the given unit
+    system, with initial temperature guesses `spread` off (relative)."""
+    from janc_amd.mechanism import NondimScales
+    U, aux = _cells(mech_si)
+    n = U.shape[1] * U.shape[2]
+    rho = U[0].ravel()
+    Y = (U[4:] / U[0]).reshape(8, -1)
+    u, v = U[1].ravel() / rho, U[2].ravel() / rho
+    e = U[3].ravel() / rho - 0.5 * (u * u + v * v)
+    T = aux[1].ravel()
+    if nondim is not None:
+        s = NondimScales(nondim)
+        e = e / s.e0
+        T = T / s.T0
+    T0 = T * np.random.default_rng(seed).uniform(1.0 - spread, 1.0 + spread, n)
+    return n, Y, e, T0
+
+
+@pytest.mark.parametrize("nondim", [None, {"P0": 1.0e5, "T0": 300.0, "R0": 287.0, "x0": 1e-3}])
+def test_grid_stop_reproduction(shim, mech_path, nondim):
+    """The grid-wide stop (thermo_model.py:140-142) as the kernels reproduce
+    it: N* = max of the per-cell update counts, a cell with n < N* takes its
+    state after one more update.  Against the oracle's grid-wide get_T at
+    1e-13, in SI and nondimensional units; with nondimensional scales the
+    per-cell stop alone is visibly off (~1e-10 relative), in SI it is not."""
+    from oracle.mech import load_mechanism
+    from oracle.thermo import Thermo
+    mech = Mechanism(mech_path, nondim)
+    n, Y, e, T0 = _halley_inputs(Mechanism(mech_path), nondim)
+    g, T, g1, T1 = (np.zeros(n) for _ in range(4))
+    nit, conv, ok1 = (np.zeros(n, dtype=np.int32) for _ in range(3))
+    I = ctypes.POINTER(ctypes.c_int)
+    shim.host_halley_more(_p(mech.const_array()), n, _p(Y), _p(e), _p(T0), _p(g), _p(T),
+                          nit.ctypes.data_as(I), _p(g1), _p(T1), conv.ctypes.data_as(I),
+                          ok1.ctypes.data_as(I))
+    assert conv.all() and ok1.all()
+    ns = nit.max()
+    assert (nit < ns).any()                      # some cells stop early
+    Tg = np.where(nit == ns, T, T1)
+    gg = np.where(nit == ns, g, g1)
+    _, th, _ = load_mechanism(mech_path, nondim)
+    ref = Thermo(th).get_T(e[None], Y, T0[None])
+    assert _rel(Tg, ref[1]) < 1e-13
+    assert _rel(gg, ref[0]) < 1e-13
+    per_cell = _rel(T, ref[1])
+    if nondim is None:
+        assert per_cell < 1e-14
+    else:
+        assert per_cell > 1e-12
+
+
+def test_lu_solve_pivot_check(shim):
+    """lu_solve against numpy.linalg.solve on pivoting systems; its pivot
+    check flags a singular system (a zero column) and a non-finite one."""
+    rng = np.random.default_rng(8)
+    for _ in range(20):
+        A = rng.standard_normal((8, 8)) * 10.0 ** rng.uniform(-3, 3, (8, 1))
+        b = rng.standard_normal(8)
+        x = b.copy()
+        assert shim.host_lu_solve8(_p(A), _p(x)) == 1
+        np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-9, atol=1e-12)
+    A = rng.standard_normal((8, 8))
+    A[:, 3] = 0.0
+    assert shim.host_lu_solve8(_p(A), _p(np.ones(8))) == 0
+    A = rng.standard_normal((8, 8))
+    A[5, 2] = np.nan
+    assert shim.host_lu_solve8(_p(A), _p(np.ones(8))) == 0
