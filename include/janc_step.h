@@ -290,6 +290,47 @@ int jh_batch_advance(jh_batch* batch, double dt, double cfl, int nsteps, double*
 int jh_batch_health(jh_batch* batch, unsigned int* flags);
 const char* jh_batch_last_error(const jh_batch* batch);
 
+/* ---------------------------------------------------------------- AMR blocks
+ * The AMR branch of the reference's step (AMR_Simulator,
+ * simulation/set_simulator.py:496-532): one refinement level's same-size
+ * blocks, with flux_func and update_func vmap-ed over the block axis
+ * (set_rhs with is_amr, :37-42), advanced by advance_one_step_amr
+ * (time_step/TVD_RK3.py:15-37) and the detailed-chemistry AMR branch of
+ * advance_one_step (:107-111).  Blocks are the reference's blk_data layout,
+ * (n_blk, 14, PX, PY) = U (12 planes) and aux (gamma, T), C-order.  Between
+ * RK stages the reference refills the ghost layers with jaxamr's
+ * get_ghost_block_data and keeps the interiors with update_external_boundary
+ * (the AMR hierarchy, a third-party library): the caller does both, these
+ * entry points do the per-block arithmetic.  Every block has its own
+ * Lax-Friedrichs maxima over its padded field, as vmap gives; temperature
+ * solves stop per cell (JH_HALLEY_PER_CELL). */
+typedef struct jh_blocks jh_blocks;
+/* Blocks of bnx x bny interior cells, grid spacing (dx, dy) of the level. */
+int jh_blocks_create(int bnx, int bny, double dx, double dy, int precision,
+                     const double* mech_consts, int n_consts, void* stream, jh_blocks** out);
+int jh_blocks_destroy(jh_blocks* blocks);
+int jh_blocks_set_stream(jh_blocks* blocks, void* stream);
+const char* jh_blocks_last_error(const jh_blocks* blocks);
+/* flux_func of the AMR branch (set_simulator.py:37-40): blk (n_blk, 14,
+ * bnx+6, bny+6) with filled ghost layers -> rhs (n_blk, 12, bnx+6, bny+6) =
+ * jnp.pad(dt * total_flux(U, aux, dx, dy), 3) per block (zero ghosts). */
+int jh_blocks_flux(jh_blocks* blocks, int n_blk, const void* blk, double dt, void* rhs);
+/* One stage of advance_one_step_amr (TVD_RK3.py:21-35) with the AMR
+ * flux_func and update_func: blk_s = the stage's input (ghosts filled),
+ * blk_n = the step's input U (ghosts filled; read for stages 1 and 2) ->
+ * blk_out (n_blk, 14, bnx+6, bny+6): U_s + rhs (stage 0), 3/4 U_n + 1/4
+ * (U_s + rhs) (1), 1/3 U_n + 2/3 (U_s + rhs) (2), rhs zero on the ghosts, and
+ * (gamma, T) = update_aux(U_out, aux_s) over the padded block.  blk_out must
+ * not alias the inputs. */
+int jh_blocks_stage(jh_blocks* blocks, int n_blk, int stage, const void* blk_s,
+                    const void* blk_n, double dt, void* blk_out);
+/* Chemistry of the AMR branch (set_simulator.py:107-111), in place on
+ * interior blocks (n_blk, 14, bnx, bny): U += reaction_source_terms(U, aux,
+ * dt) (reaction_model.py:183-203), aux = update_aux(U, aux). */
+int jh_blocks_chem(jh_blocks* blocks, int n_blk, void* blk, double dt);
+/* OR of every block's health flags since the last call; synchronises. */
+int jh_blocks_health(jh_blocks* blocks, unsigned int* flags);
+
 #ifdef __cplusplus
 }
 #endif

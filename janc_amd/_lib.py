@@ -82,6 +82,15 @@ SIGNATURES = {
     "jh_batch_advance": (_c_int, [_vp, _c_double, _c_double, _c_int, _vp, _c_int]),
     "jh_batch_health": (_c_int, [_vp, ctypes.POINTER(ctypes.c_uint)]),
     "jh_batch_last_error": (ctypes.c_char_p, [_vp]),
+    "jh_blocks_create": (_c_int, [_c_int, _c_int, _c_double, _c_double, _c_int,
+                                  ctypes.POINTER(_c_double), _c_int, _vp, ctypes.POINTER(_vp)]),
+    "jh_blocks_destroy": (_c_int, [_vp]),
+    "jh_blocks_set_stream": (_c_int, [_vp, _vp]),
+    "jh_blocks_last_error": (ctypes.c_char_p, [_vp]),
+    "jh_blocks_flux": (_c_int, [_vp, _c_int, _vp, _c_double, _vp]),
+    "jh_blocks_stage": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_double, _vp]),
+    "jh_blocks_chem": (_c_int, [_vp, _c_int, _vp, _c_double]),
+    "jh_blocks_health": (_c_int, [_vp, ctypes.POINTER(ctypes.c_uint)]),
 }
 
 _lib = None

@@ -1,0 +1,174 @@
+"""The AMR branch of the reference's step: one refinement level's same-size
+blocks advanced together on the GPU.
+
+Mirrors ``AMR_Simulator`` (src/janc_v2/simulation/set_simulator.py:496-532):
+``set_rhs(..., is_amr=True)`` vmaps ``flux_func`` and ``update_func`` over a
+leading block axis (:37-42) -- blocks arrive with their 3 ghost layers
+already filled, no boundary conditions are applied, every block has its own
+Lax-Friedrichs maxima --, ``advance_one_step_amr`` (time_step/TVD_RK3.py:
+15-37) runs the RK stages with the ghost layers refilled between them, and
+the detailed-chemistry branch of ``advance_one_step`` (:107-111) reacts the
+blocks' interiors.  The block arithmetic runs in libjanc_amd.so
+(``jh_blocks_*``); the AMR hierarchy itself -- ``get_ghost_block_data`` and
+``update_external_boundary`` of jaxamr (``amr`` in the reference), a
+third-party library -- is the caller's, passed in as an object with those two
+functions.
+
+Arrays are torch tensors on the GPU in the reference's ``blk_data`` layout:
+(n_blk, 14, bnx, bny) = U (12) and aux (gamma, T); with ghost layers
+(n_blk, 14, bnx + 6, bny + 6).
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from .solver import Simulator, _ptr
+
+
+class _Blocks:
+    """Owns one jh_blocks (block shape, level spacing, precision)."""
+
+    def __init__(self, bnx, bny, dx, dy, precision, mech):
+        lib = _lib.load()
+        consts = mech.const_array()
+        carr = consts.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        h = ctypes.c_void_p()
+        stream = torch.cuda.current_stream().cuda_stream
+        rc = lib.jh_blocks_create(bnx, bny, dx, dy, precision, carr, consts.size,
+                                  ctypes.c_void_p(stream), ctypes.byref(h))
+        if rc != _lib.JH_OK:
+            raise _lib.JancError("jh_blocks_create: " + lib.jh_blocks_last_error(None).decode())
+        self.h, self.lib = h, lib
+
+    def call(self, name, *args):
+        self.lib.jh_blocks_set_stream(self.h,
+                                      ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        rc = getattr(self.lib, name)(self.h, *args)
+        if rc != _lib.JH_OK:
+            raise _lib.JancError("%s: %s" % (name, self.lib.jh_blocks_last_error(self.h).decode()))
+
+    def close(self):
+        if self.h:
+            self.lib.jh_blocks_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BlockSolver:
+    """The vmapped block functions of one configuration (the reference's
+    simulation_config dict, read as ``Simulator`` reads it; LF flux splitting
+    with WENO5_JS, detailed chemistry).  ``dx``/``dy`` are per call, as in
+    ``advance_func_amr`` (each refinement level has its own spacing)."""
+
+    def __init__(self, simulation_config):
+        sim = Simulator(simulation_config)
+        if sim.flux != (0, 0, 0, 0):
+            raise NotImplementedError("AMR blocks run LF flux splitting with WENO5_JS")
+        if sim.halley_stop == "grid":
+            raise NotImplementedError(
+                "AMR blocks stop the temperature solve per cell; set "
+                "computation_config['halley_stop'] = 'per_cell'")
+        self.sim = sim
+        self.mech, self.precision, self.dtype = sim.mech, sim.precision, sim.dtype
+        self._ctx = {}
+
+    def _blocks(self, bnx, bny, dx, dy):
+        key = (bnx, bny, float(dx), float(dy))
+        b = self._ctx.get(key)
+        if b is None:
+            b = _Blocks(bnx, bny, float(dx), float(dy), self.precision, self.mech)
+            self._ctx[key] = b
+        return b
+
+    def _check(self, blk, padded):
+        if not isinstance(blk, torch.Tensor) and hasattr(blk, "__dlpack__"):
+            blk = torch.from_dlpack(blk)
+        if blk.device.type != "cuda" or blk.dtype != self.dtype or not blk.is_contiguous():
+            raise TypeError("blocks must be contiguous %s CUDA tensors" % self.dtype)
+        if blk.dim() != 4 or blk.shape[1] != 4 + self.mech.ns - 1 + 2:
+            raise ValueError("blocks must be (n_blk, %d, nx, ny)" % (4 + self.mech.ns + 1))
+        g = 6 if padded else 0
+        return blk, int(blk.shape[0]), int(blk.shape[2]) - g, int(blk.shape[3]) - g
+
+    def flux_func(self, blk, dx, dy, dt):
+        """set_rhs(is_amr=True).flux_func (set_simulator.py:37-40) on padded
+        blocks (U and aux concatenated): (n_blk, 12, bnx+6, bny+6)."""
+        blk, nb, bnx, bny = self._check(blk, True)
+        out = torch.empty((nb, blk.shape[1] - 2) + tuple(blk.shape[2:]), dtype=blk.dtype,
+                          device=blk.device)
+        self._blocks(bnx, bny, dx, dy).call("jh_blocks_flux", nb, _ptr(blk), float(dt),
+                                            _ptr(out))
+        return out
+
+    def stage(self, s, blk_s, blk_n, dx, dy, dt):
+        """One stage of advance_one_step_amr (TVD_RK3.py:21-35) on padded
+        blocks; returns the padded stage output."""
+        blk_s, nb, bnx, bny = self._check(blk_s, True)
+        blk_n, nb2, _, _ = self._check(blk_n, True)
+        if blk_n.shape != blk_s.shape:
+            raise ValueError("blk_s and blk_n must have the same shape")
+        out = torch.empty_like(blk_s)
+        self._blocks(bnx, bny, dx, dy).call("jh_blocks_stage", nb, int(s), _ptr(blk_s),
+                                            _ptr(blk_n), float(dt), _ptr(out))
+        return out
+
+    def chemistry(self, blk, dx, dy, dt):
+        """The detailed-chemistry AMR branch (set_simulator.py:107-111) on
+        interior blocks, in place."""
+        blk, nb, bnx, bny = self._check(blk, False)
+        self._blocks(bnx, bny, dx, dy).call("jh_blocks_chem", nb, _ptr(blk), float(dt))
+        return blk
+
+    def advance_one_step(self, level, blk_data, dx, dy, dt, ref_blk_data, ref_blk_info,
+                         theta=None, amr=None):
+        """advance_func_amr (set_simulator.py:100-111 with TVD_RK3.py:15-37):
+        blk_data = the parent level's blocks, ref_blk_data / ref_blk_info = this
+        level's blocks and their block information; ``amr`` provides
+        get_ghost_block_data(blocks, info) and
+        update_external_boundary(level, parent_blocks, interiors, info)."""
+        if amr is None:
+            raise ValueError("advance_one_step needs the AMR hierarchy's ghost functions "
+                             "(jaxamr's amr module or an object with the same two functions)")
+        g0 = amr.get_ghost_block_data(ref_blk_data, ref_blk_info)
+        b = None
+        gs = g0
+        for s in range(3):
+            out = self.stage(s, gs, g0, dx, dy, dt)
+            b = amr.update_external_boundary(level, blk_data, out[..., 3:-3, 3:-3], ref_blk_info)
+            if s < 2:
+                gs = amr.get_ghost_block_data(b, ref_blk_info)
+        b = b.contiguous()
+        return self.chemistry(b, dx, dy, dt)
+
+    def health(self):
+        """OR of every block context's health flags since the last query."""
+        f = 0
+        for b in self._ctx.values():
+            v = ctypes.c_uint()
+            b.call("jh_blocks_health", ctypes.byref(v))
+            f |= int(v.value)
+        return f
+
+
+def AMR_Simulator(simulation_config):
+    """set_simulator.py:496-532: (advance_func_amr, advance_func_base).  The
+    first takes the reference's arguments plus ``amr`` (the hierarchy's ghost
+    functions); the second advances the base level's single block
+    (blk_data[0] = (14, nx, ny)) with the boundary conditions of the
+    configuration, as the reference's advance_func_base."""
+    solver = BlockSolver(simulation_config)
+    sim = solver.sim
+
+    def advance_func_base(blk_data, dx, dy, dt, theta=None):
+        n = 4 + sim.mech.ns - 1
+        U, aux = blk_data[0, :n], blk_data[0, n:]
+        U, aux = sim.advance_one_step(U.contiguous(), aux.contiguous(), dx, dy, dt, theta)
+        return torch.cat([U, aux], 0)[None]
+
+    return solver.advance_one_step, advance_func_base

@@ -223,7 +223,7 @@ __device__ __forceinline__ void godunov_stage_body(const StepArgs<Real>& a, int 
     }
     auxo[o] = (Real)gamma;
     auxo[n + o] = (Real)T;
-    fl |= halley_flags(rho, T, gamma, conv);
+    fl |= halley_flags(rho, e, R, T, gamma, conv);
   }
   // every lane of the wave reaches here (waves w >= TX left above)
   raise_flags(a.sc, fl);

@@ -12,6 +12,7 @@
 #include "../../include/janc_step.h"
 #include "adjoint.hpp"
 #include "batch.hpp"
+#include "blocks.hpp"
 #include "godunov.hpp"
 #include "stage.hpp"
 
@@ -763,6 +764,91 @@ void drop_graph(jh_batch* b) {
 
 }  // namespace
 
+// ------------------------------------------------------------------ AMR blocks
+
+struct jh_blocks {
+  int nx = 0, ny = 0;                 // block interior
+  double dx = 0, dy = 0;
+  int precision = JH_PREC_FP64;
+  hipStream_t stream = nullptr;
+  MechConst* mc = nullptr;
+  StepScalars* sc = nullptr;          // one per block
+  int cap = 0;                        // blocks sc holds
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_blocks_err;
+
+int kfail(jh_blocks* b, int code, const std::string& msg) {
+  if (b) b->err = msg; else g_blocks_err = msg;
+  return code;
+}
+
+#define KHIP_TRY(b, call)                                                      \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return kfail(b, JH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Room for n_blk blocks' scalars (flags start at zero).
+int blocks_reserve(jh_blocks* b, int n_blk) {
+  if (n_blk <= b->cap) return JH_OK;
+  KHIP_TRY(b, hipStreamSynchronize(b->stream));
+  StepScalars* sc = nullptr;
+  KHIP_TRY(b, hipMalloc(&sc, sizeof(StepScalars) * n_blk));
+  KHIP_TRY(b, hipMemset(sc, 0, sizeof(StepScalars) * n_blk));
+  if (b->sc) (void)hipFree(b->sc);
+  b->sc = sc;
+  b->cap = n_blk;
+  return JH_OK;
+}
+
+template <typename Real>
+StepArgs<Real> blocks_args(const jh_blocks* b, double dt) {
+  StepArgs<Real> a;
+  std::memset(&a, 0, sizeof(a));
+  a.nx = b->nx;
+  a.ny = b->ny;
+  a.dx = b->dx;
+  a.dy = b->dy;
+  a.dt_fixed = dt;
+  a.mc = b->mc;
+  a.sc = b->sc;
+  return a;
+}
+
+template <typename Real, int MODE>
+int blocks_stage_impl(jh_blocks* b, int n_blk, int stage, const Real* Bs, const Real* Bn,
+                      Real* Bo, double dt) {
+  const long long P = (long long)(b->nx + 6) * (b->ny + 6);
+  const long long bs = (NU + 2) * P, bo = MODE == MODE_RHS ? NU * P : (NU + 2) * P;
+  auto a = blocks_args<Real>(b, dt);
+  hipStream_t st = b->stream;
+  blocks_alpha_kernel<Real><<<n_blk, 256, 0, st>>>(a, stage, bs, Bs);
+  const int tiles = ((b->nx + kBlkRows - 1) / kBlkRows) * ((b->ny + TY - 1) / TY);
+  blocks_stage_kernel<Real, MODE, kBlkRows, 4>
+      <<<n_blk * tiles, Tile<kBlkRows>::NT, 0, st>>>(a, stage, tiles, bs, bo, Bs, Bn, Bo);
+  const int nring = (int)(P - (long long)b->nx * b->ny);
+  blocks_ring_kernel<Real, MODE>
+      <<<dim3((nring + 255) / 256, n_blk), 256, 0, st>>>(a, stage, bs, bo, Bs, Bn, Bo);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return kfail(b, JH_ERR_HIP, std::string("block kernels: ") +
+                                                       hipGetErrorString(e));
+  return JH_OK;
+}
+
+int blocks_checks(jh_blocks* b, int n_blk) {
+  if (n_blk < 1) return kfail(b, JH_ERR_ARG, "n_blk must be >= 1");
+  if ((long long)n_blk * (NU + 2) * (b->nx + 6) * (b->ny + 6) >= (1ll << 31))
+    return kfail(b, JH_ERR_ARG, "block batch too large for 32-bit offsets");
+  return blocks_reserve(b, n_blk);
+}
+
+}  // namespace
+
 extern "C" {
 
 int jh_mech_const_count(void) { return (int)(sizeof(MechConst) / sizeof(double)); }
@@ -1397,6 +1483,125 @@ int jh_batch_health(jh_batch* b, unsigned int* flags) {
 
 const char* jh_batch_last_error(const jh_batch* b) {
   return b ? b->err.c_str() : g_batch_err.c_str();
+}
+
+// ------------------------------------------------------------------ AMR blocks API
+
+int jh_blocks_create(int bnx, int bny, double dx, double dy, int precision, const double* mech,
+                     int n_mech, void* stream, jh_blocks** out) {
+  if (!out) return kfail(nullptr, JH_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (bnx < 1 || bny < 1) return kfail(nullptr, JH_ERR_ARG, "block sizes must be >= 1");
+  if (!(dx > 0) || !(dy > 0)) return kfail(nullptr, JH_ERR_ARG, "dx, dy must be > 0");
+  if (precision != JH_PREC_FP64 && precision != JH_PREC_MIXED)
+    return kfail(nullptr, JH_ERR_ARG, "unknown precision");
+  if (!mech) return kfail(nullptr, JH_ERR_ARG, "mech is NULL");
+  if (n_mech != jh_mech_const_count())
+    return kfail(nullptr, JH_ERR_MECH, "mechanism constant count mismatch");
+  jh_blocks* b = new jh_blocks();
+  b->nx = bnx;
+  b->ny = bny;
+  b->dx = dx;
+  b->dy = dy;
+  b->precision = precision;
+  b->stream = static_cast<hipStream_t>(stream);
+  hipError_t e = hipMalloc(&b->mc, sizeof(MechConst));
+  if (!e) e = hipMemcpy(b->mc, mech, sizeof(MechConst), hipMemcpyHostToDevice);
+  if (e) {
+    g_blocks_err = std::string("allocation: ") + hipGetErrorString(e);
+    jh_blocks_destroy(b);
+    return JH_ERR_HIP;
+  }
+  *out = b;
+  return JH_OK;
+}
+
+int jh_blocks_destroy(jh_blocks* b) {
+  if (!b) return JH_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream); else (void)hipDeviceSynchronize();
+  if (b->mc) (void)hipFree(b->mc);
+  if (b->sc) (void)hipFree(b->sc);
+  delete b;
+  return JH_OK;
+}
+
+int jh_blocks_set_stream(jh_blocks* b, void* stream) {
+  if (!b) return JH_ERR_ARG;
+  b->stream = static_cast<hipStream_t>(stream);
+  return JH_OK;
+}
+
+const char* jh_blocks_last_error(const jh_blocks* b) {
+  return b ? b->err.c_str() : g_blocks_err.c_str();
+}
+
+int jh_blocks_flux(jh_blocks* b, int n_blk, const void* blk, double dt, void* rhs) {
+  if (!b) return JH_ERR_ARG;
+  if (!blk || !rhs) return kfail(b, JH_ERR_ARG, "NULL pointer");
+  if (blk == rhs) return kfail(b, JH_ERR_ARG, "rhs must not alias the blocks");
+  if (!(dt > 0)) return kfail(b, JH_ERR_ARG, "dt must be > 0");
+  int rc = blocks_checks(b, n_blk);
+  if (rc) return rc;
+  if (b->precision == JH_PREC_FP64)
+    return blocks_stage_impl<double, MODE_RHS>(b, n_blk, 0, (const double*)blk,
+                                               (const double*)blk, (double*)rhs, dt);
+  return blocks_stage_impl<float, MODE_RHS>(b, n_blk, 0, (const float*)blk, (const float*)blk,
+                                            (float*)rhs, dt);
+}
+
+int jh_blocks_stage(jh_blocks* b, int n_blk, int stage, const void* blk_s, const void* blk_n,
+                    double dt, void* blk_out) {
+  if (!b) return JH_ERR_ARG;
+  if (!blk_s || !blk_n || !blk_out) return kfail(b, JH_ERR_ARG, "NULL pointer");
+  if (blk_out == blk_s || blk_out == blk_n)
+    return kfail(b, JH_ERR_ARG, "blk_out must not alias the inputs");
+  if (stage < 0 || stage > 2) return kfail(b, JH_ERR_ARG, "stage must be 0, 1 or 2");
+  if (!(dt > 0)) return kfail(b, JH_ERR_ARG, "dt must be > 0");
+  int rc = blocks_checks(b, n_blk);
+  if (rc) return rc;
+  if (b->precision == JH_PREC_FP64)
+    return blocks_stage_impl<double, MODE_RK>(b, n_blk, stage, (const double*)blk_s,
+                                              (const double*)blk_n, (double*)blk_out, dt);
+  return blocks_stage_impl<float, MODE_RK>(b, n_blk, stage, (const float*)blk_s,
+                                           (const float*)blk_n, (float*)blk_out, dt);
+}
+
+int jh_blocks_chem(jh_blocks* b, int n_blk, void* blk, double dt) {
+  if (!b) return JH_ERR_ARG;
+  if (!blk) return kfail(b, JH_ERR_ARG, "NULL pointer");
+  if (!(dt > 0)) return kfail(b, JH_ERR_ARG, "dt must be > 0");
+  int rc = blocks_checks(b, n_blk);
+  if (rc) return rc;
+  const int n = b->nx * b->ny;
+  const long long bs = (long long)(NU + 2) * n;
+  const dim3 grid((n + 255) / 256, n_blk);
+  if (b->precision == JH_PREC_FP64)
+    blocks_chem_kernel<double><<<grid, 256, 0, b->stream>>>(blocks_args<double>(b, dt), bs,
+                                                            (double*)blk);
+  else
+    blocks_chem_kernel<float><<<grid, 256, 0, b->stream>>>(blocks_args<float>(b, dt), bs,
+                                                           (float*)blk);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return kfail(b, JH_ERR_HIP, std::string("block chemistry: ") +
+                                                       hipGetErrorString(e));
+  return JH_OK;
+}
+
+int jh_blocks_health(jh_blocks* b, unsigned int* flags) {
+  if (!b || !flags) return JH_ERR_ARG;
+  *flags = 0;
+  if (!b->cap) return JH_OK;
+  std::vector<StepScalars> h(b->cap);
+  KHIP_TRY(b, hipStreamSynchronize(b->stream));
+  KHIP_TRY(b, hipMemcpy(h.data(), b->sc, sizeof(StepScalars) * b->cap, hipMemcpyDeviceToHost));
+  unsigned int f = 0;
+  for (auto& x : h) {
+    f |= x.flags;
+    x.flags = 0;
+  }
+  KHIP_TRY(b, hipMemcpy(b->sc, h.data(), sizeof(StepScalars) * b->cap, hipMemcpyHostToDevice));
+  *flags = f;
+  return JH_OK;
 }
 
 }  // extern "C"

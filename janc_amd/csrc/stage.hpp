@@ -117,12 +117,14 @@ __device__ __forceinline__ void block_max_bits_multi(unsigned long long (&v)[K],
 
 __device__ __forceinline__ double nan_to_zero(double v) { return v == v ? v : 0.0; }
 
-// Health flags of one temperature solve's result (JH_FLAG_*).
-__device__ __forceinline__ unsigned int halley_flags(double rho, double T, double gamma,
-                                                     bool conv) {
+// Health flags of one temperature solve and the state it solved for
+// (JH_FLAG_*): e and R carry a non-finite conserved value (a NaN residual
+// ends the Halley loop at once and leaves T, gamma finite).
+__device__ __forceinline__ unsigned int halley_flags(double rho, double e, double R, double T,
+                                                     double gamma, bool conv) {
   unsigned int f = 0u;
   if (rho <= 0.0) f |= JH_FLAG_NEGATIVE_RHO;
-  if (!(isfinite(T) && isfinite(gamma))) f |= JH_FLAG_NONFINITE;
+  if (!(isfinite(e) && isfinite(R) && isfinite(T) && isfinite(gamma))) f |= JH_FLAG_NONFINITE;
   else if (!(T > 0.0)) f |= JH_FLAG_NONPOSITIVE_T;
   if (!conv) f |= JH_FLAG_HALLEY_NOCONV;
   return f;
@@ -690,6 +692,24 @@ __device__ __forceinline__ CellRef<Real> locate(const StepArgs<Real>& a, const R
   return r;
 }
 
+// A cell of a block that carries its own 3 ghost layers (the AMR branch's
+// blocks, (nx + 6) x (ny + 6) planes, interior at offset (3, 3)): cells
+// within the ghost layers are read in place, cells beyond them do not exist.
+template <typename Real>
+__device__ __forceinline__ CellRef<Real> locate_padded(const StepArgs<Real>& a,
+                                                       const Real* __restrict__ U,
+                                                       const Real* __restrict__ aux, int gx,
+                                                       int gy) {
+  const int py = a.ny + 6;
+  CellRef<Real> r;
+  r.ok = gx >= -3 && gx < a.nx + 3 && gy >= -3 && gy < a.ny + 3;
+  const int o = r.ok ? (gx + 3) * py + gy + 3 : 0;
+  r.u = U + o;
+  r.x = aux + o;
+  r.stride = (a.nx + 6) * py;
+  return r;
+}
+
 // Physical flux of component cv in direction x (dir 0) or y (dir 1) at a
 // cell, LF.py:12 (rho*u, rho*u^2+p, rho*u*v, u*(rhoE+p), rho*u*Y).
 template <typename W>
@@ -781,9 +801,11 @@ struct StageLds {
 // W the arithmetic type of the split fluxes, reconstructions, divergence and
 // RK combine (Real, the storage type, unless stated); GRID the grid-wide
 // temperature-solve stop (StepArgs::hside set), compiled separately so the
-// per-cell kernels keep their register budget.
+// per-cell kernels keep their register budget; PAD the state planes are one
+// AMR block with its own ghost layers ((nx + 6) x (ny + 6), no ghost strips,
+// no next-stage maxima: the next stage's ghosts come from the caller).
 template <typename Real, int MODE, int TX, int RECON = RECON_WENO5, int SPLIT = SPLIT_LF,
-          typename W = Real, bool GRID = false>
+          typename W = Real, bool GRID = false, bool PAD = false>
 __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
                                            const Real* __restrict__ Us,
                                            const Real* __restrict__ auxs, const Real* Un,
@@ -807,7 +829,7 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
 
   const MechConst& c = *a.mc;
   if (MODE != MODE_RHS) load_mix_tables(c, L.mix);   // read after the phase barriers
-  const int nx = a.nx, ny = a.ny, n = nx * ny;
+  const int nx = a.nx, ny = a.ny, n = PAD ? (nx + 6) * (ny + 6) : nx * ny;
   const int t = threadIdx.x, ly = t & 63, w = t >> 6;
   const int tiles_y = (ny + TY - 1) / TY;
   const int x0 = (tile / tiles_y) * TX;
@@ -849,7 +871,8 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
   Real nxt[NSL];
 #pragma unroll
   for (int s = 0; s < NSL; ++s) {
-    ref[s] = locate<Real>(a, Us, auxs, gxs[s], gys[s]);
+    ref[s] = PAD ? locate_padded<Real>(a, Us, auxs, gxs[s], gys[s])
+                 : locate<Real>(a, Us, auxs, gxs[s], gys[s]);
     if (!has[s]) ref[s].ok = false;
     if (ref[s].ok) {
       q[s] = cell_prim<W, SPLIT == SPLIT_SW, Real>(c, ref[s].u, ref[s].x, ref[s].stride);
@@ -865,7 +888,7 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     own[s] = gxs[s] < nx && gys[s] < ny;
-    oidx[s] = gxs[s] * ny + gys[s];
+    oidx[s] = PAD ? (gxs[s] + 3) * (ny + 6) + gys[s] + 3 : gxs[s] * ny + gys[s];
   }
 
   for (int cv = 0; cv < NU; ++cv) {
@@ -1038,8 +1061,8 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
     }
     auxo[oidx[s]] = (Real)gamma;
     auxo[n + oidx[s]] = (Real)T;
-    fl |= halley_flags(rho, T, gamma, conv);
-    if (stage < 2) {
+    fl |= halley_flags(rho, e, R, T, gamma, conv);
+    if (!PAD && stage < 2) {
       // next stage's alpha from the stored (rounded) values
       const double gs = (double)(Real)gamma, Ts = (double)(Real)T;
       const double p = rho * R * Ts;
@@ -1063,7 +1086,7 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
     // (the reduction's first barrier)
     block_max_bits_multi<NT, kGridSlotsStage>(gv, &a.sc->gs[a.parity][stage].nstar,
                                               reinterpret_cast<unsigned long long*>(&L.Hxp[0][0]));
-  } else if (stage < 2) {
+  } else if (!PAD && stage < 2) {
     block_max_atomic<NT>(mx, &a.sc->amax[a.parity][stage + 1][0], red);
     block_max_atomic<NT>(my, &a.sc->amax[a.parity][stage + 1][1], red);
   }
@@ -1163,7 +1186,7 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
     }
     aux[i] = (Real)gamma;
     aux[n + i] = (Real)T;
-    fl |= halley_flags(r2, T, gamma, conv);
+    fl |= halley_flags(r2, e, R, T, gamma, conv);
     // next step: CFL_2D and stage-1 LF maxima from the stored values
     const double gs = (double)(Real)gamma, Ts = (double)(Real)T;
     const double p = r2 * R * Ts;
@@ -1248,7 +1271,7 @@ __global__ __launch_bounds__(256) void update_aux_kernel(StepArgs<Real> a, const
     }
     aux_out[i] = (Real)gamma;
     aux_out[n + i] = (Real)T;
-    fl |= halley_flags(rho, T, gamma, conv);
+    fl |= halley_flags(rho, e, R, T, gamma, conv);
   }
   raise_flags(a.sc, fl);
   if (a.hside)

@@ -10,7 +10,13 @@ seconds.
   [0.8, 1.2], 40-column 20 atm / 2800 K slab) on its 400 x 50 corner crop
   (slab and phi variation included) at the full tube's CFL-0.3 dt: the mixed
   step against the FP64 oracle every 25 steps for 100 steps, within the
-  bounds stated below, and the FP64 step at the 1e-12 bar.
+  bounds stated below, and the FP64 step against the oracle over the same
+  100 steps: (gamma, T) at the 1e-12 bar, U within 1e-10.  The oracle's own
+  state moves by 4.4e-11 (max-norm relative, rho*v: small next to its
+  rounding) after 5 steps when its initial state is perturbed by one ulp, so
+  over a hundred steps the per-operation rounding differences of two correct
+  FP64 implementations reach that size; measured GPU-vs-oracle 5.9e-12 after
+  25 steps.
 """
 import numpy as np
 import pytest
@@ -24,6 +30,7 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not torch.cuda.is_available(), reason="needs a GPU")]
 
 FP64_TOL = 1e-12
+FP64_LONG_U = 1e-10     # U after tens of steps (see the module docstring)
 
 
 def rel_err(got, ref):
@@ -152,7 +159,7 @@ def test_c3_crop_mixed_and_fp64_vs_oracle(mech_path, mech):
         print("step %d fp64 %.2e %.2e | mixed " % ((s,) + fd) +
               " ".join("%s %.1e/%.1e" % (k, *v) for k, v in row.items()), flush=True)
         worst = row
-        assert fd[0] <= FP64_TOL and fd[1] <= FP64_TOL
+        assert fd[0] <= FP64_LONG_U and fd[1] <= FP64_TOL
     assert sm.health(Um) == 0 and sd.health(Ud) == 0
     for k, (l2, linf) in worst.items():
         assert l2 <= MIXED_L2, (k, l2)
