@@ -324,6 +324,17 @@ int jh_blocks_flux(jh_blocks* blocks, int n_blk, const void* blk, double dt, voi
  * not alias the inputs. */
 int jh_blocks_stage(jh_blocks* blocks, int n_blk, int stage, const void* blk_s,
                     const void* blk_n, double dt, void* blk_out);
+/* jh_blocks_stage without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  jh_blocks_stage grows the handle's
+ * per-block scalars (LF maxima, health flags) with hipMalloc when a call has
+ * more blocks than any before; this entry keeps them in a file-scope device
+ * array of 2048 blocks instead (each GPU holds it until the process ends) and
+ * fails with JH_ERR_ARG for n_blk > 2048.  Same kernels, same results; its
+ * blocks' health flags are not reported.  Calls that overlap on one GPU share
+ * the array and race. */
+int jh_blocks_stage_noalloc(jh_blocks* blocks, int n_blk, int stage, const void* blk_s,
+                            const void* blk_n, double dt, void* blk_out);
 /* Chemistry of the AMR branch (set_simulator.py:107-111), in place on
  * interior blocks (n_blk, 14, bnx, bny): U += reaction_source_terms(U, aux,
  * dt) (reaction_model.py:183-203), aux = update_aux(U, aux). */

@@ -89,6 +89,7 @@ SIGNATURES = {
     "jh_blocks_last_error": (ctypes.c_char_p, [_vp]),
     "jh_blocks_flux": (_c_int, [_vp, _c_int, _vp, _c_double, _vp]),
     "jh_blocks_stage": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_double, _vp]),
+    "jh_blocks_stage_noalloc": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_double, _vp]),
     "jh_blocks_chem": (_c_int, [_vp, _c_int, _vp, _c_double]),
     "jh_blocks_health": (_c_int, [_vp, ctypes.POINTER(ctypes.c_uint)]),
 }

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -774,12 +775,42 @@ struct jh_blocks {
   MechConst* mc = nullptr;
   StepScalars* sc = nullptr;          // one per block
   int cap = 0;                        // blocks sc holds
+  StepScalars* scratch = nullptr;     // device address of `scratch` on this handle's GPU
   std::string err;
 };
+
+// Per-block scalars of jh_blocks_stage_noalloc: a file-scope device array,
+// so that entry point allocates nothing.  Every GPU the library runs on holds
+// its own copy (kNoallocBlocks x sizeof(StepScalars) = 3.4 MB) until the
+// process ends; calls that overlap on one GPU share it and race.
+constexpr int kNoallocBlocks = 2048;
+static __device__ __align__(256) StepScalars scratch[kNoallocBlocks];
 
 namespace {
 
 thread_local std::string g_blocks_err;
+
+// scratch's device address per GPU, taken once per device
+// (hipGetSymbolAddress with that device current); the host-side name
+// `scratch` is not a device address and is never passed to a kernel.
+std::mutex g_scratch_mu;
+std::vector<StepScalars*> g_scratch_addr;
+
+hipError_t scratch_address(StepScalars** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_addr.size() <= dev) g_scratch_addr.resize(dev + 1, nullptr);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[dev] = static_cast<StepScalars*>(p);
+  }
+  *out = g_scratch_addr[dev];
+  return hipSuccess;
+}
 
 int kfail(jh_blocks* b, int code, const std::string& msg) {
   if (b) b->err = msg; else g_blocks_err = msg;
@@ -806,8 +837,9 @@ int blocks_reserve(jh_blocks* b, int n_blk) {
   return JH_OK;
 }
 
+// sc: the per-block scalars (the handle's, or the no-allocation scratch)
 template <typename Real>
-StepArgs<Real> blocks_args(const jh_blocks* b, double dt) {
+StepArgs<Real> blocks_args(const jh_blocks* b, double dt, StepScalars* sc) {
   StepArgs<Real> a;
   std::memset(&a, 0, sizeof(a));
   a.nx = b->nx;
@@ -816,16 +848,16 @@ StepArgs<Real> blocks_args(const jh_blocks* b, double dt) {
   a.dy = b->dy;
   a.dt_fixed = dt;
   a.mc = b->mc;
-  a.sc = b->sc;
+  a.sc = sc;
   return a;
 }
 
 template <typename Real, int MODE>
 int blocks_stage_impl(jh_blocks* b, int n_blk, int stage, const Real* Bs, const Real* Bn,
-                      Real* Bo, double dt) {
+                      Real* Bo, double dt, StepScalars* sc) {
   const long long P = (long long)(b->nx + 6) * (b->ny + 6);
   const long long bs = (NU + 2) * P, bo = MODE == MODE_RHS ? NU * P : (NU + 2) * P;
-  auto a = blocks_args<Real>(b, dt);
+  auto a = blocks_args<Real>(b, dt, sc);
   hipStream_t st = b->stream;
   blocks_alpha_kernel<Real><<<n_blk, 256, 0, st>>>(a, stage, bs, Bs);
   const int tiles = ((b->nx + kBlkRows - 1) / kBlkRows) * ((b->ny + TY - 1) / TY);
@@ -840,11 +872,23 @@ int blocks_stage_impl(jh_blocks* b, int n_blk, int stage, const Real* Bs, const 
   return JH_OK;
 }
 
-int blocks_checks(jh_blocks* b, int n_blk) {
+// reserve: grow the handle's per-block scalars (allocates); the
+// no-allocation entry point checks sizes only
+int blocks_checks(jh_blocks* b, int n_blk, bool reserve = true) {
   if (n_blk < 1) return kfail(b, JH_ERR_ARG, "n_blk must be >= 1");
   if ((long long)n_blk * (NU + 2) * (b->nx + 6) * (b->ny + 6) >= (1ll << 31))
     return kfail(b, JH_ERR_ARG, "block batch too large for 32-bit offsets");
-  return blocks_reserve(b, n_blk);
+  return reserve ? blocks_reserve(b, n_blk) : JH_OK;
+}
+
+int stage_args_check(jh_blocks* b, int stage, const void* blk_s, const void* blk_n, double dt,
+                     const void* blk_out) {
+  if (!blk_s || !blk_n || !blk_out) return kfail(b, JH_ERR_ARG, "NULL pointer");
+  if (blk_out == blk_s || blk_out == blk_n)
+    return kfail(b, JH_ERR_ARG, "blk_out must not alias the inputs");
+  if (stage < 0 || stage > 2) return kfail(b, JH_ERR_ARG, "stage must be 0, 1 or 2");
+  if (!(dt > 0)) return kfail(b, JH_ERR_ARG, "dt must be > 0");
+  return JH_OK;
 }
 
 }  // namespace
@@ -1507,6 +1551,8 @@ int jh_blocks_create(int bnx, int bny, double dx, double dy, int precision, cons
   b->stream = static_cast<hipStream_t>(stream);
   hipError_t e = hipMalloc(&b->mc, sizeof(MechConst));
   if (!e) e = hipMemcpy(b->mc, mech, sizeof(MechConst), hipMemcpyHostToDevice);
+  // set-up for jh_blocks_stage_noalloc: this device's address of `scratch`
+  if (!e) e = scratch_address(&b->scratch);
   if (e) {
     g_blocks_err = std::string("allocation: ") + hipGetErrorString(e);
     jh_blocks_destroy(b);
@@ -1544,26 +1590,43 @@ int jh_blocks_flux(jh_blocks* b, int n_blk, const void* blk, double dt, void* rh
   if (rc) return rc;
   if (b->precision == JH_PREC_FP64)
     return blocks_stage_impl<double, MODE_RHS>(b, n_blk, 0, (const double*)blk,
-                                               (const double*)blk, (double*)rhs, dt);
+                                               (const double*)blk, (double*)rhs, dt, b->sc);
   return blocks_stage_impl<float, MODE_RHS>(b, n_blk, 0, (const float*)blk, (const float*)blk,
-                                            (float*)rhs, dt);
+                                            (float*)rhs, dt, b->sc);
 }
 
 int jh_blocks_stage(jh_blocks* b, int n_blk, int stage, const void* blk_s, const void* blk_n,
                     double dt, void* blk_out) {
   if (!b) return JH_ERR_ARG;
-  if (!blk_s || !blk_n || !blk_out) return kfail(b, JH_ERR_ARG, "NULL pointer");
-  if (blk_out == blk_s || blk_out == blk_n)
-    return kfail(b, JH_ERR_ARG, "blk_out must not alias the inputs");
-  if (stage < 0 || stage > 2) return kfail(b, JH_ERR_ARG, "stage must be 0, 1 or 2");
-  if (!(dt > 0)) return kfail(b, JH_ERR_ARG, "dt must be > 0");
-  int rc = blocks_checks(b, n_blk);
+  int rc = stage_args_check(b, stage, blk_s, blk_n, dt, blk_out);
+  if (!rc) rc = blocks_checks(b, n_blk);
   if (rc) return rc;
   if (b->precision == JH_PREC_FP64)
     return blocks_stage_impl<double, MODE_RK>(b, n_blk, stage, (const double*)blk_s,
-                                              (const double*)blk_n, (double*)blk_out, dt);
+                                              (const double*)blk_n, (double*)blk_out, dt, b->sc);
   return blocks_stage_impl<float, MODE_RK>(b, n_blk, stage, (const float*)blk_s,
-                                           (const float*)blk_n, (float*)blk_out, dt);
+                                           (const float*)blk_n, (float*)blk_out, dt, b->sc);
+}
+
+int jh_blocks_stage_noalloc(jh_blocks* b, int n_blk, int stage, const void* blk_s,
+                            const void* blk_n, double dt, void* blk_out) {
+  if (!b) return JH_ERR_ARG;
+  int rc = stage_args_check(b, stage, blk_s, blk_n, dt, blk_out);
+  if (!rc && n_blk > kNoallocBlocks)
+    rc = kfail(b, JH_ERR_ARG, "n_blk exceeds the no-allocation entry's " +
+                                  std::to_string(kNoallocBlocks) + "-block scratch");
+  if (!rc) rc = blocks_checks(b, n_blk, false);
+  if (rc) return rc;
+  if (!b->scratch) return kfail(b, JH_ERR_HIP, "no device address for the scratch array");
+  // nothing in the scratch needs clearing: the LF maxima are stored (not
+  // folded) by blocks_alpha_kernel before the stage kernel reads them, and
+  // the health flags it also holds are not reported by this entry point
+  if (b->precision == JH_PREC_FP64)
+    return blocks_stage_impl<double, MODE_RK>(b, n_blk, stage, (const double*)blk_s,
+                                              (const double*)blk_n, (double*)blk_out, dt,
+                                              b->scratch);
+  return blocks_stage_impl<float, MODE_RK>(b, n_blk, stage, (const float*)blk_s,
+                                           (const float*)blk_n, (float*)blk_out, dt, b->scratch);
 }
 
 int jh_blocks_chem(jh_blocks* b, int n_blk, void* blk, double dt) {
@@ -1576,10 +1639,10 @@ int jh_blocks_chem(jh_blocks* b, int n_blk, void* blk, double dt) {
   const long long bs = (long long)(NU + 2) * n;
   const dim3 grid((n + 255) / 256, n_blk);
   if (b->precision == JH_PREC_FP64)
-    blocks_chem_kernel<double><<<grid, 256, 0, b->stream>>>(blocks_args<double>(b, dt), bs,
-                                                            (double*)blk);
+    blocks_chem_kernel<double><<<grid, 256, 0, b->stream>>>(blocks_args<double>(b, dt, b->sc),
+                                                            bs, (double*)blk);
   else
-    blocks_chem_kernel<float><<<grid, 256, 0, b->stream>>>(blocks_args<float>(b, dt), bs,
+    blocks_chem_kernel<float><<<grid, 256, 0, b->stream>>>(blocks_args<float>(b, dt, b->sc), bs,
                                                            (float*)blk);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return kfail(b, JH_ERR_HIP, std::string("block chemistry: ") +
