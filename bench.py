@@ -501,9 +501,25 @@ def _adjoint_leg(mech, steps, every, ndir, torch):
         r = (adjoint.pressure(mech, UN, aN) - p_target) / p_ref
         return q + 0.5 * (r * r).sum() / cells
 
-    def J_fwd(Tx, Yx):
+    def J_fwd(Tx, Yx, trace=False, clip=None):
+        """J after the rollout; with trace, also the per-step records of
+        where the LF maxima are attained and which clip branches are active
+        (Simulator.trace); clip = ('record' | 'replay', masks) records the
+        chemistry clip's branches or makes the rollout take recorded ones
+        (Simulator.clip_branches)."""
         U0, a0 = adjoint.initial_state(mech, Tx, pt, Yx, ut, vt)
-        UN, aN = sim.advance(U0.clone(), a0.clone(), steps)[:2]
+        Ur, ar = U0.clone(), a0.clone()
+        rec = None
+        if trace:
+            rec = sim.trace(Ur, torch.zeros((steps, 22), dtype=torch.int64, device=dev))
+        if clip is not None:
+            sim.clip_branches(Ur, clip[1], clip[0])
+        UN, aN = sim.advance(Ur, ar, steps)[:2]
+        if clip is not None:
+            sim.clip_branches(Ur, None, "off")
+        if trace:
+            sim.trace(Ur, None)
+            return float(J(U0, UN, aN)), rec.cpu().numpy()
         return float(J(U0, UN, aN))
 
     # forward rollout alone (fused library call)
@@ -557,20 +573,70 @@ def _adjoint_leg(mech, steps, every, ndir, torch):
     # two-point quotient's own truncation error (chemistry makes J strongly
     # nonlinear in T) falls as eps^2 while the rounding of J grows as 1/eps;
     # the four-point quotient's truncation error falls as eps^4
-    errs = {"1e-06": [], "1e-07": [], "1e-06_4pt": []}
+    # the unperturbed trajectory's branches (where each stage's LF maxima
+    # are attained, the active clip branches)
+    _, rec0 = J_fwd(Tt, Yt, trace=True)
+    # ... and every cell's clip branches along it, replayed by the
+    # branch-frozen runs: the function the adjoint differentiates
+    masks = torch.zeros((steps, cells), dtype=torch.int16, device=dev)
+    J_fwd(Tt, Yt, clip=("record", masks))
+    errs = {"1e-06": [], "1e-07": [], "1e-06_4pt": [], "frozen_1e-06": [],
+            "frozen_1e-06_4pt": []}
+    switches = []
+    # record columns: per stage (x index, x count, y index, y count) of the
+    # LF maxima; clip hash, clip count; per temperature solve (hash, count)
+    # of the cells below the NASA-7 switch
+    LF, CLIP, RANGE = slice(0, 12), slice(12, 14), slice(14, 22)
     for d in range(ndir):
         dT = Tt * torch.tensor(rng.standard_normal(T.shape), device=dev)
         dY = Yt * torch.tensor(rng.standard_normal(Y.shape), device=dev)
         ad = float((Tg.grad * dT).sum() + (Yg.grad * dY).sum())
-        Jp = {}
-        for k in (-2, -1, 1, 2):
-            Jp[k] = J_fwd(Tt + k * 1e-6 * dT, Yt + k * 1e-6 * dY)
+        Jp = {k: J_fwd(Tt + k * 1e-6 * dT, Yt + k * 1e-6 * dY) for k in (-2, -1, 1, 2)}
         fds = {"1e-06": (Jp[1] - Jp[-1]) / 2e-6,
                "1e-07": (J_fwd(Tt + 1e-7 * dT, Yt + 1e-7 * dY) -
                          J_fwd(Tt - 1e-7 * dT, Yt - 1e-7 * dY)) / 2e-7,
                "1e-06_4pt": (8.0 * (Jp[1] - Jp[-1]) - (Jp[2] - Jp[-2])) / 12e-6}
+        # branch-frozen runs (the unperturbed trajectory's clip branches),
+        # traced: which other branches the +-eps, +-2 eps runs change
+        Jf, recs = {}, {}
+        for k in (-2, -1, 1, 2):
+            Jf[k], recs[k] = J_fwd(Tt + k * 1e-6 * dT, Yt + k * 1e-6 * dY, trace=True,
+                                   clip=("replay", masks))
+        fds["frozen_1e-06"] = (Jf[1] - Jf[-1]) / 2e-6
+        fds["frozen_1e-06_4pt"] = (8.0 * (Jf[1] - Jf[-1]) - (Jf[2] - Jf[-2])) / 12e-6
         for key, fd in fds.items():
             errs[key].append(abs(fd - ad) / max(abs(fd), abs(ad), 1e-300))
+        sw = {}
+        for name, cols in (("lf", LF), ("clip", CLIP), ("range", RANGE)):
+            sw[name + "_2pt"] = bool(any((recs[k][:, cols] != rec0[:, cols]).any()
+                                         for k in (-1, 1)))
+            sw[name + "_4pt"] = bool(any((recs[k][:, cols] != rec0[:, cols]).any()
+                                         for k in (-2, -1, 1, 2)))
+        switches.append(sw)
+    free2 = [i for i, sw in enumerate(switches) if not (sw["lf_2pt"] or sw["range_2pt"])]
+    free4 = [i for i, sw in enumerate(switches) if not (sw["lf_4pt"] or sw["range_4pt"])]
+
+    def _stats(key, idx):
+        if not idx:
+            return {"directions": 0}
+        v = [errs[key][i] for i in idx]
+        return {"directions": len(idx), "max_rel_err": max(v),
+                "median_rel_err": float(np.median(v)), "within_1e-6": bool(max(v) <= 1e-6)}
+
+    fd_free = {"2pt_frozen_eps1e-6": _stats("frozen_1e-06", free2),
+               "4pt_frozen_eps1e-6": _stats("frozen_1e-06_4pt", free4),
+               "note": "directions whose branch-frozen (clip) runs change no LF-maximum "
+                       "cell and move no cell across the NASA-7 switch temperature"}
+    per_dir = [dict(sw, rel_err_1e6=errs["1e-06"][i], rel_err_1e7=errs["1e-07"][i],
+                    rel_err_4pt=errs["1e-06_4pt"][i], frozen_1e6=errs["frozen_1e-06"][i],
+                    frozen_4pt=errs["frozen_1e-06_4pt"][i])
+               for i, sw in enumerate(switches)]
+    frozen = ({"max_rel_err_2pt": max(errs["frozen_1e-06"]),
+               "max_rel_err_4pt": max(errs["frozen_1e-06_4pt"]),
+               "median_rel_err_4pt": float(np.median(errs["frozen_1e-06_4pt"])),
+               "bar": 1e-6, "within_bar": bool(max(errs["frozen_1e-06_4pt"]) <= 1e-6),
+               "note": "finite differences of rollouts that replay the unperturbed "
+                       "trajectory's chemistry clip branches"} if ndir else None)
     return {"workload": "1024x256 tube (dx 1.25e-5 m, 51-column 30 atm/3000 K driver, 0.5%% T "
                         "and 3 m/s velocity noise), FP64, fixed dt %.3e s, %d steps, checkpoints "
                         "every %d; J = heat release + pressure mismatch; grad wrt initial T, Y_k"
@@ -585,15 +651,84 @@ def _adjoint_leg(mech, steps, every, ndir, torch):
             "adjoint_over_fwd_rollout": (t_total - t_roll) / t_fwd,
             "fd_directions": ndir,
             "fd_max_rel_err": {e: max(v) for e, v in errs.items()} if ndir else None,
+            "fd_switch_free": fd_free if ndir else None,
+            "fd_branch_frozen": frozen,
+            "fd_switches": dict({k: sum(sw[k] for sw in switches) for k in switches[0]},
+                                clip_branches_active_per_step=float(rec0[:, 13].mean()),
+                                cells_below_nasa7_switch_per_solve=float(
+                                    rec0[:, 15:22:2].mean())) if ndir else None,
+            "fd_per_direction": per_dir if ndir else None,
             "fd_median_rel_err": ({e: float(np.median(v)) for e, v in errs.items()}
                                   if ndir else None)}
 
 
 
 
+def _amr_leg(mech, U3, aux3, g3, dt, steps, warmup, torch):
+    """The AMR branch (advance_func_amr, set_simulator.py:496-532) on one
+    level: the 8000 x 500 tube, FP64, tiled by same-size blocks, each step
+    three block stages with the uniform block grid's ghost refill between them
+    and the blocks' chemistry.  Timed: whole steps, and the library's block
+    calls alone (torch events on the stream they run on)."""
+    from janc_amd import cases
+    from janc_amd.amr import BlockSolver, UniformBlockGrid
+    nx, ny = U3.shape[1], U3.shape[2]
+    bs = BlockSolver(cases.simulation_config(g3, dt=dt, precision="fp64"))
+    dx, dy = g3["dx"], g3["dy"]
+    out = {"workload": "8000x500 tube (FP64, fixed dt %.4g s) tiled by same-size blocks; "
+                       "per step 3 block stages + ghost refills + block chemistry" % dt,
+           "steps": steps}
+    Ut = torch.as_tensor(U3).cuda()
+    at = torch.as_tensor(aux3).cuda()
+    for bnx, bny in ((40, 50), (20, 20)):
+        grid = UniformBlockGrid(g3["boundary_config"], nx // bnx, ny // bny, bnx, bny)
+        blk0 = grid.split(Ut, at)
+        lib_ms = [0.0]
+
+        def timed(fn, *a):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            r = fn(*a)
+            e1.record()
+            ev.append((e0, e1))
+            return r
+
+        def step(b):
+            g0 = grid.get_ghost_block_data(b)
+            gs = g0
+            for s in range(3):
+                o = timed(bs.stage, s, gs, g0, dx, dy, dt)
+                b = grid.update_external_boundary(1, None, o[..., 3:-3, 3:-3])
+                if s < 2:
+                    gs = grid.get_ghost_block_data(b)
+            return timed(bs.chemistry, b.contiguous(), dx, dy, dt)
+
+        b = blk0.clone()
+        ev = []
+        for _ in range(warmup):
+            b = step(b)
+        torch.cuda.synchronize()
+        ev = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b = step(b)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        lib = sum(e0.elapsed_time(e1) for e0, e1 in ev) / steps
+        cells = nx * ny
+        out["blocks_%dx%d" % (bnx, bny)] = {
+            "n_blk": grid.n_blk, "value": cells * steps / el, "ms_per_step": el / steps * 1e3,
+            "library_ms_per_step": lib, "library_value": cells / (lib * 1e-3),
+            "health_flags": bs.health()}
+        del blk0, b, grid
+        torch.cuda.empty_cache()
+    return out
+
+
 def _summary_adjoint(adj):
     keys = ("adjoint_over_fwd_step", "adjoint_over_fwd_rollout", "step_fwd_ms", "step_vjp_ms",
-            "fd_directions", "fd_max_rel_err", "fd_median_rel_err")
+            "fd_directions", "fd_max_rel_err", "fd_median_rel_err", "fd_switch_free",
+            "fd_branch_frozen", "fd_switches")
     return {k: adj[k] for k in keys if k in adj}
 
 
@@ -618,6 +753,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--amr-steps", type=int, default=10)
+    ap.add_argument("--no-amr", action="store_true")
     ap.add_argument("--fd-directions", type=int, default=32)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="one 256 MiB device-to-device copy before the timed region "
@@ -637,7 +774,7 @@ def main():
         sys.exit(2)
     only = args.headline_only or args.annulus_only
     if only:
-        args.no_cpu = args.no_batch = args.no_adjoint = True
+        args.no_cpu = args.no_batch = args.no_adjoint = args.no_amr = True
         args.no_schemes = args.no_snapshots = True
         args.accuracy_steps = args.crop_steps = 0
         args.no_annulus = args.headline_only
@@ -758,6 +895,12 @@ def _gpu_legs(args, mech, U3, aux3, g3, cells, pool, cpu, only):
         bat = _batch_leg(mech, args.batch_steps, tune, torch)
         emit("batch", bat)
         torch.cuda.empty_cache()
+    amr_out = None
+    if not args.no_amr:
+        dt_amr = dt_fixed if dt_fixed is not None else head["fp64"]["dt_first"]
+        amr_out = _amr_leg(mech, U3, aux3, g3, dt_amr, args.amr_steps, 2, torch)
+        emit("amr_blocks", amr_out)
+        torch.cuda.empty_cache()
     adj = None
     if not args.no_adjoint:
         adj = _adjoint_leg(mech, args.adjoint_steps, 10, args.fd_directions, torch)
@@ -816,6 +959,10 @@ def _gpu_legs(args, mech, U3, aux3, g3, cells, pool, cpu, only):
         summary["batch"] = {"value": bat["value"], "speedup_vs_sequential":
                             bat["speedup_vs_sequential"],
                             "bit_identical_to_sequential": bat["bit_identical_to_sequential"]}
+    if amr_out is not None:
+        summary["amr_blocks_fp64"] = {
+            k: {"value": v["value"], "library_value": v["library_value"], "n_blk": v["n_blk"]}
+            for k, v in amr_out.items() if k.startswith("blocks_")}
     if ann is not None:
         summary["fp64_annulus_4000x400"] = {"value": ann["value"],
                                             "ms_per_step": ann["ms_per_step"]}

@@ -129,6 +129,32 @@ int jh_ctx_get_tile(const jh_ctx* ctx);
 enum jh_halley_stop { JH_HALLEY_PER_CELL = 0, JH_HALLEY_GRID = 1 };
 int jh_ctx_set_halley_stop(jh_ctx* ctx, int mode);
 int jh_ctx_get_halley_stop(const jh_ctx* ctx);
+
+/* Step tracing, no counterpart in the reference: while set, each step of
+ * jh_advance (up to max_steps) writes a record of 22 int64 into records
+ * (device): per RK stage, the first ghost-padded index (P = (x+3)(ny+6) +
+ * y+3) attaining max|u|+a and the number of padded cells attaining it, the
+ * same for max|v|+a (the nanmax of LF.py:10-11, corners excluded); then a
+ * hash and the count of the active branches of the chemistry clip
+ * (reaction_model.py:191); then, after each of the step's temperature solves
+ * (three stages, chemistry), a hash and the count of the cells below the
+ * NASA-7 switch temperature (thermo_model.py:101-104; the two fits are not
+ * continuous there).  The step is only piecewise smooth in these: reverse
+ * mode, and jh_step_vjp, follow the branch taken, so a finite-difference
+ * check of the adjoint holds only across perturbations that change none of
+ * them.  Adds kernels to every traced step; records = NULL stops tracing. */
+int jh_ctx_set_trace(jh_ctx* ctx, long long* records, int max_steps);
+
+/* Chemistry clip branches (reaction_model.py:191), no counterpart in the
+ * reference: JH_CLIP_RECORD writes, for each following step of jh_advance
+ * (up to max_steps), the branch every cell's clip takes (uint16 per cell:
+ * bit k = lower bound for species k, bit 8 + k = upper bound) into masks
+ * (device, (max_steps, nx*ny)); JH_CLIP_REPLAY makes the chemistry take
+ * those branches instead of comparing.  A replayed rollout from a perturbed
+ * state is the branch-wise smooth function the adjoint differentiates (a
+ * finite-difference check of jh_step_vjp without the clip's kinks). */
+enum jh_clip_mode { JH_CLIP_OFF = 0, JH_CLIP_RECORD = 1, JH_CLIP_REPLAY = 2 };
+int jh_ctx_set_clip_branches(jh_ctx* ctx, unsigned short* masks, int max_steps, int mode);
 int jh_tile_variants(void);
 int jh_tile_info(int variant, int* rows, int* threads, int* waves);
 const char* jh_last_error(const jh_ctx* ctx);

@@ -49,6 +49,8 @@ SIGNATURES = {
     "jh_ctx_get_tile": (_c_int, [_vp]),
     "jh_ctx_set_halley_stop": (_c_int, [_vp, _c_int]),
     "jh_ctx_get_halley_stop": (_c_int, [_vp]),
+    "jh_ctx_set_trace": (_c_int, [_vp, _vp, _c_int]),
+    "jh_ctx_set_clip_branches": (_c_int, [_vp, _vp, _c_int, _c_int]),
     "jh_tile_variants": (_c_int, []),
     "jh_tile_info": (_c_int, [_c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),
                               ctypes.POINTER(_c_int)]),

@@ -172,3 +172,110 @@ def AMR_Simulator(simulation_config):
         return torch.cat([U, aux], 0)[None]
 
     return solver.advance_one_step, advance_func_base
+
+
+class UniformBlockGrid:
+    """One refinement level tiled by nbx x nby same-size blocks of bnx x bny
+    cells (block k = (i, j), k = i * nby + j), with the configuration's
+    boundary kinds at the domain's edges: the ghost refill advance_func_amr
+    takes from the AMR hierarchy (get_ghost_block_data,
+    update_external_boundary), for a uniform single-level grid.
+
+    A block's ghost layers are its neighbours' cells; at the domain's edges
+    they are the reference's boundary padding (boundary.py:126-132): the
+    off-by-one periodic pre-pad (boundary_padding.py:20-25; left ghosts =
+    cells N-4..N-2, right = 1..3), then each non-periodic side's strip over
+    the other axis' interior range -- zero_gradient mirrors (zero_gradient.py),
+    slip_wall copies the wall cell with the normal momentum negated
+    (slip_wall.py), no_slip_wall mirrors with both momenta negated
+    (no_slip_wall.py); corners keep the periodic pre-pad.  Pressure outlets,
+    injection inlets and callables compute their ghosts and are not offered.
+    One gather per refill (torch, on the blocks' device)."""
+
+    KINDS = ("periodic", "zero_gradient", "slip_wall", "no_slip_wall")
+
+    def __init__(self, boundary_config, nbx, nby, bnx, bny, device="cuda"):
+        kinds = [boundary_config[s + "_boundary"] for s in ("left", "right", "bottom", "top")]
+        for k in kinds:
+            if callable(k) or k not in self.KINDS:
+                raise NotImplementedError("UniformBlockGrid fills copied ghost kinds only (%s), "
+                                          "not %r" % (", ".join(self.KINDS), k))
+        self.nbx, self.nby, self.bnx, self.bny = nbx, nby, bnx, bny
+        NX, NY = nbx * bnx, nby * bny
+        gx = torch.arange(-3, NX + 3)
+        gy = torch.arange(-3, NY + 3)
+
+        def per(g, N):                     # boundary_padding.py:20-25
+            return torch.where(g < 0, N - 1 + g, torch.where(g >= N, g - N + 1, g))
+
+        def side(g, N, lo, hi):
+            """source index and (normal, tangential) momentum signs along one axis."""
+            src = per(g, N)
+            sn = torch.ones_like(g, dtype=torch.float64)
+            st = torch.ones_like(g, dtype=torch.float64)
+            for kind, mask, mirror, wall in ((lo, g < 0, -g - 1, torch.zeros_like(g)),
+                                             (hi, g >= N, 2 * N - 1 - g,
+                                              torch.full_like(g, N - 1))):
+                if kind == "zero_gradient":
+                    src = torch.where(mask, mirror, src)
+                elif kind == "slip_wall":
+                    src = torch.where(mask, wall, src)
+                    sn = torch.where(mask, -1.0, sn)
+                elif kind == "no_slip_wall":
+                    src = torch.where(mask, mirror, src)
+                    sn = torch.where(mask, -1.0, sn)
+                    st = torch.where(mask, -1.0, st)
+            return src, sn, st
+
+        sx, snx, stx = side(gx, NX, kinds[0], kinds[1])
+        sy, sny, sty = side(gy, NY, kinds[2], kinds[3])
+        inx = (gx >= 0) & (gx < NX)
+        iny = (gy >= 0) & (gy < NY)
+        # corners (both axes outside) keep the periodic pre-pad; a strip uses
+        # its own axis' kind over the other axis' interior
+        px = per(gx, NX)
+        py = per(gy, NY)
+        X = torch.where(iny[None, :], sx[:, None], px[:, None])
+        Y = torch.where(inx[:, None], sy[None, :], py[None, :])
+        one = torch.ones((), dtype=torch.float64)
+        s_u = torch.where(iny[None, :], snx[:, None], one) * \
+            torch.where(inx[:, None], sty[None, :], one)
+        s_v = torch.where(inx[:, None], sny[None, :], one) * \
+            torch.where(iny[None, :], stx[:, None], one)
+        # column of domain cell (x, y) in the block-major interior flattening
+        col = ((X // bnx) * nby + Y // bny) * (bnx * bny) + (X % bnx) * bny + Y % bny
+        PX, PY = bnx + 6, bny + 6
+        bi = torch.arange(nbx)[:, None, None, None] * bnx + torch.arange(PX)[None, None, :, None]
+        bj = torch.arange(nby)[None, :, None, None] * bny + torch.arange(PY)[None, None, None, :]
+        self.idx = col[bi, bj].reshape(-1).to(device)             # (n_blk * PX * PY,)
+        self.s_u = s_u[bi, bj].reshape(nbx * nby, PX, PY).to(device)
+        self.s_v = s_v[bi, bj].reshape(nbx * nby, PX, PY).to(device)
+        self.n_blk = nbx * nby
+
+    def split(self, U, aux):
+        """Domain state (U (12, NX, NY), aux (2, NX, NY)) -> interior blocks."""
+        full = torch.cat([U, aux], 0)
+        nv = full.shape[0]
+        b = full.reshape(nv, self.nbx, self.bnx, self.nby, self.bny)
+        return b.permute(1, 3, 0, 2, 4).reshape(self.n_blk, nv, self.bnx, self.bny).contiguous()
+
+    def assemble(self, blk):
+        """Interior blocks -> the domain's (U, aux)."""
+        nv = blk.shape[1]
+        b = blk.reshape(self.nbx, self.nby, nv, self.bnx, self.bny)
+        full = b.permute(2, 0, 3, 1, 4).reshape(nv, self.nbx * self.bnx, self.nby * self.bny)
+        return full[:-2], full[-2:]
+
+    def get_ghost_block_data(self, blk, info=None):
+        """Interior blocks (n_blk, 14, bnx, bny) -> padded blocks (n_blk, 14,
+        bnx + 6, bny + 6)."""
+        nb, nv = blk.shape[0], blk.shape[1]
+        flat = blk.reshape(nb, nv, -1).permute(1, 0, 2).reshape(nv, -1)
+        g = flat.index_select(1, self.idx).reshape(nv, nb, self.bnx + 6, self.bny + 6)
+        g[1] *= self.s_u.to(g.dtype)
+        g[2] *= self.s_v.to(g.dtype)
+        return g.permute(1, 0, 2, 3).contiguous()
+
+    def update_external_boundary(self, level, parent, interiors, info=None):
+        """A single level: the interiors are the blocks."""
+        return interiors.contiguous()

@@ -103,13 +103,18 @@ __host__ __device__ __forceinline__ bool lu_solve(double (&A)[N][N], double (&b)
 // Increment of rho*Y_k for the active species (detailed_reaction, :183-193).
 // rhoY_in: the state's rho*Y_k for the NS-1 transported species; gt: the
 // Gibbs tables (c itself, or its copy in LDS).  Returns lu_solve's pivot
-// check.
+// check.  clip (optional): bit k set where the clip of species k takes its
+// lower bound -Y_k, bit JH_NA + k where it takes 1 - Y_k.  branches
+// (optional): take the clip's branches from these bits instead of comparing
+// (a diagnostic that keeps a perturbed state on the branches of another).
 template <typename GTab>
 __host__ __device__ __forceinline__ bool chem_increment(const MechConst& c, const GTab& gt,
                                                         double rho,
                                                         const double (&rhoY_in)[JH_NS - 1],
                                                         double T, double dt,
-                                                        double (&drhoY)[JH_NA]) {
+                                                        double (&drhoY)[JH_NA],
+                                                        unsigned int* clip = nullptr,
+                                                        const unsigned int* branches = nullptr) {
   const double irho = rcp(rho);
   double Y[JH_NS];
 #pragma unroll
@@ -145,12 +150,21 @@ __host__ __device__ __forceinline__ bool chem_increment(const MechConst& c, cons
     for (int n = 0; n < JH_NA; ++n) J[k][n] = ((k == n) ? idt : 0.0) - J[k][n];
   }
   const bool ok = lu_solve<JH_NA>(J, w);
+  unsigned int cm = 0u;
 #pragma unroll
   for (int k = 0; k < JH_NA; ++k) {
-    double dY = w[k] * irho;
-    dY = fmin(fmax(dY, -Y[k]), 1.0 - Y[k]);
+    const double x = w[k] * irho;
+    if (clip) cm |= (x < -Y[k] ? 1u << k : 0u) | (x > 1.0 - Y[k] ? 1u << (JH_NA + k) : 0u);
+    double dY;
+    if (branches) {
+      const unsigned int b = *branches;
+      dY = (b >> k) & 1u ? -Y[k] : (b >> (JH_NA + k)) & 1u ? 1.0 - Y[k] : x;
+    } else {
+      dY = fmin(fmax(x, -Y[k]), 1.0 - Y[k]);
+    }
     drhoY[k] = rho * dY;
   }
+  if (clip) *clip = cm;
   return ok;
 }
 

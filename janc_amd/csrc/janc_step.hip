@@ -44,6 +44,14 @@ struct jh_ctx {
   int halley = JH_HALLEY_PER_CELL;
   void* hside = nullptr;
   unsigned char* hnit = nullptr;
+  // step tracing (jh_ctx_set_trace): records of kTraceRecord int64 per step
+  long long* trace = nullptr;
+  int trace_cap = 0, trace_step = 0;
+  TraceScratch* trs = nullptr;
+  // chemistry clip branches recorded / replayed per step
+  // (jh_ctx_set_clip_branches): (max steps, nx*ny) masks
+  unsigned short* clipm = nullptr;
+  int clip_mode = 0, clip_cap = 0, clip_step = 0;
   std::string err;
   // kernel timing with HIP events on the context stream (jh_timing_*)
   bool timing = false;             // record events around the current launches
@@ -134,6 +142,7 @@ StepArgs<Real> make_args(jh_ctx* c, double dt_fixed, double cfl) {
   const bool grid = c->halley == JH_HALLEY_GRID;
   a.hside = grid ? static_cast<Real*>(c->hside) : nullptr;
   a.hnit = grid ? c->hnit : nullptr;
+  a.clipm = nullptr;
   return a;
 }
 
@@ -277,6 +286,23 @@ int prologue(jh_ctx* c, const Real* U, const Real* aux, double dt, double cfl) {
   return check_launch(c, "reduce_kernel");
 }
 
+// Tracing around one RK stage's input (its ghost strips filled): where the
+// stage's LF maxima are attained, into rec (4 values).
+template <typename Real>
+void trace_range(jh_ctx* c, const StepArgs<Real>& a, const Real* aux, long long* rec) {
+  zero_i64_kernel<<<1, 64, 0, c->stream>>>(rec, 2);
+  trace_range_kernel<Real><<<grid_all(c), 256, 0, c->stream>>>(a, aux, rec);
+}
+
+template <typename Real>
+void trace_stage(jh_ctx* c, const StepArgs<Real>& a, const Real* Us, const Real* As,
+                 long long* rec) {
+  const int nb = std::min(((c->nx + 6) * (c->ny + 6) + 255) / 256, 1024);
+  trace_max_kernel<Real><<<nb, 256, 0, c->stream>>>(a, Us, As, c->trs);
+  trace_arg_kernel<Real><<<nb, 256, 0, c->stream>>>(a, Us, As, c->trs);
+  trace_store_kernel<<<1, 64, 0, c->stream>>>(c->trs, rec);
+}
+
 template <typename Real>
 int advance_impl(jh_ctx* c, const Real* U, const Real* aux, Real* Uo, Real* auxo, double dt,
                  double cfl, int nsteps, double* dt_hist) {
@@ -296,27 +322,54 @@ int advance_impl(jh_ctx* c, const Real* U, const Real* aux, Real* Uo, Real* auxo
     ++c->timed_steps;
     auto a = make_args<Real>(c, dt, cfl);
     const int bb = bc_blocks(c);
+    long long* rec = (c->trace && c->trace_step < c->trace_cap)
+                         ? c->trace + (size_t)kTraceRecord * c->trace_step++ : nullptr;
     // stage 1: (Un, An) -> (U1, A1)
     { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, Un, An, 0); }
+    if (rec) trace_stage<Real>(c, a, Un, An, rec);
     { Timed t(c, JH_KERNEL_STAGE);
       launch_stage<Real, MODE_RK>(c, a, 0, Un, An, Un, U1, A1);
       halley_fix<Real>(c, 0, c->parity, 1, 0, A1); }
     // stage 2: (U1, A1) -> (U2, A2)
     { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, U1, A1, 1); }
+    if (rec) {
+      trace_range<Real>(c, a, A1, rec + 14);
+      trace_stage<Real>(c, a, U1, A1, rec + 4);
+    }
     { Timed t(c, JH_KERNEL_STAGE);
       launch_stage<Real, MODE_RK>(c, a, 1, U1, A1, Un, U2, A2);
       halley_fix<Real>(c, 1, c->parity, 2, 0, A2); }
     // stage 3: (U2, A2) -> (Uo, auxo); Uo may alias Un (each thread reads
     // Un at its own cell before writing it)
     { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, U2, A2, 2); }
+    if (rec) {
+      trace_range<Real>(c, a, A2, rec + 16);
+      trace_stage<Real>(c, a, U2, A2, rec + 8);
+    }
     { Timed t(c, JH_KERNEL_STAGE);
       launch_stage<Real, MODE_RK>(c, a, 2, U2, A2, Un, Uo, auxo);
       halley_fix<Real>(c, 2, c->parity, -1, 0, auxo); }
     // chemistry + final update_aux in place, next step's maxima
+    if (rec) {
+      trace_range<Real>(c, a, auxo, rec + 18);
+      zero_i64_kernel<<<1, 64, 0, c->stream>>>(rec + 12, 2);
+      trace_clip_kernel<Real><<<grid_all(c), 256, 0, c->stream>>>(a, Uo, auxo, rec + 12);
+    }
+    const bool replay = c->clip_mode == JH_CLIP_REPLAY && c->clip_step < c->clip_cap;
+    if (c->clip_mode && c->clip_step < c->clip_cap) {
+      a.clipm = c->clipm + (size_t)n * c->clip_step++;
+      if (c->clip_mode == JH_CLIP_RECORD)
+        clip_mask_kernel<Real><<<grid_all(c), 256, 0, c->stream>>>(a, Uo, auxo);
+    }
     { Timed t(c, JH_KERNEL_CHEM);
-      chem_kernel<Real, CHEM_STEP><<<grid_all(c), 256, 0, c->stream>>>(
-          a, Uo, auxo, nullptr, dt_hist ? dt_hist + s : nullptr);
+      if (replay)
+        chem_kernel<Real, CHEM_STEP, true><<<grid_all(c), 256, 0, c->stream>>>(
+            a, Uo, auxo, nullptr, dt_hist ? dt_hist + s : nullptr);
+      else
+        chem_kernel<Real, CHEM_STEP><<<grid_all(c), 256, 0, c->stream>>>(
+            a, Uo, auxo, nullptr, dt_hist ? dt_hist + s : nullptr);
       halley_fix<Real>(c, JH_PROD_CHEM, c->parity ^ 1, 0, 1, auxo); }
+    if (rec) trace_range<Real>(c, a, auxo, rec + 20);
     rc = check_launch(c, "step kernels");
     if (rc) {
       c->timing = timing;
@@ -965,6 +1018,7 @@ int jh_ctx_destroy(jh_ctx* c) {
   }
   for (auto e : c->ev_free) (void)hipEventDestroy(e);
   if (c->adj) (void)hipFree(c->adj);
+  if (c->trs) (void)hipFree(c->trs);
   if (c->hside) (void)hipFree(c->hside);
   if (c->hnit) (void)hipFree(c->hnit);
   (void)hipFree(c->mc);
@@ -1017,6 +1071,34 @@ int jh_ctx_set_halley_stop(jh_ctx* c, int mode) {
 }
 
 int jh_ctx_get_halley_stop(const jh_ctx* c) { return c ? c->halley : -1; }
+
+int jh_ctx_set_clip_branches(jh_ctx* c, unsigned short* masks, int max_steps, int mode) {
+  if (!c) return JH_ERR_ARG;
+  if (mode != JH_CLIP_OFF && mode != JH_CLIP_RECORD && mode != JH_CLIP_REPLAY)
+    return fail(c, JH_ERR_ARG, "unknown clip-branch mode");
+  if (mode != JH_CLIP_OFF && (!masks || max_steps < 1))
+    return fail(c, JH_ERR_ARG, "clip-branch masks need a buffer of max_steps >= 1 steps");
+  c->clipm = mode ? masks : nullptr;
+  c->clip_mode = mode;
+  c->clip_cap = mode ? max_steps : 0;
+  c->clip_step = 0;
+  return JH_OK;
+}
+
+int jh_ctx_set_trace(jh_ctx* c, long long* records, int max_steps) {
+  if (!c) return JH_ERR_ARG;
+  if (records && max_steps < 1) return fail(c, JH_ERR_ARG, "max_steps must be >= 1");
+  if (records && !c->trs) {
+    HIP_TRY(c, hipMalloc(&c->trs, sizeof(TraceScratch)));
+    trace_store_kernel<<<1, 64, 0, c->stream>>>(c->trs, nullptr);
+    const int rc = check_launch(c, "trace_store_kernel");
+    if (rc) return rc;
+  }
+  c->trace = records;
+  c->trace_cap = records ? max_steps : 0;
+  c->trace_step = 0;
+  return JH_OK;
+}
 
 int jh_ctx_set_flux(jh_ctx* c, int solver_type, int split_method, int reconstruction,
                     int riemann_solver) {

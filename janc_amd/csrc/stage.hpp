@@ -243,6 +243,10 @@ struct StepArgs {
   // applies N*.
   Real* hside;
   unsigned char* hnit;
+  // Chemistry clip branches of this step, one mask per cell (clip bits of
+  // chem_increment): written by clip_mask_kernel, read by the replaying
+  // chemistry kernel (jh_ctx_set_clip_branches); null otherwise.
+  unsigned short* clipm;
 };
 
 template <typename Real>
@@ -1111,7 +1115,7 @@ enum ChemMode : int { CHEM_STEP = 0, CHEM_SOURCE = 1 };
 //   U <- U + reaction_source_terms(U, aux, dt); aux <- update_aux(U, aux)
 // plus the next step's CFL and stage-1 LF maxima.  CHEM_SOURCE writes the
 // source S (12 planes, first 4 zero) instead.
-template <typename Real, int MODE>
+template <typename Real, int MODE, bool REPLAY = false>
 __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real* aux, Real* S,
                                           double* dt_hist, int blk, double* red,
                                           unsigned long long* redb, unsigned long long* redg,
@@ -1146,7 +1150,13 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
 #pragma unroll
     for (int k = 0; k < JH_NS - 1; ++k) rY[k] = Uc[4 + k];
     double d[JH_NA];
-    const bool ok = chem_increment(c, gt, rho, rY, T, dt, d);
+    bool ok;
+    if constexpr (REPLAY) {
+      const unsigned int br = a.clipm[i];
+      ok = chem_increment(c, gt, rho, rY, T, dt, d, nullptr, &br);
+    } else {
+      ok = chem_increment(c, gt, rho, rY, T, dt, d);
+    }
     if (MODE == CHEM_SOURCE) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) S[v * n + i] = (Real)0;
@@ -1224,7 +1234,7 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
 // Two waves per SIMD: the point-implicit solve holds the 8x8 Jacobian and
 // the rates in registers (256 VGPRs at one wave); at two waves a few values
 // spill to scratch, and the kernel still runs ~1.3x faster than at one.
-template <typename Real, int MODE>
+template <typename Real, int MODE, bool REPLAY = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void chem_kernel(StepArgs<Real> a, Real* U, Real* aux,
                                                    Real* S, double* dt_hist) {
   __shared__ double red[4];
@@ -1232,7 +1242,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
   __shared__ unsigned long long redg[kGridSlotsChem * 4];
   __shared__ MixTables mix;
   __shared__ GibbsTables gt;
-  chem_body<Real, MODE>(a, U, aux, S, dt_hist, blockIdx.x, red, redb, redg, mix, gt);
+  chem_body<Real, MODE, REPLAY>(a, U, aux, S, dt_hist, blockIdx.x, red, redb, redg, mix, gt);
 }
 
 // update_aux alone (aux_func.py:21-29), for the unit entry point; in grid
@@ -1276,6 +1286,182 @@ __global__ __launch_bounds__(256) void update_aux_kernel(StepArgs<Real> a, const
   raise_flags(a.sc, fl);
   if (a.hside)
     block_max_bits_multi<256, 1>(nb, &a.sc->gs[a.parity][JH_PROD_UPDATE].nstar, redg);
+}
+
+// ------------------------------------------------------------ tracing
+// Per step, where each stage's LF maxima are attained and which chemistry
+// clip branches are active (jh_ctx_set_trace).  The step is only piecewise
+// smooth: reverse mode (and the adjoint kernels) differentiate the branch
+// taken, so a finite difference agrees with the adjoint only across
+// perturbations under which none of these switch.  A record is 22 int64:
+// per stage s, (first padded index attaining max|u|+a, number of padded cells
+// attaining it, the same for max|v|+a); then a hash and the count of the
+// (cell, species, bound) triples whose clip is active; then, after each of
+// the step's four temperature solves (the three stages, the chemistry), a
+// hash and the count of the cells below the NASA-7 switch temperature (the
+// two fits differ there by up to ~4e-7 in cp: a cell that changes side
+// changes its state discontinuously).
+
+struct TraceScratch {
+  unsigned long long mx[2];    // maxima (bit patterns) of the stage being traced
+  unsigned long long idx[2];   // first padded index attaining each
+  unsigned long long cnt[2];   // cells attaining each
+};
+constexpr int kTraceRecord = 22;
+
+// |u| + a and |v| + a of padded cell P (interior cell or non-corner ghost
+// cell from the strips), from the stored values.
+template <typename Real>
+__device__ __forceinline__ bool trace_speeds(const StepArgs<Real>& a, const Real* U,
+                                             const Real* aux, int P, double& sx, double& sy) {
+  const int py = a.ny + 6;
+  const int gx = P / py - 3, gy = P - (P / py) * py - 3;
+  const CellRef<Real> r = locate<Real>(a, U, aux, gx, gy);
+  if (!r.ok) return false;
+  const Prim<double> q = cell_prim<double, true>(*a.mc, r.u, r.x, r.stride);
+  sx = nan_to_zero(fabs(q.u) + q.a);
+  sy = nan_to_zero(fabs(q.v) + q.a);
+  return true;
+}
+
+// Pass 1: the maxima over the padded field (grid-stride).
+template <typename Real>
+__global__ __launch_bounds__(256) void trace_max_kernel(StepArgs<Real> a, const Real* U,
+                                                        const Real* aux, TraceScratch* ts) {
+  __shared__ double red[4];
+  const int np = (a.nx + 6) * (a.ny + 6);
+  double mx = 0.0, my = 0.0;
+  for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < np; P += gridDim.x * blockDim.x) {
+    double sx, sy;
+    if (trace_speeds(a, U, aux, P, sx, sy)) {
+      mx = fmax(mx, sx);
+      my = fmax(my, sy);
+    }
+  }
+  block_max_atomic<256>(mx, &ts->mx[0], red);
+  block_max_atomic<256>(my, &ts->mx[1], red);
+}
+
+// Pass 2: the first padded index attaining each maximum and their number.
+template <typename Real>
+__global__ __launch_bounds__(256) void trace_arg_kernel(StepArgs<Real> a, const Real* U,
+                                                        const Real* aux, TraceScratch* ts) {
+  const int np = (a.nx + 6) * (a.ny + 6);
+  const unsigned long long m0 = ts->mx[0], m1 = ts->mx[1];
+  for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < np; P += gridDim.x * blockDim.x) {
+    double sx, sy;
+    if (!trace_speeds(a, U, aux, P, sx, sy)) continue;
+    if (dbits(sx) == m0) {
+      atomicMin(&ts->idx[0], (unsigned long long)P);
+      atomicAdd(&ts->cnt[0], 1ull);
+    }
+    if (dbits(sy) == m1) {
+      atomicMin(&ts->idx[1], (unsigned long long)P);
+      atomicAdd(&ts->cnt[1], 1ull);
+    }
+  }
+}
+
+// Moves one stage's result into its record slots (rec: 4 values) and resets
+// the scratch for the next stage; rec == nullptr only resets.
+__global__ void trace_store_kernel(TraceScratch* ts, long long* rec) {
+  if (threadIdx.x != 0) return;
+  if (rec) {
+    rec[0] = (long long)ts->idx[0];
+    rec[1] = (long long)ts->cnt[0];
+    rec[2] = (long long)ts->idx[1];
+    rec[3] = (long long)ts->cnt[1];
+  }
+  ts->mx[0] = ts->mx[1] = 0ull;
+  ts->idx[0] = ts->idx[1] = ~0ull;
+  ts->cnt[0] = ts->cnt[1] = 0ull;
+}
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// The chemistry's active clip branches at the state the chemistry kernel
+// will react: rec[0] += a hash of every (cell, bit), rec[1] += their count
+// (rec zeroed beforehand).  One cell per thread.
+template <typename Real>
+__global__ __launch_bounds__(256) void trace_clip_kernel(StepArgs<Real> a, const Real* U,
+                                                         const Real* aux, long long* rec) {
+  const MechConst& c = *a.mc;
+  const int n = a.nx * a.ny;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long h = 0ull, cnt = 0ull;
+  if (i < n) {
+    const double rho = (double)U[i];
+    double rY[JH_NS - 1];
+#pragma unroll
+    for (int k = 0; k < JH_NS - 1; ++k) rY[k] = (double)U[(4 + k) * n + i];
+    double d[JH_NA];
+    unsigned int cm = 0u;
+    (void)chem_increment(c, c, rho, rY, (double)aux[n + i], step_dt(a), d, &cm);
+    for (int b = 0; b < 2 * JH_NA; ++b)
+      if ((cm >> b) & 1u) {
+        h += mix64(((unsigned long long)i << 5) | (unsigned long long)b);
+        cnt += 1ull;
+      }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    h += __shfl_xor(h, off, 64);
+    cnt += __shfl_xor(cnt, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && cnt) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(rec), h);
+    atomicAdd(reinterpret_cast<unsigned long long*>(rec + 1), cnt);
+  }
+}
+
+// The clip branches the chemistry kernel will take at this state, one mask
+// per cell, into a.clipm (recording for jh_ctx_set_clip_branches).
+template <typename Real>
+__global__ __launch_bounds__(256) void clip_mask_kernel(StepArgs<Real> a, const Real* U,
+                                                        const Real* aux) {
+  const MechConst& c = *a.mc;
+  const int n = a.nx * a.ny;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double rY[JH_NS - 1];
+#pragma unroll
+  for (int k = 0; k < JH_NS - 1; ++k) rY[k] = (double)U[(4 + k) * n + i];
+  double d[JH_NA];
+  unsigned int cm = 0u;
+  (void)chem_increment(c, c, (double)U[i], rY, (double)aux[n + i], step_dt(a), d, &cm);
+  a.clipm[i] = (unsigned short)cm;
+}
+
+// Cells of aux's temperature field below the NASA-7 switch: rec[0] += a
+// hash of their indices, rec[1] += their count (rec zeroed beforehand).
+template <typename Real>
+__global__ __launch_bounds__(256) void trace_range_kernel(StepArgs<Real> a, const Real* aux,
+                                                          long long* rec) {
+  const int n = a.nx * a.ny;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long h = 0ull, cnt = 0ull;
+  if (i < n && (double)aux[n + i] < a.mc->Tcr) {
+    h = mix64((unsigned long long)i);
+    cnt = 1ull;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    h += __shfl_xor(h, off, 64);
+    cnt += __shfl_xor(cnt, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && cnt) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(rec), h);
+    atomicAdd(reinterpret_cast<unsigned long long*>(rec + 1), cnt);
+  }
+}
+
+__global__ void zero_i64_kernel(long long* p, int m) {
+  if ((int)threadIdx.x < m) p[threadIdx.x] = 0;
 }
 
 // Grid-wide Halley stop (thermo_model.py:140-142), after a producer of

@@ -486,6 +486,39 @@ class Simulator:
         _lib.check(ctx.lib.jh_timing_read(ctx.h, ms, cnt), ctx.h)
         return {k: (ms[i], cnt[i]) for i, k in enumerate(_lib.KERNEL_KINDS)}
 
+    def trace(self, U, records=None):
+        """Record, for each following fused step (jh_ctx_set_trace), where the
+        LF maxima of every RK stage are attained and which chemistry clip
+        branches are active and which cells are below the NASA-7 switch
+        temperature after each temperature solve: ``records`` is an int64 CUDA
+        tensor of shape (steps, 22) filled step by step; None stops tracing."""
+        ctx = self._context(U, None)
+        if records is None:
+            _lib.check(ctx.lib.jh_ctx_set_trace(ctx.h, None, 0), ctx.h)
+            return None
+        if records.dtype != torch.int64 or not records.is_cuda or not records.is_contiguous() \
+                or records.dim() != 2 or records.shape[1] != 22:
+            raise ValueError("records must be a contiguous int64 CUDA tensor (steps, 22)")
+        _lib.check(ctx.lib.jh_ctx_set_trace(ctx.h, _ptr(records), int(records.shape[0])), ctx.h)
+        return records
+
+    def clip_branches(self, U, masks=None, mode="off"):
+        """Record ('record') or replay ('replay') the chemistry clip's branches
+        of the following fused steps (jh_ctx_set_clip_branches); ``masks`` is
+        an int16 CUDA tensor (steps, nx*ny); 'off' stops."""
+        ctx = self._context(U, None)
+        m = {"off": 0, "record": 1, "replay": 2}[mode]
+        if m == 0:
+            _lib.check(ctx.lib.jh_ctx_set_clip_branches(ctx.h, None, 0, 0), ctx.h)
+            return None
+        n = int(U.shape[1]) * int(U.shape[2])
+        if masks.dtype != torch.int16 or not masks.is_cuda or not masks.is_contiguous() \
+                or masks.dim() != 2 or masks.shape[1] != n:
+            raise ValueError("masks must be a contiguous int16 CUDA tensor (steps, nx*ny)")
+        _lib.check(ctx.lib.jh_ctx_set_clip_branches(ctx.h, _ptr(masks), int(masks.shape[0]), m),
+                   ctx.h)
+        return masks
+
     def health(self, U):
         """Device health flags since the last query (bits of
         ``_lib.HEALTH_FLAGS``; ``_lib.health_names`` spells them out)."""

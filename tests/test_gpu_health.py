@@ -105,3 +105,74 @@ def test_reaction_source_pivot_flag(tube):
     sim.reaction_source_terms(U1, aux, 3e-9)
     f = _flags(sim, U1)
     assert f == F["singular_pivot"], _lib.health_names(f)
+
+
+def test_trace_records(mech_path, tube):
+    """jh_ctx_set_trace: the first step's stage-1 record names the padded
+    cell where the oracle's nanmax of |u|+a (|v|+a) over the padded field is
+    attained and how many cells attain it; the clip record counts the active
+    clip branches of the oracle's chemistry at the state it reacts; two runs
+    give identical records."""
+    from oracle import flux as oflux
+    from oracle.step import Oracle
+    sim, U, aux = tube
+    # velocity noise: the driver's cells share one sound speed (same T and
+    # composition), so without it the maxima are ties at the rounding level
+    U = U.copy()
+    rng = np.random.default_rng(12)
+    U[1] += U[0] * 30.0 * rng.standard_normal(U[0].shape)
+    U[2] += U[0] * 30.0 * rng.standard_normal(U[0].shape)
+    U[3] += 0.5 * (U[1] ** 2 + U[2] ** 2) / U[0]
+    o = Oracle(mech_path, sim_bc(sim), sim.dx, sim.dy)
+    Ug, ag = o.boundary(U, aux)
+    with np.errstate(invalid="ignore"):
+        _, u, v, _, _, a = oflux.U_to_prim(Ug, ag, o.thermo)
+    PX, PY = Ug.shape[1], Ug.shape[2]
+    corner = np.zeros((PX, PY), dtype=bool)
+    corner[:3, :3] = corner[:3, -3:] = corner[-3:, :3] = corner[-3:, -3:] = True
+    recs = []
+    for _ in range(2):
+        Ut, at = torch.as_tensor(U).cuda(), torch.as_tensor(aux).cuda()
+        rec = sim.trace(Ut, torch.zeros((2, 22), dtype=torch.int64, device="cuda"))
+        sim.advance(Ut, at, 2)
+        sim.trace(Ut, None)
+        recs.append(rec.cpu().numpy())
+    assert np.array_equal(recs[0], recs[1])
+    r = recs[0][0]
+    for col, s in ((0, np.abs(u[0]) + a[0]), (2, np.abs(v[0]) + a[0])):
+        s = np.where(corner, -1.0, s)
+        m = s.max()
+        assert r[col] == int(np.flatnonzero(s.ravel() == m)[0]), (col, r[col])
+        assert r[col + 1] == int((s == m).sum())
+    # clip branches at the state the first step's chemistry reacts
+    U3, a3 = o.advance_flux(U, aux, 3e-9)
+    rho = U3[0:1]
+    Y = o.thermo.fill_Y(U3[4:] / rho)
+    X = rho * Y / o.thermo.Mex.reshape(-1, 1, 1)
+    x = o.reaction.construct_matrix_equation(a3[1:2], X, 3e-9) / rho
+    n_clip = int((x < -Y[:-1]).sum() + (x > 1 - Y[:-1]).sum())
+    assert r[13] == n_clip, (r[13], n_clip)
+
+
+def sim_bc(sim):
+    return {s + "_boundary": k for s, k in zip(("left", "right", "bottom", "top"),
+                                                 [{v: k for k, v in _lib.BC_KINDS.items()}[b]
+                                                  for b in sim.bc_kinds])}
+
+
+def test_clip_branch_record_and_replay(tube):
+    """Recording a rollout's chemistry clip branches and replaying them from
+    the same state reproduces it bit for bit."""
+    sim, U, aux = tube
+    n = U.shape[1] * U.shape[2]
+    masks = torch.zeros((3, n), dtype=torch.int16, device="cuda")
+    Ua, aa = torch.as_tensor(U).cuda(), torch.as_tensor(aux).cuda()
+    sim.clip_branches(Ua, masks, "record")
+    sim.advance(Ua, aa, 3)
+    sim.clip_branches(Ua, None, "off")
+    assert int((masks != 0).sum()) > 0
+    Ub, ab = torch.as_tensor(U).cuda(), torch.as_tensor(aux).cuda()
+    sim.clip_branches(Ub, masks, "replay")
+    sim.advance(Ub, ab, 3)
+    sim.clip_branches(Ub, None, "off")
+    assert torch.equal(Ua, Ub) and torch.equal(aa, ab)
