@@ -101,31 +101,36 @@ __host__ __device__ __forceinline__ bool lu_solve(double (&A)[N][N], double (&b)
 }
 
 // Increment of rho*Y_k for the active species (detailed_reaction, :183-193).
-// rhoY_in: the state's rho*Y_k for the NS-1 transported species; gt: the
-// Gibbs tables (c itself, or its copy in LDS).  Returns lu_solve's pivot
-// check.  clip (optional): bit k set where the clip of species k takes its
-// lower bound -Y_k, bit JH_NA + k where it takes 1 - Y_k.  branches
-// (optional): take the clip's branches from these bits instead of comparing
-// (a diagnostic that keeps a perturbed state on the branches of another).
-template <typename GTab>
-__host__ __device__ __forceinline__ bool chem_increment(const MechConst& c, const GTab& gt,
-                                                        double rho,
-                                                        const double (&rhoY_in)[JH_NS - 1],
-                                                        double T, double dt,
-                                                        double (&drhoY)[JH_NA],
-                                                        unsigned int* clip = nullptr,
-                                                        const unsigned int* branches = nullptr) {
-  const double irho = rcp(rho);
-  double Y[JH_NS];
-#pragma unroll
-  for (int k = 0; k < JH_NS - 1; ++k) Y[k] = rhoY_in[k] * irho;
-  double sum = 0.0;
-#pragma unroll
-  for (int k = 0; k < JH_NS - 1; ++k) sum += Y[k];
-  Y[JH_NS - 1] = 1.0 - sum;
+// ld.rho(), ld(k): the state's rho and rho*Y_k for the NS-1 transported
+// species (register values, or loads from the state: they are read again
+// after the solve, behind a compiler barrier, so the state is not held in
+// registers across the Jacobian and its factorisation); gt: the Gibbs tables (c
+// itself, or its copy in LDS); JT: the Jacobian's storage type (float in the
+// mixed-precision form: assembled in FP32 from FP64 terms, factorised and
+// solved in FP64).  Returns lu_solve's pivot check.  clip (optional): bit k
+// set where the clip of species k takes its lower bound -Y_k, bit JH_NA + k
+// where it takes 1 - Y_k.  branches (optional): take the clip's branches
+// from these bits instead of comparing (a diagnostic that keeps a perturbed
+// state on the branches of another).
+template <typename JT = double, typename GTab, typename LoadRY>
+__host__ __device__ __forceinline__ bool chem_increment_ld(const MechConst& c, const GTab& gt,
+                                                           const LoadRY& ld, double T, double dt,
+                                                           double (&drhoY)[JH_NA],
+                                                           unsigned int* clip = nullptr,
+                                                           const unsigned int* branches = nullptr) {
   double X[JH_NS];
+  {
+    const double rho = ld.rho(), irho = rcp(rho);
+    double Y[JH_NS];
 #pragma unroll
-  for (int k = 0; k < JH_NS; ++k) X[k] = (rho * Y[k]) * c.R[k];
+    for (int k = 0; k < JH_NS - 1; ++k) Y[k] = ld(k) * irho;
+    double sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < JH_NS - 1; ++k) sum += Y[k];
+    Y[JH_NS - 1] = 1.0 - sum;
+#pragma unroll
+    for (int k = 0; k < JH_NS; ++k) X[k] = (rho * Y[k]) * c.R[k];
+  }
   double Xc[JH_NA], invMX[JH_NA];
 #pragma unroll
   for (int k = 0; k < JH_NA; ++k) {
@@ -141,31 +146,57 @@ __host__ __device__ __forceinline__ bool chem_increment(const MechConst& c, cons
     eg[k] = exp(g[k]);
     ieg[k] = rcp(eg[k]);
   }
-  double w[JH_NA], J[JH_NA][JH_NA];
-  mech_rates_jacobian(c, T, invT, log(T), X, Xc, invMX, eg, ieg, w, J);
+  double w[JH_NA];
+  JT J[JH_NA][JH_NA];
+  mech_rates_jacobian<JT>(c, T, invT, log(T), X, Xc, invMX, eg, ieg, w, J);
   const double idt = rcp(dt);
+  double A[JH_NA][JH_NA];
 #pragma unroll
   for (int k = 0; k < JH_NA; ++k) {
 #pragma unroll
-    for (int n = 0; n < JH_NA; ++n) J[k][n] = ((k == n) ? idt : 0.0) - J[k][n];
+    for (int n = 0; n < JH_NA; ++n) A[k][n] = ((k == n) ? idt : 0.0) - (double)J[k][n];
   }
-  const bool ok = lu_solve<JH_NA>(J, w);
+  const bool ok = lu_solve<JH_NA>(A, w);
+#if defined(__HIP_DEVICE_COMPILE__)
+  __asm__ volatile("" ::: "memory");   // ld below reloads
+#endif
+  const double rho = ld.rho(), irho = rcp(rho);
   unsigned int cm = 0u;
 #pragma unroll
   for (int k = 0; k < JH_NA; ++k) {
+    const double Yk = ld(k) * irho;
     const double x = w[k] * irho;
-    if (clip) cm |= (x < -Y[k] ? 1u << k : 0u) | (x > 1.0 - Y[k] ? 1u << (JH_NA + k) : 0u);
+    if (clip) cm |= (x < -Yk ? 1u << k : 0u) | (x > 1.0 - Yk ? 1u << (JH_NA + k) : 0u);
     double dY;
     if (branches) {
       const unsigned int b = *branches;
-      dY = (b >> k) & 1u ? -Y[k] : (b >> (JH_NA + k)) & 1u ? 1.0 - Y[k] : x;
+      dY = (b >> k) & 1u ? -Yk : (b >> (JH_NA + k)) & 1u ? 1.0 - Yk : x;
     } else {
-      dY = fmin(fmax(x, -Y[k]), 1.0 - Y[k]);
+      dY = fmin(fmax(x, -Yk), 1.0 - Yk);
     }
     drhoY[k] = rho * dY;
   }
   if (clip) *clip = cm;
   return ok;
+}
+
+// chem_increment_ld's state from register values.
+struct RegState {
+  double r;
+  const double (&y)[JH_NS - 1];
+  __host__ __device__ double rho() const { return r; }
+  __host__ __device__ double operator()(int k) const { return y[k]; }
+};
+
+template <typename JT = double, typename GTab>
+__host__ __device__ __forceinline__ bool chem_increment(const MechConst& c, const GTab& gt,
+                                                        double rho,
+                                                        const double (&rhoY_in)[JH_NS - 1],
+                                                        double T, double dt,
+                                                        double (&drhoY)[JH_NA],
+                                                        unsigned int* clip = nullptr,
+                                                        const unsigned int* branches = nullptr) {
+  return chem_increment_ld<JT>(c, gt, RegState{rho, rhoY_in}, T, dt, drhoY, clip, branches);
 }
 
 __host__ __device__ __forceinline__ bool chem_increment(const MechConst& c, double rho,

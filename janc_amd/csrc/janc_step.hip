@@ -170,6 +170,25 @@ int grid_cells(const jh_ctx* c) {
 // one thread per cell (pointwise kernels without a grid-stride loop)
 int grid_all(const jh_ctx* c) { return (c->nx * c->ny + 255) / 256; }
 
+// The step's chemistry launch: the grid-wide Halley stop's epilogue (a.hside
+// set) and the clip-branch replay are separate instantiations.
+template <typename Real>
+void launch_chem_step(const jh_ctx* c, const StepArgs<Real>& a, hipStream_t st, Real* U,
+                      Real* aux, double* dt_hist, bool replay) {
+  const int g = grid_all(c);
+  if (a.hside) {
+    if (replay)
+      chem_kernel<Real, CHEM_STEP, true, true><<<g, 256, 0, st>>>(a, U, aux, nullptr, dt_hist);
+    else
+      chem_kernel<Real, CHEM_STEP, false, true><<<g, 256, 0, st>>>(a, U, aux, nullptr, dt_hist);
+  } else {
+    if (replay)
+      chem_kernel<Real, CHEM_STEP, true, false><<<g, 256, 0, st>>>(a, U, aux, nullptr, dt_hist);
+    else
+      chem_kernel<Real, CHEM_STEP, false, false><<<g, 256, 0, st>>>(a, U, aux, nullptr, dt_hist);
+  }
+}
+
 int tile_rows(const jh_ctx* c);
 
 // Rows of the Godunov stage kernel's tile (measured on the 4000 x 400 FP64
@@ -362,12 +381,8 @@ int advance_impl(jh_ctx* c, const Real* U, const Real* aux, Real* Uo, Real* auxo
         clip_mask_kernel<Real><<<grid_all(c), 256, 0, c->stream>>>(a, Uo, auxo);
     }
     { Timed t(c, JH_KERNEL_CHEM);
-      if (replay)
-        chem_kernel<Real, CHEM_STEP, true><<<grid_all(c), 256, 0, c->stream>>>(
-            a, Uo, auxo, nullptr, dt_hist ? dt_hist + s : nullptr);
-      else
-        chem_kernel<Real, CHEM_STEP><<<grid_all(c), 256, 0, c->stream>>>(
-            a, Uo, auxo, nullptr, dt_hist ? dt_hist + s : nullptr);
+      launch_chem_step<Real>(c, a, c->stream, Uo, auxo, dt_hist ? dt_hist + s : nullptr,
+                             replay);
       halley_fix<Real>(c, JH_PROD_CHEM, c->parity ^ 1, 0, 1, auxo); }
     if (rec) trace_range<Real>(c, a, auxo, rec + 20);
     rc = check_launch(c, "step kernels");
@@ -578,8 +593,7 @@ int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, cons
                               st));
     HIP_TRY(c, hipMemcpyAsync(w.A[4], w.A[3], 2 * n * sizeof(double), hipMemcpyDeviceToDevice,
                               st));
-    chem_kernel<double, CHEM_STEP><<<grid_all(c), 256, 0, st>>>(a, w.U[4], w.A[4], nullptr,
-                                                                nullptr);
+    launch_chem_step<double>(c, a, st, w.U[4], w.A[4], nullptr, false);
     halley_fix<double>(c, JH_PROD_CHEM, P ^ 1, 0, 1, w.A[4]);
   }
   rc = check_launch(c, "vjp forward");
@@ -1364,13 +1378,11 @@ int jh_step_finish(jh_ctx* c, void* U, void* aux, double dt, double cfl, double*
   else dt = 0.0;
   if (c->precision == JH_PREC_FP64) {
     auto a = make_args<double>(c, dt, cfl);
-    chem_kernel<double, CHEM_STEP><<<grid_all(c), 256, 0, c->stream>>>(
-        a, (double*)U, (double*)aux, nullptr, dt_dev);
+    launch_chem_step<double>(c, a, c->stream, (double*)U, (double*)aux, dt_dev, false);
     halley_fix<double>(c, JH_PROD_CHEM, c->parity ^ 1, 0, 1, (double*)aux);
   } else {
     auto a = make_args<float>(c, dt, cfl);
-    chem_kernel<float, CHEM_STEP><<<grid_all(c), 256, 0, c->stream>>>(
-        a, (float*)U, (float*)aux, nullptr, dt_dev);
+    launch_chem_step<float>(c, a, c->stream, (float*)U, (float*)aux, dt_dev, false);
     halley_fix<float>(c, JH_PROD_CHEM, c->parity ^ 1, 0, 1, (float*)aux);
   }
   const int rc = check_launch(c, "chem_kernel");

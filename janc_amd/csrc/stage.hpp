@@ -1111,11 +1111,22 @@ void stage_kernel(StepArgs<Real> a, int stage, const Real* __restrict__ Us,
 
 enum ChemMode : int { CHEM_STEP = 0, CHEM_SOURCE = 1 };
 
+// chem_increment_ld's state read from the conserved planes of cell i.
+template <typename Real>
+struct MemState {
+  const Real* U;
+  int n, i;
+  __device__ double rho() const { return (double)U[i]; }
+  __device__ double operator()(int k) const { return (double)U[(4 + k) * n + i]; }
+};
+
 // Point-implicit chemistry after RK3 (set_simulator.py:117-122):
 //   U <- U + reaction_source_terms(U, aux, dt); aux <- update_aux(U, aux)
 // plus the next step's CFL and stage-1 LF maxima.  CHEM_SOURCE writes the
-// source S (12 planes, first 4 zero) instead.
-template <typename Real, int MODE, bool REPLAY = false>
+// source S (12 planes, first 4 zero) instead.  GRID: the grid-wide Halley
+// stop's epilogue (a.hside set), compiled apart so that the per-cell form
+// keeps its registers.
+template <typename Real, int MODE, bool REPLAY = false, bool GRID = false>
 __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real* aux, Real* S,
                                           double* dt_hist, int blk, double* red,
                                           unsigned long long* redb, unsigned long long* redg,
@@ -1130,7 +1141,7 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
     a.sc->dt_last = dt;
     if (dt_hist) *dt_hist = dt;
   }
-  const bool grid = a.hside != nullptr;
+  constexpr bool grid = GRID;
   double mx = 0.0, my = 0.0;
   unsigned long long cm = 0ull;
   unsigned long long gv[kGridSlotsChem];   // grid-wide stop: N*, x, y and CFL candidates
@@ -1141,22 +1152,19 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
   // mechanism constant in registers across iterations
   const int i = blk * blockDim.x + threadIdx.x;
   if (i < n) {
-    double Uc[NU];
-#pragma unroll
-    for (int v = 0; v < NU; ++v) Uc[v] = (double)U[v * n + i];
-    double T = (double)aux[n + i];
-    const double rho = Uc[0];
-    double rY[JH_NS - 1];
-#pragma unroll
-    for (int k = 0; k < JH_NS - 1; ++k) rY[k] = Uc[4 + k];
+    // the state is read before the Jacobian and again after the solve
+    // instead of being held across it
+    const MemState<Real> ld{U, n, i};
     double d[JH_NA];
     bool ok;
+    // Jacobian storage: FP32 in the mixed-precision form (Real = float)
     if constexpr (REPLAY) {
       const unsigned int br = a.clipm[i];
-      ok = chem_increment(c, gt, rho, rY, T, dt, d, nullptr, &br);
+      ok = chem_increment_ld<Real>(c, gt, ld, (double)aux[n + i], dt, d, nullptr, &br);
     } else {
-      ok = chem_increment(c, gt, rho, rY, T, dt, d);
+      ok = chem_increment_ld<Real>(c, gt, ld, (double)aux[n + i], dt, d);
     }
+    double T = (double)aux[n + i];
     if (MODE == CHEM_SOURCE) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) S[v * n + i] = (Real)0;
@@ -1184,7 +1192,7 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
     double gamma;
     HalleyMore hm{0, 0.0, 0.0, true, true};
     bool conv;
-    if (grid) {
+    if constexpr (grid) {
       conv = halley_T<MixTables, true>(mix, Y, R, e, T, gamma, &hm);
       a.hside[i] = (Real)hm.g1;
       a.hside[n + i] = (Real)hm.T1;
@@ -1202,7 +1210,7 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
     const double p = r2 * R * Ts;
     const double snd = sqrt(gs * p * ir);
     const double sx = fabs(u) + snd, sy = fabs(v) + snd;
-    if (grid) {
+    if constexpr (grid) {
       const double g1 = (double)(Real)hm.g1, T1 = (double)(Real)hm.T1;
       const double p1 = r2 * R * T1;
       const double snd1 = sqrt(g1 * p1 * ir);
@@ -1220,7 +1228,7 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
   if (MODE == CHEM_STEP) {
     raise_flags(a.sc, fl);
     const int o = a.parity ^ 1;
-    if (grid) {
+    if constexpr (grid) {
       block_max_bits_multi<256, kGridSlotsChem>(gv, &a.sc->gs[a.parity][JH_PROD_CHEM].nstar,
                                                 redg);
     } else {
@@ -1232,17 +1240,19 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
 }
 
 // Two waves per SIMD: the point-implicit solve holds the 8x8 Jacobian and
-// the rates in registers (256 VGPRs at one wave); at two waves a few values
-// spill to scratch, and the kernel still runs ~1.3x faster than at one.
-template <typename Real, int MODE, bool REPLAY = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void chem_kernel(StepArgs<Real> a, Real* U, Real* aux,
+// the rates in registers (256 VGPRs at one wave).
+template <typename Real> struct ChemWaves { static constexpr int value = 2; };
+template <> struct ChemWaves<float> { static constexpr int value = 2; };
+template <typename Real, int MODE, bool REPLAY = false, bool GRID = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ChemWaves<Real>::value))) void chem_kernel(StepArgs<Real> a, Real* U, Real* aux,
                                                    Real* S, double* dt_hist) {
   __shared__ double red[4];
   __shared__ unsigned long long redb[4];
-  __shared__ unsigned long long redg[kGridSlotsChem * 4];
+  __shared__ unsigned long long redg[GRID ? kGridSlotsChem * 4 : 1];
   __shared__ MixTables mix;
   __shared__ GibbsTables gt;
-  chem_body<Real, MODE, REPLAY>(a, U, aux, S, dt_hist, blockIdx.x, red, redb, redg, mix, gt);
+  chem_body<Real, MODE, REPLAY, GRID>(a, U, aux, S, dt_hist, blockIdx.x, red, redb, redg, mix,
+                                      gt);
 }
 
 // update_aux alone (aux_func.py:21-29), for the unit entry point; in grid
@@ -1401,7 +1411,7 @@ __global__ __launch_bounds__(256) void trace_clip_kernel(StepArgs<Real> a, const
     for (int k = 0; k < JH_NS - 1; ++k) rY[k] = (double)U[(4 + k) * n + i];
     double d[JH_NA];
     unsigned int cm = 0u;
-    (void)chem_increment(c, c, rho, rY, (double)aux[n + i], step_dt(a), d, &cm);
+    (void)chem_increment<Real>(c, c, rho, rY, (double)aux[n + i], step_dt(a), d, &cm);
     for (int b = 0; b < 2 * JH_NA; ++b)
       if ((cm >> b) & 1u) {
         h += mix64(((unsigned long long)i << 5) | (unsigned long long)b);
@@ -1433,7 +1443,7 @@ __global__ __launch_bounds__(256) void clip_mask_kernel(StepArgs<Real> a, const 
   for (int k = 0; k < JH_NS - 1; ++k) rY[k] = (double)U[(4 + k) * n + i];
   double d[JH_NA];
   unsigned int cm = 0u;
-  (void)chem_increment(c, c, (double)U[i], rY, (double)aux[n + i], step_dt(a), d, &cm);
+  (void)chem_increment<Real>(c, c, (double)U[i], rY, (double)aux[n + i], step_dt(a), d, &cm);
   a.clipm[i] = (unsigned short)cm;
 }
 

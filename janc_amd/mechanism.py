@@ -255,18 +255,22 @@ def _generate(m):
     e("// invMX[n] = 1/(M_n*Xc_n); eg/ieg: exp(+-g) of the Gibbs s/R - h/(RT) of the active")
     e("// species, so exp(-sum_k dnu_k g_k) = prod_k (eg_k or ieg_k)^|dnu_k| (formed here as")
     e("// a quotient of eg products; ieg is read by the VJP).")
+    e("// JT: the Jacobian's storage type.  Rates, exponentials and every d_n are")
+    e("// FP64; JT = float rounds each d_n once and accumulates J in FP32 (the")
+    e("// mixed-precision form: half the registers of an FP64 J).")
+    e("template <typename JT>")
     e("__host__ __device__ __forceinline__ void mech_rates_jacobian(")
     e("    const MechConst& __restrict__ c, const double T, const double invT, const double logT,")
     e("    const double (&X)[JH_NS], const double (&Xc)[JH_NA], const double (&invMX)[JH_NA],")
     e("    const double (&eg)[JH_NA], const double (&ieg)[JH_NA], double (&w)[JH_NA],")
-    e("    double (&J)[JH_NA][JH_NA])")
+    e("    JT (&J)[JH_NA][JH_NA])")
     e("{")
     e("  const double Pf = c.Pref * invT;   // 101325/P0/T")
     e("  const double iPf = jh::rcp(Pf);")
     e("#pragma unroll")
     e("  for (int k = 0; k < JH_NA; ++k) { w[k] = 0.0;")
     e("#pragma unroll")
-    e("    for (int n = 0; n < JH_NA; ++n) J[k][n] = 0.0; }")
+    e("    for (int n = 0; n < JH_NA; ++n) J[k][n] = JT(0); }")
     # third-body concentrations first: X is not needed after them
     for i, r in enumerate(m.reactions):
         if r.three_body:
@@ -361,17 +365,17 @@ def _generate(m):
             if parts:
                 dn[n] = " + ".join(parts)
         for n, expr in dn.items():
-            e("    const double d%d = %s;" % (n, expr))
+            e("    const JT d%d = JT(%s);" % (n, expr))
         for k, d in dnu.items():
             for n in dn:
                 op = "+=" if d > 0 else "-="
-                val = "d%d" % n if abs(d) == 1.0 else "2.0 * d%d" % n
+                val = "d%d" % n if abs(d) == 1.0 else "JT(2.0) * d%d" % n
                 e("    J[%d][%d] %s %s;" % (k, n, op, val))
         e("  }")
     e("#pragma unroll")
     e("  for (int k = 0; k < JH_NA; ++k) { w[k] = c.M[k] * w[k];")
     e("#pragma unroll")
-    e("    for (int n = 0; n < JH_NA; ++n) J[k][n] = c.M[k] * J[k][n]; }")
+    e("    for (int n = 0; n < JH_NA; ++n) J[k][n] = JT(c.M[k]) * J[k][n]; }")
     e("}")
     return "\n".join(L) + "\n\n" + _generate_vjp(m)
 
