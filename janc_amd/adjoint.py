@@ -48,10 +48,9 @@ class _Rollout(torch.autograd.Function):
         U = U.detach().contiguous().clone()
         aux = aux.detach().contiguous().clone()
         ckpt = []
-        for s in range(nsteps):
-            if s % every == 0:
-                ckpt.append((U.clone(), aux.clone()))
-            U, aux = sim.advance_one_step(U, aux, sim.dx, sim.dy, dt)
+        for s in range(0, nsteps, every):          # one fused call per segment
+            ckpt.append((U.clone(), aux.clone()))
+            sim.advance(U, aux, min(every, nsteps - s), dt=dt)
         ctx.sim, ctx.dt, ctx.nsteps, ctx.every = sim, dt, nsteps, every
         ctx.ckpt = ckpt
         ctx.final = (U.clone(), aux.clone())

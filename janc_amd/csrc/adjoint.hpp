@@ -222,7 +222,7 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
     AdjArgs a, const double* __restrict__ U, const double* __restrict__ aux,
     const double* __restrict__ rbar, const unsigned long long* alpha_bits,
     double* __restrict__ Ut, double* __restrict__ At, double* __restrict__ Ubp,
-    double* __restrict__ abp, double* __restrict__ part) {
+    double* __restrict__ abp, double* __restrict__ spd, double* __restrict__ part) {
   using G = AdjTile<TXA>;
   __shared__ AdjLds<TXA> L;
   const MechConst& c = *a.mc;
@@ -458,6 +458,8 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
     const double snd = sqrt(qf.gamma * qf.p * qf.ir);
     rxo = fabs(qf.u) + snd;
     ryo = fabs(qf.v) + snd;
+    spd[P * PY + Q] = rxo;         // alpha_apply_kernel's tie test reads these
+    spd[np + P * PY + Q] = ryo;
     if (work) prim_to_U_vjp(c, qf, rhob, ub, vb, pb, 0.0, Yb, Ub, Tb, gb);
     if (pad_interior(a, P, Q)) {   // its own source: straight into the output
       const int s = src[0];
@@ -587,6 +589,7 @@ __global__ __launch_bounds__(256) void alpha_reduce_kernel(const double* __restr
 
 __global__ __launch_bounds__(256) void alpha_apply_kernel(AdjArgs a, const double* __restrict__ U,
                                                           const double* __restrict__ aux,
+                                                          const double* __restrict__ spd,
                                                           const unsigned long long* alpha_bits,
                                                           const double* scal,
                                                           const unsigned int* cnt,
@@ -601,11 +604,12 @@ __global__ __launch_bounds__(256) void alpha_apply_kernel(AdjArgs a, const doubl
   int sx, sy, flip;
   const int P = t / PY, Q = t - P * PY;
   if (!pad_src(a, P, Q, sx, sy, flip)) return;
-  const PrimD q = prim_src(c, U, aux, a.nx * a.ny, sx * a.ny + sy, flip);
-  const double snd = sqrt(q.gamma * q.p * q.ir);
-  const bool mx = cnt[0] > 0 && is_max(fabs(q.u) + snd, bitsd(alpha_bits[0]));
-  const bool my = cnt[1] > 0 && is_max(fabs(q.v) + snd, bitsd(alpha_bits[1]));
+  // the tie test compares the very values the maxima were reduced from
+  // (stored by flux_vjp_tile_kernel), not a recomputation
+  const bool mx = cnt[0] > 0 && is_max(spd[t], bitsd(alpha_bits[0]));
+  const bool my = cnt[1] > 0 && is_max(spd[np + t], bitsd(alpha_bits[1]));
   if (!mx && !my) return;
+  const PrimD q = prim_src(c, U, aux, a.nx * a.ny, sx * a.ny + sy, flip);
   const double wx = mx ? scal[0] / cnt[0] : 0.0, wy = my ? scal[1] / cnt[1] : 0.0;
   // d(|u| + a) and d(|v| + a); sign(0) = 0 as in JAX
   const double su = q.u > 0 ? 1.0 : (q.u < 0 ? -1.0 : 0.0);

@@ -442,6 +442,7 @@ struct AdjWork {
   double *U[5], *A[5];          // forward states of the step: 0 = input, 1-3 RK, 4 = output
   double *Ub, *Uacc, *ab, *rb;  // cotangents
   double *Ubp, *abp;            // ghost cells' cotangents, on the padded grid
+  double* spd;                  // |u| + a, |v| + a of every padded cell (2 planes)
   double *part, *scal;
   unsigned int* cnt;
   unsigned long long* amx;      // recomputed LF maxima (bit patterns)
@@ -454,7 +455,7 @@ int adj_work(jh_ctx* c, const double* U0, const double* A0, AdjWork& w) {
   const size_t np = (size_t)(c->nx + 6) * (c->ny + 6);
   const int nblk = adj_tiles(c);
   const size_t ng = np - n;      // ghost cells
-  const size_t len = 4 * (NU + 2) * n + (3 * NU + 2) * n + (NU + 2) * np +
+  const size_t len = 4 * (NU + 2) * n + (3 * NU + 2) * n + (NU + 4) * np +
                      6 * (size_t)nblk + 2 + 2 + 2 + (3 * ng + 2) / 2;
   if (c->adj_len < len) {
     if (c->adj) (void)hipFree(c->adj);
@@ -477,6 +478,7 @@ int adj_work(jh_ctx* c, const double* U0, const double* A0, AdjWork& w) {
   w.ab = p; p += 2 * n;
   w.Ubp = p; p += NU * np;
   w.abp = p; p += 2 * np;
+  w.spd = p; p += 2 * np;
   w.part = p; p += 6 * nblk;
   w.scal = p; p += 2;
   w.cnt = reinterpret_cast<unsigned int*>(p);
@@ -551,10 +553,10 @@ void flux_vjp(jh_ctx* c, const AdjArgs& a, AdjWork& w, const double* Us, const d
               const double* rb, const unsigned long long* alpha, double* Ut, double* At) {
   hipStream_t st = c->stream;
   flux_vjp_tile_kernel<kAdjRows><<<w.nblk, AdjTile<kAdjRows>::NT, 0, st>>>(
-      a, Us, As, rb, alpha, Ut, At, w.Ubp, w.abp, w.part);
+      a, Us, As, rb, alpha, Ut, At, w.Ubp, w.abp, w.spd, w.part);
   alpha_reduce_kernel<<<1, 256, 0, st>>>(w.part, w.nblk, w.scal, w.amx, w.cnt);
-  alpha_apply_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, Us, As, w.amx, w.scal, w.cnt, Ut,
-                                                         At, w.Ubp, w.abp);
+  alpha_apply_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, Us, As, w.spd, w.amx, w.scal, w.cnt,
+                                                         Ut, At, w.Ubp, w.abp);
   const int m = c->adj_fold_m;
   const int* fl = w.fold;
   if (m > 0)

@@ -14,8 +14,10 @@ CPU path.
 
 Configuration errors follow the reference: unsupported keys raise
 ``KeyError``/``AssertionError``/``FileNotFoundError`` at construction.
-Paths the library does not implement (Godunov/HLLC, viscosity, 1-D, AMR,
-user callables inside the fused step) raise ``NotImplementedError``.
+Paths the library does not implement (AUSMPW+, viscous fluxes, 1-D/3-D,
+constant-gamma thermo, user-defined reaction sources, multi-device
+decomposition) raise ``NotImplementedError``; the AMR branch's block step is
+``janc_amd.amr``.
 """
 import ctypes
 import os
@@ -337,22 +339,24 @@ class Simulator:
         dt = dt[0]
         return Uo, ao, t + dt, dt
 
-    def advance(self, U, aux, nsteps, theta=None, dt_hist=None):
-        """nsteps calls of advance_func fused into one library call, in place.
+    def advance(self, U, aux, nsteps, theta=None, dt_hist=None, dt=None):
+        """nsteps calls of advance_func fused into one library call, in place
+        (dt: a fixed step instead of the configuration's dt or CFL).
 
         Returns the per-step dt as a device tensor."""
         U, aux = self._state(U, aux)
         ctx = self._context(U, theta)
         if dt_hist is None:
             dt_hist = torch.empty((nsteps,), dtype=torch.float64, device=U.device)
+        fixed_dt = self.fixed_dt if dt is None else float(dt)
         if self.user_bc:
-            fixed = self.fixed_dt is not None
+            fixed = fixed_dt is not None
             for s in range(nsteps):
-                self._step_pieces(ctx, U, aux, U, aux, self.fixed_dt if fixed else 0.0,
+                self._step_pieces(ctx, U, aux, U, aux, fixed_dt if fixed else 0.0,
                                   0.0 if fixed else self.cfl, dt_hist[s:s + 1], theta)
             return U, aux, dt_hist
-        if self.fixed_dt is not None:
-            ctx.call("jh_advance", _ptr(U), _ptr(aux), _ptr(U), _ptr(aux), self.fixed_dt, 0.0,
+        if fixed_dt is not None:
+            ctx.call("jh_advance", _ptr(U), _ptr(aux), _ptr(U), _ptr(aux), fixed_dt, 0.0,
                      int(nsteps), _ptr(dt_hist))
         else:
             ctx.call("jh_advance", _ptr(U), _ptr(aux), _ptr(U), _ptr(aux), 0.0, self.cfl,
