@@ -195,9 +195,11 @@ int jh_lf_alpha(jh_ctx* ctx, const void* U, const void* aux, double* alpha_dev);
  * output, writes the cotangents of U and aux (U_in_bar, aux_in_bar); the
  * step is recomputed, and its output is also written to U_out / aux_out when
  * they are non-NULL.  FP64 contexts with periodic / zero_gradient /
- * slip_wall / no_slip_wall boundaries only.  A context's first call
- * allocates its workspace and uploads its ghost-fold tables, synchronising
- * the context stream once; later calls neither allocate nor synchronise. */
+ * slip_wall / no_slip_wall boundaries only.  U_in_bar / aux_in_bar may be
+ * the same arrays as U_bar / aux_bar (in place) but must not overlap the
+ * state.  A context's first call allocates its workspace and uploads its
+ * ghost-fold tables, synchronising the context stream once; later calls
+ * neither allocate nor synchronise. */
 int jh_step_vjp(jh_ctx* ctx, const void* U, const void* aux, double dt, const void* U_bar,
                 const void* aux_bar, void* U_in_bar, void* aux_in_bar, void* U_out,
                 void* aux_out);

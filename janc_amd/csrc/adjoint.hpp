@@ -184,16 +184,18 @@ struct AdjTile {
   static constexpr int NHS = 8 / NW;           // x-halo rows per wave
 };
 
-template <int TXA>
+// SEQ: the x faces' products are gathered before the y faces' are formed,
+// through one buffer (two more barriers per variable, ~2/3 of the LDS).
+template <int TXA, bool SEQ>
 struct AdjLds {
+  static constexpr int NCX = 2 * (TXA + 4) * 5 * TY, NCY = 2 * TXA * 5 * (TY + 4);
   double Hxp[TXA + 8][TY], Hxm[TXA + 8][TY];
   double Hyp[TXA][TY + 8], Hym[TXA][TY + 8];
   double Fbx[TXA + 5][TY];
   double Fby[TXA][TY + 5];
   // F_bar * dq/dv_m of the faces reaching the tile: x faces
   // [side][k][m][TY] (k < TXA+4), y faces [side][row][m][TY+4]
-  double Cx[2 * (TXA + 4) * 5 * TY];
-  double Cy[2 * TXA * 5 * (TY + 4)];
+  double C[SEQ ? (NCX > NCY ? NCX : NCY) : NCX + NCY];
   double red[4][TXA];
   unsigned int ties[2][TXA];
 };
@@ -217,14 +219,16 @@ __device__ __forceinline__ double phys_flux_l(int cv, int dir, const PrimL& q, d
   }
 }
 
-template <int TXA>
+template <int TXA, bool SEQ>
 __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_eu(2))) void flux_vjp_tile_kernel(
     AdjArgs a, const double* __restrict__ U, const double* __restrict__ aux,
     const double* __restrict__ rbar, const unsigned long long* alpha_bits,
     double* __restrict__ Ut, double* __restrict__ At, double* __restrict__ Ubp,
     double* __restrict__ abp, double* __restrict__ spd, double* __restrict__ part) {
   using G = AdjTile<TXA>;
-  __shared__ AdjLds<TXA> L;
+  __shared__ AdjLds<TXA, SEQ> L;
+  double* Cx = L.C;
+  double* Cy = SEQ ? L.C : L.C + AdjLds<TXA, SEQ>::NCX;
   const MechConst& c = *a.mc;
   const int nx = a.nx, ny = a.ny, n = nx * ny, PX = nx + 6, PY = ny + 6, np = PX * PY;
   const int t = threadIdx.x, ly = t & 63, w = t >> 6;
@@ -291,13 +295,64 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
   for (int k = 0; k < JH_NS - 1; ++k) Yb[k] = 0.0;
   double sx_part = 0.0, sy_part = 0.0;
 
+  // This thread's face-cotangent entries: x faces e = t + r NT (r < NEX) of
+  // the (NFX, TY) table, y faces e = t + r NT (r < NEY) of the (TXA, NFY)
+  // table; F_bar = dt/dx (rbar_f - rbar_{f-1}) with rbar zero outside the
+  // grid.  The rbar and state loads of variable cv + 1 are issued while
+  // variable cv's gradients run; they are unconditional (clamped offsets,
+  // zeroed afterwards) so no load sits on a divergent branch.
+  constexpr int NEX = (G::NFX * TY + G::NT - 1) / G::NT;
+  constexpr int NEY = (TXA * G::NFY + G::NT - 1) / G::NT;
+  int xoff[NEX], yoff[NEY];
+  unsigned xfl[NEX], yfl[NEY];   // bit 0: entry exists, 1: face valid, 2: left, 3: right
+#pragma unroll
+  for (int r = 0; r < NEX; ++r) {
+    const int e = t + r * G::NT;
+    const int k = e >> 6, col = e & 63;
+    const int f = X0 - 5 + k, jj = Y0 + col - 3;
+    const bool ex = e < G::NFX * TY;
+    const bool ok = ex && f >= 0 && f <= nx && jj >= 0 && jj < ny;
+    const bool lo = ok && f - 1 >= 0, hi = ok && f < nx;
+    xfl[r] = (ex ? 1u : 0u) | (ok ? 2u : 0u) | (lo ? 4u : 0u) | (hi ? 8u : 0u);
+    xoff[r] = ok ? (f - 1) * ny + jj : 0;          // right value at + ny
+  }
+#pragma unroll
+  for (int r = 0; r < NEY; ++r) {
+    const int e = t + r * G::NT;
+    const int row = e / G::NFY, k = e - row * G::NFY;
+    const int gg = Y0 - 5 + k, ii = X0 + row - 3;
+    const bool ex = e < TXA * G::NFY;
+    const bool ok = ex && gg >= 0 && gg <= ny && ii >= 0 && ii < nx;
+    const bool lo = ok && gg - 1 >= 0, hi = ok && gg < ny;
+    yfl[r] = (ex ? 1u : 0u) | (ok ? 2u : 0u) | (lo ? 4u : 0u) | (hi ? 8u : 0u);
+    yoff[r] = ok ? ii * ny + gg - 1 : 0;           // right value at + 1
+  }
+  double uraw[NS_], rxl[NEX], rxr[NEX], ryl[NEY], ryr[NEY];
+  auto load_raw = [&](int v) {
+    const size_t o = (size_t)v * n;
+#pragma unroll
+    for (int s = 0; s < NS_; ++s) uraw[s] = U[o + (in[s] ? src[s] : 0)];
+#pragma unroll
+    for (int r = 0; r < NEX; ++r) {
+      const int b = xoff[r];
+      rxl[r] = rbar[o + (xfl[r] & 4u ? b : 0)];
+      rxr[r] = rbar[o + (xfl[r] & 8u ? b + ny : 0)];
+    }
+#pragma unroll
+    for (int r = 0; r < NEY; ++r) {
+      const int b = yoff[r];
+      ryl[r] = rbar[o + (yfl[r] & 4u ? b : 0)];
+      ryr[r] = rbar[o + (yfl[r] & 8u ? b + 1 : 0)];
+    }
+  };
+  load_raw(0);
+
   for (int cv = 0; cv < NU; ++cv) {
     // ---- H+- of the region and the face cotangents ----
     double ucv_own = 0.0;
 #pragma unroll
     for (int s = 0; s < NS_; ++s) {
-      double ucv = 0.0;
-      if (in[s]) ucv = flip_var(cv, fl[s], U[(size_t)cv * n + src[s]]);
+      const double ucv = in[s] ? flip_var(cv, fl[s], uraw[s]) : 0.0;
       if (s == 0) ucv_own = ucv;
       if (xr[s] >= 0) {
         double hp = 0.0, hm = 0.0;
@@ -321,37 +376,30 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
         L.Hym[row][yc[s]] = hm;
       }
     }
-    // x faces f = X0-5+k at column Q: F_bar = dt/dx (rbar_f - rbar_{f-1})
-    for (int e = t; e < G::NFX * TY; e += G::NT) {
-      const int k = e >> 6, col = e & 63;
-      const int f = X0 - 5 + k, jj = Y0 + col - 3;
-      double fb = 0.0;
-      if (f >= 0 && f <= nx && jj >= 0 && jj < ny) {
-        const double rl = f - 1 >= 0 ? rbar[(size_t)cv * n + (f - 1) * ny + jj] : 0.0;
-        const double rr = f < nx ? rbar[(size_t)cv * n + f * ny + jj] : 0.0;
-        fb = dtdx * (rr - rl);
-      }
-      L.Fbx[k][col] = fb;
+#pragma unroll
+    for (int r = 0; r < NEX; ++r) {
+      if (!(xfl[r] & 1u)) continue;
+      const int e = t + r * G::NT;
+      const double rl = xfl[r] & 4u ? rxl[r] : 0.0, rr = xfl[r] & 8u ? rxr[r] : 0.0;
+      L.Fbx[e >> 6][e & 63] = xfl[r] & 2u ? dtdx * (rr - rl) : 0.0;
     }
-    // y faces g = Y0-5+k at row P
-    for (int e = t; e < TXA * G::NFY; e += G::NT) {
-      const int row = e / G::NFY, k = e - row * G::NFY;
-      const int g = Y0 - 5 + k, ii = X0 + row - 3;
-      double fb = 0.0;
-      if (g >= 0 && g <= ny && ii >= 0 && ii < nx) {
-        const double rl = g - 1 >= 0 ? rbar[(size_t)cv * n + ii * ny + (g - 1)] : 0.0;
-        const double rr = g < ny ? rbar[(size_t)cv * n + ii * ny + g] : 0.0;
-        fb = dtdy * (rr - rl);
-      }
-      L.Fby[row][k] = fb;
+#pragma unroll
+    for (int r = 0; r < NEY; ++r) {
+      if (!(yfl[r] & 1u)) continue;
+      const int e = t + r * G::NT;
+      const int row = e / G::NFY;
+      const double rl = yfl[r] & 4u ? ryl[r] : 0.0, rr = yfl[r] & 8u ? ryr[r] : 0.0;
+      L.Fby[row][e - row * G::NFY] = yfl[r] & 2u ? dtdy * (rr - rl) : 0.0;
     }
     __syncthreads();
+    if (cv + 1 < NU) load_raw(cv + 1);
     // ---- x and y faces: F_bar * dq/dv_m into LDS ----
     for (int e = t; e < 2 * (TXA + 4) * TY; e += G::NT) {
       const int side = e / ((TXA + 4) * TY), rem = e - side * ((TXA + 4) * TY);
       const int kk = rem >> 6, col = rem & 63;
       const int f = side == 0 ? X0 - 4 + kk : X0 - 5 + kk;   // face index
       const int jj = Y0 + col - 3;
+      double* out = &Cx[((side * (TXA + 4) + kk) * 5) * TY + col];
       if (f >= 0 && f <= nx && jj >= 0 && jj < ny) {
         double g[5];
         if (side == 0)   // left-biased, cells f..f+4 = region rows kk..kk+4
@@ -361,16 +409,36 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
           weno5_grad(L.Hxm[kk + 4][col], L.Hxm[kk + 3][col], L.Hxm[kk + 2][col],
                      L.Hxm[kk + 1][col], L.Hxm[kk][col], g);
         const double fb = L.Fbx[side == 0 ? kk + 1 : kk][col];
-        double* out = &L.Cx[((side * (TXA + 4) + kk) * 5) * TY + col];
 #pragma unroll
         for (int m = 0; m < 5; ++m) out[m * TY] = fb * g[m];
+      } else {         // a face outside the grid contributes nothing
+#pragma unroll
+        for (int m = 0; m < 5; ++m) out[m * TY] = 0.0;
       }
+    }
+    // ---- the own cell's sums over the faces that reach it ----
+    // (faces outside the grid hold zeros, so every read is unconditional
+    // and the ten reads of a direction are in flight together)
+    double hpx = 0.0, hmx = 0.0, hpy = 0.0, hmy = 0.0;
+    const auto gather_x = [&]() {
+      if (work && xrow) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) hpx += Cx[((w + 4 - m) * 5 + m) * TY + ly];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) hmx += Cx[(((TXA + 4) + w + m) * 5 + m) * TY + ly];
+      }
+    };
+    if constexpr (SEQ) {
+      __syncthreads();
+      gather_x();
+      __syncthreads();
     }
     for (int e = t; e < 2 * TXA * (TY + 4); e += G::NT) {
       const int side = e / (TXA * (TY + 4)), rem = e - side * (TXA * (TY + 4));
       const int row = rem / (TY + 4), kk = rem - row * (TY + 4);
       const int gf = side == 0 ? Y0 - 4 + kk : Y0 - 5 + kk;
       const int ii = X0 + row - 3;
+      double* out = &Cy[((side * TXA + row) * 5) * (TY + 4) + kk];
       if (gf >= 0 && gf <= ny && ii >= 0 && ii < nx) {
         double g[5];
         if (side == 0)
@@ -380,31 +448,20 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
           weno5_grad(L.Hym[row][kk + 4], L.Hym[row][kk + 3], L.Hym[row][kk + 2],
                      L.Hym[row][kk + 1], L.Hym[row][kk], g);
         const double fb = L.Fby[row][side == 0 ? kk + 1 : kk];
-        double* out = &L.Cy[((side * TXA + row) * 5) * (TY + 4) + kk];
 #pragma unroll
         for (int m = 0; m < 5; ++m) out[m * (TY + 4)] = fb * g[m];
+      } else {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) out[m * (TY + 4)] = 0.0;
       }
     }
     __syncthreads();
-    // ---- the own cell's sums over the faces that reach it ----
-    double hpx = 0.0, hmx = 0.0, hpy = 0.0, hmy = 0.0;
-    if (work && xrow) {
-#pragma unroll
-      for (int m = 0; m < 5; ++m)
-        if (P - m >= 0 && P - m <= nx) hpx += L.Cx[((w + 4 - m) * 5 + m) * TY + ly];
-#pragma unroll
-      for (int m = 0; m < 5; ++m)
-        if (P - 5 + m >= 0 && P - 5 + m <= nx)
-          hmx += L.Cx[(((TXA + 4) + w + m) * 5 + m) * TY + ly];
-    }
+    if constexpr (!SEQ) gather_x();
     if (work && ycol) {
 #pragma unroll
-      for (int m = 0; m < 5; ++m)
-        if (Q - m >= 0 && Q - m <= ny) hpy += L.Cy[(w * 5 + m) * (TY + 4) + ly + 4 - m];
+      for (int m = 0; m < 5; ++m) hpy += Cy[(w * 5 + m) * (TY + 4) + ly + 4 - m];
 #pragma unroll
-      for (int m = 0; m < 5; ++m)
-        if (Q - 5 + m >= 0 && Q - 5 + m <= ny)
-          hmy += L.Cy[((TXA + w) * 5 + m) * (TY + 4) + ly + m];
+      for (int m = 0; m < 5; ++m) hmy += Cy[((TXA + w) * 5 + m) * (TY + 4) + ly + m];
     }
     if (work) {
       const PrimL& qo = q[0];
@@ -444,9 +501,9 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
       }
     }
     // No barrier before the next variable: its H and F_bar writes only
-    // conflict with this variable's face loops (finished before the barrier
-    // above), and its face loops' Cx/Cy writes come after its own first
-    // barrier, which every thread reaches after these reads.
+    // conflict with this variable's face loops (finished before the last
+    // barrier above), and its face loops' product writes come after its own
+    // first barrier, which every thread reaches after these reads.
   }
   // the own cell's |u| + a and |v| + a: the padded field's LF maxima are
   // reduced from these (per block here, then alpha_reduce_kernel), with the
@@ -647,25 +704,21 @@ __global__ __launch_bounds__(256) void fold_ghosts_kernel(AdjArgs a, const int* 
                                                           const double* __restrict__ abp,
                                                           double* __restrict__ Ut,
                                                           double* __restrict__ At) {
+  // one thread per (target, variable): blockIdx.y = variable (NU + 2 of
+  // them), so the few thousand targets still fill the GPU
   const int n = a.nx * a.ny, np = (a.nx + 6) * (a.ny + 6);
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x, v = blockIdx.y;
   if (k >= m) return;
-  double acc[NU + 2];
-#pragma unroll
-  for (int v = 0; v < NU + 2; ++v) acc[v] = 0.0;
+  const double* src = v < NU ? Ubp + (size_t)v * np : abp + (size_t)(v - NU) * np;
+  double acc = 0.0;
   const int e1 = beg[k + 1];
   for (int e = beg[k]; e < e1; ++e) {
     const int pt = ent[e] >> 2, flip = ent[e] & 3;
-#pragma unroll
-    for (int v = 0; v < NU; ++v) acc[v] += flip_var(v, flip, Ubp[(size_t)v * np + pt]);
-    acc[NU] += abp[pt];
-    acc[NU + 1] += abp[np + pt];
+    acc += v < NU ? flip_var(v, flip, src[pt]) : src[pt];
   }
   const int t = tgt[k];
-#pragma unroll
-  for (int v = 0; v < NU; ++v) Ut[(size_t)v * n + t] += acc[v];
-  At[t] += acc[NU];
-  At[n + t] += acc[NU + 1];
+  double* dst = v < NU ? Ut + (size_t)v * n : At + (size_t)(v - NU) * n;
+  dst[t] += acc;
 }
 
 // Cotangent of (e, Y) -> U: e = U3/rho - (u^2+v^2)/2, Y = U_k/rho.
@@ -726,24 +779,63 @@ __global__ __launch_bounds__(256) void getT_vjp_kernel(AdjArgs a, const double* 
   for (int v = 0; v < NU; ++v) Ub[v * n + t] = Ubar[v * n + t] + Ubc[v];
 }
 
-// U4 = U3 + S(U3, T3): Ub3 = Ub4 + S_vjp, Tb3 = S_vjp_T (model/reaction_model.py:183-193).
-// Ub3 may alias Ub4 (each thread reads its cell before writing it).
-__global__ __launch_bounds__(256) void chem_vjp_kernel(AdjArgs a, const double* __restrict__ U3,
-                                                       const double* __restrict__ aux3,
-                                                       const double* Ub4, double* Ub3,
-                                                       double* __restrict__ ab3) {
+// U4 = U3 + S(U3, T3): Ub3 = Ub4 + S_vjp, Tb3 = S_vjp_T (model/reaction_model.py:183-193),
+// in two kernels: chem_vjp_solve_kernel writes x = A^-1 w and lambda =
+// A^-T x_bar (8 planes each), chem_vjp_rates_kernel the cotangents.  Split,
+// each half fits two waves per SIMD without scratch (one kernel held 512
+// registers per lane and still spilled).
+// chem_vjp_*'s state of cell t read from the stage state U and the
+// cotangent Ub of the step's output.
+// (plane base + lane index: the base is wave-uniform, so every plane is
+// addressed from scalar registers and one shared lane offset)
+struct AdjCellState {
+  const double* U;
+  const double* Ub;
+  int n, t;
+  __device__ double rho() const { return U[t]; }
+  __device__ double operator()(int k) const { return (U + (size_t)(4 + k) * n)[t]; }
+  __device__ double sb(int k) const { return (Ub + (size_t)(4 + k) * n)[t]; }
+};
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void chem_vjp_solve_kernel(
+    AdjArgs a, const double* __restrict__ U3, const double* __restrict__ aux3,
+    const double* __restrict__ Ub4, double* __restrict__ xs, double* __restrict__ ls) {
+  __shared__ GibbsTables gt;
   const MechConst& c = *a.mc;
+  load_gibbs_tables(c, gt);
+  __syncthreads();
   const int n = a.nx * a.ny;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const double rho = U3[t];
-  double rY[JH_NS - 1], Sb[JH_NA];
+  double x[JH_NA], lam[JH_NA];
+  chem_vjp_solve(c, gt, AdjCellState{U3, Ub4, n, t}, aux3[n + t], a.dt, x, lam);
 #pragma unroll
-  for (int k = 0; k < JH_NS - 1; ++k) rY[k] = U3[(4 + k) * n + t];
+  for (int k = 0; k < JH_NA; ++k) {
+    xs[(size_t)k * n + t] = x[k];
+    ls[(size_t)k * n + t] = lam[k];
+  }
+}
+
+// Ub3 may alias Ub4 (each thread reads its cell before writing it).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void chem_vjp_rates_kernel(
+    AdjArgs a, const double* __restrict__ U3, const double* __restrict__ aux3, const double* Ub4,
+    const double* __restrict__ xs, const double* __restrict__ ls, double* Ub3,
+    double* __restrict__ ab3) {
+  __shared__ GibbsTables gt;
+  const MechConst& c = *a.mc;
+  load_gibbs_tables(c, gt);
+  __syncthreads();
+  const int n = a.nx * a.ny;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double x[JH_NA], lam[JH_NA];
 #pragma unroll
-  for (int k = 0; k < JH_NA; ++k) Sb[k] = Ub4[(4 + k) * n + t];
+  for (int k = 0; k < JH_NA; ++k) {
+    x[k] = xs[(size_t)k * n + t];
+    lam[k] = ls[(size_t)k * n + t];
+  }
   double rhob, rYb[JH_NS - 1], Tb;
-  chem_increment_vjp(c, rho, rY, aux3[n + t], a.dt, Sb, rhob, rYb, Tb);
+  chem_vjp_rates(c, gt, AdjCellState{U3, Ub4, n, t}, aux3[n + t], x, lam, rhob, rYb, Tb);
 #pragma unroll
   for (int v = 0; v < NU; ++v) {
     double g = Ub4[v * n + t];

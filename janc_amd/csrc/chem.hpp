@@ -140,15 +140,12 @@ __host__ __device__ __forceinline__ bool chem_increment_ld(const MechConst& c, c
   const double invT = rcp(T);
   double g[JH_NA];
   gibbs(gt, T, invT, g);
-  double eg[JH_NA], ieg[JH_NA];
+  double eg[JH_NA];
 #pragma unroll
-  for (int k = 0; k < JH_NA; ++k) {
-    eg[k] = exp(g[k]);
-    ieg[k] = rcp(eg[k]);
-  }
+  for (int k = 0; k < JH_NA; ++k) eg[k] = exp(g[k]);
   double w[JH_NA];
   JT J[JH_NA][JH_NA];
-  mech_rates_jacobian<JT>(c, T, invT, log(T), X, Xc, invMX, eg, ieg, w, J);
+  mech_rates_jacobian<JT>(c, T, invT, log(T), X, Xc, invMX, eg, w, J);
   const double idt = rcp(dt);
   double A[JH_NA][JH_NA];
 #pragma unroll
@@ -322,31 +319,41 @@ __host__ __device__ __forceinline__ void clip_vjp(double v, double lo, double hi
 // rho*Y_k (the state's species densities) and T.  The forward quantities are
 // recomputed; the solve is differentiated as lambda = A^-T x_bar,
 // w_bar = lambda, J_bar = lambda x^T (model/reaction_model.py:147-193 under
-// JAX's reverse mode).
-__host__ __device__ __forceinline__ void chem_increment_vjp(
-    const MechConst& c, double rho, const double (&rhoY_in)[JH_NS - 1], double T, double dt,
-    const double (&Sb)[JH_NA], double& rhob, double (&rYb)[JH_NS - 1], double& Tb) {
-  const double irho = rcp(rho);
-  double Y[JH_NS];
+// JAX's reverse mode).  Two halves, each with its own register budget:
+// chem_vjp_solve (Jacobian, factorisation, x = A^-1 w and lambda) and
+// chem_vjp_rates (the clip's and the rates' reverse given x and lambda).
+
+// The species state shared by both halves: Y (all species), X = rho Y / M,
+// Xc = max(X, 1e-50), invMX = 1 / (M Xc).
+template <typename LD>
+__host__ __device__ __forceinline__ void chem_vjp_state(const MechConst& c, const LD& ld,
+                                                        double (&Y)[JH_NS], double (&X)[JH_NS],
+                                                        double (&Xc)[JH_NA],
+                                                        double (&invMX)[JH_NA]) {
+  const double rho = ld.rho(), irho = rcp(rho);
 #pragma unroll
-  for (int k = 0; k < JH_NS - 1; ++k) Y[k] = rhoY_in[k] * irho;
+  for (int k = 0; k < JH_NS - 1; ++k) Y[k] = ld(k) * irho;
   double sum = 0.0;
 #pragma unroll
   for (int k = 0; k < JH_NS - 1; ++k) sum += Y[k];
   Y[JH_NS - 1] = 1.0 - sum;
-  double X[JH_NS];
 #pragma unroll
   for (int k = 0; k < JH_NS; ++k) X[k] = (rho * Y[k]) * c.R[k];
-  double Xc[JH_NA], invMX[JH_NA];
 #pragma unroll
   for (int k = 0; k < JH_NA; ++k) {
     Xc[k] = fmax(X[k], 1e-50);
     invMX[k] = rcp(c.M[k] * Xc[k]);
   }
-  const double invT = rcp(T), logT = log(T);
+}
+
+// Gibbs terms at T: gp_k = dg_k/dT (= h_k / T^2) and exp(g_k); gt: the
+// Gibbs tables (MechConst, or its copy in LDS: the range index is per lane).
+template <typename Tab>
+__host__ __device__ __forceinline__ void chem_vjp_gibbs(const Tab& c, double T, double invT,
+                                                        double (&gp)[JH_NA],
+                                                        double (&eg)[JH_NA]) {
   const int rr = T < c.Tcr ? 0 : 1;
   const double lT = log(c.T0 * T);
-  double g[JH_NA], gp[JH_NA];
 #pragma unroll
   for (int k = 0; k < JH_NA; ++k) {
     double h = c.hc[rr][k][0];
@@ -356,17 +363,44 @@ __host__ __device__ __forceinline__ void chem_increment_vjp(
 #pragma unroll
     for (int j = 1; j < 5; ++j) s = s * T + c.s[rr][k][j];
     s = s + c.logc[rr][k] * lT;
-    g[k] = s - h * invT;
+    const double g = s - h * invT;
     gp[k] = h * invT * invT;
+    eg[k] = exp(g);
   }
-  double eg[JH_NA], ieg[JH_NA];
-#pragma unroll
-  for (int k = 0; k < JH_NA; ++k) {
-    eg[k] = exp(g[k]);
-    ieg[k] = rcp(eg[k]);
+}
+
+// The state of both halves: ld.rho(), ld(k) = rho*Y_k (k < NS-1) and
+// ld.sb(k) = the cotangent of drhoY_k.  Each half reads Y and the
+// cotangents where it needs them and again after its long middle part,
+// behind a compiler barrier, instead of holding them in registers.
+// JH_READY(v): v is computed here (an empty asm that takes v in a
+// register), so work whose results are tied this way stays ahead of a
+// later JH_REREAD instead of being sunk past it.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define JH_REREAD() __asm__ volatile("" ::: "memory")
+#define JH_READY(v) __asm__ volatile("" : "+v"(v))
+#else
+#define JH_REREAD() ((void)0)
+#define JH_READY(v) ((void)0)
+#endif
+
+// x = A^-1 w (the increment before the clip) and lambda = A^-T x_bar, x_bar
+// the clip's cotangent of x.
+template <typename LD, typename GTab>
+__host__ __device__ __forceinline__ void chem_vjp_solve(const MechConst& c, const GTab& gt,
+                                                        const LD& ld, double T, double dt,
+                                                        double (&x)[JH_NA],
+                                                        double (&lam)[JH_NA]) {
+  double X[JH_NS], Xc[JH_NA], invMX[JH_NA];
+  {
+    double Y[JH_NS];
+    chem_vjp_state(c, ld, Y, X, Xc, invMX);
   }
-  double w[JH_NA], J[JH_NA][JH_NA];
-  mech_rates_jacobian(c, T, invT, logT, X, Xc, invMX, eg, ieg, w, J);
+  const double invT = rcp(T), logT = log(T);
+  double gp[JH_NA], eg[JH_NA];
+  chem_vjp_gibbs(gt, T, invT, gp, eg);
+  double J[JH_NA][JH_NA];
+  mech_rates_jacobian(c, T, invT, logT, X, Xc, invMX, eg, x, J);
   const double idt = rcp(dt);
 #pragma unroll
   for (int k = 0; k < JH_NA; ++k) {
@@ -375,25 +409,41 @@ __host__ __device__ __forceinline__ void chem_increment_vjp(
   }
   int piv[JH_NA];
   lu_factor<JH_NA>(J, piv);
-  lu_apply<JH_NA>(J, piv, w);          // w <- x
-  // reverse: S = rho * clip(x/rho, -Y, 1-Y)
-  double Yb[JH_NS], xb[JH_NA];
+  lu_apply<JH_NA>(J, piv, x);          // x <- A^-1 w
 #pragma unroll
-  for (int k = 0; k < JH_NS; ++k) Yb[k] = 0.0;
-  rhob = 0.0;
-  Tb = 0.0;
+  for (int k = 0; k < JH_NA; ++k) JH_READY(x[k]);
+  JH_REREAD();
+  const double rho = ld.rho(), irho = rcp(rho);
 #pragma unroll
-  for (int k = 0; k < JH_NA; ++k) {
-    const double v = w[k] * irho, lo = -Y[k], hi = 1.0 - Y[k];
-    const double out = fmin(fmax(v, lo), hi);
-    rhob += Sb[k] * out;
+  for (int k = 0; k < JH_NA; ++k) {    // S = rho * clip(x/rho, -Y, 1-Y)
+    const double Yk = ld(k) * irho;
+    const double v = x[k] * irho;
     double gv, glo, ghi;
-    clip_vjp(v, lo, hi, Sb[k] * rho, gv, glo, ghi);
-    Yb[k] -= glo + ghi;
-    xb[k] = gv * irho;
-    rhob -= gv * w[k] * irho * irho;
+    clip_vjp(v, -Yk, 1.0 - Yk, ld.sb(k) * rho, gv, glo, ghi);
+    lam[k] = gv * irho;
   }
-  lu_apply_T<JH_NA>(J, piv, xb);       // xb <- lambda
+  lu_apply_T<JH_NA>(J, piv, lam);      // lam <- A^-T x_bar
+}
+
+// The rest of the reverse given x and lambda: the rates' reverse (w_bar =
+// lambda, J_bar = lambda x^T), the clip's cotangents of rho and Y, and the
+// chain through X = rho Y / M.
+template <typename LD, typename GTab>
+__host__ __device__ __forceinline__ void chem_vjp_rates(const MechConst& c, const GTab& gt,
+                                                        const LD& ld,
+                                                        double T, const double (&x)[JH_NA],
+                                                        const double (&lam)[JH_NA],
+                                                        double& rhob,
+                                                        double (&rYb)[JH_NS - 1],
+                                                        double& Tb) {
+  double X[JH_NS], Xc[JH_NA], invMX[JH_NA];
+  {
+    double Y[JH_NS];
+    chem_vjp_state(c, ld, Y, X, Xc, invMX);
+  }
+  const double invT = rcp(T), logT = log(T);
+  double gp[JH_NA], eg[JH_NA];
+  chem_vjp_gibbs(gt, T, invT, gp, eg);
   double Xb[JH_NS], Xcb[JH_NA], iMXb[JH_NA];
 #pragma unroll
   for (int k = 0; k < JH_NS; ++k) Xb[k] = 0.0;
@@ -402,12 +452,38 @@ __host__ __device__ __forceinline__ void chem_increment_vjp(
     Xcb[k] = 0.0;
     iMXb[k] = 0.0;
   }
-  mech_rates_vjp(c, T, invT, logT, X, Xc, invMX, eg, ieg, gp, xb, w, Xb, Xcb, iMXb, Tb);
+  Tb = 0.0;
+  mech_rates_vjp(c, T, invT, logT, X, Xc, invMX, eg, gp, lam, x, Xb, Xcb, iMXb, Tb);
 #pragma unroll
   for (int n = 0; n < JH_NA; ++n) {
     Xcb[n] -= iMXb[n] * invMX[n] * invMX[n] * c.M[n];
-    Xb[n] += X[n] > 1e-50 ? Xcb[n] : (X[n] == 1e-50 ? 0.5 * Xcb[n] : 0.0);
+    JH_READY(Xcb[n]);
   }
+#pragma unroll
+  for (int k = 0; k < JH_NS; ++k) JH_READY(Xb[k]);
+  JH_READY(Tb);
+  JH_REREAD();
+  double Y[JH_NS];
+  chem_vjp_state(c, ld, Y, X, Xc, invMX);
+  const double rho = ld.rho(), irho = rcp(rho);
+  double Yb[JH_NS];
+#pragma unroll
+  for (int k = 0; k < JH_NS; ++k) Yb[k] = 0.0;
+  rhob = 0.0;
+#pragma unroll
+  for (int k = 0; k < JH_NA; ++k) {
+    const double v = x[k] * irho, lo = -Y[k], hi = 1.0 - Y[k];
+    const double out = fmin(fmax(v, lo), hi);
+    const double sb = ld.sb(k);
+    rhob += sb * out;
+    double gv, glo, ghi;
+    clip_vjp(v, lo, hi, sb * rho, gv, glo, ghi);
+    Yb[k] -= glo + ghi;
+    rhob -= gv * x[k] * irho * irho;
+  }
+#pragma unroll
+  for (int n = 0; n < JH_NA; ++n)
+    Xb[n] += X[n] > 1e-50 ? Xcb[n] : (X[n] == 1e-50 ? 0.5 * Xcb[n] : 0.0);
 #pragma unroll
   for (int k = 0; k < JH_NS; ++k) {
     Yb[k] += Xb[k] * rho * c.R[k];
@@ -419,6 +495,25 @@ __host__ __device__ __forceinline__ void chem_increment_vjp(
     rYb[k] = yk * irho;
     rhob -= yk * Y[k] * irho;
   }
+}
+
+// chem_vjp_*'s state from register values.
+struct RegStateVjp {
+  double r;
+  const double (&y)[JH_NS - 1];
+  const double (&s)[JH_NA];
+  __host__ __device__ double rho() const { return r; }
+  __host__ __device__ double operator()(int k) const { return y[k]; }
+  __host__ __device__ double sb(int k) const { return s[k]; }
+};
+
+__host__ __device__ __forceinline__ void chem_increment_vjp(
+    const MechConst& c, double rho, const double (&rhoY_in)[JH_NS - 1], double T, double dt,
+    const double (&Sb)[JH_NA], double& rhob, double (&rYb)[JH_NS - 1], double& Tb) {
+  double x[JH_NA], lam[JH_NA];
+  const RegStateVjp ld{rho, rhoY_in, Sb};
+  chem_vjp_solve(c, c, ld, T, dt, x, lam);
+  chem_vjp_rates(c, c, ld, T, x, lam, rhob, rYb, Tb);
 }
 
 }  // namespace jh

@@ -1117,7 +1117,7 @@ struct MemState {
   const Real* U;
   int n, i;
   __device__ double rho() const { return (double)U[i]; }
-  __device__ double operator()(int k) const { return (double)U[(4 + k) * n + i]; }
+  __device__ double operator()(int k) const { return (double)(U + (size_t)(4 + k) * n)[i]; }
 };
 
 // Point-implicit chemistry after RK3 (set_simulator.py:117-122):

@@ -252,9 +252,8 @@ def _generate(m):
     e("// Net production rates w_k and dw_k/d(rho Y_n) for one cell, following")
     e("// model/reaction_model.py:91-130 (reactionConstant_i summed over i).")
     e("// X: unclipped rho*Y/M (all species); Xc: clip(X, 1e-50) for active species;")
-    e("// invMX[n] = 1/(M_n*Xc_n); eg/ieg: exp(+-g) of the Gibbs s/R - h/(RT) of the active")
-    e("// species, so exp(-sum_k dnu_k g_k) = prod_k (eg_k or ieg_k)^|dnu_k| (formed here as")
-    e("// a quotient of eg products; ieg is read by the VJP).")
+    e("// invMX[n] = 1/(M_n*Xc_n); eg: exp(g) of the Gibbs s/R - h/(RT) of the active")
+    e("// species, so exp(-sum_k dnu_k g_k) is a quotient of eg products.")
     e("// JT: the Jacobian's storage type.  Rates, exponentials and every d_n are")
     e("// FP64; JT = float rounds each d_n once and accumulates J in FP32 (the")
     e("// mixed-precision form: half the registers of an FP64 J).")
@@ -262,7 +261,7 @@ def _generate(m):
     e("__host__ __device__ __forceinline__ void mech_rates_jacobian(")
     e("    const MechConst& __restrict__ c, const double T, const double invT, const double logT,")
     e("    const double (&X)[JH_NS], const double (&Xc)[JH_NA], const double (&invMX)[JH_NA],")
-    e("    const double (&eg)[JH_NA], const double (&ieg)[JH_NA], double (&w)[JH_NA],")
+    e("    const double (&eg)[JH_NA], double (&w)[JH_NA],")
     e("    JT (&J)[JH_NA][JH_NA])")
     e("{")
     e("  const double Pf = c.Pref * invT;   // 101325/P0/T")
@@ -396,7 +395,7 @@ def _generate_vjp(m):
     e("__host__ __device__ __forceinline__ void mech_rates_vjp(")
     e("    const MechConst& __restrict__ c, const double T, const double invT, const double logT,")
     e("    const double (&X)[JH_NS], const double (&Xc)[JH_NA], const double (&invMX)[JH_NA],")
-    e("    const double (&eg)[JH_NA], const double (&ieg)[JH_NA], const double (&gp)[JH_NA],")
+    e("    const double (&eg)[JH_NA], const double (&gp)[JH_NA],")
     e("    const double (&lam)[JH_NA], const double (&x)[JH_NA],")
     e("    double (&Xb)[JH_NS], double (&Xcb)[JH_NA], double (&iMXb)[JH_NA], double& Tb)")
     e("{")
@@ -405,7 +404,17 @@ def _generate_vjp(m):
     e("  double ml[JH_NA];")
     e("#pragma unroll")
     e("  for (int k = 0; k < JH_NA; ++k) ml[k] = c.M[k] * lam[k];")
+    # third-body concentrations first: X is not needed after them
     for i, r in enumerate(m.reactions):
+        if r.three_body:
+            e("  const double cM%d = %s;" % (i, " + ".join("c.eff[%d][%d] * X[%d]" % (i, k, k)
+                                                          for k in range(ns))))
+    for i, r in enumerate(m.reactions):
+        # one reaction at a time (as in the forward): interleaved reactions
+        # hold more live values than two waves per SIMD allow
+        e("#if defined(__HIP_DEVICE_COMPILE__)")
+        e("  __builtin_amdgcn_sched_barrier(0);")
+        e("#endif")
         e("  { // R%d: %s" % (i, r.equation))
         if r.b == 0.0:
             tb, dl = None, "0.0"
@@ -435,14 +444,21 @@ def _generate_vjp(m):
             pb_terms += ["Xc[%d]" % k] * int(n)
         e("    const double PF = %s;" % " * ".join(pf_terms))
         e("    const double PB = %s;" % " * ".join(pb_terms))
-        fac, dgsum = [], []
+        num, den, dgsum = [], [], []
         for k in range(na):
             d = r.products.get(k, 0.0) - r.reactants.get(k, 0.0)
             if d != 0.0:
-                fac += [("ieg[%d]" if d > 0 else "eg[%d]") % k] * int(abs(d))
+                (den if d > 0 else num).extend(["eg[%d]" % k] * int(abs(d)))
                 dgsum.append("%s%s gp[%d]" % ("- " if d > 0 else "+ ", "" if abs(d) == 1 else "2.0 *", k))
         vs = sum(r.products.values()) - sum(r.reactants.values())
-        eq = " * ".join(fac) if fac else "1.0"
+        # exp(-sum dnu g) as a quotient of eg products (one reciprocal per
+        # reaction instead of a live table of 1/eg)
+        if num and den:
+            eq = "(%s) * jh::rcp(%s)" % (" * ".join(num), " * ".join(den))
+        elif den:
+            eq = "jh::rcp(%s)" % " * ".join(den)
+        else:
+            eq = " * ".join(num) if num else "1.0"
         if vs == 1.0:
             eq += " * iPf"
         elif vs == -1.0:
@@ -471,8 +487,7 @@ def _generate_vjp(m):
         e("    const double s = %s;" % sexpr)
         involved = sorted(set(r.reactants) | set(r.products))
         if r.three_body:
-            e("    const double cM = %s;" % " + ".join("c.eff[%d][%d] * X[%d]" % (i, k, k)
-                                                     for k in range(ns)))
+            e("    const double cM = cM%d;" % i)
             e("    double sx = 0.0;   // sum_n x_n effM_in")
             for n in range(na):
                 e("    sx += x[%d] * c.effM[%d][%d];" % (n, i, n))
