@@ -394,26 +394,43 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
     __syncthreads();
     if (cv + 1 < NU) load_raw(cv + 1);
     // ---- x and y faces: F_bar * dq/dv_m into LDS ----
-    for (int e = t; e < 2 * (TXA + 4) * TY; e += G::NT) {
-      const int side = e / ((TXA + 4) * TY), rem = e - side * ((TXA + 4) * TY);
-      const int kk = rem >> 6, col = rem & 63;
+    // x faces kk = 0..TXA+3 per side and column; with 8 rows a thread takes
+    // a run of three consecutive faces of one column (2 sides x 4 runs x 64
+    // columns = the 512 threads) and reads the run's seven stencil values
+    // once.
+    const auto xface = [&](int side, int kk, int col, const double* h) {
       const int f = side == 0 ? X0 - 4 + kk : X0 - 5 + kk;   // face index
       const int jj = Y0 + col - 3;
       double* out = &Cx[((side * (TXA + 4) + kk) * 5) * TY + col];
       if (f >= 0 && f <= nx && jj >= 0 && jj < ny) {
         double g[5];
         if (side == 0)   // left-biased, cells f..f+4 = region rows kk..kk+4
-          weno5_grad(L.Hxp[kk][col], L.Hxp[kk + 1][col], L.Hxp[kk + 2][col], L.Hxp[kk + 3][col],
-                     L.Hxp[kk + 4][col], g);
+          weno5_grad(h[0], h[1], h[2], h[3], h[4], g);
         else             // right-biased, cells f+5..f+1 = region rows kk+4..kk
-          weno5_grad(L.Hxm[kk + 4][col], L.Hxm[kk + 3][col], L.Hxm[kk + 2][col],
-                     L.Hxm[kk + 1][col], L.Hxm[kk][col], g);
+          weno5_grad(h[4], h[3], h[2], h[1], h[0], g);
         const double fb = L.Fbx[side == 0 ? kk + 1 : kk][col];
 #pragma unroll
         for (int m = 0; m < 5; ++m) out[m * TY] = fb * g[m];
       } else {         // a face outside the grid contributes nothing
 #pragma unroll
         for (int m = 0; m < 5; ++m) out[m * TY] = 0.0;
+      }
+    };
+    if constexpr (TXA == 8) {
+      const int side = t >> 8, col = t & 63, kk0 = 3 * ((t >> 6) & 3);
+      double h[7];
+#pragma unroll
+      for (int j = 0; j < 7; ++j) h[j] = side == 0 ? L.Hxp[kk0 + j][col] : L.Hxm[kk0 + j][col];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) xface(side, kk0 + i, col, h + i);
+    } else {
+      for (int e = t; e < 2 * (TXA + 4) * TY; e += G::NT) {
+        const int side = e / ((TXA + 4) * TY), rem = e - side * ((TXA + 4) * TY);
+        const int kk = rem >> 6, col = rem & 63;
+        double h[5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j) h[j] = side == 0 ? L.Hxp[kk + j][col] : L.Hxm[kk + j][col];
+        xface(side, kk, col, h);
       }
     }
     // ---- the own cell's sums over the faces that reach it ----
@@ -433,26 +450,51 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
       gather_x();
       __syncthreads();
     }
-    for (int e = t; e < 2 * TXA * (TY + 4); e += G::NT) {
-      const int side = e / (TXA * (TY + 4)), rem = e - side * (TXA * (TY + 4));
-      const int row = rem / (TY + 4), kk = rem - row * (TY + 4);
+    // y faces kk = 0..TY+3 per side and row: with 8 rows the 512 threads
+    // take the pairs (kk, kk+1), kk < TY, of every side and row, and the
+    // first 64 threads the last four faces of each
+    const auto yface = [&](int side, int row, int kk, const double* h) {
       const int gf = side == 0 ? Y0 - 4 + kk : Y0 - 5 + kk;
       const int ii = X0 + row - 3;
       double* out = &Cy[((side * TXA + row) * 5) * (TY + 4) + kk];
       if (gf >= 0 && gf <= ny && ii >= 0 && ii < nx) {
         double g[5];
         if (side == 0)
-          weno5_grad(L.Hyp[row][kk], L.Hyp[row][kk + 1], L.Hyp[row][kk + 2], L.Hyp[row][kk + 3],
-                     L.Hyp[row][kk + 4], g);
+          weno5_grad(h[0], h[1], h[2], h[3], h[4], g);
         else
-          weno5_grad(L.Hym[row][kk + 4], L.Hym[row][kk + 3], L.Hym[row][kk + 2],
-                     L.Hym[row][kk + 1], L.Hym[row][kk], g);
+          weno5_grad(h[4], h[3], h[2], h[1], h[0], g);
         const double fb = L.Fby[row][side == 0 ? kk + 1 : kk];
 #pragma unroll
         for (int m = 0; m < 5; ++m) out[m * (TY + 4)] = fb * g[m];
       } else {
 #pragma unroll
         for (int m = 0; m < 5; ++m) out[m * (TY + 4)] = 0.0;
+      }
+    };
+    if constexpr (TXA == 8) {
+      {
+        const int side = t >> 8, row = (t >> 5) & 7, kk0 = 2 * (t & 31);
+        double h[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) h[j] = side == 0 ? L.Hyp[row][kk0 + j] : L.Hym[row][kk0 + j];
+        yface(side, row, kk0, h);
+        yface(side, row, kk0 + 1, h + 1);
+      }
+      if (t < 64) {
+        const int side = t >> 5, row = (t >> 2) & 7, kk = TY + (t & 3);
+        double h[5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j) h[j] = side == 0 ? L.Hyp[row][kk + j] : L.Hym[row][kk + j];
+        yface(side, row, kk, h);
+      }
+    } else {
+      for (int e = t; e < 2 * TXA * (TY + 4); e += G::NT) {
+        const int side = e / (TXA * (TY + 4)), rem = e - side * (TXA * (TY + 4));
+        const int row = rem / (TY + 4), kk = rem - row * (TY + 4);
+        double h[5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j) h[j] = side == 0 ? L.Hyp[row][kk + j] : L.Hym[row][kk + j];
+        yface(side, row, kk, h);
       }
     }
     __syncthreads();
@@ -644,27 +686,31 @@ __global__ __launch_bounds__(256) void alpha_reduce_kernel(const double* __restr
   }
 }
 
-__global__ __launch_bounds__(256) void alpha_apply_kernel(AdjArgs a, const double* __restrict__ U,
-                                                          const double* __restrict__ aux,
-                                                          const double* __restrict__ spd,
-                                                          const unsigned long long* alpha_bits,
-                                                          const double* scal,
-                                                          const unsigned int* cnt,
-                                                          double* __restrict__ Ut,
-                                                          double* __restrict__ At,
-                                                          double* __restrict__ Ubp,
-                                                          double* __restrict__ abp) {
+// One block per flux_vjp_tile_kernel tile (same cells): a tile whose block
+// maxima (part) are not the field's holds no maximiser and exits at once.
+template <int TXA>
+__global__ __launch_bounds__(AdjTile<TXA>::NT) void alpha_apply_kernel(
+    AdjArgs a, const double* __restrict__ U, const double* __restrict__ aux,
+    const double* __restrict__ spd, const double* __restrict__ part,
+    const unsigned long long* alpha_bits, const double* scal, const unsigned int* cnt,
+    double* __restrict__ Ut, double* __restrict__ At, double* __restrict__ Ubp,
+    double* __restrict__ abp) {
+  const double gx = bitsd(alpha_bits[0]), gy = bitsd(alpha_bits[1]);
+  const double* pb = part + 6 * blockIdx.x;
+  if (!(cnt[0] > 0 && pb[2] == gx) && !(cnt[1] > 0 && pb[3] == gy)) return;
   const MechConst& c = *a.mc;
-  const int PY = a.ny + 6, np = (a.nx + 6) * PY;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= np) return;
+  const int PX = a.nx + 6, PY = a.ny + 6, np = PX * PY;
+  const int tiles_y = (PY + TY - 1) / TY;
+  const int P = (blockIdx.x / tiles_y) * TXA + (threadIdx.x >> 6);
+  const int Q = (blockIdx.x % tiles_y) * TY + (threadIdx.x & 63);
+  if (P >= PX || Q >= PY) return;
+  const int t = P * PY + Q;
   int sx, sy, flip;
-  const int P = t / PY, Q = t - P * PY;
   if (!pad_src(a, P, Q, sx, sy, flip)) return;
   // the tie test compares the very values the maxima were reduced from
   // (stored by flux_vjp_tile_kernel), not a recomputation
-  const bool mx = cnt[0] > 0 && is_max(spd[t], bitsd(alpha_bits[0]));
-  const bool my = cnt[1] > 0 && is_max(spd[np + t], bitsd(alpha_bits[1]));
+  const bool mx = cnt[0] > 0 && is_max(spd[t], gx);
+  const bool my = cnt[1] > 0 && is_max(spd[np + t], gy);
   if (!mx && !my) return;
   const PrimD q = prim_src(c, U, aux, a.nx * a.ny, sx * a.ny + sy, flip);
   const double wx = mx ? scal[0] / cnt[0] : 0.0, wy = my ? scal[1] / cnt[1] : 0.0;

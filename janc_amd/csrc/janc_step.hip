@@ -48,6 +48,8 @@ struct jh_ctx {
   long long* trace = nullptr;
   int trace_cap = 0, trace_step = 0;
   TraceScratch* trs = nullptr;
+  // per-block maxima of the prologue's reduction (3 x grid_cells blocks)
+  unsigned long long* rpart = nullptr;
   // chemistry clip branches recorded / replayed per step
   // (jh_ctx_set_clip_branches): (max steps, nx*ny) masks
   unsigned short* clipm = nullptr;
@@ -301,7 +303,9 @@ int prologue(jh_ctx* c, const Real* U, const Real* aux, double dt, double cfl) {
   Timed tm(c, JH_KERNEL_REDUCE);
   reset_kernel<<<1, 64, 0, c->stream>>>(c->sc, c->parity);
   auto a = make_args<Real>(c, dt, cfl);
-  reduce_kernel<Real><<<grid_cells(c), 256, 0, c->stream>>>(a, U, aux, c->parity, 1);
+  const int nb = grid_cells(c);
+  reduce_part_kernel<Real><<<nb, 256, 0, c->stream>>>(a, U, aux, c->rpart);
+  reduce_final_kernel<<<1, 256, 0, c->stream>>>(c->rpart, nb, c->sc, c->parity, 1);
   return check_launch(c, "reduce_kernel");
 }
 
@@ -560,8 +564,8 @@ void flux_vjp(jh_ctx* c, const AdjArgs& a, AdjWork& w, const double* Us, const d
   flux_vjp_tile_kernel<kAdjRows, kAdjSeq><<<w.nblk, AdjTile<kAdjRows>::NT, 0, st>>>(
       a, Us, As, rb, alpha, Ut, At, w.Ubp, w.abp, w.spd, w.part);
   alpha_reduce_kernel<<<1, 256, 0, st>>>(w.part, w.nblk, w.scal, w.amx, w.cnt);
-  alpha_apply_kernel<<<blocks_for(w.npad), 256, 0, st>>>(a, Us, As, w.spd, w.amx, w.scal, w.cnt,
-                                                         Ut, At, w.Ubp, w.abp);
+  alpha_apply_kernel<kAdjRows><<<w.nblk, AdjTile<kAdjRows>::NT, 0, st>>>(
+      a, Us, As, w.spd, w.part, w.amx, w.scal, w.cnt, Ut, At, w.Ubp, w.abp);
   const int m = c->adj_fold_m;
   const int* fl = w.fold;
   if (m > 0)
@@ -1015,6 +1019,7 @@ int jh_ctx_create(int nx, int ny, double dx, double dy, const int* bc_kind,
   hipError_t e = hipSuccess;
   e = e ? e : hipMalloc(&c->mc, sizeof(MechConst));
   e = e ? e : hipMalloc(&c->sc, sizeof(StepScalars));
+  e = e ? e : hipMalloc(&c->rpart, 3 * sizeof(unsigned long long) * grid_cells(c));
   for (int s = 0; s < 4 && !e; ++s) {
     const size_t len = (s < 2) ? ny : nx;
     e = hipMalloc(&c->ghost[s], es * NV * 3 * len);
@@ -1050,6 +1055,7 @@ int jh_ctx_destroy(jh_ctx* c) {
   if (c->hnit) (void)hipFree(c->hnit);
   (void)hipFree(c->mc);
   (void)hipFree(c->sc);
+  if (c->rpart) (void)hipFree(c->rpart);
   for (int s = 0; s < 4; ++s) (void)hipFree(c->ghost[s]);
   for (int i = 0; i < 2; ++i) {
     (void)hipFree(c->tmpU[i]);

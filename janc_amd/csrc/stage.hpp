@@ -308,6 +308,85 @@ __global__ __launch_bounds__(256) void reduce_kernel(StepArgs<Real> a, const Rea
   reduce_body<Real>(a, U, aux, slot_parity, with_cfl, blockIdx.x, gridDim.x, red, redb);
 }
 
+// The same maxima in two passes without same-address atomics: every block
+// of a short kernel finishes at once, so a thousand blocks' atomicMax on
+// three addresses serialise (~25 us at 1024 blocks).  Pass 1 writes each
+// block's (max|u|+a, max|v|+a, CFL max) bit patterns to part[3 blk ..];
+// reduce_final_kernel (one block) folds them into the slots.
+template <typename Real>
+__global__ __launch_bounds__(256) void reduce_part_kernel(StepArgs<Real> a,
+                                                          const Real* __restrict__ U,
+                                                          const Real* __restrict__ aux,
+                                                          unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long red[3][4];
+  const MechConst& c = *a.mc;
+  const int n = a.nx * a.ny;
+  unsigned long long v[3] = {0ull, 0ull, 0ull};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const Prim<double> q = cell_prim<double, true>(c, U + i, aux + i, n);
+    const double sx = fabs(q.u) + q.a, sy = fabs(q.v) + q.a;
+    const unsigned long long bx = dbits(nan_to_zero(sx)), by = dbits(nan_to_zero(sy));
+    const unsigned long long bc = dbits(sx / a.dx + sy / a.dy);
+    v[0] = bx > v[0] ? bx : v[0];
+    v[1] = by > v[1] ? by : v[1];
+    v[2] = bc > v[2] ? bc : v[2];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(v[k], off, 64);
+      v[k] = o > v[k] ? o : v[k];
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) red[k][wid] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    unsigned long long m = red[k][0];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) m = red[k][w] > m ? red[k][w] : m;
+    part[3 * blockIdx.x + k] = m;
+  }
+}
+
+__global__ __launch_bounds__(256) void reduce_final_kernel(const unsigned long long* __restrict__ part,
+                                                           int nblk, StepScalars* sc, int parity,
+                                                           int with_cfl) {
+  __shared__ unsigned long long red[3][4];
+  unsigned long long v[3] = {0ull, 0ull, 0ull};
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] = part[3 * b + k] > v[k] ? part[3 * b + k] : v[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(v[k], off, 64);
+      v[k] = o > v[k] ? o : v[k];
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) red[k][wid] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    unsigned long long m = red[k][0];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) m = red[k][w] > m ? red[k][w] : m;
+    unsigned long long* dst = k < 2 ? &sc->amax[parity][0][k] : &sc->cmax[parity];
+    if (k < 2 || with_cfl) atomicMax(dst, m);
+  }
+}
+
 // ------------------------------------------------------------ ghost cells
 
 // Pressure-feedback injector model (examples/janc_basic_example2.ipynb,
