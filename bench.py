@@ -726,10 +726,22 @@ def _amr_leg(mech, U3, aux3, g3, dt, steps, warmup, torch):
 
 
 def _summary_adjoint(adj):
+    """The adjoint leg's headline figures (its own line holds the rest)."""
     keys = ("adjoint_over_fwd_step", "adjoint_over_fwd_rollout", "step_fwd_ms", "step_vjp_ms",
-            "fd_directions", "fd_max_rel_err", "fd_median_rel_err", "fd_switch_free",
-            "fd_branch_frozen", "fd_switches")
-    return {k: adj[k] for k in keys if k in adj}
+            "fd_directions")
+    out = {k: adj[k] for k in keys if k in adj}
+    free = adj.get("fd_switch_free") or {}
+    four = free.get("4pt_frozen_eps1e-6")
+    if four:
+        out["fd_switch_free_4pt"] = {k: four[k] for k in ("directions", "max_rel_err",
+                                                           "within_1e-6")}
+    errs = adj.get("fd_max_rel_err") or {}
+    if "1e-06" in errs:
+        out["fd_max_rel_err_2pt_all"] = errs["1e-06"]
+    sw = adj.get("fd_switches") or {}
+    if "lf_4pt" in sw:
+        out["fd_directions_with_lf_switch"] = sw["lf_4pt"]
+    return out
 
 
 def main():
