@@ -46,8 +46,18 @@ STAGE_TRAFFIC_B_PER_CELL = {"fp64": 459.8, "mixed": 211.9}
 
 # Measured FP64 FLOP per cell per chemistry launch (rocprofv3
 # SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes / cells, FMA = 2; summary under
-# profiles/).  The chemistry is FP64 in both forms.
-CHEM_FP64_FLOP_PER_CELL = 3239.0
+# profiles/): rates, exponentials, LU and temperature solve are FP64 in both
+# forms; the mixed form assembles its Jacobian in FP32 (368 FP32 FLOP more).
+CHEM_FP64_FLOP_PER_CELL = {"mixed": 2760.2, "fp64": 3249.6}
+
+# Measured VALU wave instructions x 64 per cell per stage-kernel launch
+# (rocprofv3 SQ_INSTS_VALU; mixed form, tile variant 1; summary under
+# profiles/).  The VALU issue floor of a launch is instructions / 64 wave
+# instructions x 4 cycles (64 lanes on a 16-lane SIMD; FP32 without packed
+# math issues at the FP64 rate) / (1024 SIMDs x 2.4 GHz).
+STAGE_VALU_PER_CELL = {"mixed": 3475.3}
+SIMDS = 1024
+CLOCK_GHZ = 2.4
 
 # Per-kernel HIP events bracket the launches of every KERNEL_TIMING_EVERY-th
 # step of the timed region: events around every launch add ~5 % to a step.
@@ -872,13 +882,21 @@ def _gpu_legs(args, mech, U3, aux3, g3, cells, pool, cpu, only):
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": r["frac_hbm"],
                 "traffic": tr * cells if tr else None,
                 "algorithmic_bytes": r["algorithmic_bytes"], "avg_ms": r["avg_ms"]}
+    roof_valu = None
+    if "stage" in roof:
+        floor_ms = (STAGE_VALU_PER_CELL["mixed"] * cells / 64.0 * 4.0
+                    / (SIMDS * CLOCK_GHZ * 1e9) * 1e3)
+        roof_valu = {"kernel": "stage", "bound": "valu-issue",
+                     "instr_per_cell": STAGE_VALU_PER_CELL["mixed"],
+                     "floor_ms": floor_ms, "avg_ms": roof["stage"]["avg_ms"],
+                     "frac": floor_ms / roof["stage"]["avg_ms"]}
     chem_fv = None
     if "chem" in kt and kt["chem"][1]:
         ms, nl = kt["chem"]
-        tfs = CHEM_FP64_FLOP_PER_CELL * cells / (ms / nl * 1e-3) / 1e12
+        tfs = CHEM_FP64_FLOP_PER_CELL["mixed"] * cells / (ms / nl * 1e-3) / 1e12
         chem_fv = {"bound": "fp64-valu", "achieved": tfs, "peak": FP64_VECTOR_PEAK_TFLOPS,
                    "unit": "TFLOP/s", "frac": tfs / FP64_VECTOR_PEAK_TFLOPS,
-                   "avg_ms": ms / nl, "flop_per_cell": CHEM_FP64_FLOP_PER_CELL}
+                   "avg_ms": ms / nl, "flop_per_cell": CHEM_FP64_FLOP_PER_CELL["mixed"]}
 
     # ---- mixed accuracy: 1000 steps against the FP64 step ----
     acc = None
@@ -955,6 +973,7 @@ def _gpu_legs(args, mech, U3, aux3, g3, cells, pool, cpu, only):
                             for k, v in fp64["kernels"].items() if v["launches"]},
         "health_flags": {"mixed": mixed["health_flags"], "fp64": fp64["health_flags"]},
         "roofline": roofline,
+        "roofline_valu": roof_valu,
         "roofline_chem": chem_fv,
     })
     if acc is not None:
