@@ -1031,9 +1031,10 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
     if constexpr (kRuns) {
       // FP32 WENO5: runs of four consecutive faces (weno5_run).  The first
       // half of the block takes the x faces: lane = column, wave = run
-      // (TX / 4 runs per column; the last also takes face row TX).  The
-      // second half the y faces: 16 runs per row, run = u % 16, row = u / 16;
-      // run 15 also takes face column 64.
+      // (TX / 4 runs per column; the last also takes face row TX); its
+      // first wave also takes the y faces of column 64 (lane = row), so no
+      // y wave runs a single-face pass for a few of its lanes.  The second
+      // half the y faces 0..63: 16 runs per row, run = u % 16, row = u / 16.
       constexpr int NXT = NT / 2;
       if (t < NXT) {
         const int col = t & 63, fr0 = (t >> 6) * 4;
@@ -1045,16 +1046,17 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
                          weno5(sHxm[TX + 5][col], sHxm[TX + 4][col], sHxm[TX + 3][col],
                                sHxm[TX + 2][col], sHxm[TX + 1][col]);
         }
-      } else {
-        const int u = t - NXT, row = u >> 4, fc0 = (u & 15) * 4;
-        weno5_run<4, false, false>(&sHyp[row][fc0], 1, &sFy[row][fc0], 1);
-        weno5_run<4, true, true>(&sHym[row][fc0 + 1], 1, &sFy[row][fc0], 1);
-        if (fc0 + 4 == TY) {
+        if (fr0 == 0 && col < TX) {
+          const int row = col;
           sFy[row][TY] = weno5(sHyp[row][TY], sHyp[row][TY + 1], sHyp[row][TY + 2],
                                sHyp[row][TY + 3], sHyp[row][TY + 4]) +
                          weno5(sHym[row][TY + 5], sHym[row][TY + 4], sHym[row][TY + 3],
                                sHym[row][TY + 2], sHym[row][TY + 1]);
         }
+      } else {
+        const int u = t - NXT, row = u >> 4, fc0 = (u & 15) * 4;
+        weno5_run<4, false, false>(&sHyp[row][fc0], 1, &sFy[row][fc0], 1);
+        weno5_run<4, true, true>(&sHym[row][fc0 + 1], 1, &sFy[row][fc0], 1);
       }
     } else
     for (int it = t; it < NXF + NYF; it += NT) {
