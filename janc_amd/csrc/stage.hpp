@@ -1029,22 +1029,22 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
     }
     // ---- phase 2: face reconstructions ----
     if constexpr (kRuns) {
-      // FP32 WENO5: runs of four consecutive faces (weno5_run).  The first
-      // half of the block takes the x faces: lane = column, wave = run
-      // (TX / 4 runs per column; the last also takes face row TX); its
-      // first wave also takes the y faces of column 64 (lane = row), so no
-      // y wave runs a single-face pass for a few of its lanes.  The second
-      // half the y faces 0..63: 16 runs per row, run = u % 16, row = u / 16.
+      // FP32 WENO5: runs of consecutive faces (weno5_run).  The first half
+      // of the block takes the x faces: lane = column, wave = run (TX / 4
+      // runs of four per column, the last a run of five through face row
+      // TX); its first wave also takes the y faces of column 64 (lane =
+      // row), so no y wave runs a single-face pass for a few of its lanes.
+      // The second half the y faces 0..63: 16 runs of four per row, run =
+      // u % 16, row = u / 16.
       constexpr int NXT = NT / 2;
       if (t < NXT) {
         const int col = t & 63, fr0 = (t >> 6) * 4;
-        weno5_run<4, false, false>(&sHxp[fr0][col], TY, &sFx[fr0][col], TY);
-        weno5_run<4, true, true>(&sHxm[fr0 + 1][col], TY, &sFx[fr0][col], TY);
-        if (fr0 + 4 == TX) {
-          sFx[TX][col] = weno5(sHxp[TX][col], sHxp[TX + 1][col], sHxp[TX + 2][col],
-                               sHxp[TX + 3][col], sHxp[TX + 4][col]) +
-                         weno5(sHxm[TX + 5][col], sHxm[TX + 4][col], sHxm[TX + 3][col],
-                               sHxm[TX + 2][col], sHxm[TX + 1][col]);
+        if (fr0 + 4 == TX) {   // the last run also takes face row TX (wave-uniform)
+          weno5_run<5, false, false>(&sHxp[fr0][col], TY, &sFx[fr0][col], TY);
+          weno5_run<5, true, true>(&sHxm[fr0 + 1][col], TY, &sFx[fr0][col], TY);
+        } else {
+          weno5_run<4, false, false>(&sHxp[fr0][col], TY, &sFx[fr0][col], TY);
+          weno5_run<4, true, true>(&sHxm[fr0 + 1][col], TY, &sFx[fr0][col], TY);
         }
         if (fr0 == 0 && col < TX) {
           const int row = col;
