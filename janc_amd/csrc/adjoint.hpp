@@ -173,6 +173,26 @@ __device__ __forceinline__ void prim_to_U_vjp(const MechConst& c, const PrimD& q
 // and each cell gathers its five left- and five right-biased terms (in the
 // order m = 0..4); then the split-flux and primitive VJPs of the reference's
 // forward map.
+// Tiles are aligned to the interior: origins (X0, Y0) = (3 + TXA a, 3 + 64 b)
+// in padded coordinates, a >= -1, b >= -1, so an nx x ny interior whose
+// sides are multiples of (TXA, 64) splits into whole tiles and the 3-cell
+// ghost frame falls into thin tiles around them, whose face work is packed
+// into their first waves (a padded grid tiled from its corner puts a
+// partial tile into every band: at 1024 x 256, 516 full and 129 thin tiles
+// instead of 512 and 268).  Measured at 1024 x 256 the two tilings take the
+// same time (the kernel is bound by latency at two waves per SIMD, not by
+// its tile count).
+// Tile counts (bands a = -1 .. , b = -1 ..) and the origin of tile blk.
+template <int TXA>
+__host__ __device__ inline int adj_tiles_x(int nx) { return (nx + 3 + TXA - 1) / TXA + 1; }
+__host__ __device__ inline int adj_tiles_y(int ny) { return (ny + 3 + TY - 1) / TY + 1; }
+template <int TXA>
+__device__ inline void adj_tile_origin(int blk, int ny, int& X0, int& Y0) {
+  const int ty = adj_tiles_y(ny), a = blk / ty;
+  X0 = 3 + TXA * (a - 1);
+  Y0 = 3 + TY * (blk - a * ty - 1);
+}
+
 template <int TXA>
 struct AdjTile {
   static constexpr int NT = 64 * TXA;          // threads (one padded cell each)
@@ -232,8 +252,8 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
   const MechConst& c = *a.mc;
   const int nx = a.nx, ny = a.ny, n = nx * ny, PX = nx + 6, PY = ny + 6, np = PX * PY;
   const int t = threadIdx.x, ly = t & 63, w = t >> 6;
-  const int tiles_y = (PY + TY - 1) / TY;
-  const int X0 = (blockIdx.x / tiles_y) * TXA, Y0 = (blockIdx.x % tiles_y) * TY;
+  int X0, Y0;
+  adj_tile_origin<TXA>(blockIdx.x, ny, X0, Y0);
   const double ax = bitsd(alpha_bits[0]), ay = bitsd(alpha_bits[1]);
   const double dtdx = a.dt / a.dx, dtdy = a.dt / a.dy;
 
@@ -346,6 +366,17 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
     }
   };
   load_raw(0);
+  // x faces of this tile inside the grid lie in columns [cx0, cx0 + nxv)
+  // (jj = Y0 + col - 3 in [0, ny)).  A tile with at most half its columns
+  // there (the ghost-frame tile columns, small grids) is thin: its face
+  // work is packed into the first waves.  Its cells (columns [l0, l1) of
+  // the padded grid) read the y faces kk = ky0 .. ky1 (a cell at column ly
+  // reads ly .. ly + 4).
+  const int cx0 = Y0 < 3 ? 3 - Y0 : 0;
+  const int nxv = max(0, min(TY, ny + 3 - Y0) - cx0);
+  const bool thin = 2 * nxv <= TY;
+  const int ky0 = max(0, -Y0), ky1 = min(TY + 3, min(TY, PY - Y0) + 3);
+  const int nky = max(0, ky1 - ky0 + 1);
 
   for (int cv = 0; cv < NU; ++cv) {
     // ---- H+- of the region and the face cotangents ----
@@ -417,12 +448,21 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
       }
     };
     if constexpr (TXA == 8) {
-      const int side = t >> 8, col = t & 63, kk0 = 3 * ((t >> 6) & 3);
-      double h[7];
+      const auto xrun = [&](int side, int col, int kk0) {
+        double h[7];
 #pragma unroll
-      for (int j = 0; j < 7; ++j) h[j] = side == 0 ? L.Hxp[kk0 + j][col] : L.Hxm[kk0 + j][col];
+        for (int j = 0; j < 7; ++j) h[j] = side == 0 ? L.Hxp[kk0 + j][col] : L.Hxm[kk0 + j][col];
 #pragma unroll
-      for (int i = 0; i < 3; ++i) xface(side, kk0 + i, col, h + i);
+        for (int i = 0; i < 3; ++i) xface(side, kk0 + i, col, h + i);
+      };
+      if (!thin) {
+        xrun(t >> 8, t & 63, 3 * ((t >> 6) & 3));
+      } else if (t < 8 * nxv) {
+        // a thin tile packs the runs of its columns with x faces into the
+        // first waves instead of giving every wave a few working lanes
+        const int q = t / nxv;
+        xrun(q >> 2, cx0 + (t - q * nxv), 3 * (q & 3));
+      }
     } else {
       for (int e = t; e < 2 * (TXA + 4) * TY; e += G::NT) {
         const int side = e / ((TXA + 4) * TY), rem = e - side * ((TXA + 4) * TY);
@@ -434,8 +474,9 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
       }
     }
     // ---- the own cell's sums over the faces that reach it ----
-    // (faces outside the grid hold zeros, so every read is unconditional
-    // and the ten reads of a direction are in flight together)
+    // (every face a cell reads was written this variable, faces outside the
+    // grid as zeros, so the reads are unconditional and the ten reads of a
+    // direction are in flight together)
     double hpx = 0.0, hmx = 0.0, hpy = 0.0, hmy = 0.0;
     const auto gather_x = [&]() {
       if (work && xrow) {
@@ -472,15 +513,26 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
       }
     };
     if constexpr (TXA == 8) {
-      {
-        const int side = t >> 8, row = (t >> 5) & 7, kk0 = 2 * (t & 31);
+      const auto ypair = [&](int side, int row, int kk0) {
         double h[6];
 #pragma unroll
         for (int j = 0; j < 6; ++j) h[j] = side == 0 ? L.Hyp[row][kk0 + j] : L.Hym[row][kk0 + j];
         yface(side, row, kk0, h);
         yface(side, row, kk0 + 1, h + 1);
+      };
+      if (!thin) {
+        ypair(t >> 8, (t >> 5) & 7, 2 * (t & 31));
+      } else {         // the faces a thin tile's cells read, packed
+        for (int e = t; e < 2 * TXA * nky; e += G::NT) {
+          const int q = e / nky, side = q / TXA, row = q - side * TXA;
+          const int kk = ky0 + (e - q * nky);
+          double h[5];
+#pragma unroll
+          for (int j = 0; j < 5; ++j) h[j] = side == 0 ? L.Hyp[row][kk + j] : L.Hym[row][kk + j];
+          yface(side, row, kk, h);
+        }
       }
-      if (t < 64) {
+      if (!thin && t < 64) {
         const int side = t >> 5, row = (t >> 2) & 7, kk = TY + (t & 3);
         double h[5];
 #pragma unroll
@@ -700,10 +752,10 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) void alpha_apply_kernel(
   if (!(cnt[0] > 0 && pb[2] == gx) && !(cnt[1] > 0 && pb[3] == gy)) return;
   const MechConst& c = *a.mc;
   const int PX = a.nx + 6, PY = a.ny + 6, np = PX * PY;
-  const int tiles_y = (PY + TY - 1) / TY;
-  const int P = (blockIdx.x / tiles_y) * TXA + (threadIdx.x >> 6);
-  const int Q = (blockIdx.x % tiles_y) * TY + (threadIdx.x & 63);
-  if (P >= PX || Q >= PY) return;
+  int X0, Y0;
+  adj_tile_origin<TXA>(blockIdx.x, a.ny, X0, Y0);
+  const int P = X0 + (threadIdx.x >> 6), Q = Y0 + (threadIdx.x & 63);
+  if (P < 0 || P >= PX || Q < 0 || Q >= PY) return;
   const int t = P * PY + Q;
   int sx, sy, flip;
   if (!pad_src(a, P, Q, sx, sy, flip)) return;

@@ -433,9 +433,7 @@ int blocks_for(size_t m) { return (int)((m + 255) / 256); }
 // buffer (65 KB, two tiles per CU; more halo rows and barriers per cell).
 constexpr int kAdjRows = 8;
 constexpr bool kAdjSeq = false;
-int adj_tiles(const jh_ctx* c) {
-  return ((c->nx + 6 + kAdjRows - 1) / kAdjRows) * ((c->ny + 6 + TY - 1) / TY);
-}
+int adj_tiles(const jh_ctx* c) { return adj_tiles_x<kAdjRows>(c->nx) * adj_tiles_y(c->ny); }
 
 // JH_BC_USER sides must have been written (jh_set_ghost_strip or through
 // jh_ghost_strip's pointer) since their last use; consumes the marks.
