@@ -57,6 +57,10 @@ def _setup(mech_path, mech, name):
         aux = Oracle(mech_path, g["boundary_config"], g["dx"], g["dy"]).update_func(U, aux)
     elif name == "tube":
         U, aux, g = tie_free_state(mech_path, mech, "tube", 26, 13)
+    elif name == "wide":
+        # wide enough for whole 8 x 64 flux-VJP tiles (interior-aligned) as
+        # well as the thin tiles of the ghost frame and the ragged last band
+        U, aux, g = tie_free_state(mech_path, mech, "tube", 21, 70)
     elif name == "walls":
         U, aux, g = tie_free_state(mech_path, mech, "tube", 21, 17)
         g["boundary_config"] = {"left_boundary": "no_slip_wall",
@@ -79,7 +83,7 @@ def _cotangents(U, aux, seed=0):
     return rng.standard_normal(U.shape) / sU, rng.standard_normal(aux.shape) / sa
 
 
-@pytest.mark.parametrize("name", ["box", "box_ties", "tube", "walls"])
+@pytest.mark.parametrize("name", ["box", "box_ties", "tube", "walls", "wide"])
 def test_step_vjp_matches_twin(mech_path, mech, name):
     sim, g, U, aux = _setup(mech_path, mech, name)
     W, Wa = _cotangents(U, aux)
