@@ -597,6 +597,7 @@ def _adjoint_leg(mech, steps, every, ndir, torch):
     # LF maxima; clip hash, clip count; per temperature solve (hash, count)
     # of the cells below the NASA-7 switch
     LF, CLIP, RANGE = slice(0, 12), slice(12, 14), slice(14, 22)
+    trunc = []
     for d in range(ndir):
         dT = Tt * torch.tensor(rng.standard_normal(T.shape), device=dev)
         dY = Yt * torch.tensor(rng.standard_normal(Y.shape), device=dev)
@@ -623,6 +624,21 @@ def _adjoint_leg(mech, steps, every, ndir, torch):
             sw[name + "_4pt"] = bool(any((recs[k][:, cols] != rec0[:, cols]).any()
                                          for k in (-2, -1, 1, 2)))
         switches.append(sw)
+        # A direction with no switch within +-eps whose two-point error
+        # exceeds 1e-6: the quotient again at eps/2 (branch-frozen).  A
+        # truncation error (J's third derivative along the direction) falls
+        # 4x, and the Richardson combination (4 fd(eps/2) - fd(eps)) / 3
+        # removes it; a kink between the traced points would not behave so.
+        if not (sw["lf_2pt"] or sw["range_2pt"]) and errs["frozen_1e-06"][-1] > 1e-6:
+            Jh = {k: J_fwd(Tt + k * 5e-7 * dT, Yt + k * 5e-7 * dY, clip=("replay", masks))
+                  for k in (-1, 1)}
+            fd_h = (Jh[1] - Jh[-1]) / 1e-6
+            fd_r = (4.0 * fd_h - fds["frozen_1e-06"]) / 3.0
+            rel = lambda fd: abs(fd - ad) / max(abs(fd), abs(ad), 1e-300)  # noqa: E731
+            trunc.append({"direction": d, "rel_err_eps": errs["frozen_1e-06"][-1],
+                          "rel_err_eps_half": rel(fd_h),
+                          "ratio": errs["frozen_1e-06"][-1] / max(rel(fd_h), 1e-300),
+                          "rel_err_richardson": rel(fd_r)})
     free2 = [i for i, sw in enumerate(switches) if not (sw["lf_2pt"] or sw["range_2pt"])]
     free4 = [i for i, sw in enumerate(switches) if not (sw["lf_4pt"] or sw["range_4pt"])]
 
@@ -637,6 +653,17 @@ def _adjoint_leg(mech, steps, every, ndir, torch):
                "4pt_frozen_eps1e-6": _stats("frozen_1e-06_4pt", free4),
                "note": "directions whose branch-frozen (clip) runs change no LF-maximum "
                        "cell and move no cell across the NASA-7 switch temperature"}
+    fd_free["2pt_truncation_check"] = trunc
+    # every direction without a switch within +-eps, each at its two-point
+    # quotient, or its Richardson value where the former exceeds 1e-6
+    corr = dict((i, errs["frozen_1e-06"][i]) for i in free2)
+    for r in trunc:
+        corr[r["direction"]] = r["rel_err_richardson"]
+    if corr:
+        fd_free["2pt_truncation_corrected"] = {
+            "directions": len(corr), "max_rel_err": max(corr.values()),
+            "median_rel_err": float(np.median(list(corr.values()))),
+            "within_1e-6": bool(max(corr.values()) <= 1e-6)}
     per_dir = [dict(sw, rel_err_1e6=errs["1e-06"][i], rel_err_1e7=errs["1e-07"][i],
                     rel_err_4pt=errs["1e-06_4pt"][i], frozen_1e6=errs["frozen_1e-06"][i],
                     frozen_4pt=errs["frozen_1e-06_4pt"][i])
@@ -751,6 +778,10 @@ def _summary_adjoint(adj):
     sw = adj.get("fd_switches") or {}
     if "lf_4pt" in sw:
         out["fd_directions_with_lf_switch"] = sw["lf_4pt"]
+    corr = free.get("2pt_truncation_corrected")
+    if corr:
+        out["fd_switch_free_2pt_truncation_corrected"] = {
+            k: corr[k] for k in ("directions", "max_rel_err", "within_1e-6")}
     return out
 
 
