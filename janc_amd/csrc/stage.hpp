@@ -659,6 +659,12 @@ __device__ __forceinline__ void bc_body(const StepArgs<Real>& a, const Real* __r
         }
         ghost_maxima<Real>(c, G, mx, my);
       } else {
+        // every layer's values are loaded before any is stored, so the 3 x 14
+        // loads are in flight together (interleaved with the strip stores,
+        // which the compiler cannot prove disjoint from the state, each load
+        // would wait for the one before)
+        Real val[3][NV];
+#pragma unroll
         for (int gi = 0; gi < 3; ++gi) {
           // source cell along the normal for ghost layer gi (gi = 0 is the
           // outermost on the low side, the innermost on the high side)
@@ -670,13 +676,18 @@ __device__ __forceinline__ void bc_body(const StepArgs<Real>& a, const Real* __r
           const int ci = cell(src);
 #pragma unroll
           for (int v = 0; v < NU; ++v) {
-            Real val = U[v * n + ci];
-            if (kind == JH_BC_SLIP_WALL && v == (xside ? 1 : 2)) val = -val;
-            if (kind == JH_BC_NO_SLIP_WALL && (v == 1 || v == 2)) val = -val;
-            gs[v * gstride + gi * len + pos] = val;
+            Real x = U[v * n + ci];
+            if (kind == JH_BC_SLIP_WALL && v == (xside ? 1 : 2)) x = -x;
+            if (kind == JH_BC_NO_SLIP_WALL && (v == 1 || v == 2)) x = -x;
+            val[gi][v] = x;
           }
-          gs[NU * gstride + gi * len + pos] = aux[ci];
-          gs[(NU + 1) * gstride + gi * len + pos] = aux[n + ci];
+          val[gi][NU] = aux[ci];
+          val[gi][NU + 1] = aux[n + ci];
+        }
+#pragma unroll
+        for (int gi = 0; gi < 3; ++gi) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) gs[v * gstride + gi * len + pos] = val[gi][v];
         }
       }
     }
