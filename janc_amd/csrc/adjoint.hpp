@@ -388,9 +388,9 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
       if (xr[s] >= 0) {
         double hp = 0.0, hm = 0.0;
         if (in[s]) {
-          const double F = phys_flux_l(cv, 0, q[s], ucv);
-          hp = 0.5 * (F + ax * ucv);
-          hm = 0.5 * (F - ax * ucv);
+          const double hF = 0.5 * phys_flux_l(cv, 0, q[s], ucv);   // as the forward
+          hp = fma(0.5 * ax, ucv, hF);
+          hm = fma(-0.5 * ax, ucv, hF);
         }
         L.Hxp[xr[s]][ly] = hp;
         L.Hxm[xr[s]][ly] = hm;
@@ -399,9 +399,9 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
         const int row = s == 0 ? w : (t >> 3);
         double hp = 0.0, hm = 0.0;
         if (in[s]) {
-          const double Gf = phys_flux_l(cv, 1, q[s], ucv);
-          hp = 0.5 * (Gf + ay * ucv);
-          hm = 0.5 * (Gf - ay * ucv);
+          const double hG = 0.5 * phys_flux_l(cv, 1, q[s], ucv);
+          hp = fma(0.5 * ay, ucv, hG);
+          hm = fma(-0.5 * ay, ucv, hG);
         }
         L.Hyp[row][yc[s]] = hp;
         L.Hym[row][yc[s]] = hm;

@@ -932,6 +932,9 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
   const double idx = 1.0 / a.dx, idy = 1.0 / a.dy;
   const W ax = (W)(Real)bitsd(a.sc->amax[a.parity][stage][0]);
   const W ay = (W)(Real)bitsd(a.sc->amax[a.parity][stage][1]);
+  // H+- = (F +- alpha U) / 2 as fma(+-alpha/2, U, F/2): halving is exact, so
+  // this is the same value with one operation fewer per pair
+  const W hax = (W)0.5 * ax, hay = (W)0.5 * ay;
 
   int rrow[NSL], rcol[NSL];       // LDS coordinates: x-region row / y-region col
   int gxs[NSL], gys[NSL];
@@ -1000,9 +1003,9 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
           if (SPLIT == SPLIT_SW) {
             sw_split<W>(cv, 0, q[s], ucv, hp, hm);
           } else {
-            const W F = phys_flux<W>(cv, 0, q[s], ucv);
-            hp = (W)0.5 * (F + ax * ucv);
-            hm = (W)0.5 * (F - ax * ucv);
+            const W hF = (W)0.5 * phys_flux<W>(cv, 0, q[s], ucv);
+            hp = fma(hax, ucv, hF);
+            hm = fma(-hax, ucv, hF);
           }
         }
         sHxp[rrow[s]][ly] = hp;
@@ -1014,9 +1017,9 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
           if (SPLIT == SPLIT_SW) {
             sw_split<W>(cv, 1, q[s], ucv, hp, hm);
           } else {
-            const W Gf = phys_flux<W>(cv, 1, q[s], ucv);
-            hp = (W)0.5 * (Gf + ay * ucv);
-            hm = (W)0.5 * (Gf - ay * ucv);
+            const W hG = (W)0.5 * phys_flux<W>(cv, 1, q[s], ucv);
+            hp = fma(hay, ucv, hG);
+            hm = fma(-hay, ucv, hG);
           }
         }
         const int yr = s < 2 ? rrow[s] - 3 : rrow[s];
