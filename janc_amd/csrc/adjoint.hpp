@@ -7,16 +7,20 @@
 // routed to its active branch.
 //
 // Data path of the flux VJP (one RK stage), over the reference's ghost-padded
-// field without storing it: a padded cell reads its source interior cell
-// through the boundary map (pad_src, prim_src).  Per padded cell, H+-_bar is
-// gathered from the WENO gradients of the faces that reach it and pushed
-// through the split flux and the primitive map (flux_vjp_tile_kernel); an
-// interior cell's cotangent is added into the output, a ghost cell's is kept
-// in a padded buffer; the LF maxima's cotangent goes to the argmax cells
-// (alpha_*); the ghost-cell cotangents are then folded back onto their source
-// cells (fold_ghosts_kernel).
+// field without storing the padded state: a padded cell reads its source
+// interior cell through the boundary map (pad_src, prim_src).  Per padded
+// cell, H+-_bar is summed from the WENO gradients of the faces that reach it
+// (the x faces' sums in flux_vjp_x_kernel, kept per padded cell; the y
+// faces' in flux_vjp_y_kernel) and pushed through the split flux and the
+// primitive map (flux_vjp_y_kernel); an interior cell's cotangent is added
+// into the output, a ghost cell's is kept in a padded buffer; the LF
+// maxima's cotangent goes to the argmax cells (alpha_*); the ghost-cell
+// cotangents are then folded back onto their source cells
+// (fold_ghosts_kernel).
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "stage.hpp"
 
@@ -163,63 +167,51 @@ __device__ __forceinline__ void prim_to_U_vjp(const MechConst& c, const PrimD& q
 }
 
 // ------------------------------------------------------------------
-// Flux VJP of one RK stage, tiled: no face quantities are staged in HBM.
-// Tile = TXA padded rows x 64 padded columns, one cell per thread.  Padded
-// cell P receives, per direction and variable, H+_bar = sum_m F_bar(f = P - m)
-// dq_L/dv_m(face f) and H-_bar = sum_m F_bar(f = P - 5 + m) dq_R/dv_m(face f).
-// Per variable: H+- of the tile and its 4-cell halo and the face cotangents
-// F_bar go to LDS; then, one direction at a time, every face that reaches
-// the tile evaluates its WENO gradient once and stores F_bar * dq/dv_m in LDS,
-// and each cell gathers its five left- and five right-biased terms (in the
-// order m = 0..4); then the split-flux and primitive VJPs of the reference's
-// forward map.
-// Tiles are aligned to the interior: origins (X0, Y0) = (3 + TXA a, 3 + 64 b)
-// in padded coordinates, a >= -1, b >= -1, so an nx x ny interior whose
-// sides are multiples of (TXA, 64) splits into whole tiles and the 3-cell
-// ghost frame falls into thin tiles around them, whose face work is packed
-// into their first waves (a padded grid tiled from its corner puts a
-// partial tile into every band: at 1024 x 256, 516 full and 129 thin tiles
-// instead of 512 and 268).  Measured at 1024 x 256 the two tilings take the
-// same time (the kernel is bound by latency at two waves per SIMD, not by
-// its tile count).
-// Tile counts (bands a = -1 .. , b = -1 ..) and the origin of tile blk.
-template <int TXA>
-__host__ __device__ inline int adj_tiles_x(int nx) { return (nx + 3 + TXA - 1) / TXA + 1; }
-__host__ __device__ inline int adj_tiles_y(int ny) { return (ny + 3 + TY - 1) / TY + 1; }
-template <int TXA>
-__device__ inline void adj_tile_origin(int blk, int ny, int& X0, int& Y0) {
-  const int ty = adj_tiles_y(ny), a = blk / ty;
-  X0 = 3 + TXA * (a - 1);
-  Y0 = 3 + TY * (blk - a * ty - 1);
+// Flux VJP of one RK stage, one direction per pass.  Padded cell P receives,
+// per direction and variable, H+_bar = sum_m F_bar(f = P - m) dq_L/dv_m(f)
+// and H-_bar = sum_m F_bar(f = P - 5 + m) dq_R/dv_m(f), with the face
+// cotangent F_bar(f) = dt/dx (rbar_f - rbar_{f-1}) (rbar zero outside the
+// grid) and dq/dv_m the WENO gradient of face f with respect to its m-th
+// stencil value.  Everything after H+-_bar (the split flux's and the
+// primitive map's VJPs) is linear in it, so the x and y contributions add:
+//   adj_prim_kernel    (rho, u, v, p) of every interior cell, for the x pass;
+//   flux_vjp_x_kernel  H+-_bar of the x faces of every padded cell of every
+//                      interior column, swept along x in registers;
+//   flux_vjp_y_kernel  the y faces' H+-_bar, exchanged between the lanes of
+//                      a wave, then each padded cell's sums pushed through
+//                      the split flux and the primitive map, and the block
+//                      partials of the LF maxima's cotangent.
+// No pass uses a block barrier inside its variable loop and none keeps face
+// quantities in LDS.
+
+// x pass: one thread per (interior column, variable, side, segment of
+// kXSeg padded rows), the 64 lanes of a wave on consecutive columns.
+constexpr int kXSeg = 32;
+__host__ __device__ inline int xs_segments(int nx) { return (nx + 6 + kXSeg - 1) / kXSeg; }
+__host__ __device__ inline int xs_blocks(int nx, int ny) {   // of 4 waves
+  return 2 * NU * ((((ny + 63) / 64) * xs_segments(nx) + 3) / 4);
 }
 
-template <int TXA>
-struct AdjTile {
-  static constexpr int NT = 64 * TXA;          // threads (one padded cell each)
-  static constexpr int NW = TXA;               // waves
-  static constexpr int RX = TXA + 8;           // x-region rows (4-cell halo)
-  static constexpr int RY = TY + 8;            // y-region columns
-  static constexpr int NFX = TXA + 5;          // x-face rows reaching the tile
-  static constexpr int NFY = TY + 5;           // y-face columns reaching the tile
-  static constexpr int NHS = 8 / NW;           // x-halo rows per wave
-};
-
-// SEQ: the x faces' products are gathered before the y faces' are formed,
-// through one buffer (two more barriers per variable, ~2/3 of the LDS).
-template <int TXA, bool SEQ>
-struct AdjLds {
-  static constexpr int NCX = 2 * (TXA + 4) * 5 * TY, NCY = 2 * TXA * 5 * (TY + 4);
-  double Hxp[TXA + 8][TY], Hxm[TXA + 8][TY];
-  double Hyp[TXA][TY + 8], Hym[TXA][TY + 8];
-  double Fbx[TXA + 5][TY];
-  double Fby[TXA][TY + 5];
-  // F_bar * dq/dv_m of the faces reaching the tile: x faces
-  // [side][k][m][TY] (k < TXA+4), y faces [side][row][m][TY+4]
-  double C[SEQ ? (NCX > NCY ? NCX : NCY) : NCX + NCY];
-  double red[4][TXA];
-  unsigned int ties[2][TXA];
-};
-
+// y pass: a wave covers 64 consecutive padded columns of one padded row and
+// owns the 56 cells on lanes 4..59 (their stencils and faces lie within the
+// wave); kYRows waves (rows) per block.
+constexpr int kYRows = 4;
+constexpr int kYCells = 56;
+__host__ __device__ inline int ys_segments(int ny) { return (ny + 6 + kYCells - 1) / kYCells; }
+__host__ __device__ inline int ys_blocks(int nx, int ny) {
+  return ((nx + 6 + kYRows - 1) / kYRows) * ys_segments(ny);
+}
+// Padded cell of thread t of y-pass block blk, and whether the lane owns it.
+__host__ __device__ inline void ys_cell(int blk, int ny, int t, int& P, int& Q, bool& mine) {
+  const int ns = ys_segments(ny), rg = blk / ns, sg = blk - rg * ns, l = t & 63;
+#if defined(__HIP_DEVICE_COMPILE__)
+  P = rg * kYRows + __builtin_amdgcn_readfirstlane(t >> 6);   // wave-uniform row
+#else
+  P = rg * kYRows + (t >> 6);
+#endif
+  Q = sg * kYCells - 4 + l;
+  mine = l >= 4 && l < 4 + kYCells;
+}
 // Exact tie test of the LF maxima: JAX's max VJP splits the cotangent
 // among the elements equal to the maximum.
 __device__ __forceinline__ bool is_max(double val, double alpha) { return val == alpha; }
@@ -239,73 +231,219 @@ __device__ __forceinline__ double phys_flux_l(int cv, int dir, const PrimL& q, d
   }
 }
 
-template <int TXA, bool SEQ>
-__global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_eu(2))) void flux_vjp_tile_kernel(
+__global__ __launch_bounds__(256) void adj_prim_kernel(AdjArgs a, const double* __restrict__ U,
+                                                       const double* __restrict__ aux,
+                                                       double* __restrict__ qb) {
+  const int n = a.nx * a.ny;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const PrimD q = prim_src(*a.mc, U, aux, n, t, 0);
+  qb[t] = q.rho;
+  qb[n + t] = q.u;
+  qb[2 * n + t] = q.v;
+  qb[3 * n + t] = q.p;
+}
+
+// x faces f = 0..nx of column j; the stencil of face f is padded rows
+// b .. b+4 with b = f + side, and face f's m-th gradient entry belongs to row
+// b + m (side 0) or b + 4 - m (side 1).  A thread sweeps the faces reaching
+// its rows [c0, c0 + kXSeg), holding the five stencil values and the five
+// open row sums: row b is complete once face f = b - side is done.
+// hx[side][cv][P][j]: H+_bar (side 0) and H-_bar (side 1) of padded row P.
+// The sweep of one (variable, side) wave: CLS = min(cv, 4) selects the
+// physical flux and the momentum flips at compile time.
+template <int CLS, int SIDE>
+__device__ __forceinline__ void xs_sweep(const AdjArgs& a, const double* __restrict__ U,
+                                         const double* __restrict__ qb,
+                                         const double* __restrict__ rbar, double ax,
+                                         double* __restrict__ hx, int cv, int j, int seg) {
+  const int nx = a.nx, ny = a.ny, n = nx * ny, PX = nx + 6;
+  constexpr int side = SIDE;
+  const double hal = (side == 0 ? 0.5 : -0.5) * ax;
+  const double dtdx = a.dt / a.dx;
+  const int c0 = seg * kXSeg, c1 = min(c0 + kXSeg, PX);
+  const int fA = max(0, c0 - 4 - side), fB = min(nx, c0 + kXSeg - 1 - side);
+  const double* Uv = U + (size_t)cv * n;
+  const double* rv = rbar + (size_t)cv * n + j;
+  double* out = hx + (size_t)(side * NU + cv) * PX * ny + j;
+  // H+ (side 0) or H- (side 1) of padded row P of this column, read through
+  // the boundary map as the forward pads it (LF.py:10-12); the loads of a
+  // row are issued two faces before its value is needed
+  struct Raw {
+    double rho, u, v, p, uc;
+    int fl;   // momentum flips, -1: no source (H = 0)
+  };
+  const auto load = [&](int P, Raw& r) {
+    const int i = P - 3;
+    int s = i * ny + j, fl = 0;
+    if (i < 0 || i >= nx) {
+      int sx, sy;
+      if (pad_src(a, P, j + 3, sx, sy, fl)) {
+        s = sx * ny + sy;
+      } else {
+        s = 0;
+        fl = -1;
+      }
+    }
+    r.rho = qb[s];
+    r.u = qb[n + s];
+    r.v = qb[2 * n + s];
+    r.p = qb[3 * n + s];
+    r.uc = Uv[s];
+    r.fl = fl;
+  };
+  const auto hval = [&](const Raw& r) -> double {
+    if (r.fl < 0) return 0.0;
+    const PrimL q{r.rho, (r.fl & 1) ? -r.u : r.u, (r.fl & 2) ? -r.v : r.v, r.p};
+    const double ucv = flip_var(CLS, r.fl, r.uc);
+    return fma(hal, ucv, 0.5 * phys_flux_l(CLS, 0, q, ucv));
+  };
+  const auto rload = [&](int f) -> double {   // rbar of interior row f, zero outside
+    const double v = rv[(size_t)(f >= 0 && f < nx ? f : 0) * ny];
+    return f >= 0 && f < nx ? v : 0.0;
+  };
+  // rows before the first stencil get no face of this side
+  for (int P = c0; P < min(c1, fA + side); ++P) out[(size_t)P * ny] = 0.0;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int bB = fA + side - 1;   // last row whose stencil was swept
+  if (fA <= fB) {
+    const int b0 = fA + side, Pmax = PX - 1;
+    Raw r0, r1, r2;
+    load(b0, r0);
+    load(b0 + 1, r1);
+    load(b0 + 2, r2);
+    double h0 = hval(r0), h1 = hval(r1), h2 = hval(r2);
+    load(b0 + 3, r0);
+    load(b0 + 4, r1);
+    load(min(b0 + 5, Pmax), r2);
+    double h3 = hval(r0), h4 = hval(r1);
+    load(min(b0 + 6, Pmax), r0);
+    // r2, r0: raw rows b+5, b+6 of the current face's base b
+    double rprev = rload(fA - 1), rc0 = rload(fA), rc1 = rload(fA + 1);
+    // two faces per trip: the second face's loads and gradient overlap the
+    // first's
+#pragma unroll 2
+    for (int f = fA; f <= fB; ++f) {
+      const int b = f + side;
+      const double fb = dtdx * (rc0 - rprev);
+      rprev = rc0;
+      rc0 = rc1;
+      rc1 = rload(f + 2);
+      load(min(b + 7, Pmax), r1);
+      double g[5];
+      if constexpr (side == 0) {
+        weno5_grad(h0, h1, h2, h3, h4, g);
+      } else {
+        double gr[5];
+        weno5_grad(h4, h3, h2, h1, h0, gr);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) g[m] = gr[4 - m];
+      }
+      const double d0 = s0 + fb * g[0];
+      if (b >= c0 && b < c1) out[(size_t)b * ny] = d0;
+      s0 = s1 + fb * g[1];
+      s1 = s2 + fb * g[2];
+      s2 = s3 + fb * g[3];
+      s3 = fb * g[4];
+      h0 = h1;
+      h1 = h2;
+      h2 = h3;
+      h3 = h4;
+      h4 = hval(r2);   // row b + 5 (beyond the padded grid only after the last face)
+      r2 = r0;
+      r0 = r1;
+    }
+    bB = fB + side;
+  }
+  // rows after the last stencil: complete when no later face exists (fB =
+  // nx), else outside [c0, c1)
+  const double tail[4] = {s0, s1, s2, s3};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int P = bB + 1 + k;
+    if (P >= c0 && P < c1) out[(size_t)P * ny] = tail[k];
+  }
+  for (int P = max(c0, bB + 5); P < c1; ++P) out[(size_t)P * ny] = 0.0;
+}
+
+
+__global__ __launch_bounds__(256) void flux_vjp_x_kernel(AdjArgs a, const double* __restrict__ U,
+                                                         const double* __restrict__ qb,
+                                                         const double* __restrict__ rbar,
+                                                         const unsigned long long* alpha_bits,
+                                                         double* __restrict__ hx) {
+  const int nx = a.nx, ny = a.ny;
+  const int njb = (ny + 63) >> 6, nt = njb * xs_segments(nx), nbt = (nt + 3) / 4;
+  // a block's four waves share one (variable, side), so they run one
+  // instantiation of the sweep
+  // the wave index through readfirstlane: every quantity derived from it
+  // (segment, rows, faces, loop bounds) is then wave-uniform to the compiler
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int sc = blockIdx.x / nbt, tile = (blockIdx.x - sc * nbt) * 4 + wv;
+  if (tile >= nt) return;
+  const int side = sc & 1, cv = sc >> 1;
+  const int jb = tile % njb, seg = tile / njb;
+  const int j = (jb << 6) + (threadIdx.x & 63);
+  if (j >= ny) return;
+  const double ax = bitsd(alpha_bits[0]);
+  switch (2 * min(cv, 4) + side) {
+    case 0: xs_sweep<0, 0>(a, U, qb, rbar, ax, hx, cv, j, seg); break;
+    case 1: xs_sweep<0, 1>(a, U, qb, rbar, ax, hx, cv, j, seg); break;
+    case 2: xs_sweep<1, 0>(a, U, qb, rbar, ax, hx, cv, j, seg); break;
+    case 3: xs_sweep<1, 1>(a, U, qb, rbar, ax, hx, cv, j, seg); break;
+    case 4: xs_sweep<2, 0>(a, U, qb, rbar, ax, hx, cv, j, seg); break;
+    case 5: xs_sweep<2, 1>(a, U, qb, rbar, ax, hx, cv, j, seg); break;
+    case 6: xs_sweep<3, 0>(a, U, qb, rbar, ax, hx, cv, j, seg); break;
+    case 7: xs_sweep<3, 1>(a, U, qb, rbar, ax, hx, cv, j, seg); break;
+    case 8: xs_sweep<4, 0>(a, U, qb, rbar, ax, hx, cv, j, seg); break;
+    default: xs_sweep<4, 1>(a, U, qb, rbar, ax, hx, cv, j, seg); break;
+  }
+}
+
+// y pass and the per-cell reverse of the split flux and the primitive map.
+// Lane l at padded column Q evaluates the left-biased y face g = Q and the
+// right-biased face g = Q - 1 of its row (both stencils: columns Q .. Q+4,
+// taken from lanes l .. l+4); the cell on lane l sums the left-biased
+// products of lanes l-4 .. l and the right-biased ones of lanes l-4 .. l.
+__global__ __launch_bounds__(64 * kYRows) void flux_vjp_y_kernel(
     AdjArgs a, const double* __restrict__ U, const double* __restrict__ aux,
     const double* __restrict__ rbar, const unsigned long long* alpha_bits,
-    double* __restrict__ Ut, double* __restrict__ At, double* __restrict__ Ubp,
-    double* __restrict__ abp, double* __restrict__ spd, double* __restrict__ part) {
-  using G = AdjTile<TXA>;
-  __shared__ AdjLds<TXA, SEQ> L;
-  double* Cx = L.C;
-  double* Cy = SEQ ? L.C : L.C + AdjLds<TXA, SEQ>::NCX;
+    const double* __restrict__ hx, double* __restrict__ Ut, double* __restrict__ At,
+    double* __restrict__ Ubp, double* __restrict__ abp, double* __restrict__ spd,
+    double* __restrict__ part) {
+  __shared__ double red[4][kYRows];
+  __shared__ unsigned int ties[2][kYRows];
   const MechConst& c = *a.mc;
   const int nx = a.nx, ny = a.ny, n = nx * ny, PX = nx + 6, PY = ny + 6, np = PX * PY;
-  const int t = threadIdx.x, ly = t & 63, w = t >> 6;
-  int X0, Y0;
-  adj_tile_origin<TXA>(blockIdx.x, ny, X0, Y0);
+  const int t = threadIdx.x, w = t >> 6, ly = t & 63;
+  int P, Q;
+  bool mine;
+  ys_cell(blockIdx.x, ny, t, P, Q, mine);
   const double ax = bitsd(alpha_bits[0]), ay = bitsd(alpha_bits[1]);
-  const double dtdx = a.dt / a.dx, dtdy = a.dt / a.dy;
-
-  // region slots: 0 = own cell (x and y regions), 1..NHS = x-halo rows,
-  // NHS+1 = y-halo cell (t < 8 * TXA)
-  constexpr int NS_ = G::NHS + 2;
-  int px[NS_], py[NS_], xr[NS_], yc[NS_];
-  bool in[NS_];
-  px[0] = X0 + w;
-  py[0] = Y0 + ly;
-  xr[0] = w + 4;
-  yc[0] = ly + 4;
-#pragma unroll
-  for (int h = 0; h < G::NHS; ++h) {
-    const int hr = w + h * G::NW;                  // halo row 0..7
-    const int r = hr < 4 ? hr : TXA + hr;          // region rows 0..3, TXA+4..TXA+7
-    xr[1 + h] = r;
-    px[1 + h] = X0 - 4 + r;
-    py[1 + h] = Y0 + ly;
-    yc[1 + h] = -1;
-  }
-  {
-    constexpr int s = NS_ - 1;
-    const int row = t >> 3, k = t & 7;
-    yc[s] = t < 8 * TXA ? (k < 4 ? k : TY + k) : -1;
-    xr[s] = -1;
-    px[s] = X0 + row;
-    py[s] = Y0 - 4 + (k < 4 ? k : TY + k);
-  }
-  PrimL q[NS_];
-  int src[NS_], fl[NS_];   // source interior cell and momentum flips (pad_src)
-#pragma unroll
-  for (int s = 0; s < NS_; ++s) {
-    in[s] = px[s] >= 0 && px[s] < PX && py[s] >= 0 && py[s] < PY &&
-            (s < NS_ - 1 || t < 8 * TXA);
-    int sx = 0, sy = 0, f = 0;
-    // the adjoint admits linear boundary kinds only, so pad_src holds here
-    if (in[s]) in[s] = pad_src(a, px[s], py[s], sx, sy, f);
-    src[s] = sx * ny + sy;
-    fl[s] = f;
-    if (in[s]) {
-      const PrimD d = prim_src(c, U, aux, n, src[s], f);
-      q[s] = PrimL{d.rho, d.u, d.v, d.p};
-    } else {
-      q[s] = PrimL{1.0, 0.0, 0.0, 0.0};
-    }
-  }
-  // own cell: which face families reach it
-  const int P = px[0], Q = py[0], i = P - 3, j = Q - 3;
-  const bool own = in[0];
-  const bool xrow = own && j >= 0 && j < ny, ycol = own && i >= 0 && i < nx;
+  const double dtdy = a.dt / a.dy;
+  int sx = 0, sy = 0, fl = 0;
+  bool in = P < PX && Q >= 0 && Q < PY;
+  // the adjoint admits linear boundary kinds only, so pad_src holds here
+  if (in) in = pad_src(a, P, Q, sx, sy, fl);
+  const int src = sx * ny + sy;
+  const bool own = in && mine;
+  const int i = P - 3, j = Q - 3;
+  const bool rowin = i >= 0 && i < nx;          // wave-uniform: the row has y faces
+  const bool xrow = own && j >= 0 && j < ny, ycol = own && rowin;
   const bool work = own && (xrow || ycol);
+  PrimL q{1.0, 0.0, 0.0, 0.0};
+  if (in) {
+    const PrimD d = prim_src(c, U, aux, n, src, fl);
+    q = PrimL{d.rho, d.u, d.v, d.p};
+  }
+  // faces g = Q (left-biased) and g = Q - 1 (right-biased) exist for
+  // 0 <= g <= ny; rbar of row i at columns Q, Q-1, Q-2 (zero outside [0, ny))
+  const bool v0 = rowin && Q >= 0 && Q <= ny, v1 = rowin && Q >= 1 && Q <= ny + 1;
+  const bool okA = rowin && Q >= 0 && Q < ny, okB = rowin && Q >= 1 && Q <= ny,
+             okC = rowin && Q >= 2 && Q <= ny + 1;
+  const int rrow = (rowin ? i : 0) * ny;
+  const int oA = okA ? rrow + Q : 0, oB = okB ? rrow + Q - 1 : 0, oC = okC ? rrow + Q - 2 : 0;
+  const size_t ox0 = xrow ? (size_t)P * ny + j : 0;
   double Ub[NU];
 #pragma unroll
   for (int v = 0; v < NU; ++v) Ub[v] = 0.0;
@@ -315,297 +453,104 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
   for (int k = 0; k < JH_NS - 1; ++k) Yb[k] = 0.0;
   double sx_part = 0.0, sy_part = 0.0;
 
-  // This thread's face-cotangent entries: x faces e = t + r NT (r < NEX) of
-  // the (NFX, TY) table, y faces e = t + r NT (r < NEY) of the (TXA, NFY)
-  // table; F_bar = dt/dx (rbar_f - rbar_{f-1}) with rbar zero outside the
-  // grid.  The rbar and state loads of variable cv + 1 are issued while
-  // variable cv's gradients run; they are unconditional (clamped offsets,
-  // zeroed afterwards) so no load sits on a divergent branch.
-  constexpr int NEX = (G::NFX * TY + G::NT - 1) / G::NT;
-  constexpr int NEY = (TXA * G::NFY + G::NT - 1) / G::NT;
-  int xoff[NEX], yoff[NEY];
-  unsigned xfl[NEX], yfl[NEY];   // bit 0: entry exists, 1: face valid, 2: left, 3: right
-#pragma unroll
-  for (int r = 0; r < NEX; ++r) {
-    const int e = t + r * G::NT;
-    const int k = e >> 6, col = e & 63;
-    const int f = X0 - 5 + k, jj = Y0 + col - 3;
-    const bool ex = e < G::NFX * TY;
-    const bool ok = ex && f >= 0 && f <= nx && jj >= 0 && jj < ny;
-    const bool lo = ok && f - 1 >= 0, hi = ok && f < nx;
-    xfl[r] = (ex ? 1u : 0u) | (ok ? 2u : 0u) | (lo ? 4u : 0u) | (hi ? 8u : 0u);
-    xoff[r] = ok ? (f - 1) * ny + jj : 0;          // right value at + ny
-  }
-#pragma unroll
-  for (int r = 0; r < NEY; ++r) {
-    const int e = t + r * G::NT;
-    const int row = e / G::NFY, k = e - row * G::NFY;
-    const int gg = Y0 - 5 + k, ii = X0 + row - 3;
-    const bool ex = e < TXA * G::NFY;
-    const bool ok = ex && gg >= 0 && gg <= ny && ii >= 0 && ii < nx;
-    const bool lo = ok && gg - 1 >= 0, hi = ok && gg < ny;
-    yfl[r] = (ex ? 1u : 0u) | (ok ? 2u : 0u) | (lo ? 4u : 0u) | (hi ? 8u : 0u);
-    yoff[r] = ok ? ii * ny + gg - 1 : 0;           // right value at + 1
-  }
-  double uraw[NS_], rxl[NEX], rxr[NEX], ryl[NEY], ryr[NEY];
-  auto load_raw = [&](int v) {
-    const size_t o = (size_t)v * n;
-#pragma unroll
-    for (int s = 0; s < NS_; ++s) uraw[s] = U[o + (in[s] ? src[s] : 0)];
-#pragma unroll
-    for (int r = 0; r < NEX; ++r) {
-      const int b = xoff[r];
-      rxl[r] = rbar[o + (xfl[r] & 4u ? b : 0)];
-      rxr[r] = rbar[o + (xfl[r] & 8u ? b + ny : 0)];
-    }
-#pragma unroll
-    for (int r = 0; r < NEY; ++r) {
-      const int b = yoff[r];
-      ryl[r] = rbar[o + (yfl[r] & 4u ? b : 0)];
-      ryr[r] = rbar[o + (yfl[r] & 8u ? b + 1 : 0)];
-    }
+  // the loads of variable cv + 1 are issued before variable cv's gradients;
+  // they are unconditional (clamped offsets, masked afterwards)
+  double nu_, nxp, nxm, nrA, nrB, nrC;
+  const auto load = [&](int cv) {
+    const size_t o = (size_t)cv * n;
+    nu_ = U[o + (in ? src : 0)];
+    nxp = hx[(size_t)cv * PX * ny + ox0];
+    nxm = hx[(size_t)(NU + cv) * PX * ny + ox0];
+    nrA = rbar[o + oA];
+    nrB = rbar[o + oB];
+    nrC = rbar[o + oC];
   };
-  load_raw(0);
-  // x faces of this tile inside the grid lie in columns [cx0, cx0 + nxv)
-  // (jj = Y0 + col - 3 in [0, ny)).  A tile with at most half its columns
-  // there (the ghost-frame tile columns, small grids) is thin: its face
-  // work is packed into the first waves.  Its cells (columns [l0, l1) of
-  // the padded grid) read the y faces kk = ky0 .. ky1 (a cell at column ly
-  // reads ly .. ly + 4).
-  const int cx0 = Y0 < 3 ? 3 - Y0 : 0;
-  const int nxv = max(0, min(TY, ny + 3 - Y0) - cx0);
-  const bool thin = 2 * nxv <= TY;
-  const int ky0 = max(0, -Y0), ky1 = min(TY + 3, min(TY, PY - Y0) + 3);
-  const int nky = max(0, ky1 - ky0 + 1);
-
-  for (int cv = 0; cv < NU; ++cv) {
-    // ---- H+- of the region and the face cotangents ----
-    double ucv_own = 0.0;
-#pragma unroll
-    for (int s = 0; s < NS_; ++s) {
-      const double ucv = in[s] ? flip_var(cv, fl[s], uraw[s]) : 0.0;
-      if (s == 0) ucv_own = ucv;
-      if (xr[s] >= 0) {
-        double hp = 0.0, hm = 0.0;
-        if (in[s]) {
-          const double hF = 0.5 * phys_flux_l(cv, 0, q[s], ucv);   // as the forward
-          hp = fma(0.5 * ax, ucv, hF);
-          hm = fma(-0.5 * ax, ucv, hF);
-        }
-        L.Hxp[xr[s]][ly] = hp;
-        L.Hxm[xr[s]][ly] = hm;
+  // one variable; CLS = min(cv, 4) selects the flux, the flips and the
+  // split flux's reverse at compile time, and cv is a constant after
+  // unrolling, so Ub[cv] stays in registers
+  const auto step = [&](auto K, int cv) {
+    constexpr int CLS = decltype(K)::value;
+    const double ucv = in ? flip_var(CLS, fl, nu_) : 0.0;
+    const double xp = nxp, xm = nxm, rA = nrA, rB = nrB, rC = nrC;
+    if (cv + 1 < NU) load(cv + 1);
+    double hpy = 0.0, hmy = 0.0;
+    if (rowin) {   // wave-uniform
+      double hp = 0.0, hm = 0.0;
+      if (in) {
+        const double hG = 0.5 * phys_flux_l(CLS, 1, q, ucv);   // as the forward
+        hp = fma(0.5 * ay, ucv, hG);
+        hm = fma(-0.5 * ay, ucv, hG);
       }
-      if (yc[s] >= 0) {
-        const int row = s == 0 ? w : (t >> 3);
-        double hp = 0.0, hm = 0.0;
-        if (in[s]) {
-          const double hG = 0.5 * phys_flux_l(cv, 1, q[s], ucv);
-          hp = fma(0.5 * ay, ucv, hG);
-          hm = fma(-0.5 * ay, ucv, hG);
-        }
-        L.Hyp[row][yc[s]] = hp;
-        L.Hym[row][yc[s]] = hm;
+      const double fb0 = v0 ? dtdy * ((okA ? rA : 0.0) - (okB ? rB : 0.0)) : 0.0;
+      const double fb1 = v1 ? dtdy * ((okB ? rB : 0.0) - (okC ? rC : 0.0)) : 0.0;
+      double ep[5], em[5];
+      ep[0] = hp;
+      em[0] = hm;
+#pragma unroll
+      for (int d = 1; d < 5; ++d) {
+        ep[d] = __shfl_down(hp, d, 64);
+        em[d] = __shfl_down(hm, d, 64);
       }
-    }
+      double g0[5], g1[5];
+      weno5_grad(ep[0], ep[1], ep[2], ep[3], ep[4], g0);
+      weno5_grad(em[4], em[3], em[2], em[1], em[0], g1);
+      // left-biased: entry m of lane l belongs to lane l + m; right-biased:
+      // entry m (argument em[4-m]) to lane l + 4 - m
+      hpy = fb0 * g0[0];
 #pragma unroll
-    for (int r = 0; r < NEX; ++r) {
-      if (!(xfl[r] & 1u)) continue;
-      const int e = t + r * G::NT;
-      const double rl = xfl[r] & 4u ? rxl[r] : 0.0, rr = xfl[r] & 8u ? rxr[r] : 0.0;
-      L.Fbx[e >> 6][e & 63] = xfl[r] & 2u ? dtdx * (rr - rl) : 0.0;
-    }
+      for (int m = 1; m < 5; ++m) hpy += __shfl_up(fb0 * g0[m], m, 64);
+      hmy = __shfl_up(fb1 * g1[0], 4, 64);
 #pragma unroll
-    for (int r = 0; r < NEY; ++r) {
-      if (!(yfl[r] & 1u)) continue;
-      const int e = t + r * G::NT;
-      const int row = e / G::NFY;
-      const double rl = yfl[r] & 4u ? ryl[r] : 0.0, rr = yfl[r] & 8u ? ryr[r] : 0.0;
-      L.Fby[row][e - row * G::NFY] = yfl[r] & 2u ? dtdy * (rr - rl) : 0.0;
-    }
-    __syncthreads();
-    if (cv + 1 < NU) load_raw(cv + 1);
-    // ---- x and y faces: F_bar * dq/dv_m into LDS ----
-    // x faces kk = 0..TXA+3 per side and column; with 8 rows a thread takes
-    // a run of three consecutive faces of one column (2 sides x 4 runs x 64
-    // columns = the 512 threads) and reads the run's seven stencil values
-    // once.
-    const auto xface = [&](int side, int kk, int col, const double* h) {
-      const int f = side == 0 ? X0 - 4 + kk : X0 - 5 + kk;   // face index
-      const int jj = Y0 + col - 3;
-      double* out = &Cx[((side * (TXA + 4) + kk) * 5) * TY + col];
-      if (f >= 0 && f <= nx && jj >= 0 && jj < ny) {
-        double g[5];
-        if (side == 0)   // left-biased, cells f..f+4 = region rows kk..kk+4
-          weno5_grad(h[0], h[1], h[2], h[3], h[4], g);
-        else             // right-biased, cells f+5..f+1 = region rows kk+4..kk
-          weno5_grad(h[4], h[3], h[2], h[1], h[0], g);
-        const double fb = L.Fbx[side == 0 ? kk + 1 : kk][col];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) out[m * TY] = fb * g[m];
-      } else {         // a face outside the grid contributes nothing
-#pragma unroll
-        for (int m = 0; m < 5; ++m) out[m * TY] = 0.0;
-      }
-    };
-    if constexpr (TXA == 8) {
-      const auto xrun = [&](int side, int col, int kk0) {
-        double h[7];
-#pragma unroll
-        for (int j = 0; j < 7; ++j) h[j] = side == 0 ? L.Hxp[kk0 + j][col] : L.Hxm[kk0 + j][col];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) xface(side, kk0 + i, col, h + i);
-      };
-      if (!thin) {
-        xrun(t >> 8, t & 63, 3 * ((t >> 6) & 3));
-      } else if (t < 8 * nxv) {
-        // a thin tile packs the runs of its columns with x faces into the
-        // first waves instead of giving every wave a few working lanes
-        const int q = t / nxv;
-        xrun(q >> 2, cx0 + (t - q * nxv), 3 * (q & 3));
-      }
-    } else {
-      for (int e = t; e < 2 * (TXA + 4) * TY; e += G::NT) {
-        const int side = e / ((TXA + 4) * TY), rem = e - side * ((TXA + 4) * TY);
-        const int kk = rem >> 6, col = rem & 63;
-        double h[5];
-#pragma unroll
-        for (int j = 0; j < 5; ++j) h[j] = side == 0 ? L.Hxp[kk + j][col] : L.Hxm[kk + j][col];
-        xface(side, kk, col, h);
-      }
-    }
-    // ---- the own cell's sums over the faces that reach it ----
-    // (every face a cell reads was written this variable, faces outside the
-    // grid as zeros, so the reads are unconditional and the ten reads of a
-    // direction are in flight together)
-    double hpx = 0.0, hmx = 0.0, hpy = 0.0, hmy = 0.0;
-    const auto gather_x = [&]() {
-      if (work && xrow) {
-#pragma unroll
-        for (int m = 0; m < 5; ++m) hpx += Cx[((w + 4 - m) * 5 + m) * TY + ly];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) hmx += Cx[(((TXA + 4) + w + m) * 5 + m) * TY + ly];
-      }
-    };
-    if constexpr (SEQ) {
-      __syncthreads();
-      gather_x();
-      __syncthreads();
-    }
-    // y faces kk = 0..TY+3 per side and row: with 8 rows the 512 threads
-    // take the pairs (kk, kk+1), kk < TY, of every side and row, and the
-    // first 64 threads the last four faces of each
-    const auto yface = [&](int side, int row, int kk, const double* h) {
-      const int gf = side == 0 ? Y0 - 4 + kk : Y0 - 5 + kk;
-      const int ii = X0 + row - 3;
-      double* out = &Cy[((side * TXA + row) * 5) * (TY + 4) + kk];
-      if (gf >= 0 && gf <= ny && ii >= 0 && ii < nx) {
-        double g[5];
-        if (side == 0)
-          weno5_grad(h[0], h[1], h[2], h[3], h[4], g);
-        else
-          weno5_grad(h[4], h[3], h[2], h[1], h[0], g);
-        const double fb = L.Fby[row][side == 0 ? kk + 1 : kk];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) out[m * (TY + 4)] = fb * g[m];
-      } else {
-#pragma unroll
-        for (int m = 0; m < 5; ++m) out[m * (TY + 4)] = 0.0;
-      }
-    };
-    if constexpr (TXA == 8) {
-      const auto ypair = [&](int side, int row, int kk0) {
-        double h[6];
-#pragma unroll
-        for (int j = 0; j < 6; ++j) h[j] = side == 0 ? L.Hyp[row][kk0 + j] : L.Hym[row][kk0 + j];
-        yface(side, row, kk0, h);
-        yface(side, row, kk0 + 1, h + 1);
-      };
-      if (!thin) {
-        ypair(t >> 8, (t >> 5) & 7, 2 * (t & 31));
-      } else {         // the faces a thin tile's cells read, packed
-        for (int e = t; e < 2 * TXA * nky; e += G::NT) {
-          const int q = e / nky, side = q / TXA, row = q - side * TXA;
-          const int kk = ky0 + (e - q * nky);
-          double h[5];
-#pragma unroll
-          for (int j = 0; j < 5; ++j) h[j] = side == 0 ? L.Hyp[row][kk + j] : L.Hym[row][kk + j];
-          yface(side, row, kk, h);
-        }
-      }
-      if (!thin && t < 64) {
-        const int side = t >> 5, row = (t >> 2) & 7, kk = TY + (t & 3);
-        double h[5];
-#pragma unroll
-        for (int j = 0; j < 5; ++j) h[j] = side == 0 ? L.Hyp[row][kk + j] : L.Hym[row][kk + j];
-        yface(side, row, kk, h);
-      }
-    } else {
-      for (int e = t; e < 2 * TXA * (TY + 4); e += G::NT) {
-        const int side = e / (TXA * (TY + 4)), rem = e - side * (TXA * (TY + 4));
-        const int row = rem / (TY + 4), kk = rem - row * (TY + 4);
-        double h[5];
-#pragma unroll
-        for (int j = 0; j < 5; ++j) h[j] = side == 0 ? L.Hyp[row][kk + j] : L.Hym[row][kk + j];
-        yface(side, row, kk, h);
-      }
-    }
-    __syncthreads();
-    if constexpr (!SEQ) gather_x();
-    if (work && ycol) {
-#pragma unroll
-      for (int m = 0; m < 5; ++m) hpy += Cy[(w * 5 + m) * (TY + 4) + ly + 4 - m];
-#pragma unroll
-      for (int m = 0; m < 5; ++m) hmy += Cy[((TXA + w) * 5 + m) * (TY + 4) + ly + m];
+      for (int m = 1; m < 4; ++m) hmy += __shfl_up(fb1 * g1[m], 4 - m, 64);
+      hmy += fb1 * g1[4];
     }
     if (work) {
-      const PrimL& qo = q[0];
-      const double ucv = ucv_own;
+      const double hpx = xrow ? xp : 0.0, hmx = xrow ? xm : 0.0;
+      const PrimL& qo = q;
       Ub[cv] += 0.5 * ax * (hpx - hmx) + 0.5 * ay * (hpy - hmy);
       sx_part += 0.5 * ucv * (hpx - hmx);
       sy_part += 0.5 * ucv * (hpy - hmy);
       const double Fx = 0.5 * (hpx + hmx), Fy = 0.5 * (hpy + hmy);
-      switch (cv) {
-        case 0:
-          rhob += Fx * qo.u + Fy * qo.v;
-          ub += Fx * qo.rho;
-          vb += Fy * qo.rho;
-          break;
-        case 1:
-          rhob += Fx * qo.u * qo.u + Fy * qo.u * qo.v;
-          ub += Fx * 2.0 * qo.rho * qo.u + Fy * qo.rho * qo.v;
-          vb += Fy * qo.rho * qo.u;
-          pb += Fx;
-          break;
-        case 2:
-          rhob += Fx * qo.u * qo.v + Fy * qo.v * qo.v;
-          ub += Fx * qo.rho * qo.v;
-          vb += Fx * qo.rho * qo.u + Fy * 2.0 * qo.rho * qo.v;
-          pb += Fy;
-          break;
-        case 3:
-          ub += Fx * (ucv + qo.p);
-          vb += Fy * (ucv + qo.p);
-          Ub[3] += Fx * qo.u + Fy * qo.v;
-          pb += Fx * qo.u + Fy * qo.v;
-          break;
-        default:
-          ub += Fx * ucv;
-          vb += Fy * ucv;
-          Ub[cv] += Fx * qo.u + Fy * qo.v;
+      if constexpr (CLS == 0) {
+        rhob += Fx * qo.u + Fy * qo.v;
+        ub += Fx * qo.rho;
+        vb += Fy * qo.rho;
+      } else if constexpr (CLS == 1) {
+        rhob += Fx * qo.u * qo.u + Fy * qo.u * qo.v;
+        ub += Fx * 2.0 * qo.rho * qo.u + Fy * qo.rho * qo.v;
+        vb += Fy * qo.rho * qo.u;
+        pb += Fx;
+      } else if constexpr (CLS == 2) {
+        rhob += Fx * qo.u * qo.v + Fy * qo.v * qo.v;
+        ub += Fx * qo.rho * qo.v;
+        vb += Fx * qo.rho * qo.u + Fy * 2.0 * qo.rho * qo.v;
+        pb += Fy;
+      } else if constexpr (CLS == 3) {
+        ub += Fx * (ucv + qo.p);
+        vb += Fy * (ucv + qo.p);
+        Ub[3] += Fx * qo.u + Fy * qo.v;
+        pb += Fx * qo.u + Fy * qo.v;
+      } else {
+        ub += Fx * ucv;
+        vb += Fy * ucv;
+        Ub[cv] += Fx * qo.u + Fy * qo.v;
       }
     }
-    // No barrier before the next variable: its H and F_bar writes only
-    // conflict with this variable's face loops (finished before the last
-    // barrier above), and its face loops' product writes come after its own
-    // first barrier, which every thread reaches after these reads.
-  }
+  };
+  load(0);
+  step(std::integral_constant<int, 0>{}, 0);
+  step(std::integral_constant<int, 1>{}, 1);
+  step(std::integral_constant<int, 2>{}, 2);
+  step(std::integral_constant<int, 3>{}, 3);
+#pragma unroll
+  for (int cv = 4; cv < NU; ++cv) step(std::integral_constant<int, 4>{}, cv);
   // the own cell's |u| + a and |v| + a: the padded field's LF maxima are
   // reduced from these (per block here, then alpha_reduce_kernel), with the
   // arithmetic that alpha_apply_kernel's tie test repeats
   double rxo = 0.0, ryo = 0.0;   // raw values, for the tie test
   if (own) {
     double Tb = 0.0, gb = 0.0;
-    const PrimD qf = prim_src(c, U, aux, n, src[0], fl[0]);
+    const PrimD qf = prim_src(c, U, aux, n, src, fl);
     const double snd = sqrt(qf.gamma * qf.p * qf.ir);
     rxo = fabs(qf.u) + snd;
     ryo = fabs(qf.v) + snd;
@@ -613,11 +558,10 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
     spd[np + P * PY + Q] = ryo;
     if (work) prim_to_U_vjp(c, qf, rhob, ub, vb, pb, 0.0, Yb, Ub, Tb, gb);
     if (pad_interior(a, P, Q)) {   // its own source: straight into the output
-      const int s = src[0];
 #pragma unroll
-      for (int v = 0; v < NU; ++v) Ut[(size_t)v * n + s] += Ub[v];
-      At[s] += gb;
-      At[n + s] += Tb;
+      for (int v = 0; v < NU; ++v) Ut[(size_t)v * n + src] += Ub[v];
+      At[src] += gb;
+      At[n + src] += Tb;
     } else {                       // a ghost: kept for fold_ghosts_kernel
       const int pt = P * PY + Q;
 #pragma unroll
@@ -638,38 +582,38 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) __attribute__((amdgpu_waves_per_e
     myo = fmax(myo, __shfl_xor(myo, off, 64));
   }
   if (ly == 0) {
-    L.red[0][w] = sx_part;
-    L.red[1][w] = sy_part;
-    L.red[2][w] = mxo;
-    L.red[3][w] = myo;
+    red[0][w] = sx_part;
+    red[1][w] = sy_part;
+    red[2][w] = mxo;
+    red[3][w] = myo;
   }
   __syncthreads();
   double mx = 0.0, my = 0.0;
 #pragma unroll
-  for (int k = 0; k < TXA; ++k) {
-    mx = fmax(mx, L.red[2][k]);
-    my = fmax(my, L.red[3][k]);
+  for (int k = 0; k < kYRows; ++k) {
+    mx = fmax(mx, red[2][k]);
+    my = fmax(my, red[3][k]);
   }
   const unsigned long long tx = __ballot(own && is_max(rxo, mx));
   const unsigned long long ty = __ballot(own && is_max(ryo, my));
   if (ly == 0) {
-    L.ties[0][w] = (unsigned int)__popcll(tx);
-    L.ties[1][w] = (unsigned int)__popcll(ty);
+    ties[0][w] = (unsigned int)__popcll(tx);
+    ties[1][w] = (unsigned int)__popcll(ty);
   }
   __syncthreads();
   if (t == 0) {
-    double sx = 0.0, sy = 0.0;
+    double sxs = 0.0, sys = 0.0;
     unsigned int cx = 0, cy = 0;
 #pragma unroll
-    for (int k = 0; k < TXA; ++k) {
-      sx += L.red[0][k];
-      sy += L.red[1][k];
-      cx += L.ties[0][k];
-      cy += L.ties[1][k];
+    for (int k = 0; k < kYRows; ++k) {
+      sxs += red[0][k];
+      sys += red[1][k];
+      cx += ties[0][k];
+      cy += ties[1][k];
     }
     double* o = part + 6 * blockIdx.x;
-    o[0] = sx;
-    o[1] = sy;
+    o[0] = sxs;
+    o[1] = sys;
     o[2] = mx;
     o[3] = my;
     o[4] = (double)cx;
@@ -738,10 +682,9 @@ __global__ __launch_bounds__(256) void alpha_reduce_kernel(const double* __restr
   }
 }
 
-// One block per flux_vjp_tile_kernel tile (same cells): a tile whose block
-// maxima (part) are not the field's holds no maximiser and exits at once.
-template <int TXA>
-__global__ __launch_bounds__(AdjTile<TXA>::NT) void alpha_apply_kernel(
+// One block per flux_vjp_y_kernel block (same cells): a block whose maxima
+// (part) are not the field's holds no maximiser and exits at once.
+__global__ __launch_bounds__(64 * kYRows) void alpha_apply_kernel(
     AdjArgs a, const double* __restrict__ U, const double* __restrict__ aux,
     const double* __restrict__ spd, const double* __restrict__ part,
     const unsigned long long* alpha_bits, const double* scal, const unsigned int* cnt,
@@ -752,15 +695,15 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) void alpha_apply_kernel(
   if (!(cnt[0] > 0 && pb[2] == gx) && !(cnt[1] > 0 && pb[3] == gy)) return;
   const MechConst& c = *a.mc;
   const int PX = a.nx + 6, PY = a.ny + 6, np = PX * PY;
-  int X0, Y0;
-  adj_tile_origin<TXA>(blockIdx.x, a.ny, X0, Y0);
-  const int P = X0 + (threadIdx.x >> 6), Q = Y0 + (threadIdx.x & 63);
-  if (P < 0 || P >= PX || Q < 0 || Q >= PY) return;
+  int P, Q;
+  bool mine;
+  ys_cell(blockIdx.x, a.ny, threadIdx.x, P, Q, mine);
+  if (!mine || P >= PX || Q < 0 || Q >= PY) return;
   const int t = P * PY + Q;
   int sx, sy, flip;
   if (!pad_src(a, P, Q, sx, sy, flip)) return;
   // the tie test compares the very values the maxima were reduced from
-  // (stored by flux_vjp_tile_kernel), not a recomputation
+  // (stored by flux_vjp_y_kernel), not a recomputation
   const bool mx = cnt[0] > 0 && is_max(spd[t], gx);
   const bool my = cnt[1] > 0 && is_max(spd[np + t], gy);
   if (!mx && !my) return;
@@ -777,7 +720,7 @@ __global__ __launch_bounds__(AdjTile<TXA>::NT) void alpha_apply_kernel(
   for (int k = 0; k < JH_NS - 1; ++k) Yb[k] = 0.0;
   double Tb = 0.0, gb = 0.0;
   prim_to_U_vjp(c, q, 0.0, wx * su, wy * sv, 0.0, wx + wy, Yb, Ub, Tb, gb);
-  if (pad_interior(a, P, Q)) {     // as in flux_vjp_tile_kernel
+  if (pad_interior(a, P, Q)) {     // as in flux_vjp_y_kernel
     const int n = a.nx * a.ny, s = sx * a.ny + sy;
 #pragma unroll
     for (int v = 0; v < NU; ++v) Ut[v * n + s] += Ub[v];

@@ -426,14 +426,9 @@ __global__ void zero_u32_kernel(unsigned int* p, int m) {
 
 int blocks_for(size_t m) { return (int)((m + 255) / 256); }
 
-// Tiles of the adjoint flux kernel over the ghost-padded grid.
-// Flux-VJP tile: 8 padded rows x 64 columns, x and y products in separate
-// LDS buffers (142 KB, one tile per CU).  Measured at 1024 x 256: 244 us per
-// launch, against 268 us for 4-row tiles whose x and y products share one
-// buffer (65 KB, two tiles per CU; more halo rows and barriers per cell).
-constexpr int kAdjRows = 8;
-constexpr bool kAdjSeq = false;
-int adj_tiles(const jh_ctx* c) { return adj_tiles_x<kAdjRows>(c->nx) * adj_tiles_y(c->ny); }
+// Blocks of the adjoint's y pass (and of alpha_apply_kernel) over the
+// ghost-padded grid.
+int adj_tiles(const jh_ctx* c) { return ys_blocks(c->nx, c->ny); }
 
 // JH_BC_USER sides must have been written (jh_set_ghost_strip or through
 // jh_ghost_strip's pointer) since their last use; consumes the marks.
@@ -450,6 +445,8 @@ struct AdjWork {
   double *Ub, *Uacc, *ab, *rb;  // cotangents
   double *Ubp, *abp;            // ghost cells' cotangents, on the padded grid
   double* spd;                  // |u| + a, |v| + a of every padded cell (2 planes)
+  double* qb;                   // rho, u, v, p of the stage state (4 planes)
+  double* hx;                   // x faces' H+-_bar, [2][NU][nx + 6][ny]
   double *part, *scal;
   unsigned int* cnt;
   unsigned long long* amx;      // recomputed LF maxima (bit patterns)
@@ -462,7 +459,8 @@ int adj_work(jh_ctx* c, const double* U0, const double* A0, AdjWork& w) {
   const size_t np = (size_t)(c->nx + 6) * (c->ny + 6);
   const int nblk = adj_tiles(c);
   const size_t ng = np - n;      // ghost cells
-  const size_t len = 4 * (NU + 2) * n + (3 * NU + 2) * n + (NU + 4) * np +
+  const size_t nhx = 2 * NU * (size_t)(c->nx + 6) * c->ny;
+  const size_t len = 4 * (NU + 2) * n + (3 * NU + 2) * n + (NU + 4) * np + 4 * n + nhx +
                      6 * (size_t)nblk + 2 + 2 + 2 + (3 * ng + 2) / 2;
   if (c->adj_len < len) {
     if (c->adj) (void)hipFree(c->adj);
@@ -486,6 +484,8 @@ int adj_work(jh_ctx* c, const double* U0, const double* A0, AdjWork& w) {
   w.Ubp = p; p += NU * np;
   w.abp = p; p += 2 * np;
   w.spd = p; p += 2 * np;
+  w.qb = p; p += 4 * n;
+  w.hx = p; p += nhx;
   w.part = p; p += 6 * nblk;
   w.scal = p; p += 2;
   w.cnt = reinterpret_cast<unsigned int*>(p);
@@ -559,11 +559,14 @@ int adj_fold(jh_ctx* c, const AdjArgs& a, AdjWork& w) {
 void flux_vjp(jh_ctx* c, const AdjArgs& a, AdjWork& w, const double* Us, const double* As,
               const double* rb, const unsigned long long* alpha, double* Ut, double* At) {
   hipStream_t st = c->stream;
-  flux_vjp_tile_kernel<kAdjRows, kAdjSeq><<<w.nblk, AdjTile<kAdjRows>::NT, 0, st>>>(
-      a, Us, As, rb, alpha, Ut, At, w.Ubp, w.abp, w.spd, w.part);
+  const int n = a.nx * a.ny;
+  adj_prim_kernel<<<blocks_for(n), 256, 0, st>>>(a, Us, As, w.qb);
+  flux_vjp_x_kernel<<<xs_blocks(a.nx, a.ny), 256, 0, st>>>(a, Us, w.qb, rb, alpha, w.hx);
+  flux_vjp_y_kernel<<<w.nblk, 64 * kYRows, 0, st>>>(a, Us, As, rb, alpha, w.hx, Ut, At, w.Ubp,
+                                                    w.abp, w.spd, w.part);
   alpha_reduce_kernel<<<1, 256, 0, st>>>(w.part, w.nblk, w.scal, w.amx, w.cnt);
-  alpha_apply_kernel<kAdjRows><<<w.nblk, AdjTile<kAdjRows>::NT, 0, st>>>(
-      a, Us, As, w.spd, w.part, w.amx, w.scal, w.cnt, Ut, At, w.Ubp, w.abp);
+  alpha_apply_kernel<<<w.nblk, 64 * kYRows, 0, st>>>(a, Us, As, w.spd, w.part, w.amx, w.scal,
+                                                     w.cnt, Ut, At, w.Ubp, w.abp);
   const int m = c->adj_fold_m;
   const int* fl = w.fold;
   if (m > 0)

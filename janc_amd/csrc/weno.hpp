@@ -122,7 +122,9 @@ namespace jh {
 //   dq/dv_j = sum_k w_k dq_k/dv_j + sum_k kappa_k de_k/dv_j,
 //   kappa_k = -2 w_k (q_k - q) / e_k,  e_k = eps + IS_k,
 // in weno5's difference form (q_k - q = c_k - sum_m w_m c_m), with the three
-// 1/e_k from one reciprocal of e_1 e_2 e_3.  With A_k the halved first-
+// 1/e_k from 1/(e_1 e_2 e_3) and the weights' normaliser from the same
+// reciprocal (e^7 stays in FP64 range for any e = eps + IS_k the split
+// fluxes produce: 1e-42 at eps, 1e154 at IS = 1e22).  With A_k the halved first-
 // derivative terms (IS_k = A_k^2 + 13/12 b_k^2), de_k/dv = 2 A_k dA_k/dv +
 // 13/6 b_k db_k/dv, where dA_1 = (1/2, -2, 3/2, 0, 0), dA_2 = (0, -1/2, 0,
 // 1/2, 0), dA_3 = (0, 0, 3/2, -2, 1/2) and db_1 = (1, -2, 1, 0, 0), db_2 =
@@ -138,13 +140,14 @@ __host__ __device__ __forceinline__ double weno5_grad(double v0, double v1, doub
   const double e3 = fma(A3, A3, fma((13.0 / 12.0) * bC, bC, 1e-6));
   const double E12 = e1 * e2, E13 = e1 * e3, E23 = e2 * e3;
   const double P1 = 0.1 * (E23 * E23), P2 = 0.6 * (E13 * E13), P3 = 0.3 * (E12 * E12);
-  const double R = rcp(P1 + P2 + P3);
+  // R = 1 / sum(P) and iE = 1 / (e1 e2 e3) from one reciprocal
+  const double S = P1 + P2 + P3, E = E12 * e3, X = rcp(E * S);
+  const double R = E * X, iE = S * X;
   const double w1 = P1 * R, w2 = P2 * R, w3 = P3 * R;
   const double c1 = fma(1.0 / 3.0, bA, hB);
   const double c2 = fma(-1.0 / 6.0, bB, hC);
   const double c3 = fma(-1.0 / 6.0, bC, hC);
   const double cb = fma(w1, c1, fma(w2, c2, w3 * c3));
-  const double iE = rcp(E12 * e3);
   // 2 kappa_k A_k and 13/6 kappa_k b_k
   const double m1 = -2.0 * (w1 * (E23 * iE)) * (c1 - cb);
   const double m2 = -2.0 * (w2 * (E13 * iE)) * (c2 - cb);
