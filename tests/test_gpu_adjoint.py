@@ -58,9 +58,20 @@ def _setup(mech_path, mech, name):
     elif name == "tube":
         U, aux, g = tie_free_state(mech_path, mech, "tube", 26, 13)
     elif name == "wide":
-        # wide enough for whole 8 x 64 flux-VJP tiles (interior-aligned) as
-        # well as the thin tiles of the ghost frame and the ragged last band
+        # 76 padded columns: two 56-column segments of the flux VJP's y pass,
+        # the second ragged
         U, aux, g = tie_free_state(mech_path, mech, "tube", 21, 70)
+    elif name == "tall":
+        # 81 padded rows: three 32-row segments of the flux VJP's x pass (the
+        # last ragged), with wall ghost rows (momentum flips) at both ends
+        U, aux, g = tie_free_state(mech_path, mech, "tube", 75, 20)
+        g["boundary_config"] = {"left_boundary": "no_slip_wall",
+                                "right_boundary": "slip_wall",
+                                "bottom_boundary": "slip_wall",
+                                "top_boundary": "zero_gradient"}
+    elif name == "box_tall":
+        # periodic x ghost rows in the first and last of three x segments
+        U, aux, g = tie_free_state(mech_path, mech, "box", 70, 16)
     elif name == "walls":
         U, aux, g = tie_free_state(mech_path, mech, "tube", 21, 17)
         g["boundary_config"] = {"left_boundary": "no_slip_wall",
@@ -83,7 +94,7 @@ def _cotangents(U, aux, seed=0):
     return rng.standard_normal(U.shape) / sU, rng.standard_normal(aux.shape) / sa
 
 
-@pytest.mark.parametrize("name", ["box", "box_ties", "tube", "walls", "wide"])
+@pytest.mark.parametrize("name", ["box", "box_ties", "tube", "walls", "wide", "tall", "box_tall"])
 def test_step_vjp_matches_twin(mech_path, mech, name):
     sim, g, U, aux = _setup(mech_path, mech, name)
     W, Wa = _cotangents(U, aux)
