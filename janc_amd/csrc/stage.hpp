@@ -988,7 +988,12 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
     oidx[s] = PAD ? (gxs[s] + 3) * (ny + 6) + gys[s] + 3 : gxs[s] * ny + gys[s];
   }
 
-  for (int cv = 0; cv < NU; ++cv) {
+  // One conserved variable; CLS = min(cv, 4) selects the physical flux
+  // (and the Steger-Warming split) at compile time, so phase 1 carries no
+  // per-slot selection on cv.  Variables 0-3 are instantiated once each,
+  // the species share one loop body.
+  const auto var_step = [&](auto K, const int cv) {
+    constexpr int CLS = decltype(K)::value;
     // ---- phase 1: split fluxes into LDS ----
     W uc_own[2];
 #pragma unroll
@@ -1001,9 +1006,9 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
         W hp = (W)0, hm = (W)0;
         if (ref[s].ok) {
           if (SPLIT == SPLIT_SW) {
-            sw_split<W>(cv, 0, q[s], ucv, hp, hm);
+            sw_split<W>(CLS, 0, q[s], ucv, hp, hm);
           } else {
-            const W hF = (W)0.5 * phys_flux<W>(cv, 0, q[s], ucv);
+            const W hF = (W)0.5 * phys_flux<W>(CLS, 0, q[s], ucv);
             hp = fma(hax, ucv, hF);
             hm = fma(-hax, ucv, hF);
           }
@@ -1015,9 +1020,9 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
         W hp = (W)0, hm = (W)0;
         if (ref[s].ok) {
           if (SPLIT == SPLIT_SW) {
-            sw_split<W>(cv, 1, q[s], ucv, hp, hm);
+            sw_split<W>(CLS, 1, q[s], ucv, hp, hm);
           } else {
-            const W hG = (W)0.5 * phys_flux<W>(cv, 1, q[s], ucv);
+            const W hG = (W)0.5 * phys_flux<W>(CLS, 1, q[s], ucv);
             hp = fma(hay, ucv, hG);
             hm = fma(-hay, ucv, hG);
           }
@@ -1121,7 +1126,13 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
     // outstanding load (the next variable's prefetch included) inside the
     // face loop.  vmcnt(0), expcnt and lgkmcnt at their maxima (no wait).
     __builtin_amdgcn_s_waitcnt(0x0F70);
-  }
+  };
+  var_step(std::integral_constant<int, 0>{}, 0);
+  var_step(std::integral_constant<int, 1>{}, 1);
+  var_step(std::integral_constant<int, 2>{}, 2);
+  var_step(std::integral_constant<int, 3>{}, 3);
+#pragma unroll 1
+  for (int cv = 4; cv < NU; ++cv) var_step(std::integral_constant<int, 4>{}, cv);
   if (MODE == MODE_RHS) return;
 
   // ---- update_aux (aux_func.py:21-29) + next stage's LF maxima ----
