@@ -924,7 +924,10 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
   const MechConst& c = *a.mc;
   if (MODE != MODE_RHS) load_mix_tables(c, L.mix);   // read after the phase barriers
   const int nx = a.nx, ny = a.ny, n = PAD ? (nx + 6) * (ny + 6) : nx * ny;
-  const int t = threadIdx.x, ly = t & 63, w = t >> 6;
+  // the wave index through readfirstlane: the tile rows, slots and face
+  // runs derived from it are wave-uniform to the compiler (scalar registers
+  // and branches)
+  const int t = threadIdx.x, ly = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int tiles_y = (ny + TY - 1) / TY;
   const int x0 = (tile / tiles_y) * TX;
   const int y0 = (tile % tiles_y) * TY;
@@ -1056,8 +1059,9 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
       // The second half the y faces 0..63: 16 runs of four per row, run =
       // u % 16, row = u / 16.
       constexpr int NXT = NT / 2;
-      if (t < NXT) {
-        const int col = t & 63, fr0 = (t >> 6) * 4;
+      static_assert(NXT % 64 == 0, "the x-face half of the block is whole waves");
+      if (w < NXT / 64) {
+        const int col = ly, fr0 = w * 4;
         if (fr0 + 4 == TX) {   // the last run also takes face row TX (wave-uniform)
           weno5_run<5, false, false>(&sHxp[fr0][col], TY, &sFx[fr0][col], TY);
           weno5_run<5, true, true>(&sHxm[fr0 + 1][col], TY, &sFx[fr0][col], TY);
