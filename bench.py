@@ -55,7 +55,7 @@ CHEM_FP64_FLOP_PER_CELL = {"mixed": 2760.2, "fp64": 3249.6}
 # profiles/).  The VALU issue floor of a launch is instructions / 64 wave
 # instructions x 4 cycles (64 lanes on a 16-lane SIMD; FP32 without packed
 # math issues at the FP64 rate) / (1024 SIMDs x 2.4 GHz).
-STAGE_VALU_PER_CELL = {"mixed": 3266.9}
+STAGE_VALU_PER_CELL = {"mixed": 3231.0}
 SIMDS = 1024
 CLOCK_GHZ = 2.4
 
