@@ -212,6 +212,7 @@ __host__ __device__ inline void ys_cell(int blk, int ny, int t, int& P, int& Q, 
   Q = sg * kYCells - 4 + l;
   mine = l >= 4 && l < 4 + kYCells;
 }
+
 // Exact tie test of the LF maxima: JAX's max VJP splits the cotangent
 // among the elements equal to the maximum.
 __device__ __forceinline__ bool is_max(double val, double alpha) { return val == alpha; }
@@ -250,8 +251,8 @@ __global__ __launch_bounds__(256) void adj_prim_kernel(AdjArgs a, const double* 
 // its rows [c0, c0 + kXSeg), holding the five stencil values and the five
 // open row sums: row b is complete once face f = b - side is done.
 // hx[side][cv][P][j]: H+_bar (side 0) and H-_bar (side 1) of padded row P.
-// The sweep of one (variable, side) wave: CLS = min(cv, 4) selects the
-// physical flux and the momentum flips at compile time.
+// CLS = min(cv, 4) selects the physical flux and the momentum flips at
+// compile time.
 template <int CLS, int SIDE>
 __device__ __forceinline__ void xs_sweep(const AdjArgs& a, const double* __restrict__ U,
                                          const double* __restrict__ qb,

@@ -428,7 +428,7 @@ int blocks_for(size_t m) { return (int)((m + 255) / 256); }
 
 // Blocks of the adjoint's y pass (and of alpha_apply_kernel) over the
 // ghost-padded grid.
-int adj_tiles(const jh_ctx* c) { return ys_blocks(c->nx, c->ny); }
+int adj_blocks(const jh_ctx* c) { return ys_blocks(c->nx, c->ny); }
 
 // JH_BC_USER sides must have been written (jh_set_ghost_strip or through
 // jh_ghost_strip's pointer) since their last use; consumes the marks.
@@ -457,7 +457,7 @@ struct AdjWork {
 int adj_work(jh_ctx* c, const double* U0, const double* A0, AdjWork& w) {
   const size_t n = (size_t)c->nx * c->ny;
   const size_t np = (size_t)(c->nx + 6) * (c->ny + 6);
-  const int nblk = adj_tiles(c);
+  const int nblk = adj_blocks(c);
   const size_t ng = np - n;      // ghost cells
   const size_t nhx = 2 * NU * (size_t)(c->nx + 6) * c->ny;
   const size_t len = 4 * (NU + 2) * n + (3 * NU + 2) * n + (NU + 4) * np + 4 * n + nhx +
