@@ -191,6 +191,7 @@ __host__ __device__ inline int xs_segments(int nx) { return (nx + 6 + kXSeg - 1)
 __host__ __device__ inline int xs_blocks(int nx, int ny) {   // of 4 waves
   return 2 * NU * ((((ny + 63) / 64) * xs_segments(nx) + 3) / 4);
 }
+static_assert((2 * NU) % 8 == 0, "x-pass blocks: a multiple of the 8 XCDs");
 
 // y pass: a wave covers 64 consecutive padded columns of one padded row and
 // owns the 56 cells on lanes 4..59 (their stencils and faces lie within the
@@ -374,13 +375,19 @@ __global__ __launch_bounds__(256) void flux_vjp_x_kernel(AdjArgs a, const double
                                                          const unsigned long long* alpha_bits,
                                                          double* __restrict__ hx) {
   const int nx = a.nx, ny = a.ny;
-  const int njb = (ny + 63) >> 6, nt = njb * xs_segments(nx), nbt = (nt + 3) / 4;
-  // a block's four waves share one (variable, side), so they run one
-  // instantiation of the sweep
+  const int njb = (ny + 63) >> 6, nt = njb * xs_segments(nx);
+  // Blocks go round-robin to the 8 XCDs; logical block lb gives XCD x the
+  // contiguous run lb = x * (gridDim.x / 8) + k, and consecutive logical
+  // blocks are the 2 NU (variable, side) instances of one group of four
+  // tiles (a block's four waves share one instance), so a group's
+  // primitives, state and rhs cotangent are fetched into one L2 and read
+  // there by all of them.  gridDim.x is 2 NU times the group count, a
+  // multiple of 8.
+  const int lb = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
   // the wave index through readfirstlane: every quantity derived from it
   // (segment, rows, faces, loop bounds) is then wave-uniform to the compiler
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int sc = blockIdx.x / nbt, tile = (blockIdx.x - sc * nbt) * 4 + wv;
+  const int sc = lb % (2 * NU), tile = (lb / (2 * NU)) * 4 + wv;
   if (tile >= nt) return;
   const int side = sc & 1, cv = sc >> 1;
   const int jb = tile % njb, seg = tile / njb;
