@@ -188,10 +188,14 @@ __device__ __forceinline__ void prim_to_U_vjp(const MechConst& c, const PrimD& q
 // kXSeg padded rows), the 64 lanes of a wave on consecutive columns.
 constexpr int kXSeg = 32;
 __host__ __device__ inline int xs_segments(int nx) { return (nx + 6 + kXSeg - 1) / kXSeg; }
-__host__ __device__ inline int xs_blocks(int nx, int ny) {   // of 4 waves
+// (variable, side) instances x groups of four tiles, one block (4 waves)
+// each; the launch rounds it up to a multiple of the 8 XCDs
+__host__ __device__ inline int xs_work_blocks(int nx, int ny) {
   return 2 * NU * ((((ny + 63) / 64) * xs_segments(nx) + 3) / 4);
 }
-static_assert((2 * NU) % 8 == 0, "x-pass blocks: a multiple of the 8 XCDs");
+__host__ __device__ inline int xs_blocks(int nx, int ny) {
+  return (xs_work_blocks(nx, ny) + 7) / 8 * 8;
+}
 
 // y pass: a wave covers 64 consecutive padded columns of one padded row and
 // owns the 56 cells on lanes 4..59 (their stencils and faces lie within the
@@ -381,9 +385,10 @@ __global__ __launch_bounds__(256) void flux_vjp_x_kernel(AdjArgs a, const double
   // blocks are the 2 NU (variable, side) instances of one group of four
   // tiles (a block's four waves share one instance), so a group's
   // primitives, state and rhs cotangent are fetched into one L2 and read
-  // there by all of them.  gridDim.x is 2 NU times the group count, a
-  // multiple of 8.
+  // there by all of them.  gridDim.x is xs_blocks (a multiple of 8); the
+  // logical blocks past xs_work_blocks have no work.
   const int lb = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  if (lb >= xs_work_blocks(nx, ny)) return;
   // the wave index through readfirstlane: every quantity derived from it
   // (segment, rows, faces, loop bounds) is then wave-uniform to the compiler
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
