@@ -413,11 +413,25 @@ __global__ __launch_bounds__(256) void flux_vjp_x_kernel(AdjArgs a, const double
   }
 }
 
+// Whole-wave lane shifts of a double (DPP wave_shr:1 / wave_shl:1, two
+// 32-bit moves): lane l receives lane l-1 (shr1) or lane l+1 (shl1); the lane
+// shifted in receives 0.
+template <int CTRL>
+__device__ __forceinline__ double wave_shift1(double x) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double shr1(double x) { return wave_shift1<0x138>(x); }
+__device__ __forceinline__ double shl1(double x) { return wave_shift1<0x130>(x); }
+
 // y pass and the per-cell reverse of the split flux and the primitive map.
 // Lane l at padded column Q evaluates the left-biased y face g = Q and the
 // right-biased face g = Q - 1 of its row (both stencils: columns Q .. Q+4,
-// taken from lanes l .. l+4); the cell on lane l sums the left-biased
-// products of lanes l-4 .. l and the right-biased ones of lanes l-4 .. l.
+// taken from lanes l .. l+4 by DPP whole-wave shifts, no LDS); the cell on
+// lane l sums the left-biased products of lanes l-4 .. l and the
+// right-biased ones of lanes l-4 .. l.
 __global__ __launch_bounds__(64 * kYRows) void flux_vjp_y_kernel(
     AdjArgs a, const double* __restrict__ U, const double* __restrict__ aux,
     const double* __restrict__ rbar, const unsigned long long* alpha_bits,
@@ -500,22 +514,22 @@ __global__ __launch_bounds__(64 * kYRows) void flux_vjp_y_kernel(
       ep[0] = hp;
       em[0] = hm;
 #pragma unroll
-      for (int d = 1; d < 5; ++d) {
-        ep[d] = __shfl_down(hp, d, 64);
-        em[d] = __shfl_down(hm, d, 64);
+      for (int d = 1; d < 5; ++d) {   // lane l + d's values, one shift at a time
+        ep[d] = shl1(ep[d - 1]);
+        em[d] = shl1(em[d - 1]);
       }
       double g0[5], g1[5];
       weno5_grad(ep[0], ep[1], ep[2], ep[3], ep[4], g0);
       weno5_grad(em[4], em[3], em[2], em[1], em[0], g1);
       // left-biased: entry m of lane l belongs to lane l + m; right-biased:
-      // entry m (argument em[4-m]) to lane l + 4 - m
-      hpy = fb0 * g0[0];
-#pragma unroll
-      for (int m = 1; m < 5; ++m) hpy += __shfl_up(fb0 * g0[m], m, 64);
-      hmy = __shfl_up(fb1 * g1[0], 4, 64);
-#pragma unroll
-      for (int m = 1; m < 4; ++m) hmy += __shfl_up(fb1 * g1[m], 4 - m, 64);
-      hmy += fb1 * g1[4];
+      // entry m (argument em[4-m]) to lane l + 4 - m.  The cell on lane l
+      // gathers them Horner-fashion, one shift per term:
+      //   hpy(l) = p0(l) + p1(l-1) + p2(l-2) + p3(l-3) + p4(l-4),
+      //   hmy(l) = q4(l) + q3(l-1) + q2(l-2) + q1(l-3) + q0(l-4).
+      hpy = fb0 * g0[0] +
+            shr1(fb0 * g0[1] + shr1(fb0 * g0[2] + shr1(fb0 * g0[3] + shr1(fb0 * g0[4]))));
+      hmy = fb1 * g1[4] +
+            shr1(fb1 * g1[3] + shr1(fb1 * g1[2] + shr1(fb1 * g1[1] + shr1(fb1 * g1[0]))));
     }
     if (work) {
       const double hpx = xrow ? xp : 0.0, hmx = xrow ? xm : 0.0;
