@@ -169,6 +169,11 @@ def _cpu_baseline(pool, nworkers, steps):
         jobs.append(((w // 4) * CPU_CROP[0], (w % 4) * CPU_CROP[1], CPU_CROP[0], CPU_CROP[1],
                      steps))
     res = pool.map(_cpu_worker, jobs, chunksize=1)
+    return _cpu_result(res, nworkers, steps)
+
+
+def _cpu_result(res, nworkers, steps):
+    """The cpu_baseline object from the workers' (cell-updates, start, end)."""
     work = sum(r[0] for r in res)
     wall = max(r[2] for r in res) - min(r[1] for r in res)
     per = [r[0] / (r[2] - r[1]) for r in res]
@@ -785,6 +790,71 @@ def _summary_adjoint(adj):
     return out
 
 
+def build_summary(steps, warmup, nx, ny, mixed, fp64, roofline, roof_valu, chem_fv, acc,
+                  crop, adj, bat, amr_out, ann, cpu):
+    """The compact last line: the headline and its roofline and CPU baseline
+    beside the other legs' headline figures (each leg's own line holds the
+    rest).  Legs that did not run are None and leave their keys out."""
+    summary = {}
+    summary.update({
+        "metric": METRIC,
+        "value": mixed["value"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "mixed-precision step (FP32 state, split fluxes, WENO5; FP64 "
+                               "divergence, RK combine, chemistry, T solve), 8000x500 tube, "
+                               "dx 1.25e-5 m, phi in [0.8,1.2] smooth field, 40-column 20 atm/"
+                               "2800 K ignition slab, slip/zero-gradient/slip/slip, CFL 0.3, "
+                               "9sp-19r H2-air, LF + WENO5_JS",
+                   "nx": nx, "ny": ny, "cells": nx * ny,
+                   "precision": "mixed"},
+        "fp64_value": fp64["value"],
+        "fp64_ms_per_step": fp64["ms_per_step"],
+        "speedup_mixed_over_fp64": mixed["value"] / fp64["value"],
+        "kernels_mixed_ms": {k: v["ms_total"] / v["launches"]
+                             for k, v in mixed["kernels"].items() if v["launches"]},
+        "kernels_fp64_ms": {k: v["ms_total"] / v["launches"]
+                            for k, v in fp64["kernels"].items() if v["launches"]},
+        "health_flags": {"mixed": mixed["health_flags"], "fp64": fp64["health_flags"]},
+        "roofline": roofline,
+        "roofline_valu": roof_valu,
+        "roofline_chem": chem_fv,
+    })
+    if acc is not None:
+        summary["mixed_err_vs_fp64_gpu"] = dict(_worst(acc["every"][-1]),
+                                                steps=acc["every"][-1]["step"])
+    if crop is not None:
+        summary["mixed_err_vs_oracle_crop"] = dict(_worst(crop["every"][-1]),
+                                                   steps=crop["every"][-1]["step"],
+                                                   crop="%dx%d" % ORACLE_CROP)
+    if adj is not None:
+        summary["adjoint"] = _summary_adjoint(adj)
+        summary["adjoint_over_fwd_step"] = adj["adjoint_over_fwd_step"]
+    if bat is not None:
+        summary["batch"] = {"value": bat["value"], "speedup_vs_sequential":
+                            bat["speedup_vs_sequential"],
+                            "bit_identical_to_sequential": bat["bit_identical_to_sequential"]}
+    if amr_out is not None:
+        summary["amr_blocks_fp64"] = {
+            k: {"value": v["value"], "library_value": v["library_value"], "n_blk": v["n_blk"]}
+            for k, v in amr_out.items() if k.startswith("blocks_")}
+    if ann is not None:
+        summary["fp64_annulus_4000x400"] = {"value": ann["value"],
+                                            "ms_per_step": ann["ms_per_step"]}
+    if cpu is not None:
+        summary["cpu_baseline"] = cpu
+        summary["speedup_vs_cpu"] = mixed["value"] / cpu["value"]
+    return summary
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -886,7 +956,6 @@ def _gpu_legs(args, mech, U3, aux3, g3, cells, pool, cpu, only):
                                     ((ORACLE_CROP[0], ORACLE_CROP[1], dt_fixed,
                                       args.crop_steps, every),))
 
-    summary = {}
     if args.annulus_only:
         ann, _ = _annulus_leg(mech, args.steps, args.warmup, tune, torch)
         emit("fp64_annulus", ann)
@@ -975,63 +1044,9 @@ def _gpu_legs(args, mech, U3, aux3, g3, cells, pool, cpu, only):
         emit("mixed_vs_oracle_crop", crop)
 
     # ---- summary (last line) ----
-    summary.update({
-        "metric": METRIC,
-        "value": mixed["value"],
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": mixed["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64",
-        "data": "synthetic",
-        "config": {"workload": "mixed-precision step (FP32 state, split fluxes, WENO5; FP64 "
-                               "divergence, RK combine, chemistry, T solve), 8000x500 tube, "
-                               "dx 1.25e-5 m, phi in [0.8,1.2] smooth field, 40-column 20 atm/"
-                               "2800 K ignition slab, slip/zero-gradient/slip/slip, CFL 0.3, "
-                               "9sp-19r H2-air, LF + WENO5_JS",
-                   "nx": U3.shape[1], "ny": U3.shape[2], "cells": cells,
-                   "precision": "mixed"},
-        "fp64_value": fp64["value"],
-        "fp64_ms_per_step": fp64["ms_per_step"],
-        "speedup_mixed_over_fp64": mixed["value"] / fp64["value"],
-        "kernels_mixed_ms": {k: v["ms_total"] / v["launches"]
-                             for k, v in mixed["kernels"].items() if v["launches"]},
-        "kernels_fp64_ms": {k: v["ms_total"] / v["launches"]
-                            for k, v in fp64["kernels"].items() if v["launches"]},
-        "health_flags": {"mixed": mixed["health_flags"], "fp64": fp64["health_flags"]},
-        "roofline": roofline,
-        "roofline_valu": roof_valu,
-        "roofline_chem": chem_fv,
-    })
-    if acc is not None:
-        summary["mixed_err_vs_fp64_gpu"] = dict(_worst(acc["every"][-1]),
-                                                steps=acc["every"][-1]["step"])
-    if crop is not None:
-        summary["mixed_err_vs_oracle_crop"] = dict(_worst(crop["every"][-1]),
-                                                   steps=crop["every"][-1]["step"],
-                                                   crop="%dx%d" % ORACLE_CROP)
-    if adj is not None:
-        summary["adjoint"] = _summary_adjoint(adj)
-        summary["adjoint_over_fwd_step"] = adj["adjoint_over_fwd_step"]
-    if bat is not None:
-        summary["batch"] = {"value": bat["value"], "speedup_vs_sequential":
-                            bat["speedup_vs_sequential"],
-                            "bit_identical_to_sequential": bat["bit_identical_to_sequential"]}
-    if amr_out is not None:
-        summary["amr_blocks_fp64"] = {
-            k: {"value": v["value"], "library_value": v["library_value"], "n_blk": v["n_blk"]}
-            for k, v in amr_out.items() if k.startswith("blocks_")}
-    if ann is not None:
-        summary["fp64_annulus_4000x400"] = {"value": ann["value"],
-                                            "ms_per_step": ann["ms_per_step"]}
-    if cpu is not None:
-        summary["cpu_baseline"] = cpu
-        summary["speedup_vs_cpu"] = mixed["value"] / cpu["value"]
-    print(json.dumps(summary), flush=True)
+    print(json.dumps(build_summary(args.steps, args.warmup, U3.shape[1], U3.shape[2], mixed,
+                                   fp64, roofline, roof_valu, chem_fv, acc, crop, adj, bat,
+                                   amr_out, ann, cpu)), flush=True)
 
 
 if __name__ == "__main__":
