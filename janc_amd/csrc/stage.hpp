@@ -1245,10 +1245,6 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
   __syncthreads();
   const int n = a.nx * a.ny;
   const double dt = step_dt(a);
-  if (MODE == CHEM_STEP && blk == 0 && threadIdx.x == 0) {
-    a.sc->dt_last = dt;
-    if (dt_hist) *dt_hist = dt;
-  }
   constexpr bool grid = GRID;
   double mx = 0.0, my = 0.0;
   unsigned long long cm = 0ull;
@@ -1343,6 +1339,14 @@ __device__ __forceinline__ void chem_body(const StepArgs<Real>& a, Real* U, Real
       block_max_atomic<256>(mx, &a.sc->amax[o][0][0], red);
       block_max_atomic<256>(my, &a.sc->amax[o][0][1], red);
       block_max_bits_atomic<256>(cm, &a.sc->cmax[o], redb);
+    }
+    // the step's dt, stored last: a global store ahead of the rates would
+    // make the compiler treat every mechanism constant read after it as
+    // possibly clobbered, and fetch them with vector loads into VGPRs
+    // instead of scalar loads
+    if (blk == 0 && threadIdx.x == 0) {
+      a.sc->dt_last = dt;
+      if (dt_hist) *dt_hist = dt;
     }
   }
 }
