@@ -139,6 +139,7 @@ StepArgs<Real> make_args(jh_ctx* c, double dt_fixed, double cfl) {
     a.bc[s] = c->bc[s];
     a.bc_p[s] = c->bc_p[s];
   }
+  a.mirror = 0;
   a.inj_T0 = c->inj_T0;
   for (int k = 0; k < JH_NS - 1; ++k) a.inj_Y[k] = c->inj_Y[k];
   const bool grid = c->halley == JH_HALLEY_GRID;
@@ -239,6 +240,13 @@ void launch_split_stage(jh_ctx* c, const StepArgs<Real>& a, int stage, const Rea
   switch (c->tile) {
 #define JH_LAUNCH(V, TX, WPE)                                                          \
     case V:                                                                            \
+      if constexpr (MODE == MODE_RK && RECON == RECON_WENO5) {                         \
+        if (a.mirror) {                                                                \
+          stage_kernel<Real, MODE, TX, WPE, RECON, SPLIT_LF, Real, false, true>        \
+              <<<nb, Tile<TX>::NT, 0, c->stream>>>(a, stage, Us, As, Un, Uo, Ao);      \
+          break;                                                                       \
+        }                                                                              \
+      }                                                                                \
       stage_kernel<Real, MODE, TX, WPE, RECON><<<nb, Tile<TX>::NT, 0, c->stream>>>(   \
           a, stage, Us, As, Un, Uo, Ao);                                               \
       break;
@@ -326,6 +334,23 @@ void trace_stage(jh_ctx* c, const StepArgs<Real>& a, const Real* Us, const Real*
   trace_store_kernel<<<1, 64, 0, c->stream>>>(c->trs, rec);
 }
 
+// Whether the RK stages can read their halo cells from the source cells
+// (StepArgs::mirror) instead of ghost strips filled by bc_kernel: every side
+// a copy kind, the LF-split WENO5 stage kernel, each cell stopping its own
+// Halley solve, no tracing (whose kernels read the strips).
+bool mirror_halo(const jh_ctx* c) {
+  if (c->solver == JH_SOLVER_GODUNOV || c->split == JH_SPLIT_SW || c->recon != JH_RECON_WENO5_JS ||
+      c->halley == JH_HALLEY_GRID || c->trace)
+    return false;
+  for (int s = 0; s < 4; ++s) {
+    const int k = c->bc[s];
+    if (k != JH_BC_PERIODIC && k != JH_BC_ZERO_GRADIENT && k != JH_BC_SLIP_WALL &&
+        k != JH_BC_NO_SLIP_WALL)
+      return false;
+  }
+  return true;
+}
+
 template <typename Real>
 int advance_impl(jh_ctx* c, const Real* U, const Real* aux, Real* Uo, Real* auxo, double dt,
                  double cfl, int nsteps, double* dt_hist) {
@@ -339,22 +364,24 @@ int advance_impl(jh_ctx* c, const Real* U, const Real* aux, Real* Uo, Real* auxo
   const Real* Un = U;
   const Real* An = aux;
   const bool timing = c->timing;
+  const bool mir = mirror_halo(c);
   for (int s = 0; s < nsteps; ++s) {
     // with a sampling period, only every k-th step's launches are timed
     c->timing = timing && (c->timing_every <= 1 || c->timed_steps % c->timing_every == 0);
     ++c->timed_steps;
     auto a = make_args<Real>(c, dt, cfl);
+    a.mirror = mir ? 1 : 0;
     const int bb = bc_blocks(c);
     long long* rec = (c->trace && c->trace_step < c->trace_cap)
                          ? c->trace + (size_t)kTraceRecord * c->trace_step++ : nullptr;
     // stage 1: (Un, An) -> (U1, A1)
-    { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, Un, An, 0); }
+    if (!mir) { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, Un, An, 0); }
     if (rec) trace_stage<Real>(c, a, Un, An, rec);
     { Timed t(c, JH_KERNEL_STAGE);
       launch_stage<Real, MODE_RK>(c, a, 0, Un, An, Un, U1, A1);
       halley_fix<Real>(c, 0, c->parity, 1, 0, A1); }
     // stage 2: (U1, A1) -> (U2, A2)
-    { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, U1, A1, 1); }
+    if (!mir) { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, U1, A1, 1); }
     if (rec) {
       trace_range<Real>(c, a, A1, rec + 14);
       trace_stage<Real>(c, a, U1, A1, rec + 4);
@@ -364,7 +391,7 @@ int advance_impl(jh_ctx* c, const Real* U, const Real* aux, Real* Uo, Real* auxo
       halley_fix<Real>(c, 1, c->parity, 2, 0, A2); }
     // stage 3: (U2, A2) -> (Uo, auxo); Uo may alias Un (each thread reads
     // Un at its own cell before writing it)
-    { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, U2, A2, 2); }
+    if (!mir) { Timed t(c, JH_KERNEL_BC); bc_kernel<Real><<<bb, kBcThreads, 0, c->stream>>>(a, U2, A2, 2); }
     if (rec) {
       trace_range<Real>(c, a, A2, rec + 16);
       trace_stage<Real>(c, a, U2, A2, rec + 8);

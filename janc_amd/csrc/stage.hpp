@@ -232,6 +232,9 @@ struct StepArgs {
   const MechConst* __restrict__ mc;
   StepScalars* sc;
   Real* g[4];                 // ghost strips L, R, B, T (NV planes each)
+  // host side: 1 selects the stage kernel that reads each halo cell from
+  // its source cell (locate_src) instead of the strips (every side a copy kind)
+  int mirror;
   int bc[4];
   double bc_p[4];             // pressure_outlet back pressure / inlet P0 per side
   double inj_T0;              // injection inlet: manifold total temperature
@@ -789,6 +792,56 @@ __device__ __forceinline__ CellRef<Real> locate(const StepArgs<Real>& a, const R
 // A cell of a block that carries its own 3 ghost layers (the AMR branch's
 // blocks, (nx + 6) x (ny + 6) planes, interior at offset (3, 3)): cells
 // within the ghost layers are read in place, cells beyond them do not exist.
+// A halo cell of a copy-kind side (periodic with the reference's
+// off-by-one, slip wall, zero gradient, no-slip wall) read from the interior
+// cell bc_body copies into its ghost slot: the source cell of ghost layer gi
+// (gi = 0 outermost on the low side, innermost on the high side) and, in
+// flip, the momentum components the ghost negates (bit 0: rho u, bit 1:
+// rho v).  Negation is exact, so the values equal the strip's.
+__device__ __forceinline__ int ghost_src(int kind, bool low, int N, int gi) {
+  if (kind == JH_BC_PERIODIC) return low ? N - 4 + gi : 1 + gi;
+  if (kind == JH_BC_SLIP_WALL) return low ? 0 : N - 1;
+  return low ? 2 - gi : N - 1 - gi;   // zero gradient, no-slip wall
+}
+
+__device__ __forceinline__ unsigned int ghost_flip(int kind, bool xside) {
+  if (kind == JH_BC_SLIP_WALL) return xside ? 1u : 2u;
+  return kind == JH_BC_NO_SLIP_WALL ? 3u : 0u;
+}
+
+template <typename Real>
+__device__ __forceinline__ CellRef<Real> locate_src(const StepArgs<Real>& a,
+                                                    const Real* __restrict__ U,
+                                                    const Real* __restrict__ aux, int gx, int gy,
+                                                    unsigned int& flip) {
+  const int nx = a.nx, ny = a.ny;
+  const bool inx = gx >= 0 && gx < nx, iny = gy >= 0 && gy < ny;
+  CellRef<Real> r;
+  r.ok = (inx || (gx >= -3 && gx < nx + 3)) && (iny || (gy >= -3 && gy < ny + 3)) &&
+         (inx || iny);
+  r.stride = nx * ny;
+  flip = 0u;
+  int sx = gx, sy = gy;
+  if (!r.ok) {
+    sx = 0;
+    sy = 0;
+  } else if (!inx) {
+    const bool low = gx < 0;
+    const int kind = low ? a.bc[JH_LEFT] : a.bc[JH_RIGHT];
+    sx = ghost_src(kind, low, nx, low ? gx + 3 : gx - nx);
+    flip = ghost_flip(kind, true);
+  } else if (!iny) {
+    const bool low = gy < 0;
+    const int kind = low ? a.bc[JH_BOTTOM] : a.bc[JH_TOP];
+    sy = ghost_src(kind, low, ny, low ? gy + 3 : gy - ny);
+    flip = ghost_flip(kind, false);
+  }
+  const int o = sx * ny + sy;
+  r.u = U + o;
+  r.x = aux + o;
+  return r;
+}
+
 template <typename Real>
 __device__ __forceinline__ CellRef<Real> locate_padded(const StepArgs<Real>& a,
                                                        const Real* __restrict__ U,
@@ -899,7 +952,7 @@ struct StageLds {
 // AMR block with its own ghost layers ((nx + 6) x (ny + 6), no ghost strips,
 // no next-stage maxima: the next stage's ghosts come from the caller).
 template <typename Real, int MODE, int TX, int RECON = RECON_WENO5, int SPLIT = SPLIT_LF,
-          typename W = Real, bool GRID = false, bool PAD = false>
+          typename W = Real, bool GRID = false, bool PAD = false, bool MIRROR = false>
 __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
                                            const Real* __restrict__ Us,
                                            const Real* __restrict__ auxs, const Real* Un,
@@ -928,6 +981,11 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
   // runs derived from it are wave-uniform to the compiler (scalar registers
   // and branches)
   const int t = threadIdx.x, ly = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
+  if constexpr (MIRROR) {
+    // with the halo read from the source cells no bc_kernel runs, so the
+    // first stage zeroes the other parity's slots in its place
+    if (stage == 0 && tile == 0) zero_slots(a.sc, a.parity ^ 1);
+  }
   const int tiles_y = (ny + TY - 1) / TY;
   const int x0 = (tile / tiles_y) * TX;
   const int y0 = (tile % tiles_y) * TY;
@@ -969,13 +1027,21 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
   CellRef<Real> ref[NSL];
   Prim<W> q[NSL];
   Real nxt[NSL];
+  // bits 2s, 2s + 1: slot s is a ghost whose rho u, rho v its side negates
+  // (halo read from the source cells)
+  unsigned int flips = 0u;
 #pragma unroll
   for (int s = 0; s < NSL; ++s) {
-    ref[s] = PAD ? locate_padded<Real>(a, Us, auxs, gxs[s], gys[s])
-                 : locate<Real>(a, Us, auxs, gxs[s], gys[s]);
+    unsigned int f = 0u;
+    if constexpr (PAD) ref[s] = locate_padded<Real>(a, Us, auxs, gxs[s], gys[s]);
+    else if constexpr (MIRROR) ref[s] = locate_src<Real>(a, Us, auxs, gxs[s], gys[s], f);
+    else ref[s] = locate<Real>(a, Us, auxs, gxs[s], gys[s]);
     if (!has[s]) ref[s].ok = false;
     if (ref[s].ok) {
       q[s] = cell_prim<W, SPLIT == SPLIT_SW, Real>(c, ref[s].u, ref[s].x, ref[s].stride);
+      if (f & 1u) q[s].u = -q[s].u;
+      if (f & 2u) q[s].v = -q[s].v;
+      flips |= f << (2 * s);
       nxt[s] = ref[s].u[0];
     } else {
       q[s].rho = (W)1; q[s].u = (W)0; q[s].v = (W)0; q[s].p = (W)0; q[s].a = (W)0;
@@ -1002,7 +1068,10 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
 #pragma unroll
     for (int s = 0; s < NSL; ++s) {
       if (!has[s]) continue;
-      const W ucv = (W)nxt[s];
+      W ucv = (W)nxt[s];
+      if constexpr (CLS == 1 || CLS == 2) {
+        if ((flips >> (2 * s + CLS - 1)) & 1u) ucv = -ucv;
+      }
       if (s < 2) uc_own[s] = ucv;
       const bool xreg = s < NSL - 1, yreg = (s < 2) || (s == NSL - 1);
       if (xreg) {
@@ -1205,13 +1274,13 @@ __device__ __forceinline__ void stage_body(const StepArgs<Real>& a, int stage,
 }
 
 template <typename Real, int MODE, int TX, int WPE, int RECON, int SPLIT = SPLIT_LF,
-          typename W = Real, bool GRID = false>
+          typename W = Real, bool GRID = false, bool MIRROR = false>
 __global__ __launch_bounds__(Tile<TX>::NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 void stage_kernel(StepArgs<Real> a, int stage, const Real* __restrict__ Us,
                   const Real* __restrict__ auxs, const Real* Un, Real* Uo,
                   Real* __restrict__ auxo) {
   __shared__ StageLds<W, TX> L;
-  stage_body<Real, MODE, TX, RECON, SPLIT, W, GRID>(a, stage, Us, auxs, Un, Uo, auxo,
+  stage_body<Real, MODE, TX, RECON, SPLIT, W, GRID, false, MIRROR>(a, stage, Us, auxs, Un, Uo, auxo,
                                                     xcd_tile(blockIdx.x, gridDim.x), L);
 }
 

@@ -41,13 +41,20 @@ def _case(name, mech):
                                 "bottom_boundary": "zero_gradient",
                                 "top_boundary": "pressure_outlet"}
         theta = {"Pb": 1.5 * 101325.0}
+    elif name == "walls":   # copy kinds only, both wall kinds
+        U, aux, g = cases.detonation_tube(mech, nx=33, ny=41, driver=6, seed=3)
+        g["boundary_config"] = {"left_boundary": "no_slip_wall",
+                                "right_boundary": "zero_gradient",
+                                "bottom_boundary": "slip_wall",
+                                "top_boundary": "no_slip_wall"}
+        theta = None
     else:   # annulus crop: periodic x, injection inlet below (open and blocked
         # cells), pressure outlet above
         U, aux, g, theta = cases.rdc_annulus(mech, nx=61, ny=37, seed_cols=12)
     return U, aux, g, theta
 
 
-CASES = ["box", "tube", "outlet", "inlet"]
+CASES = ["box", "tube", "outlet", "inlet", "walls"]
 
 
 def make_oracle(mech_path, g, theta, flux_config=None):
@@ -153,6 +160,33 @@ def test_fused_steps_equal_single_steps(mech_path, mech):
     _, _, dts = sim.advance(Ub, ab, 3)
     assert torch.equal(Ua, Ub) and torch.equal(aa, ab)
     assert dts.shape == (3,) and bool(torch.all(dts > 0))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", ["box", "tube", "walls"])
+def test_halo_from_source_cells_equals_strips(mech_path, mech, name, precision):
+    """With every side a copy kind the fused step's stage kernels read their
+    halo from the source cells and no bc_kernel runs; tracing keeps the
+    ghost-strip path.  Both give the same state bit for bit."""
+    sim, o, U, aux, theta = _pair(mech_path, mech, name, precision)
+    dt = torch.float64 if precision == "fp64" else torch.float32
+    U0 = torch.as_tensor(U).cuda().to(dt)
+    a0 = torch.as_tensor(aux).cuda().to(dt)
+    Ua, aa = U0.clone(), a0.clone()
+    sim.timing(Ua, enable=1)
+    sim.advance(Ua, aa, 3)
+    kt = sim.timing(Ua)
+    sim.timing(Ua, enable=False)
+    assert kt["bc"][1] == 0 and kt["stage"][1] == 9
+    Ub, ab = U0.clone(), a0.clone()
+    sim.trace(Ub, torch.zeros((3, 22), dtype=torch.int64, device="cuda"))
+    sim.timing(Ub, enable=1)
+    sim.advance(Ub, ab, 3)
+    kt = sim.timing(Ub)
+    sim.timing(Ub, enable=False)
+    sim.trace(Ub, None)
+    assert kt["bc"][1] == 9
+    assert torch.equal(Ua, Ub) and torch.equal(aa, ab)
 
 
 MIXED_TOL = {"flux_func": 1e-6, "update_aux": 1e-6, "reaction": 1e-6, "step": 1e-5}
