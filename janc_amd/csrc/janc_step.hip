@@ -623,9 +623,10 @@ int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, cons
   if (rc) return rc;
   const int P = c->parity;
   auto a = make_args<double>(c, dt, 0.0);
+  a.mirror = mirror_halo(c) ? 1 : 0;
   const int bb = bc_blocks(c);
   for (int s = 0; s < 3; ++s) {
-    bc_kernel<double><<<bb, kBcThreads, 0, st>>>(a, w.U[s], w.A[s], s);
+    if (!a.mirror) bc_kernel<double><<<bb, kBcThreads, 0, st>>>(a, w.U[s], w.A[s], s);
     launch_stage<double, MODE_RK>(c, a, s, w.U[s], w.A[s], U0, w.U[s + 1], w.A[s + 1]);
     halley_fix<double>(c, s, P, s < 2 ? s + 1 : -1, 0, w.A[s + 1]);
   }
