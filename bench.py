@@ -42,7 +42,7 @@ WORDS = {"stage": (27 + 39 + 39) / 3.0, "chem": 23.0, "reduce": 13.0, "bc": 0.0}
 # annulus step (tile variant 3), mixed = the 8000 x 500 tube (variant 1).
 # Reads exceed the algorithmic words: each region cell's state is read for
 # its primitives and again per variable.
-STAGE_TRAFFIC_B_PER_CELL = {"fp64": 459.8, "mixed": 211.9}
+STAGE_TRAFFIC_B_PER_CELL = {"fp64": 459.8, "mixed": 210.4}
 
 # Measured FP64 FLOP per cell per chemistry launch (rocprofv3
 # SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes / cells, FMA = 2; summary under
@@ -55,7 +55,7 @@ CHEM_FP64_FLOP_PER_CELL = {"mixed": 2760.2, "fp64": 3249.6}
 # profiles/).  The VALU issue floor of a launch is instructions / 64 wave
 # instructions x 4 cycles (64 lanes on a 16-lane SIMD; FP32 without packed
 # math issues at the FP64 rate) / (1024 SIMDs x 2.4 GHz).
-STAGE_VALU_PER_CELL = {"mixed": 3231.0}
+STAGE_VALU_PER_CELL = {"mixed": 3209.0}
 SIMDS = 1024
 CLOCK_GHZ = 2.4
 

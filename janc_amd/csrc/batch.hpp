@@ -90,7 +90,9 @@ __global__ __launch_bounds__(kBcThreads) void bc_batch_kernel(const CaseDesc<Rea
 // through xcd_tile so each XCD (blocks are dealt to the 8 XCDs round-robin,
 // and j = blockIdx.x + i*gridDim.x keeps j's XCD) walks a contiguous run of
 // the list: whole cases or long runs of one case's x-adjacent tiles.
-template <typename Real, int TX, int WPE>
+// MIRROR: every case's sides are copy kinds and the halo is read from the
+// source cells (no ghost-strip launches).
+template <typename Real, int TX, int WPE, bool MIRROR = false>
 __global__ __launch_bounds__(Tile<TX>::NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 void stage_batch_kernel(const CaseDesc<Real>* __restrict__ d, const int* __restrict__ off,
                         int ncase, int stage, int parity, double dt_fixed, double cfl) {
@@ -105,7 +107,8 @@ void stage_batch_kernel(const CaseDesc<Real>* __restrict__ d, const int* __restr
     const Real* As = stage == 0 ? cd.aux : stage == 1 ? cd.A1 : cd.A2;
     Real* Uo = stage == 0 ? cd.U1 : stage == 1 ? cd.U2 : cd.U;
     Real* Ao = stage == 0 ? cd.A1 : stage == 1 ? cd.A2 : cd.aux;
-    stage_body<Real, MODE_RK, TX>(a, stage, Us, As, cd.U, Uo, Ao, t - off[k], L);
+    stage_body<Real, MODE_RK, TX, RECON_WENO5, SPLIT_LF, Real, false, false, MIRROR>(
+        a, stage, Us, As, cd.U, Uo, Ao, t - off[k], L);
     __syncthreads();   // LDS is reused by the next job
   }
 }

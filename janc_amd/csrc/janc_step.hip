@@ -695,6 +695,7 @@ struct jh_batch {
   int ncase = 0;
   int precision = JH_PREC_FP64;
   std::vector<int> nx, ny, bc;        // bc: 4 per case
+  bool mirror = false;                // every side of every case a copy kind
   std::vector<double> dx, dy, bcp;    // bcp: 4 per case
   std::vector<double> inj;            // per case: T0, Y_inj[NS-1]
   hipStream_t user = nullptr;         // the caller's stream
@@ -828,8 +829,12 @@ void launch_stage_batch(jh_batch* b, hipStream_t st, int stage, int parity, doub
   switch (v) {
 #define JH_LAUNCH(V, TX, WPE)                                                           \
     case V:                                                                             \
-      stage_batch_kernel<Real, TX, WPE><<<g, Tile<TX>::NT, 0, st>>>(d, off, b->ncase, stage, \
-                                                                   parity, dt, cfl);     \
+      if (b->mirror)                                                                    \
+        stage_batch_kernel<Real, TX, WPE, true><<<g, Tile<TX>::NT, 0, st>>>(            \
+            d, off, b->ncase, stage, parity, dt, cfl);                                  \
+      else                                                                              \
+        stage_batch_kernel<Real, TX, WPE><<<g, Tile<TX>::NT, 0, st>>>(                  \
+            d, off, b->ncase, stage, parity, dt, cfl);                                  \
       break;
     JH_TILE_VARIANTS(JH_LAUNCH)
 #undef JH_LAUNCH
@@ -851,8 +856,11 @@ int batch_enqueue(jh_batch* b, hipStream_t st, double dt, double cfl, int nsteps
   for (int s = 0; s < nsteps; ++s) {
     const int parity = s & 1;
     for (int stage = 0; stage < 3; ++stage) {
-      bc_batch_kernel<Real><<<n_bc, kBcThreads, 0, st>>>(d, b->off + b->t_bc, nc, stage, parity, dt,
-                                                  cfl);
+      // copy-kind sides only: the stage kernel reads the halo from the
+      // source cells and zeroes the other parity's slots at stage 0
+      if (!b->mirror)
+        bc_batch_kernel<Real><<<n_bc, kBcThreads, 0, st>>>(d, b->off + b->t_bc, nc, stage, parity,
+                                                           dt, cfl);
       launch_stage_batch<Real>(b, st, stage, parity, dt, cfl);
     }
     chem_batch_kernel<Real><<<n_chem, 256, 0, st>>>(d, b->off + b->t_chem, nc, parity, dt, cfl,
@@ -1486,6 +1494,11 @@ int jh_batch_create(int ncase, const int* nx, const int* ny, const double* dx, c
   b->dx.assign(dx, dx + ncase);
   b->dy.assign(dy, dy + ncase);
   b->bc.assign(bc_kind, bc_kind + 4 * ncase);
+  b->mirror = true;
+  for (int k : b->bc)
+    if (k != JH_BC_PERIODIC && k != JH_BC_ZERO_GRADIENT && k != JH_BC_SLIP_WALL &&
+        k != JH_BC_NO_SLIP_WALL)
+      b->mirror = false;
   b->bcp.assign(4 * ncase, 0.0);
   if (bc_param) b->bcp.assign(bc_param, bc_param + 4 * ncase);
   b->U.assign(ncase, nullptr);

@@ -22,7 +22,7 @@ def mech(mech_path):
     return Mechanism(mech_path)
 
 
-def _cases(mech, mech_path, precision, dt=None):
+def _cases(mech, mech_path, precision, dt=None, copy_only=False):
     out = []
     sizes = [(37, 20), (64, 40), (29, 67), (128, 32), (20, 70), (45, 22)]
     for k, (nx, ny) in enumerate(sizes):
@@ -33,7 +33,12 @@ def _cases(mech, mech_path, precision, dt=None):
             U, aux, g = cases.detonation_tube(mech, nx=nx, ny=ny, driver=5, seed=k,
                                               phi=0.8 + 0.1 * k, T_drv=2500.0 + 100 * k)
             theta = None
-        if k == 2:
+        if k == 2 and copy_only:   # walls of both kinds, no computed ghosts
+            g["boundary_config"] = {"left_boundary": "no_slip_wall",
+                                    "right_boundary": "zero_gradient",
+                                    "bottom_boundary": "zero_gradient",
+                                    "top_boundary": "slip_wall"}
+        elif k == 2:
             g["boundary_config"] = {"left_boundary": "no_slip_wall",
                                     "right_boundary": "pressure_outlet",
                                     "bottom_boundary": "zero_gradient",
@@ -69,6 +74,30 @@ def test_batch_matches_single_cases(mech_path, mech, precision, dt, graph):
         sim = Simulator(cfg)
         Ur, Ar = _dev(U, dtype), _dev(aux, dtype)
         _, _, h1 = sim.advance(Ur, Ar, nsteps)
+        assert torch.equal(Us[k], Ur), "case %d U" % k
+        assert torch.equal(As[k], Ar), "case %d aux" % k
+        assert torch.equal(hist[k], h1), "case %d dt" % k
+    assert b.health() == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_of_copy_kinds_matches_single_cases(mech_path, mech, precision):
+    """Every side of every case a copy kind: the batched stage kernel reads
+    the halo from the source cells and no ghost-strip kernel runs; still
+    bit-identical to the single cases."""
+    from janc_amd.batch import BatchSimulator
+    from janc_amd.solver import Simulator
+    cs = _cases(mech, mech_path, precision, copy_only=True)
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    Us = [_dev(U, dtype) for _, U, _ in cs]
+    As = [_dev(a, dtype) for _, _, a in cs]
+    print("batch copy kinds", precision, flush=True)
+    b = BatchSimulator([c for c, _, _ in cs], Us, As)
+    hist = b.advance(4, graph=True, dt_hist=True)
+    torch.cuda.synchronize()
+    for k, (cfg, U, aux) in enumerate(cs):
+        Ur, Ar = _dev(U, dtype), _dev(aux, dtype)
+        _, _, h1 = Simulator(cfg).advance(Ur, Ar, 4)
         assert torch.equal(Us[k], Ur), "case %d U" % k
         assert torch.equal(As[k], Ar), "case %d aux" % k
         assert torch.equal(hist[k], h1), "case %d dt" % k
