@@ -210,6 +210,21 @@ int jh_step_vjp(jh_ctx* ctx, const void* U, const void* aux, double dt, const vo
 int jh_step_vjp_known(jh_ctx* ctx, const void* U, const void* aux, double dt,
                       const void* U_next, const void* aux_next, const void* U_bar,
                       const void* aux_bar, void* U_in_bar, void* aux_in_bar);
+/* The step's residuals for reverse mode, as jax.vjp of advance_one_step
+ * (simulation/set_simulator.py:117-122) keeps them: jh_step_keep runs one
+ * FP64 step (output to U_next / aux_next) and writes into res the three RK
+ * stage states and the stages' LF maxima; jh_step_vjp_res is then the step's
+ * VJP without recomputing any of it (U, aux: the step's input; U_next,
+ * aux_next: its output; cotangent arguments as jh_step_vjp).  res holds
+ * jh_step_res_count() doubles (3 x 14 planes of nx x ny, then 6) and is read-only to
+ * jh_step_vjp_res; the results equal jh_step_vjp_known's bit for bit.  The
+ * same conditions as jh_step_vjp. */
+int jh_step_res_count(jh_ctx* ctx, long long* count);
+int jh_step_keep(jh_ctx* ctx, const void* U, const void* aux, double dt, void* U_next,
+                 void* aux_next, void* res);
+int jh_step_vjp_res(jh_ctx* ctx, const void* U, const void* aux, double dt, const void* res,
+                    const void* U_next, const void* aux_next, const void* U_bar,
+                    const void* aux_bar, void* U_in_bar, void* aux_in_bar);
 
 /* Device pointer and element count of one side's ghost strip (14 planes of
  * 3 x len: vars 0-11 = U, 12-13 = aux; left/right strips are (14, 3, ny) with

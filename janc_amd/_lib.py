@@ -63,6 +63,9 @@ SIGNATURES = {
     "jh_lf_alpha": (_c_int, [_vp, _vp, _vp, _vp]),
     "jh_step_vjp": (_c_int, [_vp, _vp, _vp, _c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jh_step_vjp_known": (_c_int, [_vp, _vp, _vp, _c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "jh_step_res_count": (_c_int, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
+    "jh_step_keep": (_c_int, [_vp, _vp, _vp, _c_double, _vp, _vp, _vp]),
+    "jh_step_vjp_res": (_c_int, [_vp, _vp, _vp, _c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jh_ghost_strip": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_int)]),
     "jh_set_ghost_strip": (_c_int, [_vp, _c_int, _vp]),
     "jh_step_begin": (_c_int, [_vp, _vp, _vp, _c_double, _c_double]),

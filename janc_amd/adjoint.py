@@ -1,13 +1,13 @@
 """Differentiable steps: torch.autograd wrappers around jh_advance / jh_step_vjp.
 
 ``step(sim, U, aux, dt)`` is one fixed-dt step as a ``torch.autograd.Function``
-(forward = the FP64 step kernels, backward = the hand-written adjoint
-kernels).  ``rollout(sim, U, aux, dt, nsteps, checkpoint_every)`` runs many
+(forward = the FP64 step kernels keeping the step's residuals, the RK stage
+states and their LF maxima, ``jh_step_keep``; backward = the hand-written
+adjoint kernels from those residuals, ``jh_step_vjp_res``, which recompute
+nothing).  ``rollout(sim, U, aux, dt, nsteps, checkpoint_every)`` runs many
 steps keeping only every ``checkpoint_every``-th state; its backward replays
-each segment from its checkpoint (the RK stages of every step are recomputed
-inside the step VJP) and sweeps the step VJPs in reverse.  Each step's VJP is
-given the step's output, which the forward pass or the replay already holds,
-so the chemistry's forward pass is not repeated (``jh_step_vjp_known``).
+each segment from its checkpoint keeping every step's residuals (one
+segment's worth at a time) and sweeps the step VJPs in reverse.
 
 These replace JAX's reverse mode over the reference's ``advance_one_step``
 (set_simulator.py:117-122), the custom VJP of ``get_T_nasa7``
@@ -16,24 +16,26 @@ These replace JAX's reverse mode over the reference's ``advance_one_step``
 import torch
 
 
-def _vjp(sim, U, aux, dt, Ub, ab, out=None):
-    return sim.step_vjp(U, aux, dt, Ub.contiguous(), ab.contiguous(), known_output=out)
+def _vjp(sim, U, aux, dt, Ub, ab, out, res):
+    return sim.step_vjp(U, aux, dt, Ub.contiguous(), ab.contiguous(), known_output=out,
+                        residuals=res)
 
 
 class _Step(torch.autograd.Function):
     @staticmethod
     def forward(ctx, U, aux, sim, dt):
-        Uo, ao = sim.advance_one_step(U.detach(), aux.detach(), sim.dx, sim.dy, dt)
-        ctx.save_for_backward(U.detach(), aux.detach(), Uo.detach(), ao.detach())
+        U, aux = U.detach().contiguous(), aux.detach().contiguous()
+        Uo, ao, res = sim.step_keep(U, aux, dt)
+        ctx.save_for_backward(U, aux, Uo, ao, res)
         ctx.sim, ctx.dt = sim, dt
         return Uo, ao
 
     @staticmethod
     def backward(ctx, Ub, ab):
-        U, aux, Uo, ao = ctx.saved_tensors
+        U, aux, Uo, ao, res = ctx.saved_tensors
         Ub = torch.zeros_like(U) if Ub is None else Ub
         ab = torch.zeros_like(aux) if ab is None else ab
-        gU, ga = _vjp(ctx.sim, U, aux, ctx.dt, Ub, ab, out=(Uo, ao))
+        gU, ga = _vjp(ctx.sim, U, aux, ctx.dt, Ub, ab, (Uo, ao), res)
         return gU, ga, None, None
 
 
@@ -61,19 +63,18 @@ class _Rollout(torch.autograd.Function):
         sim, dt, nsteps, every = ctx.sim, ctx.dt, ctx.nsteps, ctx.every
         Ub = Ub.contiguous().clone()
         ab = (torch.zeros_like(ctx.ckpt[0][1]) if ab is None else ab).contiguous().clone()
-        nxt = ctx.final                            # output of the segment's last step
         for seg in reversed(range(len(ctx.ckpt))):
             U, aux = ctx.ckpt[seg]
             first = seg * every
             last = min(first + every, nsteps)
-            states = [(U, aux)]
-            for _ in range(first, last - 1):       # replay the segment
-                U, aux = sim.advance_one_step(U, aux, sim.dx, sim.dy, dt)
-                states.append((U, aux))
-            outs = states[1:] + [nxt]
-            for (Us, As), out in zip(reversed(states), reversed(outs)):
-                Ub, ab = _vjp(sim, Us, As, dt, Ub, ab, out=out)
-            nxt = ctx.ckpt[seg]
+            steps = []
+            for _ in range(first, last):           # replay the segment, keeping residuals
+                Un, an, res = sim.step_keep(U, aux, dt)
+                steps.append((U, aux, Un, an, res))
+                U, aux = Un, an
+            for Us, As, Un, an, res in reversed(steps):
+                Ub, ab = _vjp(sim, Us, As, dt, Ub, ab, (Un, an), res)
+            steps = None
         ctx.ckpt = ctx.final = None
         return Ub, ab, None, None, None, None
 

@@ -601,23 +601,16 @@ void flux_vjp(jh_ctx* c, const AdjArgs& a, AdjWork& w, const double* Us, const d
                                                                     m, w.Ubp, w.abp, Ut, At);
 }
 
-// Uk/Ak: the step's output when the caller already has it (the RK stages are
-// still recomputed; only the chemistry's forward pass, whose output the
-// reverse sweep starts from, is skipped).
-int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, const double* Ubar,
-                  const double* Abar, double* Ub0, double* Ab0, double* Uo, double* Ao,
-                  const double* Uk = nullptr, const double* Ak = nullptr) {
-  AdjWork w;
-  int rc = adj_work(c, U0, A0, w);
-  if (rc) return rc;
-  // the sweep accumulates straight into the outputs: the input cotangents
-  // are read by the first kernel (getT_vjp_kernel) only, before either
-  // output is written, so the outputs may alias them
-  w.Uacc = static_cast<double*>(Ub0);
-  w.ab = static_cast<double*>(Ab0);
+// Forward pass of one step keeping every RK stage: w.U[1..3], w.A[1..3] the
+// stage states (the third before the chemistry), w.U[4], w.A[4] the output
+// (a copy of stage 3 put through the chemistry), or the caller's output
+// Uk/Ak, whose chemistry is then not repeated.  The stages' LF maxima are
+// left in c->sc's slots of the parity the step ran with (returned in *P).
+int vjp_forward(jh_ctx* c, AdjWork& w, const double* U0, const double* A0, double dt,
+                const double* Uk, const double* Ak, int* Pout) {
   const size_t n = (size_t)c->nx * c->ny;
-  const int bc1 = blocks_for(n);
   hipStream_t st = c->stream;
+  int rc;
   // ---- forward recompute, keeping every stage ----
   rc = prologue<double>(c, U0, A0, dt, 0.0);
   if (rc) return rc;
@@ -644,13 +637,21 @@ int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, cons
   rc = check_launch(c, "vjp forward");
   if (rc) return rc;
   c->parity ^= 1;
-  if (Uo) HIP_TRY(c, hipMemcpyAsync(Uo, w.U[4], NU * n * sizeof(double), hipMemcpyDeviceToDevice, st));
-  if (Ao) HIP_TRY(c, hipMemcpyAsync(Ao, w.A[4], 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  *Pout = P;
+  return JH_OK;
+}
+
+// Reverse sweep of one step from the forward's states in w and the stages'
+// LF maxima alpha[s] (2 bit patterns each); accumulates into w.Uacc, w.ab.
+int vjp_reverse(jh_ctx* c, AdjWork& w, double dt, const unsigned long long* const alpha[3],
+                const double* Ubar, const double* Abar) {
+  const size_t n = (size_t)c->nx * c->ny;
+  const int bc1 = blocks_for(n);
+  hipStream_t st = c->stream;
+  int rc;
   const AdjArgs aa = adj_args(c, dt);
   rc = adj_fold(c, aa, w);
   if (rc) return rc;
-  const unsigned long long* alpha[3] = {&c->sc->amax[P][0][0], &c->sc->amax[P][1][0],
-                                        &c->sc->amax[P][2][0]};
   // ---- reverse sweep ----
   // final update_aux and chemistry: Ub = U4_bar -> U3_bar
   getT_vjp_kernel<<<bc1, 256, 0, st>>>(aa, w.U[4], w.A[4], Abar, Ubar, w.Ub);
@@ -673,6 +674,79 @@ int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, cons
                                          nullptr, w.ab);
   flux_vjp(c, aa, w, w.U[0], w.A[0], w.Ub, alpha[0], w.Uacc, w.ab);
   return check_launch(c, "vjp kernels");
+}
+
+// Uk/Ak: the step's output when the caller already has it (the RK stages are
+// still recomputed; only the chemistry's forward pass, whose output the
+// reverse sweep starts from, is skipped).
+int step_vjp_impl(jh_ctx* c, const double* U0, const double* A0, double dt, const double* Ubar,
+                  const double* Abar, double* Ub0, double* Ab0, double* Uo, double* Ao,
+                  const double* Uk = nullptr, const double* Ak = nullptr) {
+  AdjWork w;
+  int rc = adj_work(c, U0, A0, w);
+  if (rc) return rc;
+  // the sweep accumulates straight into the outputs: the input cotangents
+  // are read by the first kernel (getT_vjp_kernel) only, before either
+  // output is written, so the outputs may alias them
+  w.Uacc = static_cast<double*>(Ub0);
+  w.ab = static_cast<double*>(Ab0);
+  const size_t n = (size_t)c->nx * c->ny;
+  hipStream_t st = c->stream;
+  int P = 0;
+  rc = vjp_forward(c, w, U0, A0, dt, Uk, Ak, &P);
+  if (rc) return rc;
+  if (Uo) HIP_TRY(c, hipMemcpyAsync(Uo, w.U[4], NU * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (Ao) HIP_TRY(c, hipMemcpyAsync(Ao, w.A[4], 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  const unsigned long long* const alpha[3] = {&c->sc->amax[P][0][0], &c->sc->amax[P][1][0],
+                                              &c->sc->amax[P][2][0]};
+  return vjp_reverse(c, w, dt, alpha, Ubar, Abar);
+}
+
+// Residuals of one FP64 step (jh_step_keep, jh_step_vjp_res): the three RK
+// stage states, each NU + 2 planes (U then aux; the third before the
+// chemistry), then the three stages' LF maxima (x, y) as 6 bit patterns.
+size_t res_count(const jh_ctx* c) { return 3 * (size_t)(NU + 2) * c->nx * c->ny + 6; }
+
+void res_views(const jh_ctx* c, double* res, AdjWork& w, const unsigned long long** alpha) {
+  const size_t n = (size_t)c->nx * c->ny;
+  for (int s = 1; s <= 3; ++s) {
+    w.U[s] = res + (size_t)(s - 1) * (NU + 2) * n;
+    w.A[s] = w.U[s] + NU * n;
+  }
+  const auto* bits = reinterpret_cast<const unsigned long long*>(res + 3 * (NU + 2) * n);
+  for (int s = 0; s < 3; ++s) alpha[s] = bits + 2 * s;
+}
+
+int step_keep_impl(jh_ctx* c, const double* U0, const double* A0, double dt, double* Uo,
+                   double* Ao, double* res) {
+  AdjWork w{};
+  const unsigned long long* alpha[3];
+  res_views(c, res, w, alpha);
+  w.U[0] = const_cast<double*>(U0);
+  w.A[0] = const_cast<double*>(A0);
+  w.U[4] = Uo;
+  w.A[4] = Ao;
+  int P = 0;
+  int rc = vjp_forward(c, w, U0, A0, dt, nullptr, nullptr, &P);
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpyAsync(const_cast<unsigned long long*>(alpha[0]), &c->sc->amax[P][0][0],
+                            6 * sizeof(unsigned long long), hipMemcpyDeviceToDevice, c->stream));
+  return JH_OK;
+}
+
+int step_vjp_res_impl(jh_ctx* c, const double* U0, const double* A0, double dt,
+                      const double* res, const double* Uk, const double* Ak, const double* Ubar,
+                      const double* Abar, double* Ub0, double* Ab0) {
+  AdjWork w;
+  int rc = adj_work(c, U0, A0, w);
+  if (rc) return rc;
+  const unsigned long long* alpha[3];
+  res_views(c, const_cast<double*>(res), w, alpha);
+  w.U[4] = const_cast<double*>(Uk);
+  w.A[4] = const_cast<double*>(Ak);
+  w.Uacc = Ub0;
+  w.ab = Ab0;
+  return vjp_reverse(c, w, dt, alpha, Ubar, Abar);
 }
 
 }  // namespace
@@ -1341,6 +1415,35 @@ int jh_step_vjp_known(jh_ctx* c, const void* U, const void* aux, double dt, cons
   return step_vjp_impl(c, (const double*)U, (const double*)aux, dt, (const double*)U_bar,
                        (const double*)aux_bar, (double*)U_in_bar, (double*)aux_in_bar, nullptr,
                        nullptr, (const double*)U_next, (const double*)aux_next);
+}
+
+int jh_step_res_count(jh_ctx* c, long long* count) {
+  if (!c || !count) return JH_ERR_ARG;
+  *count = (long long)res_count(c);
+  return JH_OK;
+}
+
+int jh_step_keep(jh_ctx* c, const void* U, const void* aux, double dt, void* U_next,
+                 void* aux_next, void* res) {
+  if (!c) return JH_ERR_ARG;
+  if (!U_next || !aux_next || !res) return fail(c, JH_ERR_ARG, "NULL pointer");
+  // the adjoint's conditions (the cotangent arguments are not used here)
+  int rc = vjp_checks(c, U, aux, dt, res, res, res, res);
+  if (rc) return rc;
+  return step_keep_impl(c, (const double*)U, (const double*)aux, dt, (double*)U_next,
+                        (double*)aux_next, (double*)res);
+}
+
+int jh_step_vjp_res(jh_ctx* c, const void* U, const void* aux, double dt, const void* res,
+                    const void* U_next, const void* aux_next, const void* U_bar,
+                    const void* aux_bar, void* U_in_bar, void* aux_in_bar) {
+  if (!c) return JH_ERR_ARG;
+  if (!res || !U_next || !aux_next) return fail(c, JH_ERR_ARG, "NULL pointer");
+  int rc = vjp_checks(c, U, aux, dt, U_bar, aux_bar, U_in_bar, aux_in_bar);
+  if (rc) return rc;
+  return step_vjp_res_impl(c, (const double*)U, (const double*)aux, dt, (const double*)res,
+                           (const double*)U_next, (const double*)aux_next, (const double*)U_bar,
+                           (const double*)aux_bar, (double*)U_in_bar, (double*)aux_in_bar);
 }
 
 int jh_timing_enable(jh_ctx* c, int every) {

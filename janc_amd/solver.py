@@ -434,22 +434,55 @@ class Simulator:
         ctx.call("jh_lf_alpha", _ptr(U), _ptr(aux), _ptr(out))
         return out
 
-    def step_vjp(self, U, aux, dt, U_bar, aux_bar, with_output=False, known_output=None):
+    def _adjoint_form(self):
+        if self.precision != _lib.PREC_FP64:
+            raise NotImplementedError("the adjoint is built for the FP64 form")
+        if self.flux != (0, 0, 0, 0):
+            raise NotImplementedError("the adjoint is built for LF flux splitting with WENO5_JS")
+
+    def step_keep(self, U, aux, dt):
+        """One fixed-dt FP64 step that also returns its residuals for
+        reverse mode (jh_step_keep: the RK stage states and their LF maxima,
+        what jax.vjp of set_simulator.py:117-122 keeps): (U_next, aux_next,
+        res), res a flat float64 tensor for step_vjp(..., residuals=res)."""
+        self._adjoint_form()
+        U, aux = self._state(U, aux)
+        ctx = self._context(U, None)
+        cnt = ctypes.c_longlong(0)
+        _lib.check(ctx.lib.jh_step_res_count(ctx.h, ctypes.byref(cnt)), ctx.h)
+        res = torch.empty((cnt.value,), dtype=torch.float64, device=U.device)
+        Uo, ao = torch.empty_like(U), torch.empty_like(aux)
+        ctx.call("jh_step_keep", _ptr(U), _ptr(aux), float(dt), _ptr(Uo), _ptr(ao), _ptr(res))
+        return Uo, ao, res
+
+    def step_vjp(self, U, aux, dt, U_bar, aux_bar, with_output=False, known_output=None,
+                 residuals=None):
         """Reverse-mode derivative of advance_one_step at fixed dt (FP64).
 
         Returns (U_in_bar, aux_in_bar), plus the step's output (U, aux) when
         ``with_output``: the VJP of set_simulator.py:117-122 under JAX's
         rules, get_T_nasa7's custom VJP (thermo_model.py:155-190) included.
         ``known_output`` = the step's output (U, aux) when the caller has it
-        (jh_step_vjp_known: the chemistry's forward pass is not repeated)."""
-        if self.precision != _lib.PREC_FP64:
-            raise NotImplementedError("the adjoint is built for the FP64 form")
-        if self.flux != (0, 0, 0, 0):
-            raise NotImplementedError("the adjoint is built for LF flux splitting with WENO5_JS")
+        (jh_step_vjp_known: the chemistry's forward pass is not repeated);
+        with it, ``residuals`` = step_keep's res (jh_step_vjp_res: nothing of
+        the forward is recomputed)."""
+        self._adjoint_form()
         U, aux = self._state(U, aux)
         Ub, ab = self._state(U_bar, aux_bar)
         ctx = self._context(U, None)
         gU, ga = torch.empty_like(U), torch.empty_like(aux)
+        if residuals is not None:
+            if known_output is None or with_output:
+                raise ValueError("residuals go with known_output (and without with_output)")
+            Un, an = self._state(*known_output)
+            cnt = ctypes.c_longlong(0)
+            _lib.check(ctx.lib.jh_step_res_count(ctx.h, ctypes.byref(cnt)), ctx.h)
+            if residuals.dtype != torch.float64 or not residuals.is_contiguous() \
+                    or residuals.device != U.device or residuals.numel() != cnt.value:
+                raise ValueError("residuals must be step_keep's float64 tensor for this grid")
+            ctx.call("jh_step_vjp_res", _ptr(U), _ptr(aux), float(dt), _ptr(residuals),
+                     _ptr(Un), _ptr(an), _ptr(Ub), _ptr(ab), _ptr(gU), _ptr(ga))
+            return gU, ga
         if known_output is not None and not with_output:
             Un, an = self._state(*known_output)
             ctx.call("jh_step_vjp_known", _ptr(U), _ptr(aux), float(dt), _ptr(Un), _ptr(an),

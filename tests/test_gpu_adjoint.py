@@ -171,6 +171,24 @@ def test_vjp_with_known_output_is_identical(mech_path, mech):
     assert torch.equal(gU, kU) and torch.equal(ga, ka)
 
 
+@pytest.mark.parametrize("name", ["tube", "walls"])
+def test_vjp_from_residuals_is_identical(mech_path, mech, name):
+    """jh_step_keep's output is the step's (jh_step_vjp's U_out) and
+    jh_step_vjp_res from its residuals gives jh_step_vjp_known's cotangents,
+    bit for bit; the residuals are not modified by the VJP."""
+    sim, g, U, aux = _setup(mech_path, mech, name)
+    W, Wa = _cotangents(U, aux, 11)
+    gU, ga, Uo, ao = sim.step_vjp(U, aux, DT, W, Wa, with_output=True)
+    Uk, ak, res = sim.step_keep(U, aux, DT)
+    assert torch.equal(Uk, Uo) and torch.equal(ak, ao)
+    r0 = res.clone()
+    rU, ra = sim.step_vjp(U, aux, DT, W, Wa, known_output=(Uk, ak), residuals=res)
+    assert torch.equal(rU, gU) and torch.equal(ra, ga)
+    assert torch.equal(res, r0)
+    with pytest.raises(ValueError):
+        sim.step_vjp(U, aux, DT, W, Wa, known_output=(Uk, ak), residuals=res[:-1].contiguous())
+
+
 def test_rollout_checkpointing_matches_chained_steps(mech_path, mech):
     from janc_amd import adjoint
     sim, g, U, aux = _setup(mech_path, mech, "tube")
