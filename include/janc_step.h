@@ -352,6 +352,21 @@ typedef struct jh_blocks jh_blocks;
 int jh_blocks_create(int bnx, int bny, double dx, double dy, int precision,
                      const double* mech_consts, int n_consts, void* stream, jh_blocks** out);
 int jh_blocks_destroy(jh_blocks* blocks);
+/* Ghost refill of n_blk same-size blocks (bnx x bny) tiling one uniform
+ * level: the padded blocks out (n_blk, 14, bnx + 6, bny + 6) are gathered
+ * from the level's interiors, out[b][v][p] = interior cell idx[b P + p]
+ * (flat index block * bnx * bny + x * bny + y) of plane v times its sign
+ * table (s_u for plane 1, s_v for plane 2; n_blk P entries, P = (bnx + 6)
+ * (bny + 6)).  The source is read through element strides sb (block), sv
+ * (plane), sr (row), columns contiguous: interior blocks, or the interior
+ * view of padded ones.  Replaces the ghost-layer fill of the reference's AMR
+ * hierarchy (jaxamr's get_ghost_block_data, called at time_step/TVD_RK3.py:
+ * 16,23,30) for one uniform level with the domain's boundary padding; the
+ * tables come from janc_amd.amr.UniformBlockGrid.  Runs on stream (NULL:
+ * the default stream); errors in jh_blocks_last_error(NULL). */
+int jh_blocks_fill_ghosts(void* stream, int precision, int n_blk, int bnx, int bny,
+                          const void* src, long long sb, long long sv, long long sr,
+                          const int* idx, const void* s_u, const void* s_v, void* out);
 int jh_blocks_set_stream(jh_blocks* blocks, void* stream);
 const char* jh_blocks_last_error(const jh_blocks* blocks);
 /* flux_func of the AMR branch (set_simulator.py:37-40): blk (n_blk, 14,

@@ -90,6 +90,9 @@ SIGNATURES = {
     "jh_blocks_create": (_c_int, [_c_int, _c_int, _c_double, _c_double, _c_int,
                                   ctypes.POINTER(_c_double), _c_int, _vp, ctypes.POINTER(_vp)]),
     "jh_blocks_destroy": (_c_int, [_vp]),
+    "jh_blocks_fill_ghosts": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp,
+                                       ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
+                                       _vp, _vp, _vp, _vp]),
     "jh_blocks_set_stream": (_c_int, [_vp, _vp]),
     "jh_blocks_last_error": (ctypes.c_char_p, [_vp]),
     "jh_blocks_flux": (_c_int, [_vp, _c_int, _vp, _c_double, _vp]),

@@ -190,7 +190,10 @@ class UniformBlockGrid:
     (slip_wall.py), no_slip_wall mirrors with both momenta negated
     (no_slip_wall.py); corners keep the periodic pre-pad.  Pressure outlets,
     injection inlets and callables compute their ghosts and are not offered.
-    One gather per refill (torch, on the blocks' device)."""
+    One gather per refill: on the GPU the library's blocks_fill_kernel
+    (jh_blocks_fill_ghosts, reading the interiors through their strides, so
+    update_external_boundary hands on the stage output's interior view
+    without a copy); elsewhere torch's index_select."""
 
     KINDS = ("periodic", "zero_gradient", "slip_wall", "no_slip_wall")
 
@@ -247,7 +250,13 @@ class UniformBlockGrid:
         PX, PY = bnx + 6, bny + 6
         bi = torch.arange(nbx)[:, None, None, None] * bnx + torch.arange(PX)[None, None, :, None]
         bj = torch.arange(nby)[None, :, None, None] * bny + torch.arange(PY)[None, None, None, :]
-        self.idx = col[bi, bj].reshape(-1).to(device)             # (n_blk * PX * PY,)
+        idx = col[bi, bj].reshape(-1)
+        # every entry indexes the level's interiors (the gather kernel reads
+        # through them unchecked)
+        assert int(idx.min()) >= 0 and int(idx.max()) < nbx * nby * bnx * bny
+        self.idx = idx.to(device)                                  # (n_blk * PX * PY,)
+        self._idx32 = idx.to(torch.int32).to(device)
+        self._signs = {}
         self.s_u = s_u[bi, bj].reshape(nbx * nby, PX, PY).to(device)
         self.s_v = s_v[bi, bj].reshape(nbx * nby, PX, PY).to(device)
         self.n_blk = nbx * nby
@@ -266,9 +275,36 @@ class UniformBlockGrid:
         full = b.permute(2, 0, 3, 1, 4).reshape(nv, self.nbx * self.bnx, self.nby * self.bny)
         return full[:-2], full[-2:]
 
+    def _fill_gpu(self, blk):
+        from .solver import _ptr
+        if blk.dtype not in (torch.float64, torch.float32) or blk.dim() != 4 \
+                or tuple(blk.shape) != (self.n_blk, 14, self.bnx, self.bny) \
+                or blk.stride(3) != 1 or blk.device != self._idx32.device:
+            raise ValueError("blocks must be (%d, 14, %d, %d) float CUDA tensors with "
+                             "contiguous columns on the grid's device"
+                             % (self.n_blk, self.bnx, self.bny))
+        if blk.dtype not in self._signs:
+            self._signs[blk.dtype] = (self.s_u.to(blk.dtype).contiguous(),
+                                      self.s_v.to(blk.dtype).contiguous())
+        su, sv = self._signs[blk.dtype]
+        out = torch.empty((self.n_blk, 14, self.bnx + 6, self.bny + 6), dtype=blk.dtype,
+                          device=blk.device)
+        lib = _lib.load()
+        prec = _lib.PREC_FP64 if blk.dtype == torch.float64 else _lib.PREC_MIXED
+        rc = lib.jh_blocks_fill_ghosts(ctypes.c_void_p(torch.cuda.current_stream().cuda_stream),
+                                       prec, self.n_blk, self.bnx, self.bny, _ptr(blk),
+                                       blk.stride(0), blk.stride(1), blk.stride(2),
+                                       _ptr(self._idx32), _ptr(su), _ptr(sv), _ptr(out))
+        if rc != _lib.JH_OK:
+            raise _lib.JancError("jh_blocks_fill_ghosts: "
+                                 + lib.jh_blocks_last_error(None).decode())
+        return out
+
     def get_ghost_block_data(self, blk, info=None):
         """Interior blocks (n_blk, 14, bnx, bny) -> padded blocks (n_blk, 14,
         bnx + 6, bny + 6)."""
+        if blk.is_cuda:
+            return self._fill_gpu(blk)
         nb, nv = blk.shape[0], blk.shape[1]
         flat = blk.reshape(nb, nv, -1).permute(1, 0, 2).reshape(nv, -1)
         g = flat.index_select(1, self.idx).reshape(nv, nb, self.bnx + 6, self.bny + 6)
@@ -277,5 +313,6 @@ class UniformBlockGrid:
         return g.permute(1, 0, 2, 3).contiguous()
 
     def update_external_boundary(self, level, parent, interiors, info=None):
-        """A single level: the interiors are the blocks."""
-        return interiors.contiguous()
+        """A single level: the interiors are the blocks (on the GPU the view
+        itself; get_ghost_block_data reads it through its strides)."""
+        return interiors if interiors.is_cuda else interiors.contiguous()

@@ -172,4 +172,28 @@ void blocks_chem_kernel(StepArgs<Real> a, long long bs, Real* B) {
                              red, redb, redg, mix, gt);
 }
 
+// Ghost refill of same-size blocks tiling one uniform level: out[b][v][p]
+// (padded blocks, P = (bnx + 6)(bny + 6) cells each) = the interior cell
+// idx[b P + p] of the level (flat index block * bnx * bny + x * bny + y) of
+// plane v, times the cell's momentum sign (su for v = 1, sv for v = 2).
+// The source is addressed by strides (block, plane, row; columns
+// contiguous), so a padded stage output's interior view needs no copy.
+// Grid: (ceil(n_blk P / 256), NV).
+template <typename Real>
+__global__ __launch_bounds__(256) void blocks_fill_kernel(
+    const Real* __restrict__ src, long long sb, long long sv, long long sr, int bnx, int bny,
+    const int* __restrict__ idx, const Real* __restrict__ su, const Real* __restrict__ svg,
+    Real* __restrict__ out, int n_blk, int P) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (i >= (long long)n_blk * P) return;
+  const long long b = i / P, p = i - b * P;
+  const int col = idx[i], B = bnx * bny;
+  const int cb = col / B, loc = col - cb * B, lx = loc / bny, ly = loc - lx * bny;
+  Real x = src[cb * sb + v * sv + lx * sr + ly];
+  if (v == 1) x *= su[i];
+  else if (v == 2) x *= svg[i];
+  out[(b * NV + v) * P + p] = x;
+}
+
 }  // namespace jh

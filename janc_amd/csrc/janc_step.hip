@@ -1820,6 +1820,33 @@ int jh_blocks_create(int bnx, int bny, double dx, double dy, int precision, cons
   return JH_OK;
 }
 
+int jh_blocks_fill_ghosts(void* stream, int precision, int n_blk, int bnx, int bny,
+                          const void* src, long long sb, long long sv, long long sr,
+                          const int* idx, const void* s_u, const void* s_v, void* out) {
+  if (!src || !idx || !s_u || !s_v || !out) return kfail(nullptr, JH_ERR_ARG, "NULL pointer");
+  if (n_blk < 1 || bnx < 1 || bny < 1 || sb < 1 || sv < 1 || sr < bny)
+    return kfail(nullptr, JH_ERR_ARG, "bad block count, size or strides");
+  if (precision != JH_PREC_FP64 && precision != JH_PREC_MIXED)
+    return kfail(nullptr, JH_ERR_ARG, "unknown precision");
+  const long long P = (long long)(bnx + 6) * (bny + 6), m = n_blk * P;
+  if (m >= (1ll << 31) || (long long)n_blk * bnx * bny >= (1ll << 31))
+    return kfail(nullptr, JH_ERR_ARG, "too many cells for 32-bit indices");
+  const dim3 grid((unsigned)((m + 255) / 256), NV);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (precision == JH_PREC_FP64)
+    blocks_fill_kernel<double><<<grid, 256, 0, st>>>(
+        (const double*)src, sb, sv, sr, bnx, bny, idx, (const double*)s_u, (const double*)s_v,
+        (double*)out, n_blk, (int)P);
+  else
+    blocks_fill_kernel<float><<<grid, 256, 0, st>>>(
+        (const float*)src, sb, sv, sr, bnx, bny, idx, (const float*)s_u, (const float*)s_v,
+        (float*)out, n_blk, (int)P);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return kfail(nullptr, JH_ERR_HIP, std::string("blocks_fill_kernel: ") +
+                                                             hipGetErrorString(e));
+  return JH_OK;
+}
+
 int jh_blocks_destroy(jh_blocks* b) {
   if (!b) return JH_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream); else (void)hipDeviceSynchronize();

@@ -183,3 +183,26 @@ def test_blocks_health_flag(mech_path, setup):
     f = bs.health()
     assert f & _lib.HEALTH_FLAGS["negative_rho"], _lib.health_names(f)
     assert bs.health() == 0                           # cleared by the read
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_gpu_ghost_refill_equals_torch_gather(dtype):
+    """UniformBlockGrid's GPU refill (jh_blocks_fill_ghosts) equals its torch
+    gather on the CPU bit for bit, from contiguous interior blocks and from
+    the interior view of padded blocks, for every copy kind."""
+    from janc_amd.amr import UniformBlockGrid
+    bc = {"left_boundary": "no_slip_wall", "right_boundary": "zero_gradient",
+          "bottom_boundary": "periodic", "top_boundary": "periodic"}
+    nbx, nby, bnx, bny = 3, 4, 7, 5
+    g_gpu = UniformBlockGrid(bc, nbx, nby, bnx, bny, device="cuda")
+    g_cpu = UniformBlockGrid(bc, nbx, nby, bnx, bny, device="cpu")
+    gen = torch.Generator().manual_seed(2)
+    blk = torch.randn((nbx * nby, 14, bnx, bny), generator=gen, dtype=torch.float64).to(dtype)
+    ref = g_cpu.get_ghost_block_data(blk)
+    got = g_gpu.get_ghost_block_data(blk.cuda())
+    assert torch.equal(got.cpu(), ref)
+    pad = torch.randn((nbx * nby, 14, bnx + 6, bny + 6), generator=gen,
+                      dtype=torch.float64).to(dtype)
+    pad[..., 3:-3, 3:-3] = blk
+    view = g_gpu.update_external_boundary(1, None, pad.cuda()[..., 3:-3, 3:-3])
+    assert torch.equal(g_gpu.get_ghost_block_data(view).cpu(), ref)
